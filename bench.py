@@ -1,0 +1,150 @@
+"""Benchmark: FDK backprojection throughput (Gvoxel-updates/s) on one MI355X.
+
+Workload: 256^3 volume (1 mm voxels), 360 views over 360 deg, 512 x 512 flat
+detector (1 mm pitch, SOD 600, SDD 1200 -> 0.5 mm at the isocentre).  The
+3-D Shepp-Logan phantom is forward projected and ramp filtered on the GPU
+before the timed region; a step is one FDK backprojection of all 360
+filtered views into the volume, inputs resident in HBM.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ct-research_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=120, help="views in the bounded CPU-baseline sample")
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the FDK kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel); see
+    profiles/fdk_bp_256_360_traffic.txt.  None when not measured."""
+    path = os.path.join(ROOT, "profiles", "fdk_bp_256_360_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(geom_args, n_views):
+    """The FP64 CPU restatement of the reference FDK backprojection
+    (oracle/ctr_oracle.c or_fdk_bp, OpenMP over all host threads) on a
+    bounded sample: the full 256^3 volume and the first n_views views."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nt, ns, nz, size, D, beta, p, xi = geom_args
+    beta = beta[:n_views]
+    og = O.Cone(nt, ns, nz, size, D, beta, p, xi)
+    y = np.random.default_rng(0).standard_normal(og.nray).astype(np.float32)
+    t0 = time.perf_counter()
+    O.fdk_bp(og, y)
+    dt = time.perf_counter() - t0
+    gvox = nt * ns * nz * n_views / dt / 1e9
+    return {"value": gvox, "unit": "Gvoxel-upd/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"FP64 restatement of FDK.cu BackProjection, 256^3 volume x {n_views} of 360 views "
+                      f"({dt:.1f} s, OpenMP)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import ctr
+    from ctr import geometry, phantoms
+
+    dev = torch.device("cuda:0")
+    ctx = ctr.Context(0)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+
+    g = geometry.config_cone_256(360)
+    nvox, nviews, ndet = g.nvox, len(g.beta), len(g.p) * len(g.xi)
+    vol = torch.from_numpy(phantoms.phantom3d(256, "shepp-logan", 12).ravel()).to(dev)
+    proj = ctx.cone_fp(g, vol)
+    filt = ctx.cone_filter(g, proj, filt=ctr.FILTER_SL_HALF)
+    out = torch.empty(nvox, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    # filter timing (reported separately, not part of the metric)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ctx.cone_filter(g, proj, filt=ctr.FILTER_SL_HALF, out=filt)
+    e0.record(stream)
+    for _ in range(5):
+        ctx.cone_filter(g, proj, filt=ctr.FILTER_SL_HALF, out=filt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    filter_ms = e0.elapsed_time(e1) / 5
+
+    for _ in range(args.warmup):
+        ctx.fdk_bp(g, filt, out=out)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        ctx.fdk_bp(g, filt, out=out)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps     # one kernel per step on this stream
+    ms_per_step = wall * 1000.0 / args.steps
+    value = nvox * nviews / (ms_per_step * 1e-3) / 1e9
+
+    # roofline: algorithmic bytes of the streaming dataflow with K views per pass
+    K = nviews
+    alg_bytes = math.ceil(nviews / K) * 8 * nvox + 4 * nviews * ndet
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    k1_bytes = nviews * 8 * nvox + 4 * nviews * ndet
+    res = {
+        "metric": "Gvoxel-upd/s (FDK BP)",
+        "value": round(value, 2),
+        "unit": "Gvoxel-upd/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (3-D Shepp-Logan, 12 ellipsoids, forward projected and SL-half filtered on the GPU)",
+        "config": {"workload": "fdk_bp 256^3 vol 1mm, 360 views, 512x512 flat det 1mm, SOD 600, SDD 1200",
+                   "views_per_pass": K, "filter_ms": round(filter_ms, 4), "kernel_ms": round(kernel_ms, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profiles()},
+        "roofline_k1_equiv": {"bytes": k1_bytes, "achieved": round(k1_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+                              "frac": round(k1_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline((g.nt, g.ns, g.nz, g.size, g.distance, g.beta, g.p, g.xi),
+                                           args.cpu_views)
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

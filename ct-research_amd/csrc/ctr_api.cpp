@@ -1,0 +1,1055 @@
+// ctr_api.cpp - the C ABI of libctr: contexts, argument checks, geometry
+// tables, workspaces and dispatch to the HIP kernels.  There is no CPU
+// compute path: every operator runs on the GPU or returns an error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "ctr_internal.h"
+
+using namespace ctr;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsCount };
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct ConeTables {
+    uint64_t key = 0;
+    bool valid = false;
+    std::vector<double> h_view;     // [nb][4]
+    std::vector<double> h_pd, h_xd; // positions (double)
+    std::vector<BpView> h_bp;
+    std::vector<float> h_pf, h_xf;
+    DevBuf d_view, d_p, d_xi, d_bp, d_pf, d_xf;
+};
+
+struct FanTables {
+    uint64_t key = 0;
+    bool valid = false;
+    std::vector<double> h_view, h_p, h_zero;
+    std::vector<float> h_pf, h_gam, h_colw, h_xf;
+    DevBuf d_view, d_p, d_zero, d_pf, d_gam, d_colw, d_xf;
+};
+
+struct TapTable {
+    uint64_t key = 0;
+    bool valid = false;
+    std::vector<double> h;
+    DevBuf d;
+};
+
+}  // namespace
+
+struct ctr_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    DevBuf ws[kWsCount];
+    ConeTables cone;
+    FanTables fan;
+    TapTable taps;
+    int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
+};
+
+namespace {
+
+ctr_status fail(ctr_ctx *ctx, ctr_status st, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->last_error = buf;
+    g_last_error = buf;
+    return st;
+}
+
+#define HIP_TRY(ctx, expr)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail((ctx), CTR_EHIP, "%s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+int parse_fault()
+{
+    const char *f = getenv("CTR_FAULT");
+    if (!f) return 0;
+    if (!strcmp(f, "alloc")) return 1;
+    if (!strcmp(f, "launch")) return 2;
+    if (!strcmp(f, "copy")) return 3;
+    return 0;
+}
+
+ctr_status buf_reserve(ctr_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    if (ctx->fault == 1) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=alloc: device allocation of %zu bytes", bytes);
+    if (bytes <= b.bytes && b.ptr) return CTR_OK;
+    if (b.ptr) {
+        // the old buffer may still be read by queued work
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, CTR_ENOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    b.ptr = p;
+    b.bytes = bytes;
+    return CTR_OK;
+}
+
+void buf_free(DevBuf &b)
+{
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+ctr_status upload(ctr_ctx *ctx, DevBuf &b, const void *src, size_t bytes)
+{
+    ctr_status s = buf_reserve(ctx, b, bytes);
+    if (s != CTR_OK) return s;
+    if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy: host-to-device copy");
+    HIP_TRY(ctx, hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status check_launch(ctr_ctx *ctx, hipError_t e, const char *what)
+{
+    if (ctx->fault == 2) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=launch: %s", what);
+    if (e != hipSuccess) return fail(ctx, CTR_EHIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return CTR_OK;
+}
+
+uint64_t fnv(uint64_t h, const void *data, size_t n)
+{
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }
+
+template <typename T>
+ctr_status check_axis(const T *a, int n, bool need_uniform, const char *name)
+{
+    if (!a) return fail(nullptr, CTR_EINVAL, "%s is NULL", name);
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite((double)a[i])) return fail(nullptr, CTR_EINVAL, "%s[%d] is not finite", name, i);
+    if (n < 2) return CTR_OK;
+    const double d0 = (double)a[1] - (double)a[0];
+    if (d0 == 0.0) return fail(nullptr, CTR_EINVAL, "%s is not strictly monotonic", name);
+    for (int i = 1; i < n; ++i) {
+        const double d = (double)a[i] - (double)a[i - 1];
+        if ((d > 0) != (d0 > 0) || d == 0.0) return fail(nullptr, CTR_EINVAL, "%s is not strictly monotonic", name);
+        if (need_uniform && std::fabs(d - d0) > 1e-3 * std::fabs(d0))
+            return fail(nullptr, CTR_EINVAL, "%s has a non-uniform pitch (step %d is %g, first step %g)", name, i, d, d0);
+    }
+    return CTR_OK;
+}
+
+ctr_status check_cone(const ctr_cone_geom *g, int need_uniform)
+{
+    if (!g) return fail(nullptr, CTR_EINVAL, "geometry is NULL");
+    if (g->nt < 1 || g->ns < 1 || g->nz < 1) return fail(nullptr, CTR_EINVAL, "volume dimensions must be positive");
+    if ((double)g->nt * g->ns * g->nz > (double)std::numeric_limits<int>::max())
+        return fail(nullptr, CTR_ETOOBIG, "volume has more than 2^31-1 voxels");
+    for (int a = 0; a < 3; ++a)
+        if (!finite_pos(g->size[a])) return fail(nullptr, CTR_EINVAL, "size[%d] must be positive and finite", a);
+    if (!finite_pos(g->distance)) return fail(nullptr, CTR_EINVAL, "distance must be positive and finite");
+    if (g->n_beta < 1 || g->n_p < 1 || g->n_xi < 1) return fail(nullptr, CTR_EINVAL, "detector/view counts must be positive");
+    if (need_uniform && (g->n_p < 2 || g->n_xi < 2))
+        return fail(nullptr, CTR_EINVAL, "backprojection needs at least 2 detector columns and rows");
+    if ((double)g->n_beta * g->n_p * g->n_xi > (double)std::numeric_limits<int>::max())
+        return fail(nullptr, CTR_ETOOBIG, "more than 2^31-1 detector samples");
+    const double half = 0.5 * std::sqrt(g->size[0] * g->size[0] + g->size[1] * g->size[1]);
+    if (!(g->distance > half * (1.0 + 1e-6)))
+        return fail(nullptr, CTR_EINVAL, "source orbit (distance %g) intersects the volume (half diagonal %g)",
+                    g->distance, half);
+    // view angles need not be monotonic: only finite
+    if (!g->beta) return fail(nullptr, CTR_EINVAL, "beta is NULL");
+    for (int i = 0; i < g->n_beta; ++i)
+        if (!std::isfinite(g->beta[i])) return fail(nullptr, CTR_EINVAL, "beta[%d] is not finite", i);
+    ctr_status s;
+    if ((s = check_axis(g->p, g->n_p, need_uniform != 0, "p")) != CTR_OK) return s;
+    if ((s = check_axis(g->xi, g->n_xi, need_uniform != 0, "xi")) != CTR_OK) return s;
+    return CTR_OK;
+}
+
+ctr_status check_fan(const ctr_fan_geom *g, int need_uniform)
+{
+    if (!g) return fail(nullptr, CTR_EINVAL, "geometry is NULL");
+    if (g->nt < 1 || g->ns < 1) return fail(nullptr, CTR_EINVAL, "image dimensions must be positive");
+    if ((double)g->nt * g->ns > (double)std::numeric_limits<int>::max())
+        return fail(nullptr, CTR_ETOOBIG, "image has more than 2^31-1 pixels");
+    for (int a = 0; a < 2; ++a)
+        if (!finite_pos(g->size[a])) return fail(nullptr, CTR_EINVAL, "size[%d] must be positive and finite", a);
+    if (!finite_pos(g->distance)) return fail(nullptr, CTR_EINVAL, "distance must be positive and finite");
+    if (g->n_beta < 1 || g->n_p < 1) return fail(nullptr, CTR_EINVAL, "detector/view counts must be positive");
+    if (need_uniform && g->n_p < 2) return fail(nullptr, CTR_EINVAL, "backprojection needs at least 2 detector bins");
+    const double half = 0.5 * std::sqrt(g->size[0] * g->size[0] + g->size[1] * g->size[1]);
+    if (!(g->distance > half * (1.0 + 1e-6)))
+        return fail(nullptr, CTR_EINVAL, "source orbit (distance %g) intersects the image (half diagonal %g)",
+                    g->distance, half);
+    if (!g->beta) return fail(nullptr, CTR_EINVAL, "beta is NULL");
+    for (int i = 0; i < g->n_beta; ++i)
+        if (!std::isfinite(g->beta[i])) return fail(nullptr, CTR_EINVAL, "beta[%d] is not finite", i);
+    return check_axis(g->p, g->n_p, need_uniform != 0, "p");
+}
+
+ctr_status need_ctx(ctr_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, CTR_EINVAL, "context is NULL");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return CTR_OK;
+}
+
+// ---- geometry tables -------------------------------------------------------
+
+uint64_t cone_key(const ctr_cone_geom *g)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, &g->nt, sizeof(int) * 3);
+    h = fnv(h, g->size, sizeof(double) * 3);
+    h = fnv(h, &g->distance, sizeof(double));
+    h = fnv(h, &g->n_beta, sizeof(int) * 3);
+    h = fnv(h, g->beta, sizeof(float) * g->n_beta);
+    h = fnv(h, g->p, sizeof(float) * g->n_p);
+    h = fnv(h, g->xi, sizeof(float) * g->n_xi);
+    return h;
+}
+
+ctr_status cone_tables(ctr_ctx *ctx, const ctr_cone_geom *g)
+{
+    ConeTables &T = ctx->cone;
+    const uint64_t key = cone_key(g);
+    if (T.valid && T.key == key) return CTR_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // queued work may still read the old tables
+    T.valid = false;
+    const int nb = g->n_beta;
+    const double D = g->distance, Ct = g->size[0] / 2, Cs = g->size[1] / 2;
+    const double rt = g->size[0] / g->nt, rs = g->size[1] / g->ns;
+    const float dbeta = nb >= 2 ? std::fabs(g->beta[1] - g->beta[0]) : 0.0f;
+    T.h_view.resize(4 * (size_t)nb);
+    T.h_bp.resize(nb);
+    for (int b = 0; b < nb; ++b) {
+        const double beta = (double)g->beta[b], sb = std::sin(beta), cb = std::cos(beta);
+        T.h_view[4 * b + 0] = sb;
+        T.h_view[4 * b + 1] = cb;
+        T.h_view[4 * b + 2] = Ct - D * sb;
+        T.h_view[4 * b + 3] = Cs + D * cb;
+        const double x0 = 0.5 * rt - Ct, y0 = 0.5 * rs - Cs;
+        BpView v{};
+        v.at = (float)(rt * cb);
+        v.as = (float)(rs * sb);
+        v.a0 = (float)(x0 * cb + y0 * sb);
+        v.bt = (float)(-rt * sb);
+        v.bs = (float)(rs * cb);
+        v.b0 = (float)(-x0 * sb + y0 * cb);
+        v.wv = dbeta;
+        v.st = (float)(Ct - D * sb);
+        v.ss = (float)(Cs + D * cb);
+        v.sb = (float)sb;
+        v.cb = (float)cb;
+        T.h_bp[b] = v;
+    }
+    T.h_pd.assign(g->p, g->p + g->n_p);
+    T.h_xd.assign(g->xi, g->xi + g->n_xi);
+    T.h_pf.assign(g->p, g->p + g->n_p);
+    T.h_xf.assign(g->xi, g->xi + g->n_xi);
+    ctr_status s;
+    if ((s = upload(ctx, T.d_view, T.h_view.data(), sizeof(double) * T.h_view.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_p, T.h_pd.data(), sizeof(double) * T.h_pd.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_xi, T.h_xd.data(), sizeof(double) * T.h_xd.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_bp, T.h_bp.data(), sizeof(BpView) * T.h_bp.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_pf, T.h_pf.data(), sizeof(float) * T.h_pf.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_xf, T.h_xf.data(), sizeof(float) * T.h_xf.size())) != CTR_OK) return s;
+    T.key = key;
+    T.valid = true;
+    return CTR_OK;
+}
+
+RayParams cone_ray_params(ctr_ctx *ctx, const ctr_cone_geom *g)
+{
+    const ConeTables &T = ctx->cone;
+    RayParams r{};
+    r.nt = g->nt;
+    r.ns = g->ns;
+    r.nz = g->nz;
+    r.np = g->n_p;
+    r.nx = g->n_xi;
+    r.nb = g->n_beta;
+    r.rt = g->size[0] / g->nt;
+    r.rs = g->size[1] / g->ns;
+    r.rz = g->size[2] / g->nz;
+    r.bt = g->size[0];
+    r.bs = g->size[1];
+    r.bz = g->size[2];
+    r.D = g->distance;
+    r.sz = g->size[2] / 2;
+    r.view = (const double *)T.d_view.ptr;
+    r.p = (const double *)T.d_p.ptr;
+    r.xi = (const double *)T.d_xi.ptr;
+    return r;
+}
+
+BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
+{
+    const ConeTables &T = ctx->cone;
+    BpParams P{};
+    P.nt = g->nt;
+    P.ns = g->ns;
+    P.nz = g->nz;
+    P.np = g->n_p;
+    P.nx = g->n_xi;
+    P.nb = g->n_beta;
+    P.rt = (float)(g->size[0] / g->nt);
+    P.rs = (float)(g->size[1] / g->ns);
+    P.rz = (float)(g->size[2] / g->nz);
+    P.zc0 = (float)(0.5 * g->size[2] / g->nz - g->size[2] / 2);
+    P.D = (float)g->distance;
+    const double dp = (double)g->p[1] - (double)g->p[0], dx = (double)g->xi[1] - (double)g->xi[0];
+    P.u_scale = (float)(1.0 / dp);
+    P.u_off = (float)(-(double)g->p[0] / dp);
+    P.v_scale = (float)(1.0 / dx);
+    P.v_off = (float)(-(double)g->xi[0] / dx);
+    P.umax = (float)(g->n_p - 1);
+    P.vmax = (float)(g->n_xi - 1);
+    P.ht = 0.5f * P.rt;
+    P.hs = 0.5f * P.rs;
+    P.hz = 0.5f * P.rz;
+    P.views = (const BpView *)T.d_bp.ptr;
+    P.pd = (const float *)T.d_pf.ptr;
+    P.xd = (const float *)T.d_xf.ptr;
+    P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
+    P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
+    return P;
+}
+
+uint64_t fan_key(const ctr_fan_geom *g)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, &g->nt, sizeof(int) * 2);
+    h = fnv(h, g->size, sizeof(double) * 2);
+    h = fnv(h, &g->distance, sizeof(double));
+    h = fnv(h, &g->n_beta, sizeof(int) * 2);
+    h = fnv(h, g->beta, sizeof(double) * g->n_beta);
+    h = fnv(h, g->p, sizeof(double) * g->n_p);
+    return h;
+}
+
+ctr_status fan_tables(ctr_ctx *ctx, const ctr_fan_geom *g)
+{
+    FanTables &T = ctx->fan;
+    const uint64_t key = fan_key(g);
+    if (T.valid && T.key == key) return CTR_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    T.valid = false;
+    const int nb = g->n_beta, np = g->n_p;
+    const double D = g->distance, Ct = g->size[0] / 2, Cs = g->size[1] / 2;
+    T.h_view.resize(4 * (size_t)nb);
+    for (int b = 0; b < nb; ++b) {
+        const double sb = std::sin(g->beta[b]), cb = std::cos(g->beta[b]);
+        T.h_view[4 * b + 0] = sb;
+        T.h_view[4 * b + 1] = cb;
+        T.h_view[4 * b + 2] = Ct - D * sb;
+        T.h_view[4 * b + 3] = Cs + D * cb;
+    }
+    T.h_p.assign(g->p, g->p + np);
+    T.h_zero.assign(1, 0.0);
+    T.h_pf.resize(np);
+    T.h_gam.resize(np);
+    T.h_colw.resize(np);
+    for (int i = 0; i < np; ++i) {
+        T.h_pf[i] = (float)g->p[i];
+        const double gam = std::atan(g->p[i] / D);
+        T.h_gam[i] = (float)gam;
+        T.h_colw[i] = (float)(D * std::cos(gam));
+    }
+    T.h_xf.assign(1, 0.0f);
+    ctr_status s;
+    if ((s = upload(ctx, T.d_view, T.h_view.data(), sizeof(double) * T.h_view.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_p, T.h_p.data(), sizeof(double) * np)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_zero, T.h_zero.data(), sizeof(double))) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_pf, T.h_pf.data(), sizeof(float) * np)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_gam, T.h_gam.data(), sizeof(float) * np)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_colw, T.h_colw.data(), sizeof(float) * np)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_xf, T.h_xf.data(), sizeof(float))) != CTR_OK) return s;
+    T.key = key;
+    T.valid = true;
+    return CTR_OK;
+}
+
+RayParams fan_ray_params(ctr_ctx *ctx, const ctr_fan_geom *g)
+{
+    const FanTables &T = ctx->fan;
+    RayParams r{};
+    r.nt = g->nt;
+    r.ns = g->ns;
+    r.nz = 1;
+    r.np = g->n_p;
+    r.nx = 1;
+    r.nb = g->n_beta;
+    r.rt = g->size[0] / g->nt;
+    r.rs = g->size[1] / g->ns;
+    r.rz = 1.0;
+    r.bt = g->size[0];
+    r.bs = g->size[1];
+    r.bz = 1.0;
+    r.D = g->distance;
+    r.sz = 0.5;
+    r.view = (const double *)T.d_view.ptr;
+    r.p = (const double *)T.d_p.ptr;
+    r.xi = (const double *)T.d_zero.ptr;
+    return r;
+}
+
+FanParams fan_params(ctr_ctx *ctx, const ctr_fan_geom *g, unsigned flags)
+{
+    const FanTables &T = ctx->fan;
+    FanParams P{};
+    P.nt = g->nt;
+    P.ns = g->ns;
+    P.np = g->n_p;
+    P.nb = g->n_beta;
+    P.rt = (float)(g->size[0] / g->nt);
+    P.rs = (float)(g->size[1] / g->ns);
+    P.ct = (float)(g->size[0] / 2);
+    P.cs = (float)(g->size[1] / 2);
+    P.D = (float)g->distance;
+    if (g->n_p >= 2) {
+        const double p0 = (double)T.h_pf[0], dp = (double)T.h_pf[1] - p0;
+        P.u_scale = (float)(1.0 / dp);
+        P.u_off = (float)(-p0 / dp);
+        const double g0 = std::atan(g->p[0] / g->distance), dg = std::atan(g->p[1] / g->distance) - g0;
+        P.g0 = (float)g0;
+        P.g_scale = (float)(1.0 / dg);
+    }
+    P.umax = (float)(g->n_p - 1);
+    P.gmax = (float)(g->n_p - 1);
+    P.dbeta = g->n_beta >= 2 ? (float)std::fabs(g->beta[1] - g->beta[0]) : 0.0f;
+    P.view = (const double *)T.d_view.ptr;
+    P.pf = (const float *)T.d_pf.ptr;
+    P.gam = (const float *)T.d_gam.ptr;
+    P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
+    P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
+    return P;
+}
+
+// Half-scaled ramp taps times the pitch, two-sided: h2[k] = d * h[|k-(n-1)|].
+ctr_status tap_table(ctr_ctx *ctx, int filter, int equiangular, double d, int n)
+{
+    TapTable &T = ctx->taps;
+    uint64_t key = 1469598103934665603ull;
+    key = fnv(key, &filter, sizeof filter);
+    key = fnv(key, &equiangular, sizeof equiangular);
+    key = fnv(key, &d, sizeof d);
+    key = fnv(key, &n, sizeof n);
+    if (T.valid && T.key == key) return CTR_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    T.valid = false;
+    const double pi = 3.141592653589793;
+    std::vector<double> h(n);
+    for (int k = 0; k < n; ++k) {
+        double v;
+        if (filter == CTR_FILTER_SL_HALF) {
+            v = -1.0 / ((d * pi) * (d * pi)) / (4.0 * k * k - 1.0);
+            if (!equiangular) v = (double)(float)v;   // dev_G is float (FDK.cu:6-19)
+        } else if (k == 0) {
+            v = 1.0 / (8.0 * d * d);
+        } else if (k % 2 == 0) {
+            v = 0.0;
+        } else if (equiangular) {
+            const double s = std::sin(k * d);
+            v = -1.0 / (2.0 * pi * pi * s * s);
+        } else {
+            v = -1.0 / (2.0 * pi * pi * (double)k * k * d * d);
+        }
+        h[k] = v * d;
+    }
+    T.h.resize(2 * (size_t)n - 1);
+    for (int k = 0; k < 2 * n - 1; ++k) T.h[k] = h[std::abs(k - (n - 1))];
+    ctr_status s = upload(ctx, T.d, T.h.data(), sizeof(double) * T.h.size());
+    if (s != CTR_OK) return s;
+    T.key = key;
+    T.valid = true;
+    return CTR_OK;
+}
+
+// Host/device pointer plumbing: in host mode inputs are staged into
+// workspaces and results copied back; in device mode pointers pass through.
+struct Io {
+    ctr_ctx *ctx;
+    bool dev;
+};
+
+ctr_status stage_in(ctr_ctx *ctx, bool dev, int slot, const float *src, size_t n, const float **out)
+{
+    if (!src) return fail(ctx, CTR_EINVAL, "input data pointer is NULL");
+    if (dev) {
+        *out = src;
+        return CTR_OK;
+    }
+    ctr_status s = buf_reserve(ctx, ctx->ws[slot], n * sizeof(float));
+    if (s != CTR_OK) return s;
+    if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy: host-to-device copy");
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->ws[slot].ptr, src, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    *out = (const float *)ctx->ws[slot].ptr;
+    return CTR_OK;
+}
+
+// Output buffer; with accumulate in host mode the current contents are uploaded.
+ctr_status stage_out(ctr_ctx *ctx, bool dev, bool load, int slot, float *dst, size_t n, float **out)
+{
+    if (!dst) return fail(ctx, CTR_EINVAL, "output data pointer is NULL");
+    if (dev) {
+        *out = dst;
+        return CTR_OK;
+    }
+    ctr_status s = buf_reserve(ctx, ctx->ws[slot], n * sizeof(float));
+    if (s != CTR_OK) return s;
+    if (load) {
+        if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy: host-to-device copy");
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->ws[slot].ptr, dst, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    }
+    *out = (float *)ctx->ws[slot].ptr;
+    return CTR_OK;
+}
+
+ctr_status finish_out(ctr_ctx *ctx, bool dev, const float *dbuf, float *dst, size_t n)
+{
+    if (dev) return CTR_OK;
+    if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy: device-to-host copy");
+    HIP_TRY(ctx, hipMemcpyAsync(dst, dbuf, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+size_t nvox3(const ctr_cone_geom *g) { return (size_t)g->nt * g->ns * g->nz; }
+size_t nray3(const ctr_cone_geom *g) { return (size_t)g->n_beta * g->n_p * g->n_xi; }
+size_t nvox2(const ctr_fan_geom *g) { return (size_t)g->nt * g->ns; }
+size_t nray2(const ctr_fan_geom *g) { return (size_t)g->n_beta * g->n_p; }
+
+#define TRY(expr)                          \
+    do {                                   \
+        ctr_status s_ = (expr);            \
+        if (s_ != CTR_OK) return s_;       \
+    } while (0)
+
+ctr_status cone_filter_impl(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *in, float *out)
+{
+    if (g->n_p > 2048) return fail(ctx, CTR_ETOOBIG, "ramp filter supports at most 2048 detector columns");
+    const double d = (double)std::fabs(g->p[1] - g->p[0]);
+    TRY(tap_table(ctx, filter, 0, (double)(float)d, g->n_p));
+    FilterParams F{};
+    F.np = g->n_p;
+    F.nx = g->n_xi;
+    F.rows = g->n_beta * g->n_xi;
+    F.mode = 0;
+    F.D = (float)g->distance;
+    F.pd = (const float *)ctx->cone.d_pf.ptr;
+    F.xd = (const float *)ctx->cone.d_xf.ptr;
+    F.taps = (const double *)ctx->taps.d.ptr;
+    return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "ramp filter");
+}
+
+ctr_status fan_filter_impl(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *in, float *out)
+{
+    if (g->n_p > 2048) return fail(ctx, CTR_ETOOBIG, "ramp filter supports at most 2048 detector bins");
+    double d;
+    if (detector == CTR_FAN_EQUIANGULAR) {
+        d = std::fabs(std::atan(g->p[1] / g->distance) - std::atan(g->p[0] / g->distance));
+    } else {
+        d = (double)std::fabs((float)g->p[1] - (float)g->p[0]);
+    }
+    TRY(tap_table(ctx, filter, detector == CTR_FAN_EQUIANGULAR, d, g->n_p));
+    FilterParams F{};
+    F.np = g->n_p;
+    F.nx = 1;
+    F.rows = g->n_beta;
+    F.mode = detector == CTR_FAN_EQUIANGULAR ? 1 : 0;
+    F.D = (float)g->distance;
+    F.pd = (const float *)ctx->fan.d_pf.ptr;
+    F.xd = (const float *)ctx->fan.d_xf.ptr;
+    F.colw = (const float *)ctx->fan.d_colw.ptr;
+    F.taps = (const double *)ctx->taps.d.ptr;
+    return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "fan ramp filter");
+}
+
+ctr_status check_filter_args(ctr_ctx *ctx, int filter)
+{
+    if (filter != CTR_FILTER_SL_HALF && filter != CTR_FILTER_RL_HALF)
+        return fail(ctx, CTR_EINVAL, "unknown filter %d", filter);
+    return CTR_OK;
+}
+
+ctr_status check_equiangular(ctr_ctx *ctx, const ctr_fan_geom *g)
+{
+    const double g0 = std::atan(g->p[0] / g->distance), d0 = std::atan(g->p[1] / g->distance) - g0;
+    for (int i = 1; i < g->n_p; ++i) {
+        const double d = std::atan(g->p[i] / g->distance) - std::atan(g->p[i - 1] / g->distance);
+        if (std::fabs(d - d0) > 1e-3 * std::fabs(d0))
+            return fail(ctx, CTR_EINVAL, "equiangular detector needs uniformly spaced atan(p/D)");
+    }
+    return CTR_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *ctr_version(void) { return CTR_VERSION_STRING; }
+
+const char *ctr_status_string(int status)
+{
+    switch (status) {
+    case CTR_OK: return "ok";
+    case CTR_EINVAL: return "invalid argument";
+    case CTR_ENOMEM: return "out of memory";
+    case CTR_EHIP: return "HIP runtime error";
+    case CTR_ETOOBIG: return "problem too large";
+    case CTR_EFAULT_INJECTED: return "injected fault";
+    case CTR_ENODEV: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+const char *ctr_last_error(const ctr_ctx *ctx) { return ctx ? ctx->last_error.c_str() : g_last_error.c_str(); }
+
+ctr_status ctr_ctx_create(int device, ctr_ctx **out)
+{
+    if (!out) return fail(nullptr, CTR_EINVAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(nullptr, CTR_ENODEV, "no HIP device available (%s)", hipGetErrorString(e));
+    }
+    if (device < 0 || device >= n) return fail(nullptr, CTR_EINVAL, "device %d out of range (0..%d)", device, n - 1);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, CTR_EHIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
+    ctr_ctx *ctx = new ctr_ctx();
+    ctx->device = device;
+    ctx->fault = parse_fault();
+    e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, CTR_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    ctx->stream = ctx->own;
+    *out = ctx;
+    return CTR_OK;
+}
+
+void ctr_ctx_destroy(ctr_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->ws) buf_free(b);
+    for (DevBuf *b : {&ctx->cone.d_view, &ctx->cone.d_p, &ctx->cone.d_xi, &ctx->cone.d_bp, &ctx->cone.d_pf,
+                      &ctx->cone.d_xf, &ctx->fan.d_view, &ctx->fan.d_p, &ctx->fan.d_zero, &ctx->fan.d_pf,
+                      &ctx->fan.d_gam, &ctx->fan.d_colw, &ctx->fan.d_xf, &ctx->taps.d})
+        buf_free(*b);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+}
+
+ctr_status ctr_ctx_set_stream(ctr_ctx *ctx, void *hip_stream)
+{
+    TRY(need_ctx(ctx));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own;
+    return CTR_OK;
+}
+
+void *ctr_ctx_stream(const ctr_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+ctr_status ctr_ctx_synchronize(ctr_ctx *ctx)
+{
+    TRY(need_ctx(ctx));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status ctr_device_alloc(ctr_ctx *ctx, size_t bytes, void **out)
+{
+    TRY(need_ctx(ctx));
+    if (!out) return fail(ctx, CTR_EINVAL, "out is NULL");
+    if (ctx->fault == 1) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=alloc");
+    hipError_t e = hipMalloc(out, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, CTR_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return CTR_OK;
+}
+
+ctr_status ctr_device_free(ctr_ctx *ctx, void *ptr)
+{
+    TRY(need_ctx(ctx));
+    if (ptr) {
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx, hipFree(ptr));
+    }
+    return CTR_OK;
+}
+
+ctr_status ctr_copy_to_device(ctr_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    TRY(need_ctx(ctx));
+    if (!dst || !src) return fail(ctx, CTR_EINVAL, "NULL pointer");
+    if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy");
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status ctr_copy_to_host(ctr_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    TRY(need_ctx(ctx));
+    if (!dst || !src) return fail(ctx, CTR_EINVAL, "NULL pointer");
+    if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy");
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status ctr_cone_geom_check(const ctr_cone_geom *g, int need_uniform_pitch) { return check_cone(g, need_uniform_pitch); }
+
+ctr_status ctr_fan_geom_check(const ctr_fan_geom *g, int need_uniform_pitch) { return check_fan(g, need_uniform_pitch); }
+
+// ---- cone ------------------------------------------------------------------
+
+ctr_status ctr_cone_fp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *vol, float *proj, unsigned flags)
+{
+    TRY(check_cone(g, 0));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(cone_tables(ctx, g));
+    const float *dv;
+    float *dp;
+    TRY(stage_in(ctx, dev, kWsIn, vol, nvox3(g), &dv));
+    TRY(stage_out(ctx, dev, false, kWsOut, proj, nray3(g), &dp));
+    RayParams R = cone_ray_params(ctx, g);
+    TRY(check_launch(ctx, launch_ray_fp(R, dv, dp, ctx->stream), "cone forward projection"));
+    return finish_out(ctx, dev, dp, proj, nray3(g));
+}
+
+ctr_status ctr_cone_bp_ref(ctr_ctx *ctx, const ctr_cone_geom *g, const float *proj, float *vol, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(cone_tables(ctx, g));
+    const float *dp;
+    float *dv;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray3(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
+    BpParams P = cone_bp_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_bp_tiled(kBpRef, P, dp, dv, ctx->stream), "cone reference backprojection"));
+    return finish_out(ctx, dev, dv, vol, nvox3(g));
+}
+
+ctr_status ctr_cone_bp_adjoint(ctr_ctx *ctx, const ctr_cone_geom *g, const float *proj, float *vol, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(cone_tables(ctx, g));
+    const float *dp;
+    float *dv;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray3(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
+    BpParams P = cone_bp_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_adjoint(P, dp, dv, nullptr, 0, ctx->stream), "cone adjoint"));
+    return finish_out(ctx, dev, dv, vol, nvox3(g));
+}
+
+ctr_status ctr_cone_filter(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj, float *filtered,
+                           unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(cone_tables(ctx, g));
+    const float *dp;
+    float *df;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray3(g), &dp));
+    TRY(stage_out(ctx, dev, false, kWsTmp, filtered, nray3(g), &df));
+    TRY(cone_filter_impl(ctx, g, filter, dp, df));
+    return finish_out(ctx, dev, df, filtered, nray3(g));
+}
+
+ctr_status ctr_fdk_bp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *filtered, float *vol, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    if (g->n_beta < 2) return fail(ctx, CTR_EINVAL, "FDK needs at least 2 views (dbeta = |beta[1]-beta[0]|)");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(cone_tables(ctx, g));
+    const float *dp;
+    float *dv;
+    TRY(stage_in(ctx, dev, kWsIn, filtered, nray3(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
+    BpParams P = cone_bp_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_bp_tiled(kBpFdk, P, dp, dv, ctx->stream), "FDK backprojection"));
+    return finish_out(ctx, dev, dv, vol, nvox3(g));
+}
+
+ctr_status ctr_fdk(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj, float *vol, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    if (g->n_beta < 2) return fail(ctx, CTR_EINVAL, "FDK needs at least 2 views (dbeta = |beta[1]-beta[0]|)");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(cone_tables(ctx, g));
+    const float *dp;
+    float *dv;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray3(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
+    TRY(buf_reserve(ctx, ctx->ws[kWsTmp], nray3(g) * sizeof(float)));
+    float *filt = (float *)ctx->ws[kWsTmp].ptr;
+    TRY(cone_filter_impl(ctx, g, filter, dp, filt));
+    BpParams P = cone_bp_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_bp_tiled(kBpFdk, P, filt, dv, ctx->stream), "FDK backprojection"));
+    if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
+    return finish_out(ctx, dev, dv, vol, nvox3(g));
+}
+
+// ---- fan -------------------------------------------------------------------
+
+ctr_status ctr_fan_fp(ctr_ctx *ctx, const ctr_fan_geom *g, const float *img, float *proj, unsigned flags)
+{
+    TRY(check_fan(g, 0));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(fan_tables(ctx, g));
+    const float *di;
+    float *dp;
+    TRY(stage_in(ctx, dev, kWsIn, img, nvox2(g), &di));
+    TRY(stage_out(ctx, dev, false, kWsOut, proj, nray2(g), &dp));
+    RayParams R = fan_ray_params(ctx, g);
+    TRY(check_launch(ctx, launch_ray_fp(R, di, dp, ctx->stream), "fan forward projection"));
+    return finish_out(ctx, dev, dp, proj, nray2(g));
+}
+
+ctr_status ctr_fan_bp_ref(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags)
+{
+    TRY(check_fan(g, 1));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(fan_tables(ctx, g));
+    const float *dp;
+    float *di;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray2(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, img, nvox2(g), &di));
+    FanParams P = fan_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_fan_bp_ref(P, dp, di, ctx->stream), "fan reference backprojection"));
+    return finish_out(ctx, dev, di, img, nvox2(g));
+}
+
+ctr_status ctr_fan_bp_adjoint(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags)
+{
+    TRY(check_fan(g, 1));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(fan_tables(ctx, g));
+    const float *dp;
+    float *di;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray2(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, img, nvox2(g), &di));
+    FanParams P = fan_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_fan_adjoint(P, dp, di, nullptr, 0, ctx->stream), "fan adjoint"));
+    return finish_out(ctx, dev, di, img, nvox2(g));
+}
+
+ctr_status ctr_fan_filter(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj,
+                          float *filtered, unsigned flags)
+{
+    TRY(check_fan(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    if (detector != CTR_FAN_FLAT && detector != CTR_FAN_EQUIANGULAR)
+        return fail(ctx, CTR_EINVAL, "unknown detector kind %d", detector);
+    if (detector == CTR_FAN_EQUIANGULAR) TRY(check_equiangular(ctx, g));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(fan_tables(ctx, g));
+    const float *dp;
+    float *df;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray2(g), &dp));
+    TRY(stage_out(ctx, dev, false, kWsTmp, filtered, nray2(g), &df));
+    TRY(fan_filter_impl(ctx, g, detector, filter, dp, df));
+    return finish_out(ctx, dev, df, filtered, nray2(g));
+}
+
+ctr_status ctr_fan_fbp(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj, float *img,
+                       unsigned flags)
+{
+    TRY(check_fan(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    if (detector != CTR_FAN_FLAT && detector != CTR_FAN_EQUIANGULAR)
+        return fail(ctx, CTR_EINVAL, "unknown detector kind %d", detector);
+    if (detector == CTR_FAN_EQUIANGULAR) TRY(check_equiangular(ctx, g));
+    if (g->n_beta < 2) return fail(ctx, CTR_EINVAL, "FBP needs at least 2 views (dbeta = |beta[1]-beta[0]|)");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(fan_tables(ctx, g));
+    const float *dp;
+    float *di;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nray2(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, img, nvox2(g), &di));
+    TRY(buf_reserve(ctx, ctx->ws[kWsTmp], nray2(g) * sizeof(float)));
+    float *filt = (float *)ctx->ws[kWsTmp].ptr;
+    TRY(fan_filter_impl(ctx, g, detector, filter, dp, filt));
+    FanParams P = fan_params(ctx, g, flags);
+    P.equiangular = detector == CTR_FAN_EQUIANGULAR;
+    TRY(check_launch(ctx, launch_fan_fbp_bp(P, filt, di, ctx->stream), "fan FBP backprojection"));
+    if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return finish_out(ctx, dev, di, img, nvox2(g));
+}
+
+}  // extern "C"
+
+// ---- SIRT ------------------------------------------------------------------
+
+namespace {
+
+ctr_status sirt_opts_check(ctr_ctx *ctx, const ctr_sirt_opts *o)
+{
+    if (!o) return fail(ctx, CTR_EINVAL, "options are NULL");
+    if (o->n_iter < 0) return fail(ctx, CTR_EINVAL, "n_iter must be >= 0");
+    if (!std::isfinite(o->lambda)) return fail(ctx, CTR_EINVAL, "lambda must be finite");
+    if (std::isnan(o->min_val) || std::isnan(o->max_val) || o->min_val > o->max_val)
+        return fail(ctx, CTR_EINVAL, "clamp bounds must satisfy min_val <= max_val");
+    return CTR_OK;
+}
+
+// Shared SIRT driver: fp_resid(x, r, partials, &nblk) and adj_update(r, upd)
+// close over the geometry.
+template <typename FpResid, typename AdjOnes, typename AdjUpdate>
+ctr_status sirt_loop(ctr_ctx *ctx, size_t nvox, size_t nray, const float *b_in, float *x_in,
+                            const ctr_sirt_opts *o, unsigned flags, int nblk_fp, FpResid fp_resid, AdjOnes adj_ones,
+                            AdjUpdate adj_update)
+{
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const float *b;
+    float *x;
+    TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
+    TRY(stage_out(ctx, dev, o->use_x0 != 0, kWsOut, x_in, nvox, &x));
+    TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsVinv], nvox * sizeof(float)));
+    const size_t nsum = (size_t)o->n_iter + 2;
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(nblk_fp, 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * nsum));
+    float *r = (float *)ctx->ws[kWsR].ptr, *vinv = (float *)ctx->ws[kWsVinv].ptr;
+    double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
+    if (!o->use_x0) TRY(check_launch(ctx, launch_fill(x, 0.0f, nvox, ctx->stream), "fill"));
+    // V = 1/(A'1 + 1e-9): column sums from the adjoint of ones
+    TRY(adj_ones(vinv));
+    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, ctx->stream), "reciprocal"));
+    // ||b||^2 and sum(b) for the residual normalisations (slot n_iter+1)
+    int nb2 = 0;
+    TRY(check_launch(ctx, launch_sum_sq_sum(b, nray, part, &nb2, ctx->stream), "norm"));
+    TRY(check_launch(ctx, launch_reduce_partials(part, nb2, sums + 2 * (nsum - 1), ctx->stream), "reduce"));
+    const bool want_res = o->residual_l2 || o->residual_l1;
+    SirtUpdate U{vinv, x, o->lambda, o->min_val, o->max_val};
+    for (int k = 0; k <= o->n_iter; ++k) {
+        if (k == o->n_iter && !want_res) break;
+        int nblk = 0;
+        TRY(fp_resid(x, b, r, part, &nblk));
+        TRY(check_launch(ctx, launch_reduce_partials(part, nblk, sums + 2 * k, ctx->stream), "reduce"));
+        if (k == o->n_iter) break;
+        TRY(adj_update(r, &U));
+    }
+    if (want_res) {
+        std::vector<double> hs(2 * nsum);
+        HIP_TRY(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * 2 * nsum, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        const double bn = std::sqrt(hs[2 * (nsum - 1)]);
+        const double bsum = hs[2 * (nsum - 1) + 1];   // SIRT3D.m:59 Norm = sum(R)
+        for (int k = 0; k <= o->n_iter; ++k) {
+            if (o->residual_l2) o->residual_l2[k] = std::sqrt(hs[2 * k]) / (bn > 0 ? bn : 1.0);
+            if (o->residual_l1) o->residual_l1[k] = hs[2 * k + 1] / ((bsum != 0 ? bsum : 1.0) * (double)nvox);
+        }
+    }
+    return finish_out(ctx, dev, x, x_in, nvox);
+}
+
+}  // namespace
+
+extern "C" {
+
+ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
+                      unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(sirt_opts_check(ctx, opts));
+    TRY(cone_tables(ctx, g));
+    const RayParams R = cone_ray_params(ctx, g);
+    const BpParams P = cone_bp_params(ctx, g, 0);
+    auto fp = [&](const float *xx, const float *bb, float *rr, double *part, int *nblk) {
+        return check_launch(ctx, launch_ray_fp_residual(R, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
+    };
+    auto ones = [&](float *vinv) {
+        return check_launch(ctx, launch_adjoint(P, nullptr, vinv, nullptr, 1, ctx->stream), "SIRT column sums");
+    };
+    auto upd = [&](const float *rr, SirtUpdate *U) {
+        return check_launch(ctx, launch_adjoint(P, rr, nullptr, U, 0, ctx->stream), "SIRT update");
+    };
+    return sirt_loop(ctx, nvox3(g), nray3(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd);
+}
+
+ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
+                      unsigned flags)
+{
+    TRY(check_fan(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(sirt_opts_check(ctx, opts));
+    TRY(fan_tables(ctx, g));
+    const RayParams R = fan_ray_params(ctx, g);
+    const FanParams P = fan_params(ctx, g, 0);
+    auto fp = [&](const float *xx, const float *bb, float *rr, double *part, int *nblk) {
+        return check_launch(ctx, launch_ray_fp_residual(R, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
+    };
+    auto ones = [&](float *vinv) {
+        return check_launch(ctx, launch_fan_adjoint(P, nullptr, vinv, nullptr, 1, ctx->stream), "SIRT column sums");
+    };
+    auto upd = [&](const float *rr, SirtUpdate *U) {
+        return check_launch(ctx, launch_fan_adjoint(P, rr, nullptr, U, 0, ctx->stream), "SIRT update");
+    };
+    return sirt_loop(ctx, nvox2(g), nray2(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd);
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// ctr_internal.h - kernel parameter blocks and launcher declarations shared by
+// the C ABI (ctr_api.cpp) and the HIP kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "ctr/ctr.h"
+
+namespace ctr {
+
+constexpr float kInf = __builtin_huge_valf();
+
+// Ray-driven Siddon projector (cone; the fan beam is the nz = n_xi = 1 case).
+// Box [0,bt] x [0,bs] x [0,bz] mm, voxel pitch (rt, rs, rz).
+struct RayParams {
+    int nt, ns, nz;
+    int np, nx, nb;
+    double rt, rs, rz;
+    double bt, bs, bz;
+    double D;
+    double sz;              // source z (mm, volume frame)
+    const double *view;     // [nb][4]: sin b, cos b, source t, source s
+    const double *p;        // [np] column positions at the isocentre
+    const double *xi;       // [nx] row positions at the isocentre
+};
+
+// Per-view constants of the voxel-driven backprojectors, float:
+//   dect_t(i,j) = at*i + as*j + a0, dect_s(i,j) = bt*i + bs*j + b0
+//   (i, j voxel indices; the voxel centre frame of Backprojection3D.cu:31-37),
+//   wv = view weight (dbeta for FDK), st/ss = source position (volume frame).
+struct BpView {
+    float at, as, a0, bt, bs, b0, wv, pad0;
+    float st, ss, sb, cb, pad1, pad2, pad3, pad4;
+};
+
+struct BpParams {
+    int nt, ns, nz, np, nx, nb;
+    int tiles_t, tiles_s, tiles_z;
+    float rt, rs, rz;       // voxel pitch
+    float zc0;              // (0.5 rz - Cz): z of slice-0 centre relative to the source plane
+    float D;
+    float u_scale, u_off;   // detector column coordinate u = P*u_scale + u_off (sample index units)
+    float v_scale, v_off;   // detector row coordinate v = xi*v_scale + v_off
+    float umax, vmax;       // valid iff 0 <= u < umax and 0 <= v < vmax
+    float ht, hs, hz;       // half voxel sizes (chord of BP_REF)
+    const BpView *views;
+    const float *pd;        // [np] detector columns (float, device)
+    const float *xd;        // [nx] detector rows (float, device)
+    int accumulate;
+    int quirks;
+};
+
+enum BpKind { kBpFdk = 0, kBpRef = 1 };
+
+// Exact adjoint: SIRT update fused into the epilogue when sirt != 0.
+struct SirtUpdate {
+    const float *vinv;      // [nvox] 1/(A'1 + 1e-9)
+    float *x;               // [nvox] iterate, updated in place
+    float lambda, lo, hi;
+};
+
+// Row filter (pre-weight + convolution along p).
+struct FilterParams {
+    int np, nx, rows;       // detector columns, rows per view, total rows
+    int mode;               // 0: w = D/sqrt(D^2+p^2+xi^2); 1: w = colw[p]
+    float D;
+    const float *pd;        // [np] column positions
+    const float *xd;        // [nx] row positions
+    const float *colw;      // [np] column weights (mode 1)
+    const double *taps;     // [2np-1] two-sided taps times the pitch: h2[k] = d*h[|k-(np-1)|]
+};
+
+// Fan-beam pixel-driven kernels.
+struct FanParams {
+    int nt, ns, np, nb;
+    float rt, rs;
+    float ct, cs;           // rotation centre
+    float D;
+    float u_scale, u_off, umax;
+    float dbeta;
+    float g0, g_scale, gmax;   // equiangular: gamma index = (gamma - g0)*g_scale, valid in [0,gmax)
+    const double *view;     // [nb][4]: sin, cos, source t, source s
+    const float *pf;        // [np] bin positions (float)
+    const float *gam;       // [np] bin angles (float, equiangular)
+    int accumulate;
+    int quirks;
+    int equiangular;
+};
+
+// ---- launchers (return hipError_t of the launch) --------------------------
+hipError_t launch_ray_fp(const RayParams &g, const float *vol, float *proj, hipStream_t st);
+// SIRT residual: r = (b - A x) / (rowlen + 1e-9); partials[block] = (sum e^2, sum |e|)
+hipError_t launch_ray_fp_residual(const RayParams &g, const float *vol, const float *b, float *r,
+                                  double *partials, int *nblocks, hipStream_t st);
+int ray_fp_blocks(const RayParams &g);
+
+hipError_t launch_bp_tiled(int kind, const BpParams &P, const float *proj, float *vol, hipStream_t st);
+hipError_t launch_adjoint(const BpParams &P, const float *proj, float *vol, const SirtUpdate *upd,
+                          int ones_input, hipStream_t st);
+
+hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
+
+hipError_t launch_fan_bp_ref(const FanParams &P, const float *proj, float *img, hipStream_t st);
+hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img, const SirtUpdate *upd,
+                              int ones_input, hipStream_t st);
+hipError_t launch_fan_fbp_bp(const FanParams &P, const float *filt, float *img, hipStream_t st);
+
+// Utilities.
+hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st);
+hipError_t launch_reciprocal_eps(float *v, size_t n, float eps, hipStream_t st);   // v <- 1/(v+eps)
+hipError_t launch_reduce_partials(const double *partials, int n, double *out2, hipStream_t st);
+hipError_t launch_sum_sq_sum(const float *x, size_t n, double *partials, int *nblocks, hipStream_t st);  // (sum x^2, sum x)
+
+}  // namespace ctr

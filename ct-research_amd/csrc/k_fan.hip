@@ -1,0 +1,202 @@
+// k_fan.hip - fan-beam pixel-driven kernels for gfx950: the reference
+// backprojection (linear sample x pixel chord), the exact adjoint of the
+// Siddon fan projector, and the flat / equiangular FBP backprojection.
+// One thread per pixel loops over all views (the reference launches one
+// kernel per view, BackprojectionFan2D.cu:243-314); 2-D problems are small,
+// so no LDS staging.
+#include "ctr_internal.h"
+
+namespace ctr {
+namespace {
+
+// Reference fan backprojection (BackprojectionFan2D.cu:7-116).
+__global__ __launch_bounds__(256) void k_fan_bp_ref(const float *__restrict__ proj, float *__restrict__ img, FanParams P)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= P.nt || j >= P.ns) return;
+    const float x = ((float)i + 0.5f) * P.rt - P.ct, y = ((float)j + 0.5f) * P.rs - P.cs;
+    float acc = 0.0f;
+    for (int b = 0; b < P.nb; ++b) {
+        const float sb = (float)P.view[4 * b + 0], cb = (float)P.view[4 * b + 1];
+        const float u = x * cb + y * sb, s = -x * sb + y * cb;
+        const float U = (P.D - s) / P.D;
+        const float pd = u / U;
+        const float uu = fmaf(pd, P.u_scale, P.u_off);
+        float val = 0.0f;
+        if (uu >= 0.0f && uu < P.umax) {
+            const int iu = min((int)floorf(uu), (int)P.umax - 1);
+            const float fu = uu - (float)iu;
+            const float *r = proj + (size_t)b * P.np;
+            val = fmaf(fu, r[iu + 1] - r[iu], r[iu]);
+        }
+        float chord;
+        if (!P.quirks) {
+            // the line from the source through the pixel centre: chord = 2 |d| min(h/|d_axis|)
+            const float dx = x + P.ct - (float)P.view[4 * b + 2], dy = y + P.cs - (float)P.view[4 * b + 3];
+            chord = 2.0f * sqrtf(dx * dx + dy * dy) * fminf(0.5f * P.rt / fabsf(dx), 0.5f * P.rs / fabsf(dy));
+        } else {
+            // reference arithmetic in FP64: entry face tested against 0 instead of the
+            // pixel's low edge (:95-101), distance taken to the ray's far end point (:109)
+            const double Db = P.D, st = P.view[4 * b + 2], ss = P.view[4 * b + 3];
+            const double sbd = P.view[4 * b + 0], cbd = P.view[4 * b + 1];
+            const double th = atan((double)pd / Db), Smax = 2.0 * Db;
+            const double ct = P.ct, cs = P.cs;
+            const double te = ct + Smax * sin(th) * cbd - (Db - Smax * cos(th)) * sbd;
+            const double se = cs + Smax * sin(th) * sbd + (Db - Smax * cos(th)) * cbd;
+            double t2s = (se - ss) / ((te - st) == 0.0 ? 1e-15 : (te - st));
+            double s2t = (te - st) / ((se - ss) == 0.0 ? 1e-15 : (se - ss));
+            t2s = fmin(fmax(t2s, -1e16), 1e16);
+            s2t = fmin(fmax(s2t, -1e16), 1e16);
+            const double tlo = (double)i * P.rt, thi = (double)(i + 1) * P.rt;
+            const double slo = (double)j * P.rs, shi = (double)(j + 1) * P.rs;
+            const double tlo_s = ss + (tlo - st) * t2s, slo_t = st + (slo - ss) * s2t;
+            double T1, S1;
+            if (tlo_s >= 0.0 && tlo_s <= shi) {
+                T1 = tlo;
+                S1 = tlo_s;
+            } else if (slo_t >= 0.0 && slo_t <= thi) {
+                T1 = slo_t;
+                S1 = slo;
+            } else {
+                continue;
+            }
+            chord = (float)(2.0 * sqrt((T1 - te) * (T1 - te) + (S1 - se) * (S1 - se)));
+        }
+        acc = fmaf(val, chord, acc);
+    }
+    const size_t o = (size_t)j * P.nt + i;
+    img[o] = P.accumulate ? img[o] + acc : acc;
+}
+
+// Exact adjoint of the fan Siddon projector: for each view the candidate
+// rays are the bins inside the projection of the pixel's square; the ray
+// S + lam (a, b), a = p cos + D sin, b = p sin - D cos, meets the pixel for
+// lam in the intersection of its t and s slabs; length = dlam sqrt(p^2+D^2).
+__global__ __launch_bounds__(256) void k_fan_adjoint(const float *__restrict__ proj, float *__restrict__ img,
+                                                     FanParams P, SirtUpdate U, int sirt, int ones)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= P.nt || j >= P.ns) return;
+    const float tlo = (float)i * P.rt, slo = (float)j * P.rs;
+    float acc = 0.0f;
+    for (int b = 0; b < P.nb; ++b) {
+        const float sb = (float)P.view[4 * b + 0], cb = (float)P.view[4 * b + 1];
+        const float st = (float)P.view[4 * b + 2], ss = (float)P.view[4 * b + 3];
+        float umin = kInf, umax = -kInf;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float x = tlo + ((c & 1) ? P.rt : 0.0f) - P.ct, y = slo + ((c & 2) ? P.rs : 0.0f) - P.cs;
+            const float u = x * cb + y * sb, s = -x * sb + y * cb;
+            const float pd = u * P.D / (P.D - s);
+            const float uu = fmaf(pd, P.u_scale, P.u_off);
+            umin = fminf(umin, uu);
+            umax = fmaxf(umax, uu);
+        }
+        const float eu = 1e-3f + 4e-6f * fmaxf(fabsf(umin), fabsf(umax));
+        const int c0 = max((int)ceilf(fmaxf(umin - eu, -1.0f)), 0);
+        const int c1 = min((int)floorf(fminf(umax + eu, (float)P.np)), P.np - 1);
+        const float *r = proj + (size_t)b * P.np;
+        for (int c = c0; c <= c1; ++c) {
+            const float p = P.pf[c];
+            const float a = fmaf(p, cb, P.D * sb), bq = fmaf(p, sb, -P.D * cb);
+            float lo = -kInf, hi = kInf;
+            if (a != 0.0f) {
+                const float inv = 1.0f / a;
+                const float l1 = (tlo - st) * inv, l2 = (tlo + P.rt - st) * inv;
+                lo = fmaxf(lo, fminf(l1, l2));
+                hi = fminf(hi, fmaxf(l1, l2));
+            } else if (!(st >= tlo && st < tlo + P.rt)) {
+                continue;
+            }
+            if (bq != 0.0f) {
+                const float inv = 1.0f / bq;
+                const float l1 = (slo - ss) * inv, l2 = (slo + P.rs - ss) * inv;
+                lo = fmaxf(lo, fminf(l1, l2));
+                hi = fminf(hi, fmaxf(l1, l2));
+            } else if (!(ss >= slo && ss < slo + P.rs)) {
+                continue;
+            }
+            const float len = fmaxf(hi - lo, 0.0f) * sqrtf(fmaf(p, p, P.D * P.D));
+            acc = fmaf(len, ones ? 1.0f : r[c], acc);
+        }
+    }
+    const size_t o = (size_t)j * P.nt + i;
+    if (sirt) {
+        const float xn = fmaf(U.lambda * U.vinv[o], acc, U.x[o]);
+        U.x[o] = fminf(fmaxf(xn, U.lo), U.hi);
+    } else {
+        img[o] = P.accumulate ? img[o] + acc : acc;
+    }
+}
+
+// Fan FBP backprojection.
+//   flat (FBPfan.cu:45-96): sum_b lerp(q)(p/U) / U^2 * dbeta, U = (D - s)/D
+//   equiangular (FanProjection.m:299-324): sum_b lerp(q)(gamma) / L^2 * dbeta,
+//   gamma = atan2(u, D - s), L^2 = u^2 + (D - s)^2
+__global__ __launch_bounds__(256) void k_fan_fbp_bp(const float *__restrict__ filt, float *__restrict__ img, FanParams P)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= P.nt || j >= P.ns) return;
+    const float x = ((float)i + 0.5f) * P.rt - P.ct, y = ((float)j + 0.5f) * P.rs - P.cs;
+    float acc = 0.0f;
+    for (int b = 0; b < P.nb; ++b) {
+        const float sb = (float)P.view[4 * b + 0], cb = (float)P.view[4 * b + 1];
+        const float u = x * cb + y * sb, s = -x * sb + y * cb;
+        const float *r = filt + (size_t)b * P.np;
+        if (!P.equiangular) {
+            const float U = (P.D - s) / P.D;
+            const float uu = fmaf(u / U, P.u_scale, P.u_off);
+            if (uu >= 0.0f && uu < P.umax) {
+                const int iu = min((int)floorf(uu), (int)P.umax - 1);
+                const float fu = uu - (float)iu;
+                acc += fmaf(fu, r[iu + 1] - r[iu], r[iu]) / (U * U);
+            }
+        } else {
+            const float w = P.D - s;
+            const float g = atan2f(u, fabsf(w));
+            const float gg = (g - P.g0) * P.g_scale;
+            if (gg >= 0.0f && gg < P.gmax) {
+                const int ig = min((int)floorf(gg), (int)P.gmax - 1);
+                // weights from the stored bin angles, as FanProjection.m:95-98
+                const float g1 = P.gam[ig], g2 = P.gam[ig + 1];
+                const float den = g2 - g1;
+                const float val = (fabsf(g2 - g) * r[ig] + fabsf(g - g1) * r[ig + 1]) / fabsf(den);
+                acc += val / fmaf(u, u, w * w);
+            }
+        }
+    }
+    acc *= P.dbeta;
+    const size_t o = (size_t)j * P.nt + i;
+    img[o] = P.accumulate ? img[o] + acc : acc;
+}
+
+dim3 pix_grid(const FanParams &P) { return dim3((P.nt + 15) / 16, (P.ns + 15) / 16); }
+
+}  // namespace
+
+hipError_t launch_fan_bp_ref(const FanParams &P, const float *proj, float *img, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fan_bp_ref, pix_grid(P), dim3(256), 0, st, proj, img, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img, const SirtUpdate *upd, int ones_input,
+                              hipStream_t st)
+{
+    SirtUpdate U = {};
+    int sirt = 0;
+    if (upd) {
+        U = *upd;
+        sirt = 1;
+    }
+    hipLaunchKernelGGL(k_fan_adjoint, pix_grid(P), dim3(256), 0, st, proj, img, P, U, sirt, ones_input);
+    return hipGetLastError();
+}
+
+hipError_t launch_fan_fbp_bp(const FanParams &P, const float *filt, float *img, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fan_fbp_bp, pix_grid(P), dim3(256), 0, st, filt, img, P);
+    return hipGetLastError();
+}
+
+}  // namespace ctr

@@ -1,0 +1,214 @@
+// k_forward.hip - Siddon ray-driven forward projector for gfx950.
+//
+// One thread per ray (view, row, column); all views in one launch.  A block
+// is 64 columns x 4 rows of one view, so a wavefront holds 64 neighbouring,
+// nearly parallel rays that walk neighbouring voxels (shared cache lines).
+//
+// Per ray the entry/exit of the volume box and the first crossing of each
+// axis are computed in FP64 (near-parallel rays need it); the traversal then
+// runs in FP32 relative to the entry point.  The number of plane crossings
+// of each axis is fixed by the FP64 setup, so the voxel index can only move
+// between the clamped entry and exit voxels: no bounds test per step and no
+// out-of-volume read whatever the rounding.
+//
+// Geometry: ProjectionCone_3D.cu:26-42 (source, end point at 2D along the
+// ray through (p, xi) of the virtual isocentre detector); segment weight is
+// the intersection length in mm, as the reference's :311-356.
+#include "ctr_internal.h"
+
+namespace ctr {
+namespace {
+
+constexpr int kBX = 64, kBY = 4;
+
+struct AxisSetup {
+    int i0, n;        // start index, number of plane crossings
+    float next0;      // first crossing, relative to the entry parameter
+    float step;       // parameter step between crossings
+    int stride;       // signed index stride
+};
+
+__device__ __forceinline__ AxisSetup setup_axis(double s, double d, double lam_in, double lam_out, double r, int nvox,
+                                                int stride)
+{
+    AxisSetup a;
+    const double xin = (s + lam_in * d) / r, xout = (s + lam_out * d) / r;
+    if (d > 0.0) {
+        int i0 = (int)floor(xin), i1 = (int)ceil(xout) - 1;
+        i0 = min(max(i0, 0), nvox - 1);
+        i1 = min(max(i1, i0), nvox - 1);
+        a.i0 = i0;
+        a.n = i1 - i0;
+        a.next0 = (float)(((i0 + 1) * r - s) / d - lam_in);
+        a.step = (float)(r / d);
+        a.stride = stride;
+    } else if (d < 0.0) {
+        int i0 = (int)ceil(xin) - 1, i1 = (int)floor(xout);
+        i0 = min(max(i0, 0), nvox - 1);
+        i1 = max(min(i1, i0), 0);
+        a.i0 = i0;
+        a.n = i0 - i1;
+        a.next0 = (float)((i0 * r - s) / d - lam_in);
+        a.step = (float)(-r / d);
+        a.stride = -stride;
+    } else {
+        a.i0 = min(max((int)floor(xin), 0), nvox - 1);
+        a.n = 0;
+        a.next0 = kInf;
+        a.step = 0.0f;
+        a.stride = 0;
+    }
+    return a;
+}
+
+// Slab clip of the ray s + lam*d against [0, b]; returns false on a miss.
+__device__ __forceinline__ bool clip_axis(double s, double d, double b, double &lo, double &hi)
+{
+    if (d == 0.0) return s >= 0.0 && s < b;
+    const double inv = 1.0 / d;
+    const double t1 = -s * inv, t2 = (b - s) * inv;
+    lo = fmax(lo, fmin(t1, t2));
+    hi = fmin(hi, fmax(t1, t2));
+    return true;
+}
+
+// Line integral of one ray; also returns the chord length of the ray
+// through the volume box (the row sum A*1 of the Siddon system matrix).
+__device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const RayParams &g, int ib, int ip, int ix,
+                                           float &chord)
+{
+    const double sb = g.view[4 * ib + 0], cb = g.view[4 * ib + 1];
+    const double st = g.view[4 * ib + 2], ss = g.view[4 * ib + 3];
+    const double P = g.p[ip], X = g.xi[ix];
+    double dx = P * cb + g.D * sb, dy = P * sb - g.D * cb, dz = X;
+    const double inv_n = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    dx *= inv_n;
+    dy *= inv_n;
+    dz *= inv_n;
+    double lo = -__builtin_huge_val(), hi = __builtin_huge_val();
+    bool hit = clip_axis(st, dx, g.bt, lo, hi);
+    hit = clip_axis(ss, dy, g.bs, lo, hi) && hit;
+    hit = clip_axis(g.sz, dz, g.bz, lo, hi) && hit;
+    chord = 0.0f;
+    if (!hit || !(hi > lo)) return 0.0f;
+
+    const AxisSetup at = setup_axis(st, dx, lo, hi, g.rt, g.nt, 1);
+    const AxisSetup as = setup_axis(ss, dy, lo, hi, g.rs, g.ns, g.nt);
+    const AxisSetup az = setup_axis(g.sz, dz, lo, hi, g.rz, g.nz, g.nt * g.ns);
+    const float lam_end = (float)(hi - lo);
+    chord = lam_end;
+
+    int idx = (az.i0 * g.ns + as.i0) * g.nt + at.i0;
+    int kt = 0, ks = 0, kz = 0;
+    float nt = at.n > 0 ? at.next0 : kInf;
+    float ns = as.n > 0 ? as.next0 : kInf;
+    float nz = az.n > 0 ? az.next0 : kInf;
+    float lam = 0.0f, acc = 0.0f;
+    const int total = at.n + as.n + az.n + 1;
+    for (int k = 0; k < total; ++k) {
+        const float m = fminf(fminf(nt, ns), fminf(nz, lam_end));
+        const float v = vol[idx];
+        acc = fmaf(fmaxf(m - lam, 0.0f), v, acc);
+        lam = fmaxf(lam, m);
+        // step the axis whose plane comes first (exhausted axes sit at +inf)
+        const bool bt = (nt <= ns) && (nt <= nz);
+        const bool bs = !bt && (ns <= nz);
+        if (bt) {
+            idx += at.stride;
+            ++kt;
+            nt = kt < at.n ? fmaf((float)kt, at.step, at.next0) : kInf;
+        } else if (bs) {
+            idx += as.stride;
+            ++ks;
+            ns = ks < as.n ? fmaf((float)ks, as.step, as.next0) : kInf;
+        } else {
+            idx += az.stride;
+            ++kz;
+            nz = kz < az.n ? fmaf((float)kz, az.step, az.next0) : kInf;
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(kBX *kBY) void k_ray_fp(const float *__restrict__ vol, float *__restrict__ proj, RayParams g)
+{
+    const int ip = blockIdx.x * kBX + threadIdx.x;
+    const int ix = blockIdx.y * kBY + threadIdx.y;
+    const int ib = blockIdx.z;
+    if (ip >= g.np || ix >= g.nx) return;
+    float chord;
+    const float acc = trace_ray(vol, g, ib, ip, ix, chord);
+    proj[((size_t)ib * g.nx + ix) * g.np + ip] = acc;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// SIRT residual pass: r = (b - A x) / (A 1 + 1e-9) and per-block partial sums
+// of (b - A x)^2 and |b - A x| (SIRT3D.m:80-81, :87).
+__global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual(const float *__restrict__ vol, const float *__restrict__ b,
+                                                              float *__restrict__ r, double *__restrict__ partials,
+                                                              RayParams g)
+{
+    __shared__ double red[2][kBY];
+    const int ip = blockIdx.x * kBX + threadIdx.x;
+    const int ix = blockIdx.y * kBY + threadIdx.y;
+    const int ib = blockIdx.z;
+    double e2 = 0.0, e1 = 0.0;
+    if (ip < g.np && ix < g.nx) {
+        float chord;
+        const float acc = trace_ray(vol, g, ib, ip, ix, chord);
+        const size_t ray = ((size_t)ib * g.nx + ix) * g.np + ip;
+        const float e = b[ray] - acc;
+        r[ray] = e / (chord + 1e-9f);
+        e2 = (double)e * (double)e;
+        e1 = fabs((double)e);
+    }
+    e2 = wave_sum(e2);
+    e1 = wave_sum(e1);
+    if (threadIdx.x == 0) {
+        red[0][threadIdx.y] = e2;
+        red[1][threadIdx.y] = e1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double s2 = 0.0, s1 = 0.0;
+        for (int w = 0; w < kBY; ++w) {
+            s2 += red[0][w];
+            s1 += red[1][w];
+        }
+        const size_t blk = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[2 * blk + 0] = s2;
+        partials[2 * blk + 1] = s1;
+    }
+}
+
+dim3 fp_grid(const RayParams &g) { return dim3((g.np + kBX - 1) / kBX, (g.nx + kBY - 1) / kBY, g.nb); }
+
+}  // namespace
+
+int ray_fp_blocks(const RayParams &g)
+{
+    const dim3 grid = fp_grid(g);
+    return (int)(grid.x * grid.y * grid.z);
+}
+
+hipError_t launch_ray_fp(const RayParams &g, const float *vol, float *proj, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ray_fp, fp_grid(g), dim3(kBX, kBY), 0, st, vol, proj, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_ray_fp_residual(const RayParams &g, const float *vol, const float *b, float *r, double *partials,
+                                  int *nblocks, hipStream_t st)
+{
+    const dim3 grid = fp_grid(g);
+    *nblocks = (int)(grid.x * grid.y * grid.z);
+    hipLaunchKernelGGL(k_ray_fp_residual, grid, dim3(kBX, kBY), 0, st, vol, b, r, partials, g);
+    return hipGetLastError();
+}
+
+}  // namespace ctr

@@ -1,0 +1,106 @@
+// k_util.hip - small element-wise and reduction kernels (fill, reciprocal,
+// deterministic two-level reductions of per-block partial sums).
+#include "ctr_internal.h"
+
+namespace ctr {
+namespace {
+
+__global__ __launch_bounds__(256) void k_fill(float *dst, float value, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = value;
+}
+
+__global__ __launch_bounds__(256) void k_reciprocal_eps(float *v, size_t n, float eps)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        v[i] = 1.0f / (v[i] + eps);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One block sums n pairs (fixed order per lane, then a fixed tree): deterministic.
+__global__ __launch_bounds__(256) void k_reduce_pairs(const double *partials, int n, double *out2)
+{
+    __shared__ double red[2][4];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out2[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        out2[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// Per-block (sum x^2, sum x).
+__global__ __launch_bounds__(256) void k_sum_sq_sum(const float *x, size_t n, double *partials)
+{
+    __shared__ double red[2][4];
+    double a = 0.0, b = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double v = x[i];
+        a += v * v;
+        b += v;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        partials[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+int grid_for(size_t n)
+{
+    size_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, st, dst, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reciprocal_eps(float *v, size_t n, float eps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reciprocal_eps, dim3(grid_for(n)), dim3(256), 0, st, v, n, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double *partials, int n, double *out2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, st, partials, n, out2);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_sq_sum(const float *x, size_t n, double *partials, int *nblocks, hipStream_t st)
+{
+    const int g = grid_for(n);
+    *nblocks = g;
+    hipLaunchKernelGGL(k_sum_sq_sum, dim3(g), dim3(256), 0, st, x, n, partials);
+    return hipGetLastError();
+}
+
+}  // namespace ctr

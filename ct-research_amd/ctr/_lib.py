@@ -1,0 +1,306 @@
+"""ctypes binding of libctr.so (include/ctr/ctr.h).
+
+The library must be built (ct-research_amd/build_lib.py); there is no
+Python or CPU fallback for any operator.  Data arguments are numpy arrays
+(host memory: the call stages them through the GPU and is synchronous) or
+torch CUDA tensors (device memory: the call is ordered on the current
+torch stream).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libctr.so")
+
+OK, EINVAL, ENOMEM, EHIP, ETOOBIG, EFAULT_INJECTED, ENODEV = range(7)
+DEVICE_PTRS, ACCUMULATE, REF_QUIRKS = 0x1, 0x2, 0x4
+FILTER_SL_HALF, FILTER_RL_HALF = 0, 1
+FAN_FLAT, FAN_EQUIANGULAR = 0, 1
+
+# every symbol declared in include/ctr/ctr.h
+EXPORTS = [
+    "ctr_version", "ctr_status_string", "ctr_last_error", "ctr_ctx_create", "ctr_ctx_destroy",
+    "ctr_ctx_set_stream", "ctr_ctx_stream", "ctr_ctx_synchronize", "ctr_device_alloc", "ctr_device_free",
+    "ctr_copy_to_device", "ctr_copy_to_host", "ctr_cone_geom_check", "ctr_fan_geom_check",
+    "ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_cone_filter", "ctr_fdk_bp", "ctr_fdk",
+    "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
+    "ctr_sirt3d", "ctr_sirt2d",
+]
+
+
+class CtrError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"ctr error {status}: {msg}")
+        self.status = status
+
+
+class ConeGeomC(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("nz", ctypes.c_int),
+                ("size", ctypes.c_double * 3), ("distance", ctypes.c_double),
+                ("n_beta", ctypes.c_int), ("n_p", ctypes.c_int), ("n_xi", ctypes.c_int),
+                ("beta", ctypes.c_void_p), ("p", ctypes.c_void_p), ("xi", ctypes.c_void_p)]
+
+
+class FanGeomC(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("size", ctypes.c_double * 2),
+                ("distance", ctypes.c_double), ("n_beta", ctypes.c_int), ("n_p", ctypes.c_int),
+                ("beta", ctypes.c_void_p), ("p", ctypes.c_void_p)]
+
+
+class SirtOptsC(ctypes.Structure):
+    _fields_ = [("n_iter", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
+                ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
+                ("residual_l2", ctypes.c_void_p), ("residual_l1", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libctr.so; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libctr.so is not built ({LIB_PATH}); run ct-research_amd/build_lib.py")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ctr_version.restype = ctypes.c_char_p
+        L.ctr_status_string.restype = ctypes.c_char_p
+        L.ctr_status_string.argtypes = [ctypes.c_int]
+        L.ctr_last_error.restype = ctypes.c_char_p
+        L.ctr_last_error.argtypes = [ctypes.c_void_p]
+        L.ctr_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.ctr_ctx_destroy.argtypes = [ctypes.c_void_p]
+        L.ctr_ctx_destroy.restype = None
+        L.ctr_ctx_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ctr_ctx_stream.argtypes = [ctypes.c_void_p]
+        L.ctr_ctx_stream.restype = ctypes.c_void_p
+        L.ctr_ctx_synchronize.argtypes = [ctypes.c_void_p]
+        L.ctr_cone_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ctr_fan_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        for name in ("ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_fdk_bp",
+                     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint]
+        for name in ("ctr_cone_filter", "ctr_fdk"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_uint]
+        for name in ("ctr_fan_filter", "ctr_fan_fbp"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
+        for name in ("ctr_sirt3d", "ctr_sirt2d"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_uint]
+        _lib = L
+    return _lib
+
+
+def version():
+    return lib().ctr_version().decode()
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=np.float64)
+
+
+class ConeGeom:
+    """Cone-beam geometry in the reference's argument conventions
+    (ProjectionCone_3D.cpp:5-20): volume dims, Size (mm), source distance,
+    view angles (rad), detector column / row positions at the isocentre."""
+
+    def __init__(self, nt, ns, nz, size, distance, beta, p, xi):
+        self.nt, self.ns, self.nz = int(nt), int(ns), int(nz)
+        self.size = tuple(float(s) for s in size)
+        self.distance = float(distance)
+        self.beta, self.p, self.xi = _f32(beta), _f32(p), _f32(xi)
+        self.c = ConeGeomC(self.nt, self.ns, self.nz, (ctypes.c_double * 3)(*self.size), self.distance,
+                           len(self.beta), len(self.p), len(self.xi),
+                           self.beta.ctypes.data, self.p.ctypes.data, self.xi.ctypes.data)
+
+    @property
+    def nvox(self):
+        return self.nt * self.ns * self.nz
+
+    @property
+    def nray(self):
+        return len(self.beta) * len(self.p) * len(self.xi)
+
+    @property
+    def vol_shape(self):
+        return (self.nz, self.ns, self.nt)
+
+    @property
+    def proj_shape(self):
+        return (len(self.beta), len(self.xi), len(self.p))
+
+    def check(self, need_uniform=False):
+        return lib().ctr_cone_geom_check(ctypes.byref(self.c), int(need_uniform))
+
+
+class FanGeom:
+    """Fan-beam geometry (ProjectionFan_2D.cpp:5-16)."""
+
+    def __init__(self, nt, ns, size, distance, beta, p):
+        self.nt, self.ns = int(nt), int(ns)
+        self.size = tuple(float(s) for s in size)
+        self.distance = float(distance)
+        self.beta, self.p = _f64(beta), _f64(p)
+        self.c = FanGeomC(self.nt, self.ns, (ctypes.c_double * 2)(*self.size), self.distance,
+                          len(self.beta), len(self.p), self.beta.ctypes.data, self.p.ctypes.data)
+
+    @property
+    def nvox(self):
+        return self.nt * self.ns
+
+    @property
+    def nray(self):
+        return len(self.beta) * len(self.p)
+
+    def check(self, need_uniform=False):
+        return lib().ctr_fan_geom_check(ctypes.byref(self.c), int(need_uniform))
+
+
+def _is_device(x):
+    return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
+
+
+class Context:
+    """A libctr context on one HIP device (stream, workspaces, cached tables)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        st = L.ctr_ctx_create(int(device), ctypes.byref(h))
+        if st != OK:
+            raise CtrError(st, L.ctr_last_error(None).decode())
+        self.h = h
+        self._bound_stream = None
+
+    def close(self):
+        if self.h:
+            lib().ctr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise CtrError(st, lib().ctr_last_error(self.h).decode())
+
+    def synchronize(self):
+        self._check(lib().ctr_ctx_synchronize(self.h))
+
+    def stream(self):
+        return lib().ctr_ctx_stream(self.h)
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().ctr_ctx_set_stream(self.h, stream_ptr))
+        self._bound_stream = stream_ptr
+
+    def _bind_torch_stream(self):
+        import torch
+        s = torch.cuda.current_stream().cuda_stream
+        if s != self._bound_stream:
+            self.set_stream(s)
+
+    # -- argument plumbing ------------------------------------------------------
+    def _io(self, inp, out_shape, out, accumulate=False):
+        """Return (in_ptr, out_obj, out_ptr, flags)."""
+        if _is_device(inp):
+            import torch
+            self._bind_torch_stream()
+            if out is None:
+                out = torch.zeros(out_shape, dtype=torch.float32, device=inp.device)
+            assert out.dtype == torch.float32 and out.is_contiguous() and inp.is_contiguous()
+            assert inp.dtype == torch.float32
+            return inp.data_ptr(), out, out.data_ptr(), DEVICE_PTRS
+        a = _f32(inp)
+        if out is None:
+            out = np.zeros(int(np.prod(out_shape)), dtype=np.float32)
+        else:
+            assert out.dtype == np.float32 and out.flags.c_contiguous
+        return a, out, out.ctypes.data, 0
+
+    def _run(self, fn, geom, inp, out_shape, out=None, flags=0, extra=()):
+        a, out, optr, f = self._io(inp, out_shape, out)
+        iptr = a if isinstance(a, int) else a.ctypes.data
+        self._check(fn(self.h, ctypes.byref(geom.c), *extra, iptr, optr, f | flags))
+        return out
+
+    # -- cone -------------------------------------------------------------------
+    def cone_fp(self, g, vol, out=None, flags=0):
+        return self._run(lib().ctr_cone_fp, g, vol, g.nray, out, flags)
+
+    def cone_bp_ref(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_cone_bp_ref, g, proj, g.nvox, out, flags)
+
+    def cone_bp_adjoint(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_cone_bp_adjoint, g, proj, g.nvox, out, flags)
+
+    def cone_filter(self, g, proj, filt=FILTER_SL_HALF, out=None, flags=0):
+        return self._run(lib().ctr_cone_filter, g, proj, g.nray, out, flags, extra=(int(filt),))
+
+    def fdk_bp(self, g, filtered, out=None, flags=0):
+        return self._run(lib().ctr_fdk_bp, g, filtered, g.nvox, out, flags)
+
+    def fdk(self, g, proj, filt=FILTER_SL_HALF, out=None, flags=0):
+        return self._run(lib().ctr_fdk, g, proj, g.nvox, out, flags, extra=(int(filt),))
+
+    # -- fan --------------------------------------------------------------------
+    def fan_fp(self, g, img, out=None, flags=0):
+        return self._run(lib().ctr_fan_fp, g, img, g.nray, out, flags)
+
+    def fan_bp_ref(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_fan_bp_ref, g, proj, g.nvox, out, flags)
+
+    def fan_bp_adjoint(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_fan_bp_adjoint, g, proj, g.nvox, out, flags)
+
+    def fan_filter(self, g, proj, detector=FAN_FLAT, filt=FILTER_SL_HALF, out=None, flags=0):
+        return self._run(lib().ctr_fan_filter, g, proj, g.nray, out, flags, extra=(int(detector), int(filt)))
+
+    def fan_fbp(self, g, proj, detector=FAN_FLAT, filt=FILTER_SL_HALF, out=None, flags=0):
+        return self._run(lib().ctr_fan_fbp, g, proj, g.nvox, out, flags, extra=(int(detector), int(filt)))
+
+    # -- SIRT -------------------------------------------------------------------
+    def _sirt(self, fn, g, b, n_iter, lam, lo, hi, x0, residuals, flags):
+        r2 = np.zeros(n_iter + 1) if residuals else None
+        r1 = np.zeros(n_iter + 1) if residuals else None
+        opts = SirtOptsC(int(n_iter), float(lam), float(lo), float(hi), int(x0 is not None),
+                         r2.ctypes.data if residuals else None, r1.ctypes.data if residuals else None)
+        if _is_device(b):
+            import torch
+            self._bind_torch_stream()
+            x = x0 if x0 is not None else torch.zeros(g.nvox, dtype=torch.float32, device=b.device)
+            self._check(fn(self.h, ctypes.byref(g.c), b.data_ptr(), x.data_ptr(), ctypes.byref(opts),
+                           DEVICE_PTRS | flags))
+        else:
+            bb = _f32(b)
+            x = _f32(x0).copy() if x0 is not None else np.zeros(g.nvox, dtype=np.float32)
+            self._check(fn(self.h, ctypes.byref(g.c), bb.ctypes.data, x.ctypes.data, ctypes.byref(opts), flags))
+        return x, r2, r1
+
+    def sirt3d(self, g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None, residuals=True, flags=0):
+        return self._sirt(lib().ctr_sirt3d, g, b, n_iter, lam, lo, hi, x0, residuals, flags)
+
+    def sirt2d(self, g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None, residuals=True, flags=0):
+        return self._sirt(lib().ctr_sirt2d, g, b, n_iter, lam, lo, hi, x0, residuals, flags)
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx

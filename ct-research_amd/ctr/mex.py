@@ -1,0 +1,94 @@
+"""The reference's MEX operators with their MATLAB call signatures.
+
+Each function takes the arguments in the order and meaning of the MEX
+gateway it replaces and returns a float64 column vector, as
+mxCreateDoubleMatrix(L,1) does there.  Differences from the reference,
+all deliberate: invalid arguments raise (the reference printed a message
+and returned zeros, e.g. ProjectionCone_3D.cpp:38-42); the device context
+persists between calls (no cudaDeviceReset per call, :53); the arithmetic
+quirks listed in the flags of ctr.h are off unless ref_quirks=True.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import ConeGeom, FanGeom, default_context
+
+
+def _col(a):
+    return np.asarray(a, dtype=np.float64).reshape(-1)
+
+
+def _flags(ref_quirks):
+    return _lib.REF_QUIRKS if ref_quirks else 0
+
+
+def ProjectionCone_3D(pic, Nt, Ns, Nz, Size, BetaScanRange, Pdomain, Xigamadomain, Distance, ctx=None):
+    """R = ProjectionCone_3D(vol, Nt, Ns, Nz, Size, beta, Pdomain, Xigamadomain, Distance)
+    (GPU_accelerated_Operator/ProjectionCone_3D/ProjectionCone_3D.cpp:3-59)."""
+    ctx = ctx or default_context()
+    g = ConeGeom(Nt, Ns, Nz, _col(Size)[:3], float(Distance), BetaScanRange, Pdomain, Xigamadomain)
+    return ctx.cone_fp(g, pic).astype(np.float64)
+
+
+def Backprojection(R, Xigamadomain, Pdomain, BetaScanRange, Distance, Size, Nt, Ns, Nz, ctx=None):
+    """V = Backprojection(R, Xigamadomain, Pdomain, beta, Distance, Size, Nt, Ns, Nz)
+    (GPU_accelerated_Operator/BackprojectionCone_3D/Backprojection3D.cpp:3-60)."""
+    ctx = ctx or default_context()
+    g = ConeGeom(Nt, Ns, Nz, _col(Size)[:3], float(Distance), BetaScanRange, Pdomain, Xigamadomain)
+    return ctx.cone_bp_ref(g, R).astype(np.float64)
+
+
+BackprojectionCone_3D = Backprojection
+Backprojection3D = Backprojection
+
+
+def FDK(R, Xigamadomain, Pdomain, BetaScanRange, Distance, Size, Nt, Ns, Nz, ref_quirks=False,
+        filt=_lib.FILTER_SL_HALF, ctx=None):
+    """V = FDK(R, Xigamadomain, Pdomain, beta, Distance, Size, Nt, Ns, Nz)
+    (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FDK_GPU/FDK.cpp:3-61)."""
+    ctx = ctx or default_context()
+    g = ConeGeom(Nt, Ns, Nz, _col(Size)[:3], float(Distance), BetaScanRange, Pdomain, Xigamadomain)
+    return ctx.fdk(g, R, filt=filt, flags=_flags(ref_quirks)).astype(np.float64)
+
+
+def ProjectionFan_2D(picvector, Nt, Ns, Size, BetaScanRange, Pdomain, Distance, ctx=None):
+    """R = ProjectionFan_2D(img, Nt, Ns, Size, beta, Pdomain, Distance)
+    (GPU_accelerated_Operator/ProjectionFan_2D/ProjectionFan_2D.cpp:3-54)."""
+    ctx = ctx or default_context()
+    g = FanGeom(Nt, Ns, _col(Size)[:2], float(Distance), BetaScanRange, Pdomain)
+    return ctx.fan_fp(g, picvector).astype(np.float64)
+
+
+def BackprojectionFan2D(R, BetaScanRange, Pdomain, Size, Nt, Ns, Rscan, ref_quirks=False, ctx=None):
+    """img = BackprojectionFan2D(R, beta, Pdomain, Size, Nt, Ns, Rscan)
+    (GPU_accelerated_Operator/BackprojectionFan_2D/BackprojectionFan2D.cpp:3-66).
+    beta and Pdomain are single in the reference; they are rounded to float32 here too."""
+    ctx = ctx or default_context()
+    g = FanGeom(Nt, Ns, _col(Size)[:2], float(Rscan), _col(BetaScanRange).astype(np.float32),
+                _col(Pdomain).astype(np.float32))
+    return ctx.fan_bp_ref(g, R, flags=_flags(ref_quirks)).astype(np.float64)
+
+
+BackprojectionFan_2D = BackprojectionFan2D
+
+
+def FBPfan(R, BetaScanRange, Pdomain, Size, Nt, Ns, Rscan, filt=_lib.FILTER_SL_HALF, ctx=None):
+    """img = FBPfan(R, beta, Pdomain, Size, Nt, Ns, Rscan)
+    (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FBP_fan_GPU/FBPfan.cpp:3-69)."""
+    ctx = ctx or default_context()
+    g = FanGeom(Nt, Ns, _col(Size)[:2], float(Rscan), _col(BetaScanRange).astype(np.float32),
+                _col(Pdomain).astype(np.float32))
+    return ctx.fan_fbp(g, R, detector=_lib.FAN_FLAT, filt=filt).astype(np.float64)
+
+
+def SIRT3D(R, Xigamadomain, Pdomain, BetaScanRange, Distance, Size, Nt, Ns, Nz, nIter, Lamda,
+           x0=None, minLim=0.0, maxLim=np.inf, ctx=None):
+    """[x, res] = SIRT3D(R, Xigamadomain, Pdomain, beta, Distance, Size, Nt, Ns, Nz, nIter, lambda[, x0, min, max]).
+    No reference MEX exists: the loop is Iterative_algorithm/SIRT3D.m:67-92 with
+    beta in radians.  Returns the volume and the residual history
+    ||b - A x_k|| / ||b|| for k = 0..nIter."""
+    ctx = ctx or default_context()
+    g = ConeGeom(Nt, Ns, Nz, _col(Size)[:3], float(Distance), BetaScanRange, Pdomain, Xigamadomain)
+    x, r2, _ = ctx.sirt3d(g, R, int(nIter), float(Lamda), float(minLim), float(maxLim),
+                          x0=None if x0 is None else np.asarray(x0, np.float32))
+    return x.astype(np.float64), r2

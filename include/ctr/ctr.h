@@ -1,0 +1,202 @@
+/*
+ * ctr.h - C ABI of the MI355X (gfx950) CT projector library.
+ *
+ * The library provides the cone-beam and fan-beam projector pair of
+ * CT-Research (Siddon ray-driven forward projection, voxel-driven
+ * backprojection), the FDK / fan-beam FBP ramp filter and backprojection,
+ * the exact matched adjoint of the Siddon projector and the SIRT loop
+ * built on the pair.  Every computation runs in hand-written HIP kernels
+ * on one GPU; there is no CPU fallback.
+ *
+ * Conventions shared with the reference MEX operators
+ *   - volume axes (t, s, z), t fastest: index k*Nt*Ns + j*Nt + i
+ *   - 2-D image: index j*Nt + i
+ *   - the volume occupies [0,size[0]] x [0,size[1]] x [0,size[2]] mm and
+ *     rotates about its centre size/2
+ *   - source S(beta) = (Ct - D sin(beta), Cs + D cos(beta), Cz), beta in radians
+ *   - virtual flat detector through the isocentre: ray (p, xi) runs from S
+ *     through C + p (cos beta, sin beta, 0) + xi (0, 0, 1)
+ *   - projections: 3-D beta*(n_xi*n_p) + xi*n_p + p, 2-D beta*n_p + p
+ *
+ * Pointer convention
+ *   Geometry arrays (beta, p, xi) are always host memory.  Data pointers
+ *   (volumes, images, projections) are host memory unless CTR_DEVICE_PTRS
+ *   is set.  Host-pointer calls are synchronous.  Device-pointer calls are
+ *   ordered on the context's stream and return before the GPU finishes;
+ *   call ctr_ctx_synchronize() before reading the results from the host.
+ *
+ * Errors
+ *   Every entry point returns a ctr_status.  No entry point aborts.  The
+ *   message of the last failure on a context is ctr_last_error(ctx); for a
+ *   failure with no usable context it is ctr_last_error(NULL) (per thread).
+ *
+ * Threading
+ *   A context is used by one host thread at a time.  Use one context per
+ *   thread for concurrent callers.
+ */
+#ifndef CTR_CTR_H_
+#define CTR_CTR_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTR_VERSION_STRING "ctr 0.1.0 (gfx950)"
+
+typedef enum ctr_status {
+    CTR_OK = 0,
+    CTR_EINVAL = 1,          /* invalid argument (null pointer, bad size, bad geometry) */
+    CTR_ENOMEM = 2,          /* device or pinned-host allocation failed */
+    CTR_EHIP = 3,            /* a HIP runtime call or a kernel launch failed */
+    CTR_ETOOBIG = 4,         /* problem exceeds an index or workspace limit */
+    CTR_EFAULT_INJECTED = 5, /* failure forced by the CTR_FAULT environment hook */
+    CTR_ENODEV = 6           /* no usable HIP device */
+} ctr_status;
+
+/* Call flags (bitwise OR). */
+#define CTR_DEVICE_PTRS 0x1u /* data pointers are device memory, call is stream-ordered */
+#define CTR_ACCUMULATE 0x2u  /* backprojections add into the output instead of overwriting it */
+#define CTR_REF_QUIRKS 0x4u  /* reproduce the reference's documented arithmetic quirks
+                                (fan BP entry-face test, FDK stale sample above the detector) */
+
+/* Ramp filters (half-scaled: the 1/2 redundancy weight of a 360-degree scan is folded in). */
+#define CTR_FILTER_SL_HALF 0 /* Shepp-Logan  h[n] = -1/(pi^2 d^2 (4n^2-1)) */
+#define CTR_FILTER_RL_HALF 1 /* Ram-Lak      h[0] = 1/(8 d^2), h[odd n] = -1/(2 pi^2 n^2 d^2), h[even n] = 0 */
+
+/* Fan-beam detector kinds for filtered backprojection. */
+#define CTR_FAN_FLAT 0        /* p[] are flat-detector positions at the isocentre */
+#define CTR_FAN_EQUIANGULAR 1 /* p[] are D*tan(gamma_i) with gamma_i uniformly spaced */
+
+/* Cone-beam circular-orbit geometry (the argument set of ProjectionCone_3D / Backprojection / FDK). */
+typedef struct ctr_cone_geom {
+    int nt, ns, nz;         /* voxels along t, s, z */
+    double size[3];         /* physical extent of the volume (mm) */
+    double distance;        /* source-to-isocentre distance D (mm) */
+    int n_beta, n_p, n_xi;  /* views, detector columns, detector rows */
+    const float *beta;      /* [n_beta] view angles (rad) */
+    const float *p;         /* [n_p] column positions at the isocentre (mm), monotonic */
+    const float *xi;        /* [n_xi] row positions at the isocentre (mm), monotonic */
+} ctr_cone_geom;
+
+/* Fan-beam geometry (the argument set of ProjectionFan_2D / BackprojectionFan2D / FBPfan). */
+typedef struct ctr_fan_geom {
+    int nt, ns;             /* pixels along t, s */
+    double size[2];         /* physical extent of the image (mm) */
+    double distance;        /* source-to-isocentre distance (mm) */
+    int n_beta, n_p;        /* views, detector bins */
+    const double *beta;     /* [n_beta] view angles (rad) */
+    const double *p;        /* [n_p] bin positions at the isocentre (mm), monotonic */
+} ctr_fan_geom;
+
+/* SIRT options (Iterative_algorithm/SIRT3D.m, SIRT.m). */
+typedef struct ctr_sirt_opts {
+    int n_iter;             /* iterations (SIRT3D.m "Times") */
+    float lambda;           /* relaxation factor (SIRT3D.m "Lamda") */
+    float min_val;          /* lower clamp of every iterate (SIRT3D.m: 0) */
+    float max_val;          /* upper clamp (SIRT.m: 1); +INFINITY for none */
+    int use_x0;             /* nonzero: x holds the starting iterate, else start from 0 */
+    double *residual_l2;    /* optional host [n_iter+1]: ||b - A x_k||_2 / ||b||_2, k = 0..n_iter */
+    double *residual_l1;    /* optional host [n_iter+1]: sum|b - A x_k| / (sum(b) * n_voxels) */
+} ctr_sirt_opts;
+
+typedef struct ctr_ctx ctr_ctx;
+
+/* ---- library and context --------------------------------------------- */
+
+const char *ctr_version(void);
+const char *ctr_status_string(int status);
+const char *ctr_last_error(const ctr_ctx *ctx);
+
+/* Create a context on HIP device `device`: a stream, cached device workspaces
+ * and geometry tables.  Replaces the per-call cudaSetDevice(0) / cudaMalloc /
+ * cudaDeviceReset of every reference launcher (e.g.
+ * GPU_accelerated_Operator/ProjectionCone_3D/ProjectionCone_3D.cu:433-519,
+ * ProjectionCone_3D.cpp:53). */
+ctr_status ctr_ctx_create(int device, ctr_ctx **out);
+void ctr_ctx_destroy(ctr_ctx *ctx);
+/* Use an external hipStream_t (NULL = the context's own stream). */
+ctr_status ctr_ctx_set_stream(ctr_ctx *ctx, void *hip_stream);
+void *ctr_ctx_stream(const ctr_ctx *ctx);
+ctr_status ctr_ctx_synchronize(ctr_ctx *ctx);
+
+/* Device memory helpers for callers without another allocator. */
+ctr_status ctr_device_alloc(ctr_ctx *ctx, size_t bytes, void **out);
+ctr_status ctr_device_free(ctr_ctx *ctx, void *ptr);
+ctr_status ctr_copy_to_device(ctr_ctx *ctx, void *dst, const void *src, size_t bytes);
+ctr_status ctr_copy_to_host(ctr_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* Argument checks only (no device work). */
+ctr_status ctr_cone_geom_check(const ctr_cone_geom *g, int need_uniform_pitch);
+ctr_status ctr_fan_geom_check(const ctr_fan_geom *g, int need_uniform_pitch);
+
+/* ---- cone beam --------------------------------------------------------- */
+
+/* Siddon line integrals of vol [nz*ns*nt] into proj [n_beta*n_xi*n_p].
+ * Replaces ProjectionCone_3D() (GPU_accelerated_Operator/ProjectionCone_3D/ProjectionCone_3D.h:32-35,
+ * kernel ProjectionCone_3D.cu:8-379). */
+ctr_status ctr_cone_fp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *vol, float *proj, unsigned flags);
+
+/* Voxel-driven backprojection: bilinear detector sample x chord of the
+ * source-to-voxel-centre line through the voxel.  Replaces BackPro()
+ * (GPU_accelerated_Operator/BackprojectionCone_3D/Backprojection3D.h:31-33, kernel Backprojection3D.cu:8-291). */
+ctr_status ctr_cone_bp_ref(ctr_ctx *ctx, const ctr_cone_geom *g, const float *proj, float *vol, unsigned flags);
+
+/* Exact transpose of ctr_cone_fp (voxel-driven gather of Siddon segment
+ * lengths).  No reference counterpart: SIRT3D.m:81 uses SysMatrix' of the
+ * same Siddon system matrix. */
+ctr_status ctr_cone_bp_adjoint(ctr_ctx *ctx, const ctr_cone_geom *g, const float *proj, float *vol, unsigned flags);
+
+/* FDK pre-weight D/sqrt(D^2+p^2+xi^2) and ramp filtering along p, scaled by
+ * the column pitch.  Replaces GFunction + PreWeightFilter
+ * (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FDK_GPU/FDK.cu:6-48). */
+ctr_status ctr_cone_filter(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj, float *filtered,
+                           unsigned flags);
+
+/* FDK backprojection of filtered projections: bilinear sample x mag^2 x dbeta.
+ * Replaces the BackProjection kernel of FDK_GPU/FDK.cu:50-126. */
+ctr_status ctr_fdk_bp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *filtered, float *vol, unsigned flags);
+
+/* Filter + backprojection.  Replaces FDKpro() (FDK_GPU/FDK.h:29-31). */
+ctr_status ctr_fdk(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj, float *vol, unsigned flags);
+
+/* ---- fan beam ---------------------------------------------------------- */
+
+/* Replaces ProjectionFan_2D() (GPU_accelerated_Operator/ProjectionFan_2D/ProjectionFan_2D.h:32-34). */
+ctr_status ctr_fan_fp(ctr_ctx *ctx, const ctr_fan_geom *g, const float *img, float *proj, unsigned flags);
+
+/* Linear detector sample x pixel chord.  Replaces FBPfan() of
+ * GPU_accelerated_Operator/BackprojectionFan_2D/BackprojectionFan2D.cu:119 (kernel :7-116). */
+ctr_status ctr_fan_bp_ref(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags);
+
+/* Exact transpose of ctr_fan_fp. */
+ctr_status ctr_fan_bp_adjoint(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags);
+
+/* Fan pre-weight and ramp filter (flat: FBP_fan_GPU/FBPfan.cu:21-43; equiangular:
+ * FBP_type_analytic_algorithm/FanProjection.m:229-293). */
+ctr_status ctr_fan_filter(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj,
+                          float *filtered, unsigned flags);
+
+/* Filtered backprojection.  Flat replaces FBPfan() (FBP_fan_GPU/FBPfan.h:31-32);
+ * equiangular follows FanProjection.m:295-324 with weight 1/2 for a full scan. */
+ctr_status ctr_fan_fbp(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj, float *img,
+                       unsigned flags);
+
+/* ---- SIRT -------------------------------------------------------------- */
+
+/* x <- clamp(x + lambda * V .* A'(W .* (b - A x))), W = 1/(A 1 + 1e-9), V = 1/(A' 1 + 1e-9),
+ * with A the Siddon projector and A' its exact transpose (Iterative_algorithm/SIRT3D.m:78-92).
+ * b: [n_beta*n_xi*n_p], x: [nz*ns*nt] (in/out). */
+ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
+                      unsigned flags);
+
+/* 2-D fan-beam SIRT (Iterative_algorithm/SIRT.m:126-154). */
+ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
+                      unsigned flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTR_CTR_H_ */

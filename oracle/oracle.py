@@ -1,0 +1,219 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/ctr_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker (or the timed CPU
+baseline), never as the product path.  The product library (libctr.so)
+never loads this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+class _Cone(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("nz", ctypes.c_int),
+                ("size", ctypes.c_double * 3), ("D", ctypes.c_double),
+                ("nb", ctypes.c_int), ("np", ctypes.c_int), ("nx", ctypes.c_int),
+                ("beta", ctypes.c_void_p), ("p", ctypes.c_void_p), ("xi", ctypes.c_void_p)]
+
+
+class _Fan(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("size", ctypes.c_double * 2),
+                ("D", ctypes.c_double), ("nb", ctypes.c_int), ("np", ctypes.c_int),
+                ("beta", ctypes.c_void_p), ("p", ctypes.c_void_p)]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Cone:
+    """Cone geometry in the reference's argument conventions."""
+
+    def __init__(self, nt, ns, nz, size, distance, beta, p, xi):
+        self.beta, self.p, self.xi = _f32(beta), _f32(p), _f32(xi)
+        self.s = _Cone(nt, ns, nz, (ctypes.c_double * 3)(*size), distance,
+                       len(self.beta), len(self.p), len(self.xi),
+                       _ptr(self.beta), _ptr(self.p), _ptr(self.xi))
+        self.nvox = nt * ns * nz
+        self.nray = len(self.beta) * len(self.p) * len(self.xi)
+
+
+class Fan:
+    def __init__(self, nt, ns, size, distance, beta, p):
+        self.beta, self.p = _f64(beta), _f64(p)
+        self.s = _Fan(nt, ns, (ctypes.c_double * 2)(*size), distance, len(self.beta), len(self.p),
+                      _ptr(self.beta), _ptr(self.p))
+        self.nvox = nt * ns
+        self.nray = len(self.beta) * len(self.p)
+
+
+def _call(name, g, *args):
+    fn = getattr(lib(), name)
+    fn.restype = None
+    fn(ctypes.byref(g.s), *args)
+
+
+def cone_fp_exact(g, vol):
+    out = np.zeros(g.nray)
+    v = _f32(vol)
+    _call("or_cone_fp_exact", g, _ptr(v), _ptr(out))
+    return out
+
+
+def cone_fp_ref(g, vol):
+    out = np.zeros(g.nray)
+    v = _f32(vol)
+    _call("or_cone_fp_ref", g, _ptr(v), _ptr(out))
+    return out
+
+
+def cone_rowsum(g):
+    out = np.zeros(g.nray)
+    _call("or_cone_rowsum", g, _ptr(out))
+    return out
+
+
+def cone_adjoint(g, proj):
+    out = np.zeros(g.nvox)
+    y = _f32(proj)
+    _call("or_cone_adjoint", g, _ptr(y), _ptr(out))
+    return out
+
+
+def cone_ray_brute(g, vol, b, ip, ix):
+    fn = lib().or_cone_ray_brute
+    fn.restype = ctypes.c_double
+    v = _f32(vol)
+    return fn(ctypes.byref(g.s), _ptr(v), b, ip, ix)
+
+
+def cone_bp_ref(g, proj, quirks=False):
+    out = np.zeros(g.nvox)
+    y = _f32(proj)
+    _call("or_cone_bp_ref", g, _ptr(y), _ptr(out), ctypes.c_int(int(quirks)))
+    return out
+
+
+def cone_filter(g, proj, filt=0):
+    out = np.zeros(g.nray)
+    y = _f32(proj)
+    _call("or_cone_filter", g, ctypes.c_int(filt), _ptr(y), _ptr(out))
+    return out
+
+
+def fdk_bp(g, filtered, quirks=False):
+    out = np.zeros(g.nvox)
+    y = _f32(filtered)
+    _call("or_fdk_bp", g, _ptr(y), _ptr(out), ctypes.c_int(int(quirks)))
+    return out
+
+
+def filter_taps(filt, equiangular, d, n):
+    out = np.zeros(n)
+    fn = lib().or_filter_taps
+    fn.restype = None
+    fn(ctypes.c_int(filt), ctypes.c_int(int(equiangular)), ctypes.c_double(d), ctypes.c_int(n), _ptr(out))
+    return out
+
+
+def fan_fp_exact(g, img):
+    out = np.zeros(g.nray)
+    v = _f32(img)
+    _call("or_fan_fp_exact", g, _ptr(v), _ptr(out))
+    return out
+
+
+def fan_fp_ref(g, img):
+    out = np.zeros(g.nray)
+    v = _f32(img)
+    _call("or_fan_fp_ref", g, _ptr(v), _ptr(out))
+    return out
+
+
+def fan_rowsum(g):
+    out = np.zeros(g.nray)
+    _call("or_fan_rowsum", g, _ptr(out))
+    return out
+
+
+def fan_adjoint(g, proj):
+    out = np.zeros(g.nvox)
+    y = _f32(proj)
+    _call("or_fan_adjoint", g, _ptr(y), _ptr(out))
+    return out
+
+
+def fan_bp_ref(g, proj, quirks=False):
+    out = np.zeros(g.nvox)
+    y = _f32(proj)
+    _call("or_fan_bp_ref", g, _ptr(y), _ptr(out), ctypes.c_int(int(quirks)))
+    return out
+
+
+def fan_filter(g, proj, equiangular=False, filt=0):
+    out = np.zeros(g.nray)
+    y = _f32(proj)
+    _call("or_fan_filter", g, ctypes.c_int(int(equiangular)), ctypes.c_int(filt), _ptr(y), _ptr(out))
+    return out
+
+
+def fan_fbp_bp(g, filtered, equiangular=False):
+    out = np.zeros(g.nvox)
+    y = _f32(filtered)
+    _call("or_fan_fbp_bp", g, ctypes.c_int(int(equiangular)), _ptr(y), _ptr(out))
+    return out
+
+
+def sirt3d(g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None):
+    x = np.zeros(g.nvox) if x0 is None else _f64(x0).copy()
+    r2 = np.zeros(n_iter + 1)
+    r1 = np.zeros(n_iter + 1)
+    bb = _f32(b)
+    _call("or_sirt3d", g, _ptr(bb), _ptr(x), ctypes.c_int(n_iter), ctypes.c_double(lam),
+          ctypes.c_double(lo), ctypes.c_double(hi), _ptr(r2), _ptr(r1))
+    return x, r2, r1
+
+
+def sirt2d(g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None):
+    x = np.zeros(g.nvox) if x0 is None else _f64(x0).copy()
+    r2 = np.zeros(n_iter + 1)
+    r1 = np.zeros(n_iter + 1)
+    bb = _f32(b)
+    _call("or_sirt2d", g, _ptr(bb), _ptr(x), ctypes.c_int(n_iter), ctypes.c_double(lam),
+          ctypes.c_double(lo), ctypes.c_double(hi), _ptr(r2), _ptr(r1))
+    return x, r2, r1
+
+
+def num_threads():
+    fn = lib().or_num_threads
+    fn.restype = ctypes.c_int
+    return fn()

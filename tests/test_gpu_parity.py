@@ -1,0 +1,287 @@
+"""GPU parity: every HIP operator against the FP64 CPU oracle on the same
+seeded inputs, through the C ABI.  Tolerances are stated per test; the
+HIP path computes in FP32 (FP64 per-ray setup for the projector, FP64
+accumulation in the ramp filter), the oracle in FP64."""
+import numpy as np
+import pytest
+
+from conftest import errs, report
+
+pytestmark = pytest.mark.gpu
+
+
+def cone_case(name):
+    """Small cone geometries that exercise every kernel path: partial tiles,
+    anisotropic voxels, truncated detectors, a detector so fine that tile
+    footprints overflow the LDS buffer, descending sample arrays."""
+    d = dict(nt=40, ns=36, nz=20, size=(40.0, 36 * 1.1, 20 * 0.9), D=120.0, nb=30, np_=64, nx=48,
+             pp=1.0, px=0.8, desc=False)
+    if name == "truncated":
+        d.update(np_=40, nx=20, pp=0.7, px=0.7)
+    elif name == "fine":
+        d.update(nt=24, ns=20, nz=18, size=(24.0, 20.0, 18.0), np_=200, nx=150, pp=0.15, px=0.15, nb=12)
+    elif name == "descending":
+        d.update(desc=True)
+    beta = (2 * np.pi / d["nb"]) * np.arange(1, d["nb"] + 1)
+    p = d["pp"] * (np.arange(d["np_"]) - (d["np_"] - 1) / 2)
+    xi = d["px"] * (np.arange(d["nx"]) - (d["nx"] - 1) / 2) + 0.05
+    if d["desc"]:
+        p = p[::-1].copy()
+        xi = xi[::-1].copy()
+    return d, beta.astype(np.float32), p.astype(np.float32), xi.astype(np.float32)
+
+
+def make_cone(name, O):
+    import ctr
+    d, beta, p, xi = cone_case(name)
+    g = ctr.ConeGeom(d["nt"], d["ns"], d["nz"], d["size"], d["D"], beta, p, xi)
+    og = O.Cone(d["nt"], d["ns"], d["nz"], d["size"], d["D"], beta, p, xi)
+    return g, og
+
+
+CASES = ["base", "truncated", "fine", "descending"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_cone_fp(ctx, oracle, case):
+    g, og = make_cone(case, oracle)
+    print(f"[case] cone_fp {case} vol={g.vol_shape} proj={g.proj_shape}", flush=True)
+    rng = np.random.default_rng(1)
+    vol = rng.random(g.nvox).astype(np.float32)
+    gpu = ctx.cone_fp(g, vol)
+    ex = oracle.cone_fp_exact(og, vol)
+    rf = oracle.cone_fp_ref(og, vol)
+    e1, m1 = errs(gpu, ex)
+    e2, m2 = errs(gpu, rf)
+    report(f"cone_fp[{case}]", rel_l2_exact=e1, max_rel_exact=m1, rel_l2_ref=e2, max_rel_ref=m2)
+    assert e1 < 2e-6 and m1 < 2e-5
+    assert e2 < 2e-6 and m2 < 2e-5
+
+
+def test_cone_fp_nonuniform_detector(ctx, oracle):
+    """The forward projector reads per-sample positions (ProjectionCone_3D.cu:20-21),
+    so an equiangular (p = D tan gamma) detector works for projection."""
+    import ctr
+    D = 150.0
+    gam = 0.004 * (np.arange(70) - 34.5)
+    p = (D * np.tan(gam)).astype(np.float32)
+    xi = (0.9 * (np.arange(30) - 14.5)).astype(np.float32)
+    beta = ((2 * np.pi / 17) * np.arange(1, 18)).astype(np.float32)
+    g = ctr.ConeGeom(32, 32, 16, (32.0, 32.0, 16.0), D, beta, p, xi)
+    og = oracle.Cone(32, 32, 16, (32.0, 32.0, 16.0), D, beta, p, xi)
+    vol = np.random.default_rng(2).random(g.nvox).astype(np.float32)
+    e, m = errs(ctx.cone_fp(g, vol), oracle.cone_fp_exact(og, vol))
+    report("cone_fp[nonuniform]", rel_l2=e, max_rel=m)
+    assert e < 2e-6 and m < 2e-5
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_cone_bp_ref(ctx, oracle, case):
+    g, og = make_cone(case, oracle)
+    print(f"[case] cone_bp_ref {case}", flush=True)
+    y = np.random.default_rng(3).random(g.nray).astype(np.float32)
+    gpu = ctx.cone_bp_ref(g, y)
+    ref = oracle.cone_bp_ref(og, y)
+    e, m = errs(gpu, ref)
+    report(f"cone_bp_ref[{case}]", rel_l2=e, max_rel=m)
+    assert e < 1e-5 and m < 2e-4
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fdk_bp(ctx, oracle, case):
+    g, og = make_cone(case, oracle)
+    print(f"[case] fdk_bp {case}", flush=True)
+    y = np.random.default_rng(4).standard_normal(g.nray).astype(np.float32)
+    gpu = ctx.fdk_bp(g, y)
+    ref = oracle.fdk_bp(og, y)
+    e, m = errs(gpu, ref)
+    report(f"fdk_bp[{case}]", rel_l2=e, max_rel=m)
+    assert e < 1e-5 and m < 2e-4
+
+
+def test_fdk_bp_ref_quirks(ctx, oracle):
+    """CTR_REF_QUIRKS reproduces the stale sample above the detector (FDK.cu:100-123)."""
+    import ctr
+    g, og = make_cone("truncated", oracle)
+    y = np.random.default_rng(5).random(g.nray).astype(np.float32)
+    gpu = ctx.fdk_bp(g, y, flags=ctr.REF_QUIRKS)
+    ref_q = oracle.fdk_bp(og, y, quirks=True)
+    ref_n = oracle.fdk_bp(og, y, quirks=False)
+    e, m = errs(gpu, ref_q)
+    report("fdk_bp[quirks]", rel_l2=e, max_rel=m, quirk_effect=errs(ref_q, ref_n)[0])
+    assert errs(ref_q, ref_n)[0] > 1e-2   # the quirk is visible on this geometry
+    assert e < 1e-5 and m < 2e-4
+
+
+def test_fdk_accumulate(ctx, oracle):
+    """View-chunked FDK (FDK_GPU.m:78-104): two half scans accumulated equal one full scan."""
+    import ctr
+    g, og = make_cone("base", oracle)
+    y = np.random.default_rng(6).random(g.nray).astype(np.float32).reshape(g.proj_shape)
+    full = ctx.fdk_bp(g, y)
+    nb = len(g.beta)
+    # same dbeta for both chunks: each chunk keeps two consecutive views first
+    g1 = ctr.ConeGeom(g.nt, g.ns, g.nz, g.size, g.distance, g.beta[: nb // 2], g.p, g.xi)
+    g2 = ctr.ConeGeom(g.nt, g.ns, g.nz, g.size, g.distance, g.beta[nb // 2:], g.p, g.xi)
+    out = ctx.fdk_bp(g1, y[: nb // 2])
+    ctx.fdk_bp(g2, y[nb // 2:], out=out, flags=ctr.ACCUMULATE)
+    e, m = errs(out, full)
+    report("fdk_bp[accumulate]", rel_l2=e, max_rel=m)
+    assert e < 1e-6
+
+
+@pytest.mark.parametrize("filt", [0, 1])
+def test_cone_filter(ctx, oracle, filt):
+    g, og = make_cone("base", oracle)
+    print(f"[case] cone_filter filt={filt}", flush=True)
+    y = np.random.default_rng(7).random(g.nray).astype(np.float32)
+    gpu = ctx.cone_filter(g, y, filt=filt)
+    ref = oracle.cone_filter(og, y, filt=filt)
+    e, m = errs(gpu, ref)
+    report(f"cone_filter[{filt}]", rel_l2=e, max_rel=m)
+    assert e < 1e-6 and m < 1e-5
+
+
+@pytest.mark.parametrize("np_", [33, 130, 512, 1000, 2048])
+def test_filter_widths(ctx, oracle, np_):
+    """Every template width of the row filter (K = 1..16, one and two rows per block)."""
+    import ctr
+    p = (0.37 * (np.arange(np_) - (np_ - 1) / 2)).astype(np.float32)
+    xi = np.array([-0.5, 0.5], np.float32)
+    beta = np.array([0.1, 0.2, 0.3], np.float32)
+    g = ctr.ConeGeom(8, 8, 4, (8.0, 8.0, 4.0), 1000.0, beta, p, xi)
+    og = oracle.Cone(8, 8, 4, (8.0, 8.0, 4.0), 1000.0, beta, p, xi)
+    y = np.random.default_rng(np_).random(g.nray).astype(np.float32)
+    for filt in (0, 1):
+        e, m = errs(ctx.cone_filter(g, y, filt=filt), oracle.cone_filter(og, y, filt=filt))
+        report(f"filter[np={np_},f={filt}]", rel_l2=e, max_rel=m)
+        assert e < 1e-6 and m < 1e-5
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_cone_adjoint(ctx, oracle, case):
+    g, og = make_cone(case, oracle)
+    print(f"[case] cone_adjoint {case}", flush=True)
+    rng = np.random.default_rng(8)
+    y = rng.random(g.nray).astype(np.float32)
+    gpu = ctx.cone_bp_adjoint(g, y)
+    ref = oracle.cone_adjoint(og, y)
+    e, m = errs(gpu, ref)
+    # adjoint identity with the GPU pair itself
+    x = rng.random(g.nvox).astype(np.float32)
+    ax = ctx.cone_fp(g, x).astype(np.float64)
+    lhs = np.dot(ax, y.astype(np.float64))
+    rhs = np.dot(x.astype(np.float64), gpu.astype(np.float64))
+    adj = abs(lhs - rhs) / (np.linalg.norm(ax) * np.linalg.norm(y))
+    report(f"cone_adjoint[{case}]", rel_l2=e, max_rel=m, adjoint_test=adj)
+    assert adj < 1e-5
+    assert e < 1e-4 and m < 1e-3
+
+
+def fan_case(equiangular=False, n=48, nb=40, nbins=80):
+    D = 100.0
+    size = (n * 0.9, n * 0.9)
+    beta = (2 * np.pi / nb) * np.arange(1, nb + 1)
+    if equiangular:
+        gam = 0.008 * (np.arange(nbins) - (nbins - 1) / 2)
+        p = D * np.tan(gam)
+    else:
+        p = 0.7 * (np.arange(nbins) - (nbins - 1) / 2)
+    return n, size, D, beta, p
+
+
+def make_fan(O, equiangular=False, single=False):
+    import ctr
+    n, size, D, beta, p = fan_case(equiangular)
+    if single:
+        beta = beta.astype(np.float32).astype(np.float64)
+        p = p.astype(np.float32).astype(np.float64)
+    return ctr.FanGeom(n, n, size, D, beta, p), O.Fan(n, n, size, D, beta, p)
+
+
+def test_fan_fp(ctx, oracle):
+    for eq in (False, True):
+        g, og = make_fan(oracle, eq)
+        img = np.random.default_rng(9).random(g.nvox).astype(np.float32)
+        gpu = ctx.fan_fp(g, img)
+        e1, m1 = errs(gpu, oracle.fan_fp_exact(og, img))
+        e2, m2 = errs(gpu, oracle.fan_fp_ref(og, img))
+        report(f"fan_fp[eq={eq}]", rel_l2_exact=e1, max_rel_exact=m1, rel_l2_ref=e2, max_rel_ref=m2)
+        assert e1 < 2e-6 and m1 < 2e-5 and e2 < 2e-6 and m2 < 2e-5
+
+
+@pytest.mark.parametrize("quirks", [False, True])
+def test_fan_bp_ref(ctx, oracle, quirks):
+    import ctr
+    g, og = make_fan(oracle, single=True)
+    y = np.random.default_rng(10).random(g.nray).astype(np.float32)
+    gpu = ctx.fan_bp_ref(g, y, flags=ctr.REF_QUIRKS if quirks else 0)
+    ref = oracle.fan_bp_ref(og, y, quirks=quirks)
+    e, m = errs(gpu, ref)
+    report(f"fan_bp_ref[quirks={quirks}]", rel_l2=e, max_rel=m)
+    assert e < 1e-5 and m < 2e-4
+
+
+def test_fan_adjoint(ctx, oracle):
+    g, og = make_fan(oracle)
+    rng = np.random.default_rng(11)
+    y = rng.random(g.nray).astype(np.float32)
+    gpu = ctx.fan_bp_adjoint(g, y)
+    e, m = errs(gpu, oracle.fan_adjoint(og, y))
+    x = rng.random(g.nvox).astype(np.float32)
+    ax = ctx.fan_fp(g, x).astype(np.float64)
+    adj = abs(np.dot(ax, y) - np.dot(x.astype(np.float64), gpu)) / (np.linalg.norm(ax) * np.linalg.norm(y))
+    report("fan_adjoint", rel_l2=e, max_rel=m, adjoint_test=adj)
+    assert adj < 1e-5 and e < 1e-4 and m < 1e-3
+
+
+@pytest.mark.parametrize("eq", [False, True])
+@pytest.mark.parametrize("filt", [0, 1])
+def test_fan_fbp(ctx, oracle, eq, filt):
+    import ctr
+    g, og = make_fan(oracle, eq, single=not eq)
+    y = np.random.default_rng(12).random(g.nray).astype(np.float32)
+    det = ctr.FAN_EQUIANGULAR if eq else ctr.FAN_FLAT
+    f_gpu = ctx.fan_filter(g, y, detector=det, filt=filt)
+    f_ref = oracle.fan_filter(og, y, equiangular=eq, filt=filt)
+    e1, m1 = errs(f_gpu, f_ref)
+    img = ctx.fan_fbp(g, y, detector=det, filt=filt)
+    ref = oracle.fan_fbp_bp(og, f_ref.astype(np.float32), equiangular=eq)
+    e2, m2 = errs(img, ref)
+    report(f"fan_fbp[eq={eq},f={filt}]", filt_rel_l2=e1, filt_max_rel=m1, fbp_rel_l2=e2, fbp_max_rel=m2)
+    assert e1 < 1e-6 and m1 < 1e-5
+    assert e2 < 2e-5 and m2 < 2e-4
+
+
+def test_sirt3d(ctx, oracle):
+    import ctr
+    d, beta, p, xi = cone_case("base")
+    g = ctr.ConeGeom(24, 24, 12, (24.0, 24.0, 12.0), 120.0, beta[::2], p, xi)
+    og = oracle.Cone(24, 24, 12, (24.0, 24.0, 12.0), 120.0, beta[::2], p, xi)
+    vol = np.zeros((12, 24, 24), np.float32)
+    vol[3:9, 6:18, 5:19] = 1.0
+    vol[5:7, 10:14, 9:13] = 0.5
+    b = oracle.cone_fp_exact(og, vol.ravel()).astype(np.float32)
+    x, r2, r1 = ctx.sirt3d(g, b, 20, lam=1.0, lo=0.0)
+    xo, r2o, r1o = oracle.sirt3d(og, b, 20, lam=1.0, lo=0.0)
+    e, m = errs(x, xo)
+    report("sirt3d", rel_l2=e, max_rel=m, res_gpu=float(r2[-1]), res_oracle=float(r2o[-1]))
+    assert e < 1e-4
+    assert np.allclose(r2, r2o, rtol=1e-3, atol=1e-7) and np.allclose(r1, r1o, rtol=1e-3, atol=1e-9)
+    assert np.all(np.diff(r2) <= 1e-7)   # monotone residual with the exact adjoint, lambda in (0, 2)
+    assert x.min() >= 0.0
+
+
+def test_sirt2d(ctx, oracle):
+    g, og = make_fan(oracle)
+    img = np.zeros((g.ns, g.nt), np.float32)
+    img[10:38, 12:36] = 0.8
+    img[20:26, 18:30] = 0.3
+    b = oracle.fan_fp_exact(og, img.ravel()).astype(np.float32)
+    x, r2, r1 = ctx.sirt2d(g, b, 30, lam=2.0, lo=0.0, hi=1.0)
+    xo, r2o, r1o = oracle.sirt2d(og, b, 30, lam=2.0, lo=0.0, hi=1.0)
+    e, m = errs(x, xo)
+    report("sirt2d", rel_l2=e, max_rel=m, res_gpu=float(r2[-1]), res_oracle=float(r2o[-1]))
+    assert e < 1e-4
+    assert np.allclose(r2, r2o, rtol=1e-3, atol=1e-7)
+    assert x.min() >= 0.0 and x.max() <= 1.0
