@@ -332,7 +332,11 @@ BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
     P.rz = (float)(g->size[2] / g->nz);
     P.zc0 = (float)(0.5 * g->size[2] / g->nz - g->size[2] / 2);
     P.D = (float)g->distance;
-    const double dp = (double)g->p[1] - (double)g->p[0], dx = (double)g->xi[1] - (double)g->xi[0];
+    // uniform grid through the end samples: a pitch taken from the first two
+    // float samples (as Backprojection3D.cu:304-305 does) drifts by up to
+    // (n-1)*ulp/pitch samples across the detector when the pitch is not exact in float
+    const double dp = ((double)g->p[g->n_p - 1] - (double)g->p[0]) / (g->n_p - 1);
+    const double dx = ((double)g->xi[g->n_xi - 1] - (double)g->xi[0]) / (g->n_xi - 1);
     P.u_scale = (float)(1.0 / dp);
     P.u_off = (float)(-(double)g->p[0] / dp);
     P.v_scale = (float)(1.0 / dx);
@@ -442,10 +446,13 @@ FanParams fan_params(ctr_ctx *ctx, const ctr_fan_geom *g, unsigned flags)
     P.cs = (float)(g->size[1] / 2);
     P.D = (float)g->distance;
     if (g->n_p >= 2) {
-        const double p0 = (double)T.h_pf[0], dp = (double)T.h_pf[1] - p0;
+        // uniform grids through the end samples (see cone_bp_params)
+        const int n = g->n_p;
+        const double p0 = (double)T.h_pf[0], dp = ((double)T.h_pf[n - 1] - p0) / (n - 1);
         P.u_scale = (float)(1.0 / dp);
         P.u_off = (float)(-p0 / dp);
-        const double g0 = std::atan(g->p[0] / g->distance), dg = std::atan(g->p[1] / g->distance) - g0;
+        const double g0 = std::atan(g->p[0] / g->distance);
+        const double dg = (std::atan(g->p[n - 1] / g->distance) - g0) / (n - 1);
         P.g0 = (float)g0;
         P.g_scale = (float)(1.0 / dg);
     }
@@ -897,7 +904,9 @@ ctr_status ctr_fan_bp_adjoint(ctr_ctx *ctx, const ctr_fan_geom *g, const float *
 ctr_status ctr_fan_filter(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj,
                           float *filtered, unsigned flags)
 {
-    TRY(check_fan(g, 1));
+    // an equiangular detector is uniform in atan(p/D), not in p
+    TRY(check_fan(g, detector == CTR_FAN_FLAT));
+    if (g->n_p < 2) return fail(ctx, CTR_EINVAL, "filtering needs at least 2 detector bins");
     TRY(need_ctx(ctx));
     TRY(check_filter_args(ctx, filter));
     if (detector != CTR_FAN_FLAT && detector != CTR_FAN_EQUIANGULAR)
@@ -916,7 +925,8 @@ ctr_status ctr_fan_filter(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int
 ctr_status ctr_fan_fbp(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj, float *img,
                        unsigned flags)
 {
-    TRY(check_fan(g, 1));
+    TRY(check_fan(g, detector == CTR_FAN_FLAT));
+    if (g->n_p < 2) return fail(ctx, CTR_EINVAL, "filtered backprojection needs at least 2 detector bins");
     TRY(need_ctx(ctx));
     TRY(check_filter_args(ctx, filter));
     if (detector != CTR_FAN_FLAT && detector != CTR_FAN_EQUIANGULAR)

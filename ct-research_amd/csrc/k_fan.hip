@@ -17,30 +17,38 @@ __global__ __launch_bounds__(256) void k_fan_bp_ref(const float *__restrict__ pr
     const float x = ((float)i + 0.5f) * P.rt - P.ct, y = ((float)j + 0.5f) * P.rs - P.cs;
     float acc = 0.0f;
     for (int b = 0; b < P.nb; ++b) {
-        const float sb = (float)P.view[4 * b + 0], cb = (float)P.view[4 * b + 1];
-        const float u = x * cb + y * sb, s = -x * sb + y * cb;
-        const float U = (P.D - s) / P.D;
-        const float pd = u / U;
-        const float uu = fmaf(pd, P.u_scale, P.u_off);
-        float val = 0.0f;
-        if (uu >= 0.0f && uu < P.umax) {
-            const int iu = min((int)floorf(uu), (int)P.umax - 1);
-            const float fu = uu - (float)iu;
-            const float *r = proj + (size_t)b * P.np;
-            val = fmaf(fu, r[iu + 1] - r[iu], r[iu]);
-        }
-        float chord;
+        float val = 0.0f, chord;
+        const float *r = proj + (size_t)b * P.np;
         if (!P.quirks) {
+            const float sb = (float)P.view[4 * b + 0], cb = (float)P.view[4 * b + 1];
+            const float u = x * cb + y * sb, s = -x * sb + y * cb;
+            const float U = (P.D - s) / P.D;
+            const float uu = fmaf(u / U, P.u_scale, P.u_off);
+            if (uu >= 0.0f && uu < P.umax) {
+                const int iu = min((int)floorf(uu), (int)P.umax - 1);
+                const float fu = uu - (float)iu;
+                val = fmaf(fu, r[iu + 1] - r[iu], r[iu]);
+            }
             // the line from the source through the pixel centre: chord = 2 |d| min(h/|d_axis|)
             const float dx = x + P.ct - (float)P.view[4 * b + 2], dy = y + P.cs - (float)P.view[4 * b + 3];
             chord = 2.0f * sqrtf(dx * dx + dy * dy) * fminf(0.5f * P.rt / fabsf(dx), 0.5f * P.rs / fabsf(dy));
         } else {
-            // reference arithmetic in FP64: entry face tested against 0 instead of the
+            // the reference's arithmetic in FP64 (its face test is discontinuous, so
+            // rounding must follow it): entry face tested against 0 instead of the
             // pixel's low edge (:95-101), distance taken to the ray's far end point (:109)
             const double Db = P.D, st = P.view[4 * b + 2], ss = P.view[4 * b + 3];
             const double sbd = P.view[4 * b + 0], cbd = P.view[4 * b + 1];
-            const double th = atan((double)pd / Db), Smax = 2.0 * Db;
             const double ct = P.ct, cs = P.cs;
+            const double xd = ((double)i + 0.5) * P.rt - ct, yd = ((double)j + 0.5) * P.rs - cs;
+            const double ud = xd * cbd + yd * sbd, sd = -xd * sbd + yd * cbd;
+            const double pd = ud / ((Db - sd) / Db);
+            const double uu = pd * (double)P.u_scale + (double)P.u_off;
+            if (uu >= 0.0 && uu < (double)P.umax) {
+                const int iu = min((int)floor(uu), (int)P.umax - 1);
+                const double fu = uu - (double)iu;
+                val = (float)((1.0 - fu) * (double)r[iu] + fu * (double)r[iu + 1]);
+            }
+            const double th = atan(pd / Db), Smax = 2.0 * Db;
             const double te = ct + Smax * sin(th) * cbd - (Db - Smax * cos(th)) * sbd;
             const double se = cs + Smax * sin(th) * sbd + (Db - Smax * cos(th)) * cbd;
             double t2s = (se - ss) / ((te - st) == 0.0 ? 1e-15 : (te - st));
@@ -156,11 +164,10 @@ __global__ __launch_bounds__(256) void k_fan_fbp_bp(const float *__restrict__ fi
             const float g = atan2f(u, fabsf(w));
             const float gg = (g - P.g0) * P.g_scale;
             if (gg >= 0.0f && gg < P.gmax) {
+                // linear interpolation in gamma (FanProjection.m:93-98) on the uniform angle grid
                 const int ig = min((int)floorf(gg), (int)P.gmax - 1);
-                // weights from the stored bin angles, as FanProjection.m:95-98
-                const float g1 = P.gam[ig], g2 = P.gam[ig + 1];
-                const float den = g2 - g1;
-                const float val = (fabsf(g2 - g) * r[ig] + fabsf(g - g1) * r[ig + 1]) / fabsf(den);
+                const float fg = gg - (float)ig;
+                const float val = fmaf(fg, r[ig + 1] - r[ig], r[ig]);
                 acc += val / fmaf(u, u, w * w);
             }
         }

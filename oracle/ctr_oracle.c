@@ -375,13 +375,28 @@ void or_cone_fp_ref(const or_cone *g, const float *vol, double *proj)
  * acceptance window from the end samples
  * (Backprojection3D.cu:304-311, FDK.cu:140-147).
  * ---------------------------------------------------------------------- */
-typedef struct { double first, pitch, lo, hi; int n; const float *pos; } or_axis;
+typedef struct { double first, pitch, step, lo, hi; int n; const float *pos; } or_axis;
+
+/* Interpolation grid of the voxel-driven backprojectors.
+ *   0 (reference): index floor(|x - x0| / |x1 - x0|) with the pitch of the
+ *     first two float samples, weights from the sample positions
+ *     (FDK.cu:102-112).  For a pitch that is not exact in float32 this pitch
+ *     drifts across the detector and near sample boundaries far from x0 the
+ *     index is one too low, where the |.| weights then over-weight.
+ *   1 (uniform): linear interpolation on the uniform grid through the end
+ *     samples x0 + k (x_{n-1} - x0)/(n-1), as the HIP kernels compute.
+ * The two agree when the positions are equally spaced in float32. */
+static int g_interp_mode = 0;
+
+void or_set_interp_mode(int mode) { g_interp_mode = mode; }
 
 static or_axis make_axis(const float *pos, int n)
 {
     or_axis a;
     a.first = pos[0];
     a.pitch = (double)fabsf(pos[1] - pos[0]);
+    a.step = ((double)pos[n - 1] - (double)pos[0]) / (n - 1);
+    if (g_interp_mode == 1) a.pitch = fabs(a.step);
     a.lo = (double)(pos[0] < pos[n - 1] ? pos[0] : pos[n - 1]);
     a.hi = (double)(pos[0] > pos[n - 1] ? pos[0] : pos[n - 1]);
     a.n = n;
@@ -390,10 +405,21 @@ static or_axis make_axis(const float *pos, int n)
 }
 
 /* Linear interpolation coordinates of x on an axis (FDK.cu:102-112): lower
- * sample index and the two distance weights. */
+ * sample index and the two distance weights (each divided by the pitch). */
 static int axis_lerp(const or_axis *a, double x, int *i1, double *w1, double *w2)
 {
     if (!(x >= a->lo && x < a->hi)) return 0;
+    if (g_interp_mode == 1) {
+        double t = (x - a->first) / a->step;
+        int i = (int)floor(t);
+        if (i > a->n - 2) i = a->n - 2;
+        if (i < 0) i = 0;
+        double f = t - i;
+        *i1 = i;
+        *w2 = f * a->pitch;
+        *w1 = (1.0 - f) * a->pitch;
+        return 1;
+    }
     int i = (int)floor(fabs(x - a->first) / a->pitch);
     if (i > a->n - 2) i = a->n - 2; /* the reference would read past the end here */
     *i1 = i;
@@ -698,7 +724,7 @@ void or_fan_fp_ref(const or_fan *g, const float *img, double *proj)
 /* Fan-geometry detector axis from double positions rounded to float, as the
  * BackprojectionFan2D / FBPfan gateways receive single arrays
  * (BackprojectionFan2D.cpp:8-11). */
-typedef struct { double first, pitch, lo, hi; int n; float *pos; } or_faxis;
+typedef struct { double first, pitch, step, lo, hi; int n; float *pos; } or_faxis;
 
 static or_faxis make_faxis(const double *p, int n)
 {
@@ -707,6 +733,7 @@ static or_faxis make_faxis(const double *p, int n)
     for (int i = 0; i < n; i++) a.pos[i] = (float)p[i];
     a.first = a.pos[0];
     a.pitch = (double)fabsf(a.pos[1] - a.pos[0]);
+    a.step = ((double)a.pos[n - 1] - (double)a.pos[0]) / (n - 1);
     a.lo = (double)(a.pos[0] < a.pos[n - 1] ? a.pos[0] : a.pos[n - 1]);
     a.hi = (double)(a.pos[0] > a.pos[n - 1] ? a.pos[0] : a.pos[n - 1]);
     a.n = n;
@@ -716,6 +743,17 @@ static or_faxis make_faxis(const double *p, int n)
 static int faxis_lerp(const or_faxis *a, double x, int *i1, double *wl, double *wu)
 {
     if (!(x >= a->lo && x < a->hi)) return 0;
+    if (g_interp_mode == 1) {
+        double t = (x - a->first) / a->step;
+        int i = (int)floor(t);
+        if (i > a->n - 2) i = a->n - 2;
+        if (i < 0) i = 0;
+        double f = t - i;
+        *i1 = i;
+        *wl = (1.0 - f) * a->pitch;
+        *wu = f * a->pitch;
+        return 1;
+    }
     int i = (int)floor(fabs(x - a->first) / a->pitch);
     if (i > a->n - 2) i = a->n - 2;
     *i1 = i;
@@ -835,9 +873,18 @@ void or_fan_fbp_bp(const or_fan *g, int equiangular, const float *filt, double *
                 } else {
                     double gb = atan2(u, fabs(w - D)), L2 = u * u + (w - D) * (w - D);
                     if (gb >= glo && gb < ghi) {
-                        int i = (int)floor(fabs(gb - g0) / dg);
-                        if (i > g->np - 2) i = g->np - 2;
-                        acc += (fabs(gam[i + 1] - gb) * R[i] + fabs(gb - gam[i]) * R[i + 1]) / (L2 * dg);
+                        if (g_interp_mode == 1) {
+                            double t = (gb - g0) / ((gam[g->np - 1] - g0) / (g->np - 1));
+                            int i = (int)floor(t);
+                            if (i > g->np - 2) i = g->np - 2;
+                            if (i < 0) i = 0;
+                            double f = t - i;
+                            acc += ((1.0 - f) * R[i] + f * R[i + 1]) / L2;
+                        } else {
+                            int i = (int)floor(fabs(gb - g0) / dg);
+                            if (i > g->np - 2) i = g->np - 2;
+                            acc += (fabs(gam[i + 1] - gb) * R[i] + fabs(gb - gam[i]) * R[i + 1]) / (L2 * dg);
+                        }
                     }
                 }
             }

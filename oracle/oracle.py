@@ -213,6 +213,27 @@ def sirt2d(g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None):
     return x, r2, r1
 
 
+def set_interp_mode(mode):
+    """0: the reference's first-pair detector grid; 1: the uniform grid through
+    the end samples (what the HIP backprojectors interpolate on)."""
+    fn = lib().or_set_interp_mode
+    fn.restype = None
+    fn(ctypes.c_int(int(mode)))
+
+
+class interp_mode:
+    """Context manager selecting the oracle's interpolation grid."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        set_interp_mode(self.mode)
+
+    def __exit__(self, *exc):
+        set_interp_mode(0)
+
+
 def num_threads():
     fn = lib().or_num_threads
     fn.restype = ctypes.c_int

@@ -22,9 +22,11 @@ def cone_case(name):
         d.update(nt=24, ns=20, nz=18, size=(24.0, 20.0, 18.0), np_=200, nx=150, pp=0.15, px=0.15, nb=12)
     elif name == "descending":
         d.update(desc=True)
+    elif name == "exact":
+        d.update(pp=0.5, px=0.5)
     beta = (2 * np.pi / d["nb"]) * np.arange(1, d["nb"] + 1)
     p = d["pp"] * (np.arange(d["np_"]) - (d["np_"] - 1) / 2)
-    xi = d["px"] * (np.arange(d["nx"]) - (d["nx"] - 1) / 2) + 0.05
+    xi = d["px"] * (np.arange(d["nx"]) - (d["nx"] - 1) / 2) + (0.0 if name == "exact" else 0.05)
     if d["desc"]:
         p = p[::-1].copy()
         xi = xi[::-1].copy()
@@ -39,7 +41,7 @@ def make_cone(name, O):
     return g, og
 
 
-CASES = ["base", "truncated", "fine", "descending"]
+CASES = ["base", "truncated", "fine", "descending", "exact"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -81,10 +83,15 @@ def test_cone_bp_ref(ctx, oracle, case):
     print(f"[case] cone_bp_ref {case}", flush=True)
     y = np.random.default_rng(3).random(g.nray).astype(np.float32)
     gpu = ctx.cone_bp_ref(g, y)
-    ref = oracle.cone_bp_ref(og, y)
+    with oracle.interp_mode(1):
+        ref = oracle.cone_bp_ref(og, y)
     e, m = errs(gpu, ref)
     report(f"cone_bp_ref[{case}]", rel_l2=e, max_rel=m)
-    assert e < 1e-5 and m < 2e-4
+    assert e < 1e-5 and m < 1e-4
+    if case == "exact":   # float-exact grid: identical to the reference's own grid
+        e0, m0 = errs(gpu, oracle.cone_bp_ref(og, y))
+        report("cone_bp_ref[exact,ref-grid]", rel_l2=e0, max_rel=m0)
+        assert e0 < 1e-5 and m0 < 1e-4
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -93,10 +100,15 @@ def test_fdk_bp(ctx, oracle, case):
     print(f"[case] fdk_bp {case}", flush=True)
     y = np.random.default_rng(4).standard_normal(g.nray).astype(np.float32)
     gpu = ctx.fdk_bp(g, y)
-    ref = oracle.fdk_bp(og, y)
+    with oracle.interp_mode(1):
+        ref = oracle.fdk_bp(og, y)
     e, m = errs(gpu, ref)
     report(f"fdk_bp[{case}]", rel_l2=e, max_rel=m)
-    assert e < 1e-5 and m < 2e-4
+    assert e < 1e-5 and m < 1e-4
+    if case == "exact":
+        e0, m0 = errs(gpu, oracle.fdk_bp(og, y))
+        report("fdk_bp[exact,ref-grid]", rel_l2=e0, max_rel=m0)
+        assert e0 < 1e-5 and m0 < 1e-4
 
 
 def test_fdk_bp_ref_quirks(ctx, oracle):
@@ -105,8 +117,9 @@ def test_fdk_bp_ref_quirks(ctx, oracle):
     g, og = make_cone("truncated", oracle)
     y = np.random.default_rng(5).random(g.nray).astype(np.float32)
     gpu = ctx.fdk_bp(g, y, flags=ctr.REF_QUIRKS)
-    ref_q = oracle.fdk_bp(og, y, quirks=True)
-    ref_n = oracle.fdk_bp(og, y, quirks=False)
+    with oracle.interp_mode(1):
+        ref_q = oracle.fdk_bp(og, y, quirks=True)
+        ref_n = oracle.fdk_bp(og, y, quirks=False)
     e, m = errs(gpu, ref_q)
     report("fdk_bp[quirks]", rel_l2=e, max_rel=m, quirk_effect=errs(ref_q, ref_n)[0])
     assert errs(ref_q, ref_n)[0] > 1e-2   # the quirk is visible on this geometry
@@ -216,7 +229,8 @@ def test_fan_bp_ref(ctx, oracle, quirks):
     g, og = make_fan(oracle, single=True)
     y = np.random.default_rng(10).random(g.nray).astype(np.float32)
     gpu = ctx.fan_bp_ref(g, y, flags=ctr.REF_QUIRKS if quirks else 0)
-    ref = oracle.fan_bp_ref(og, y, quirks=quirks)
+    with oracle.interp_mode(1):
+        ref = oracle.fan_bp_ref(og, y, quirks=quirks)
     e, m = errs(gpu, ref)
     report(f"fan_bp_ref[quirks={quirks}]", rel_l2=e, max_rel=m)
     assert e < 1e-5 and m < 2e-4
@@ -246,7 +260,8 @@ def test_fan_fbp(ctx, oracle, eq, filt):
     f_ref = oracle.fan_filter(og, y, equiangular=eq, filt=filt)
     e1, m1 = errs(f_gpu, f_ref)
     img = ctx.fan_fbp(g, y, detector=det, filt=filt)
-    ref = oracle.fan_fbp_bp(og, f_ref.astype(np.float32), equiangular=eq)
+    with oracle.interp_mode(1):
+        ref = oracle.fan_fbp_bp(og, f_ref.astype(np.float32), equiangular=eq)
     e2, m2 = errs(img, ref)
     report(f"fan_fbp[eq={eq},f={filt}]", filt_rel_l2=e1, filt_max_rel=m1, fbp_rel_l2=e2, fbp_max_rel=m2)
     assert e1 < 1e-6 and m1 < 1e-5

@@ -1,0 +1,302 @@
+"""Pins the FP64 oracle (oracle/ctr_oracle.c) before it is trusted as the
+checker: brute force on tiny inputs, closed forms, invariants and an
+independent dense-matrix SIRT.  CPU only."""
+import numpy as np
+import pytest
+
+
+def small_cone(O, nt=7, ns=6, nz=5, size=(7.0, 6.6, 4.5), D=25.0, nb=9, np_=11, nx=9, pp=0.9, px=0.7,
+               beta0=None):
+    beta = (2 * np.pi / nb) * np.arange(1, nb + 1) if beta0 is None else np.asarray(beta0)
+    p = pp * (np.arange(np_) - (np_ - 1) / 2)
+    xi = px * (np.arange(nx) - (nx - 1) / 2)
+    return O.Cone(nt, ns, nz, size, D, beta.astype(np.float32), p.astype(np.float32), xi.astype(np.float32))
+
+
+def rays(og):
+    """Source and unit direction of every ray, computed independently in numpy."""
+    beta = og.beta.astype(np.float64)[:, None, None]
+    p = og.p.astype(np.float64)[None, None, :]
+    xi = og.xi.astype(np.float64)[None, :, None]
+    D = og.s.D
+    size = np.array(og.s.size[:])
+    S = np.stack(np.broadcast_arrays(size[0] / 2 - D * np.sin(beta), size[1] / 2 + D * np.cos(beta),
+                                     size[2] / 2 + 0 * beta), -1)
+    d = np.stack(np.broadcast_arrays(p * np.cos(beta) + D * np.sin(beta), p * np.sin(beta) - D * np.cos(beta),
+                                     xi + 0 * beta), -1)
+    d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    S = np.broadcast_to(S, d.shape)
+    return S.reshape(-1, 3), d.reshape(-1, 3), size
+
+
+def test_siddon_exact_matches_brute_force(oracle):
+    rng = np.random.default_rng(0)
+    for kw in [{}, dict(size=(7.0, 7.0, 7.0), nz=7), dict(D=60.0, pp=0.3, px=0.25, nb=5)]:
+        og = small_cone(oracle, **kw)
+        vol = rng.random(og.nvox).astype(np.float32)
+        ex = oracle.cone_fp_exact(og, vol)
+        nb, nx, np_ = len(og.beta), len(og.xi), len(og.p)
+        br = np.array([oracle.cone_ray_brute(og, vol, b, ip, ix) for b in range(nb) for ix in range(nx)
+                       for ip in range(np_)])
+        assert np.abs(ex - br).max() <= 1e-10 * np.abs(br).max()
+
+
+def test_fp_of_ones_is_box_chord(oracle):
+    """FP of a constant-1 volume = chord of the ray through the volume box (slab method)."""
+    og = small_cone(oracle, nt=9, ns=8, nz=6, size=(9.0, 8.8, 5.4), np_=15, nx=13, pp=1.1, px=0.9)
+    S, d, size = rays(og)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t1 = (0 - S) / d
+        t2 = (size - S) / d
+    lo = np.nanmax(np.where(d != 0, np.minimum(t1, t2), -np.inf), axis=1)
+    hi = np.nanmin(np.where(d != 0, np.maximum(t1, t2), np.inf), axis=1)
+    chord = np.maximum(hi - lo, 0)
+    fp1 = oracle.cone_fp_exact(og, np.ones(og.nvox, np.float32))
+    assert np.abs(fp1 - chord).max() < 1e-10
+    assert np.abs(oracle.cone_rowsum(og) - chord).max() < 1e-10
+    assert (chord > 0).sum() > 0.25 * len(chord)
+
+
+def test_fp_of_ball_converges_to_analytic_chord(oracle):
+    """Line integrals of a rasterised ball approach 2 sqrt(r^2 - dist^2) (ray-sphere closed form)."""
+    n = 96
+    size = (48.0, 48.0, 48.0)
+    og = small_cone(oracle, nt=n, ns=n, nz=n, size=size, D=200.0, nb=3, np_=21, nx=5, pp=1.7, px=3.1)
+    c = np.array(size) / 2
+    r = 15.0
+    g = (np.arange(n) + 0.5) * 0.5
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")   # [z][s][t]
+    vol = (((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) <= r * r).astype(np.float32).ravel()
+    fp = oracle.cone_fp_exact(og, vol)
+    S, d, _ = rays(og)
+    w = S - c
+    tca = -(w * d).sum(1)
+    dist2 = (w * w).sum(1) - tca ** 2
+    chord = 2 * np.sqrt(np.maximum(r * r - dist2, 0))
+    inside = chord > 10
+    assert inside.sum() > 50
+    # staircase boundary of a 0.5 mm raster: mean error well under a voxel, totals to 0.3 %
+    assert np.abs(fp[inside] - chord[inside]).mean() < 0.25
+    assert abs(fp[inside].sum() / chord[inside].sum() - 1) < 3e-3
+
+
+def test_ref_restatement_agrees_with_exact(oracle):
+    rng = np.random.default_rng(1)
+    og = small_cone(oracle, nt=12, ns=11, nz=7, size=(12.0, 12.1, 6.3), np_=23, nx=15, pp=0.8, px=0.6)
+    vol = rng.random(og.nvox).astype(np.float32)
+    ex, rf = oracle.cone_fp_exact(og, vol), oracle.cone_fp_ref(og, vol)
+    assert np.abs(ex - rf).max() < 1e-7 * np.abs(ex).max()
+    # no ray lies exactly in a voxel plane (those are assigned to either side by rounding)
+    fg = oracle.Fan(12, 11, (12.0, 12.1), 30.0, (2 * np.pi / 13) * np.arange(1, 14), 0.8 * (np.arange(25) - 12) + 0.1)
+    img = rng.random(fg.nvox).astype(np.float32)
+    assert np.abs(oracle.fan_fp_exact(fg, img) - oracle.fan_fp_ref(fg, img)).max() < 1e-7 * 12
+
+
+def test_mirror_symmetry(oracle):
+    """At beta = 0 the t-mirror of the volume maps ray p to ray -p, the z-mirror maps xi to -xi."""
+    # even sample counts: no ray lies in the mirror plane itself
+    og = small_cone(oracle, nt=10, ns=9, nz=8, size=(10.0, 9.0, 8.0), beta0=[0.0], np_=16, nx=12, pp=0.9, px=0.8)
+    vol = np.random.default_rng(2).random(og.nvox).astype(np.float32).reshape(8, 9, 10)
+    a = oracle.cone_fp_exact(og, vol.ravel()).reshape(12, 16)
+    b = oracle.cone_fp_exact(og, vol[:, :, ::-1].ravel()).reshape(12, 16)
+    c = oracle.cone_fp_exact(og, vol[::-1].ravel()).reshape(12, 16)
+    assert np.allclose(a, b[:, ::-1], rtol=1e-12, atol=1e-12)
+    assert np.allclose(a, c[::-1, :], rtol=1e-12, atol=1e-12)
+
+
+def test_adjoint_identity(oracle):
+    rng = np.random.default_rng(3)
+    og = small_cone(oracle)
+    x = rng.random(og.nvox).astype(np.float32)
+    y = rng.random(og.nray).astype(np.float32)
+    lhs = np.dot(oracle.cone_fp_exact(og, x), y.astype(np.float64))
+    rhs = np.dot(x.astype(np.float64), oracle.cone_adjoint(og, y))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    fg = oracle.Fan(9, 8, (9.0, 8.8), 30.0, (2 * np.pi / 11) * np.arange(1, 12), 0.7 * (np.arange(19) - 9))
+    x2 = rng.random(fg.nvox).astype(np.float32)
+    y2 = rng.random(fg.nray).astype(np.float32)
+    assert abs(np.dot(oracle.fan_fp_exact(fg, x2), y2) - np.dot(x2, oracle.fan_adjoint(fg, y2))) < 1e-10
+
+
+def _response(h, w):
+    n = np.arange(1, len(h))
+    return h[0] + 2 * np.sum(h[1:, None] * np.cos(np.outer(n, w)), axis=0)
+
+
+def test_filter_taps_frequency_response(oracle):
+    """Half Shepp-Logan: H(w) = |sin(w/2)|/(2 pi); half Ram-Lak: H(w) = |w|/(4 pi), |w| <= pi:
+    both are half the ramp |w|/(2 pi) at low frequency (d = 1 here; the taps scale as
+    1/d^2 and the convolution multiplies by d)."""
+    N = 200000
+    w = np.linspace(0.05, np.pi, 9)
+    sl = oracle.filter_taps(0, False, 1.0, N)
+    rl = oracle.filter_taps(1, False, 1.0, N)
+    assert np.allclose(_response(sl, w), np.abs(np.sin(w / 2)) / (2 * np.pi), atol=2e-6)
+    assert np.allclose(_response(rl, w), np.abs(w) / (4 * np.pi), atol=2e-5)
+    d = 0.37
+    assert np.allclose(oracle.filter_taps(1, False, d, 50), rl[:50] / d ** 2)
+    # equiangular Ram-Lak: h(n d) = -1/(2 pi^2 sin^2(n d)) for odd n (FanProjection.m:241-254 halved)
+    eq = oracle.filter_taps(1, True, 0.01, 8)
+    assert np.isclose(eq[3], -1 / (2 * np.pi ** 2 * np.sin(0.03) ** 2)) and eq[2] == 0 and np.isclose(eq[0], 1 / (8e-4))
+
+
+def test_cone_filter_matches_numpy(oracle):
+    og = small_cone(oracle, np_=37, nx=5, pp=0.6, px=0.9, nb=3)
+    y = np.random.default_rng(4).random(og.nray).astype(np.float32)
+    for filt in (0, 1):
+        out = oracle.cone_filter(og, y, filt).reshape(3, 5, 37)
+        d = float(np.float32(abs(og.p[1] - og.p[0])))
+        h = oracle.filter_taps(filt, False, d, 37)
+        h2 = np.concatenate([h[:0:-1], h])
+        Y = y.reshape(3, 5, 37).astype(np.float64)
+        P, X = og.p.astype(np.float64), og.xi.astype(np.float64)
+        D = og.s.D
+        wgt = D / np.sqrt(D * D + P[None, :] ** 2 + X[:, None] ** 2)
+        W = (Y * wgt[None]).astype(np.float32).astype(np.float64)
+        ref = np.stack([[d * np.convolve(W[b, x], h2)[36:73] for x in range(5)] for b in range(3)])
+        assert np.abs(out - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def _mag_geometry(og):
+    nt, ns, nz = og.s.nt, og.s.ns, og.s.nz
+    size = np.array(og.s.size[:])
+    r = size / np.array([nt, ns, nz])
+    x = (np.arange(nt) + 0.5) * r[0] - size[0] / 2
+    y = (np.arange(ns) + 0.5) * r[1] - size[1] / 2
+    z = (np.arange(nz) + 0.5) * r[2] - size[2] / 2
+    return x, y, z, r
+
+
+def test_fdk_bp_constant_closed_form(oracle):
+    """Constant filtered projections c: every voxel whose sample lies on the detector gets
+    c * mag^2 * dbeta per view (FDK.cu:116-120)."""
+    og = small_cone(oracle, nt=8, ns=9, nz=7, size=(8.0, 9.0, 7.0), np_=13, nx=11, pp=1.0, px=1.0, D=30.0)
+    c = 0.75
+    out = oracle.fdk_bp(og, np.full(og.nray, c, np.float32)).reshape(7, 9, 8)
+    x, y, z, r = _mag_geometry(og)
+    D = og.s.D
+    dbeta = abs(float(og.beta[1]) - float(og.beta[0]))
+    ref = np.zeros((7, 9, 8))
+    for b in og.beta.astype(np.float64):
+        u = x[None, :] * np.cos(b) + y[:, None] * np.sin(b)
+        s = -x[None, :] * np.sin(b) + y[:, None] * np.cos(b)
+        m = D / (D - s)
+        P = u * m
+        X = z[:, None, None] * m[None]
+        ok = (P[None] >= og.p[0]) & (P[None] < og.p[-1]) & (X >= og.xi[0]) & (X < og.xi[-1])
+        ref += np.where(ok, c * m[None] ** 2 * dbeta, 0.0)
+    assert np.abs(out - ref).max() < 1e-9 * ref.max()
+
+
+def test_bp_ref_ones_is_sum_of_chords(oracle):
+    """All-ones projections: BP = sum over views of the chord of the source-to-centre line
+    through the voxel, 2 |d| min_a(h_a/|d_a|) (Backprojection3D.cu:243-288)."""
+    og = small_cone(oracle, nt=6, ns=7, nz=5, size=(6.0, 7.7, 4.0), np_=15, nx=13, pp=1.0, px=1.0, D=30.0)
+    out = oracle.cone_bp_ref(og, np.ones(og.nray, np.float32)).reshape(5, 7, 6)
+    x, y, z, r = _mag_geometry(og)
+    D = og.s.D
+    ref = np.zeros((5, 7, 6))
+    for b in og.beta.astype(np.float64):
+        dx = x[None, None, :] + D * np.sin(b)
+        dy = y[None, :, None] - D * np.cos(b)
+        dz = z[:, None, None] + 0 * dx
+        dx, dy = dx + 0 * dz, dy + 0 * dz
+        n = np.sqrt(dx ** 2 + dy ** 2 + dz ** 2)
+        with np.errstate(divide="ignore"):
+            lam = np.minimum(np.minimum(r[0] / 2 / np.abs(dx), r[1] / 2 / np.abs(dy)), r[2] / 2 / np.abs(dz))
+        u = x[None, None, :] * np.cos(b) + y[None, :, None] * np.sin(b)
+        s = -x[None, None, :] * np.sin(b) + y[None, :, None] * np.cos(b)
+        m = D / (D - s)
+        ok = (u * m >= og.p[0]) & (u * m < og.p[-1]) & (dz * m >= og.xi[0]) & (dz * m < og.xi[-1])
+        ref += np.where(ok, 2 * n * lam, 0.0)
+    assert np.abs(out - ref).max() < 1e-9 * ref.max()
+
+
+def test_fdk_recovers_ball_amplitude(oracle):
+    """FDK of a centred uniform ball recovers its amplitude near the centre."""
+    n = 40
+    og = small_cone(oracle, nt=n, ns=n, nz=n, size=(40.0, 40.0, 40.0), D=200.0, nb=120, np_=96, nx=72,
+                    pp=0.5, px=0.5)
+    g = (np.arange(n) + 0.5) - 20.0
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    vol = ((X ** 2 + Y ** 2 + Z ** 2) <= 12.0 ** 2).astype(np.float32)
+    proj = oracle.cone_fp_exact(og, vol.ravel()).astype(np.float32)
+    rec = oracle.fdk_bp(og, oracle.cone_filter(og, proj, 1).astype(np.float32)).reshape(n, n, n)
+    core = rec[16:24, 16:24, 16:24]
+    assert abs(core.mean() - 1.0) < 0.03, core.mean()
+    assert abs(rec[2:6, 2:6, 18:22].mean()) < 0.05
+
+
+@pytest.mark.parametrize("equiangular", [False, True])
+@pytest.mark.parametrize("filt", [0, 1])
+def test_fan_fbp_recovers_disk(oracle, equiangular, filt):
+    """Flat and equiangular fan FBP (both filters) of a uniform disk recover its amplitude
+    inside and zero outside: pins every scale factor (dbeta, pitch, 1/U^2 or 1/L^2, the
+    half-filter convention) at once."""
+    import ctr
+    n = 128
+    g0 = np.arange(n) + 0.5 - 64
+    Y, X = np.meshgrid(g0, g0, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2)
+    disk = (r <= 40).astype(np.float32)
+    if equiangular:
+        g = ctr.geometry.fan_equiangular(n, 1.0, 360, 256, 1.0, 500.0, 1000.0)
+    else:
+        g = ctr.geometry.fan_flat(n, 1.0, 360, 256, 1.0, 500.0, 1000.0)
+    og = oracle.Fan(n, n, g.size, g.distance, g.beta, g.p)
+    proj = oracle.fan_fp_exact(og, disk.ravel()).astype(np.float32)
+    f = oracle.fan_filter(og, proj, equiangular, filt).astype(np.float32)
+    rec = oracle.fan_fbp_bp(og, f, equiangular).reshape(n, n)
+    assert abs(rec[r < 30].mean() - 1.0) < 5e-3
+    assert abs(rec[(r > 48) & (r < 60)].mean()) < 5e-3
+    # the Shepp-Logan phantom as a whole
+    img = ctr.phantoms.shepp_logan_2d(n).astype(np.float32)
+    rec2 = oracle.fan_fbp_bp(og, oracle.fan_filter(og, oracle.fan_fp_exact(og, img.ravel()).astype(np.float32),
+                                                   equiangular, filt).astype(np.float32), equiangular).reshape(n, n)
+    assert np.linalg.norm(rec2 - img) / np.linalg.norm(img) < 0.2
+
+
+def test_dense_matrix_sirt(oracle):
+    """The oracle's matrix-free SIRT equals SIRT3D.m:80-82 / SIRT.m:129-132 run on the explicit
+    system matrix, whose columns are the projections of unit vectors."""
+    og = small_cone(oracle, nt=5, ns=4, nz=3, size=(5.0, 4.0, 3.0), np_=9, nx=7, pp=0.8, px=0.6, nb=7, D=20.0)
+    A = np.stack([oracle.cone_fp_exact(og, np.eye(og.nvox, dtype=np.float32)[j]) for j in range(og.nvox)], 1)
+    rng = np.random.default_rng(5)
+    xt = rng.random(og.nvox)
+    b32 = (A @ xt).astype(np.float32)
+    b = b32.astype(np.float64)
+    x = np.zeros(og.nvox)
+    rows, cols = A.sum(1), A.sum(0)
+    res = []
+    for k in range(15):
+        e = b - A @ x
+        res.append(np.linalg.norm(e) / np.linalg.norm(b))
+        x = np.maximum(x + 1.0 * (A.T @ (e / (rows + 1e-9))) / (cols + 1e-9), 0.0)
+    res.append(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+    xo, r2, r1 = oracle.sirt3d(og, b32, 15, 1.0, 0.0)
+    assert np.abs(xo - x).max() < 1e-10
+    assert np.allclose(r2, res, rtol=1e-9)
+    assert np.all(np.diff(r2) <= 1e-12)
+    # L1 form of SIRT3D.m:87
+    assert np.isclose(r1[-1], np.abs(b - A @ x).sum() / (b.sum() * og.nvox))
+
+
+def test_dense_matrix_sirt2d(oracle):
+    fg = oracle.Fan(6, 5, (6.0, 5.5), 20.0, (2 * np.pi / 9) * np.arange(1, 10), 0.6 * (np.arange(15) - 7))
+    A = np.stack([oracle.fan_fp_exact(fg, np.eye(fg.nvox, dtype=np.float32)[j]) for j in range(fg.nvox)], 1)
+    b = (A @ np.random.default_rng(6).random(fg.nvox)).astype(np.float32)
+    x = np.zeros(fg.nvox)
+    for _ in range(10):
+        x = np.clip(x + 2.0 * (A.T @ ((b - A @ x) / (A.sum(1) + 1e-9))) / (A.sum(0) + 1e-9), 0.0, 1.0)
+    xo, _, _ = oracle.sirt2d(fg, b, 10, 2.0, 0.0, 1.0)
+    assert np.abs(xo - x).max() < 1e-10
+
+
+def test_grid_modes_agree_on_exact_grids(oracle):
+    """The reference's first-pair grid and the end-sample grid coincide on float-exact positions."""
+    og = small_cone(oracle, np_=16, nx=12, pp=0.5, px=0.25)
+    y = np.random.default_rng(7).random(og.nray).astype(np.float32)
+    a = oracle.fdk_bp(og, y)
+    with oracle.interp_mode(1):
+        b = oracle.fdk_bp(og, y)
+    assert np.abs(a - b).max() < 1e-12 * np.abs(a).max()
