@@ -27,7 +27,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-views", type=int, default=120, help="views in the bounded CPU-baseline sample")
+    ap.add_argument("--cpu-views", type=int, default=360, help="views in the CPU-baseline sample")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="before the timed region, copy this many MiB device-to-device (a plain copy kernel of "
+                         "known size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -87,6 +90,11 @@ def main():
     proj = ctx.cone_fp(g, vol)
     filt = ctx.cone_filter(g, proj, filt=ctr.FILTER_SL_HALF)
     out = torch.empty(nvox, dtype=torch.float32, device=dev)
+    if args.calib_copy_mb:
+        a = torch.empty(args.calib_copy_mb << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        del a, b
     torch.cuda.synchronize()
 
     # filter timing (reported separately, not part of the metric)

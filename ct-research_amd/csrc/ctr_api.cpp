@@ -444,6 +444,10 @@ FanParams fan_params(ctr_ctx *ctx, const ctr_fan_geom *g, unsigned flags)
     P.rs = (float)(g->size[1] / g->ns);
     P.ct = (float)(g->size[0] / 2);
     P.cs = (float)(g->size[1] / 2);
+    P.rt_d = g->size[0] / g->nt;
+    P.rs_d = g->size[1] / g->ns;
+    P.ct_d = g->size[0] / 2;
+    P.cs_d = g->size[1] / 2;
     P.D = (float)g->distance;
     if (g->n_p >= 2) {
         // uniform grids through the end samples (see cone_bp_params)
@@ -451,6 +455,8 @@ FanParams fan_params(ctr_ctx *ctx, const ctr_fan_geom *g, unsigned flags)
         const double p0 = (double)T.h_pf[0], dp = ((double)T.h_pf[n - 1] - p0) / (n - 1);
         P.u_scale = (float)(1.0 / dp);
         P.u_off = (float)(-p0 / dp);
+        P.u_scale_d = 1.0 / dp;
+        P.u_off_d = -p0 / dp;
         const double g0 = std::atan(g->p[0] / g->distance);
         const double dg = (std::atan(g->p[n - 1] / g->distance) - g0) / (n - 1);
         P.g0 = (float)g0;
@@ -694,7 +700,7 @@ ctr_status ctr_ctx_set_stream(ctr_ctx *ctx, void *hip_stream)
 {
     TRY(need_ctx(ctx));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own;
+    ctx->stream = (hipStream_t)hip_stream;
     return CTR_OK;
 }
 

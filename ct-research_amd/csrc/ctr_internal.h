@@ -82,6 +82,7 @@ struct FanParams {
     float u_scale, u_off, umax;
     float dbeta;
     float g0, g_scale, gmax;   // equiangular: gamma index = (gamma - g0)*g_scale, valid in [0,gmax)
+    double rt_d, rs_d, ct_d, cs_d, u_scale_d, u_off_d;   // FP64 copies for the reference-quirk path
     const double *view;     // [nb][4]: sin, cos, source t, source s
     const float *pf;        // [np] bin positions (float)
     const float *gam;       // [np] bin angles (float, equiangular)

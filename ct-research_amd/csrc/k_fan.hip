@@ -38,11 +38,11 @@ __global__ __launch_bounds__(256) void k_fan_bp_ref(const float *__restrict__ pr
             // pixel's low edge (:95-101), distance taken to the ray's far end point (:109)
             const double Db = P.D, st = P.view[4 * b + 2], ss = P.view[4 * b + 3];
             const double sbd = P.view[4 * b + 0], cbd = P.view[4 * b + 1];
-            const double ct = P.ct, cs = P.cs;
-            const double xd = ((double)i + 0.5) * P.rt - ct, yd = ((double)j + 0.5) * P.rs - cs;
+            const double ct = P.ct_d, cs = P.cs_d;
+            const double xd = ((double)i + 0.5) * P.rt_d - ct, yd = ((double)j + 0.5) * P.rs_d - cs;
             const double ud = xd * cbd + yd * sbd, sd = -xd * sbd + yd * cbd;
             const double pd = ud / ((Db - sd) / Db);
-            const double uu = pd * (double)P.u_scale + (double)P.u_off;
+            const double uu = pd * P.u_scale_d + P.u_off_d;
             if (uu >= 0.0 && uu < (double)P.umax) {
                 const int iu = min((int)floor(uu), (int)P.umax - 1);
                 const double fu = uu - (double)iu;
@@ -55,8 +55,8 @@ __global__ __launch_bounds__(256) void k_fan_bp_ref(const float *__restrict__ pr
             double s2t = (te - st) / ((se - ss) == 0.0 ? 1e-15 : (se - ss));
             t2s = fmin(fmax(t2s, -1e16), 1e16);
             s2t = fmin(fmax(s2t, -1e16), 1e16);
-            const double tlo = (double)i * P.rt, thi = (double)(i + 1) * P.rt;
-            const double slo = (double)j * P.rs, shi = (double)(j + 1) * P.rs;
+            const double tlo = (double)i * P.rt_d, thi = (double)(i + 1) * P.rt_d;
+            const double slo = (double)j * P.rs_d, shi = (double)(j + 1) * P.rs_d;
             const double tlo_s = ss + (tlo - st) * t2s, slo_t = st + (slo - ss) * s2t;
             double T1, S1;
             if (tlo_s >= 0.0 && tlo_s <= shi) {

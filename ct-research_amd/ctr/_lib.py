@@ -179,7 +179,7 @@ class Context:
         if st != OK:
             raise CtrError(st, L.ctr_last_error(None).decode())
         self.h = h
-        self._bound_stream = None
+        self._bound_stream = L.ctr_ctx_stream(h)   # the context's own stream
 
     def close(self):
         if self.h:

@@ -116,7 +116,10 @@ const char *ctr_last_error(const ctr_ctx *ctx);
  * ProjectionCone_3D.cpp:53). */
 ctr_status ctr_ctx_create(int device, ctr_ctx **out);
 void ctr_ctx_destroy(ctr_ctx *ctx);
-/* Use an external hipStream_t (NULL = the context's own stream). */
+/* Order the context's work on hip_stream (a hipStream_t; NULL is HIP's null
+ * stream, e.g. torch's default stream).  The context's own stream is the
+ * value ctr_ctx_stream() returns right after creation; pass it back to
+ * restore it. */
 ctr_status ctr_ctx_set_stream(ctr_ctx *ctx, void *hip_stream);
 void *ctr_ctx_stream(const ctr_ctx *ctx);
 ctr_status ctr_ctx_synchronize(ctr_ctx *ctx);

@@ -98,17 +98,19 @@ def test_cone_bp_ref(ctx, oracle, case):
 def test_fdk_bp(ctx, oracle, case):
     g, og = make_cone(case, oracle)
     print(f"[case] fdk_bp {case}", flush=True)
+    # zero-mean data like filtered projections: the per-view terms cancel, so the FP32
+    # relative error of the sum is a few times that of the terms (bound 3e-5)
     y = np.random.default_rng(4).standard_normal(g.nray).astype(np.float32)
     gpu = ctx.fdk_bp(g, y)
     with oracle.interp_mode(1):
         ref = oracle.fdk_bp(og, y)
     e, m = errs(gpu, ref)
     report(f"fdk_bp[{case}]", rel_l2=e, max_rel=m)
-    assert e < 1e-5 and m < 1e-4
+    assert e < 3e-5 and m < 1e-4
     if case == "exact":
         e0, m0 = errs(gpu, oracle.fdk_bp(og, y))
         report("fdk_bp[exact,ref-grid]", rel_l2=e0, max_rel=m0)
-        assert e0 < 1e-5 and m0 < 1e-4
+        assert e0 < 3e-5 and m0 < 1e-4
 
 
 def test_fdk_bp_ref_quirks(ctx, oracle):

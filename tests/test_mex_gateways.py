@@ -1,0 +1,165 @@
+"""The MEX gateways (ct-research_amd/mex/*.cpp) through a fake MEX runtime
+(tests/fakemex): argument checks and error ids on the CPU, numerics on the
+GPU against the C ABI and the oracle.  The argument lists are the
+reference's MATLAB call signatures."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, errs, gpu_available, report
+
+FM = os.path.join(ROOT, "tests", "fakemex", "build")
+DOUBLE, SINGLE, INT32 = 6, 7, 12
+
+
+class Mex:
+    _fm = None
+
+    def __init__(self, name):
+        if Mex._fm is None:
+            Mex._fm = ctypes.CDLL(os.path.join(FM, "libfakemex.so"), mode=ctypes.RTLD_GLOBAL)
+            f = Mex._fm
+            f.fm_create.restype = ctypes.c_void_p
+            f.fm_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p]
+            f.fm_destroy.argtypes = [ctypes.c_void_p]
+            f.fm_data.restype = ctypes.c_void_p
+            f.fm_data.argtypes = [ctypes.c_void_p]
+            f.fm_m.restype = f.fm_n.restype = ctypes.c_size_t
+            f.fm_m.argtypes = f.fm_n.argtypes = [ctypes.c_void_p]
+            f.fm_call.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_char_p, ctypes.c_char_p]
+        self.lib = ctypes.CDLL(os.path.join(FM, f"mex_{name}.so"))
+        self.fn = ctypes.cast(self.lib.mexFunction, ctypes.c_void_p)
+
+    @staticmethod
+    def arr(x, cls=None, row=False):
+        x = np.asarray(x)
+        if cls is None:
+            cls = SINGLE if x.dtype == np.float32 else DOUBLE
+        dt = {DOUBLE: np.float64, SINGLE: np.float32, INT32: np.int32}[cls]
+        x = np.ascontiguousarray(x.reshape(-1), dtype=dt)
+        m, n = (1, x.size) if row else (x.size, 1)
+        return MxArray(Mex._fm.fm_create(cls, m, n, x.ctypes.data))
+
+    def __call__(self, *args, nlhs=1):
+        prhs = [(a if isinstance(a, MxArray) else self.arr(a)).ptr for a in args]
+        arr_in = (ctypes.c_void_p * len(prhs))(*prhs)
+        arr_out = (ctypes.c_void_p * max(nlhs, 1))()
+        eid, emsg = ctypes.create_string_buffer(128), ctypes.create_string_buffer(1024)
+        rc = Mex._fm.fm_call(self.fn, nlhs, arr_out, len(prhs), arr_in, eid, emsg)
+        for p in prhs:
+            Mex._fm.fm_destroy(p)
+        if rc:
+            raise MexError(eid.value.decode(), emsg.value.decode())
+        outs = []
+        for k in range(nlhs):
+            a = arr_out[k]
+            n = Mex._fm.fm_m(a) * Mex._fm.fm_n(a)
+            outs.append(np.ctypeslib.as_array((ctypes.c_double * n).from_address(Mex._fm.fm_data(a))).copy())
+            Mex._fm.fm_destroy(a)
+        return outs[0] if nlhs == 1 else outs
+
+
+class MxArray:
+    """An mxArray built by the harness (lets a test pass a chosen class)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+class MexError(Exception):
+    def __init__(self, ident, msg):
+        super().__init__(f"{ident}: {msg}")
+        self.ident = ident
+
+
+def cone_args():
+    beta = ((2 * np.pi / 20) * np.arange(1, 21)).astype(np.float32)
+    p = (0.5 * (np.arange(48) - 23.5)).astype(np.float32)
+    xi = (0.5 * (np.arange(32) - 15.5)).astype(np.float32)
+    return 24, 20, 12, np.array([24.0, 20.0, 12.0]), beta, p, xi, 80.0
+
+
+def test_gateway_argument_errors():
+    """Wrong argument count, class or size raise ctr:<op>:<what> before any device work."""
+    nt, ns, nz, size, beta, p, xi, D = cone_args()
+    vol = np.zeros(nt * ns * nz, np.float32)
+    fp = Mex("ProjectionCone_3D")
+    with pytest.raises(MexError) as e:
+        fp(vol, nt, ns, nz, size, beta, p, xi)
+    assert e.value.ident == "ctr:ProjectionCone_3D:nrhs"
+    with pytest.raises(MexError) as e:
+        fp(vol[:-1], nt, ns, nz, size, beta, p, xi, D)
+    assert e.value.ident == "ctr:ProjectionCone_3D:size"
+    with pytest.raises(MexError) as e:
+        fp(Mex.arr(vol.astype(np.int32), INT32), nt, ns, nz, size, beta, p, xi, D)
+    assert e.value.ident == "ctr:ProjectionCone_3D:class"
+    with pytest.raises(MexError) as e:
+        fp(vol, 0, ns, nz, size, beta, p, xi, D)
+    assert e.value.ident == "ctr:ProjectionCone_3D:value"
+    with pytest.raises(MexError) as e:       # source orbit inside the volume
+        fp(vol, nt, ns, nz, size, beta, p, xi, 5.0)
+    assert e.value.ident == "ctr:ProjectionCone_3D:EINVAL"
+    bp = Mex("Backprojection")
+    nonuni = np.array([0, 0.5, 1.0, 1.7, 2.0], np.float32)
+    with pytest.raises(MexError) as e:
+        bp(np.zeros(len(beta) * len(nonuni) * len(xi), np.float32), xi, nonuni, beta, D, size, nt, ns, nz)
+    assert e.value.ident == "ctr:Backprojection:EINVAL" and "non-uniform" in str(e.value)
+    s3 = Mex("SIRT3D")
+    with pytest.raises(MexError) as e:
+        s3(np.zeros(len(beta) * len(p) * len(xi), np.float32), xi, p, beta, D, size, nt, ns, nz, 5, 1.0, [], 0.0)
+    assert e.value.ident == "ctr:SIRT3D:nrhs"
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_gateway_reports_missing_device():
+    nt, ns, nz, size, beta, p, xi, D = cone_args()
+    with pytest.raises(MexError) as e:
+        Mex("ProjectionCone_3D")(np.zeros(nt * ns * nz, np.float32), nt, ns, nz, size, beta, p, xi, D)
+    assert e.value.ident == "ctr:ProjectionCone_3D:ENODEV"
+
+
+@pytest.mark.gpu
+def test_cone_gateways(ctx, oracle):
+    import ctr
+    nt, ns, nz, size, beta, p, xi, D = cone_args()
+    print("[case] cone gateways", flush=True)
+    g = ctr.ConeGeom(nt, ns, nz, size, D, beta, p, xi)
+    og = oracle.Cone(nt, ns, nz, tuple(size), D, beta, p, xi)
+    vol = ctr.phantoms.rasterize_3d(ctr.phantoms.ellipsoids_3d()[:4], 24)[:nz, :ns, :nt].ravel().copy()
+    R = Mex("ProjectionCone_3D")(vol, nt, ns, nz, size, beta, p, xi, D)
+    assert R.dtype == np.float64 and R.shape == (g.nray,)
+    assert errs(R, ctx.cone_fp(g, vol))[1] == 0.0
+    assert errs(R, oracle.cone_fp_exact(og, vol))[0] < 2e-6
+    R32 = R.astype(np.float32)
+    V = Mex("Backprojection")(R32, xi, p, beta, D, size, nt, ns, nz)
+    assert errs(V, ctx.cone_bp_ref(g, R32))[1] == 0.0
+    F = Mex("FDK")(R32, xi, p, beta, D, size, nt, ns, nz)
+    Fr = oracle.fdk_bp(og, oracle.cone_filter(og, R32, 0).astype(np.float32))
+    e, m = errs(F, Fr)
+    report("mex FDK vs oracle", rel_l2=e, max_rel=m)
+    assert e < 3e-5
+    F1 = Mex("FDK")(R32, xi, p, beta, D, size, nt, ns, nz, 1)
+    assert errs(F1, ctx.fdk(g, R32, filt=ctr.FILTER_RL_HALF))[1] == 0.0
+    X, res = Mex("SIRT3D")(R32, xi, p, beta, D, size, nt, ns, nz, 10, 1.0, nlhs=2)
+    assert len(res) == 11 and np.all(np.diff(res) <= 1e-7) and X.min() >= 0
+
+
+@pytest.mark.gpu
+def test_fan_gateways(ctx, oracle):
+    import ctr
+    n = 40
+    beta = (2 * np.pi / 36) * np.arange(1, 37)
+    p = 0.5 * (np.arange(96) - 47.5)
+    img = ctr.phantoms.shepp_logan_2d(n).astype(np.float32).ravel()
+    R = Mex("ProjectionFan_2D")(img, n, n, np.array([40.0, 40.0]), beta, p, 120.0)
+    og = oracle.Fan(n, n, (40.0, 40.0), 120.0, beta, p)
+    assert errs(R, oracle.fan_fp_exact(og, img))[0] < 2e-6
+    R32 = R.astype(np.float32)
+    bp = Mex("BackprojectionFan2D")(R32, beta.astype(np.float32), p.astype(np.float32), [40.0, 40.0], n, n, 120.0)
+    gs = ctr.FanGeom(n, n, (40.0, 40.0), 120.0, beta.astype(np.float32), p.astype(np.float32))
+    assert errs(bp, ctx.fan_bp_ref(gs, R32))[1] == 0.0
+    fbp = Mex("FBPfan")(R32, beta.astype(np.float32), p.astype(np.float32), [40.0, 40.0], n, n, 120.0)
+    assert errs(fbp, ctx.fan_fbp(gs, R32))[1] == 0.0
