@@ -46,6 +46,12 @@ struct FanTables {
     DevBuf d_view, d_p, d_zero, d_pf, d_gam, d_colw, d_xf;
 };
 
+struct FootCache {
+    DevBuf buf;
+    uint64_t key = 0;
+    BpPlan plan{};
+};
+
 struct TapTable {
     uint64_t key = 0;
     bool valid = false;
@@ -64,6 +70,7 @@ struct ctr_ctx {
     ConeTables cone;
     FanTables fan;
     TapTable taps;
+    FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
 };
 
@@ -352,6 +359,23 @@ BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
     P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
     P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
     return P;
+}
+
+// Footprint table of the tiled backprojection for the current cone geometry
+// (computed on the device once per geometry, reused by every later call).
+ctr_status ensure_footprints(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint)
+{
+    const BpPlan L = adjoint ? adjoint_plan(P) : bp_plan(P);
+    uint64_t key = ctx->cone.key;
+    key = fnv(key, &L, sizeof L);
+    if (fc.buf.ptr && fc.key == key) return CTR_OK;
+    ctr_status s = buf_reserve(ctx, fc.buf, L.nfoot * sizeof(int4));
+    if (s != CTR_OK) return s;
+    s = check_launch(ctx, launch_bp_footprints(P, L, (int4 *)fc.buf.ptr, ctx->stream), "footprint table");
+    if (s != CTR_OK) return s;
+    fc.key = key;
+    fc.plan = L;
+    return CTR_OK;
 }
 
 uint64_t fan_key(const ctr_fan_geom *g)
@@ -690,7 +714,7 @@ void ctr_ctx_destroy(ctr_ctx *ctx)
     for (auto &b : ctx->ws) buf_free(b);
     for (DevBuf *b : {&ctx->cone.d_view, &ctx->cone.d_p, &ctx->cone.d_xi, &ctx->cone.d_bp, &ctx->cone.d_pf,
                       &ctx->cone.d_xf, &ctx->fan.d_view, &ctx->fan.d_p, &ctx->fan.d_zero, &ctx->fan.d_pf,
-                      &ctx->fan.d_gam, &ctx->fan.d_colw, &ctx->fan.d_xf, &ctx->taps.d})
+                      &ctx->fan.d_gam, &ctx->fan.d_colw, &ctx->fan.d_xf, &ctx->taps.d, &ctx->foot_bp.buf, &ctx->foot_adj.buf})
         buf_free(*b);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -788,7 +812,9 @@ ctr_status ctr_cone_bp_ref(ctr_ctx *ctx, const ctr_cone_geom *g, const float *pr
     TRY(stage_in(ctx, dev, kWsIn, proj, nray3(g), &dp));
     TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
     BpParams P = cone_bp_params(ctx, g, flags);
-    TRY(check_launch(ctx, launch_bp_tiled(kBpRef, P, dp, dv, ctx->stream), "cone reference backprojection"));
+    TRY(ensure_footprints(ctx, P, ctx->foot_bp, false));
+    TRY(check_launch(ctx, launch_bp_tiled(kBpRef, P, ctx->foot_bp.plan, (const int4 *)ctx->foot_bp.buf.ptr, dp, dv, ctx->stream),
+                     "cone reference backprojection"));
     return finish_out(ctx, dev, dv, vol, nvox3(g));
 }
 
@@ -803,7 +829,11 @@ ctr_status ctr_cone_bp_adjoint(ctr_ctx *ctx, const ctr_cone_geom *g, const float
     TRY(stage_in(ctx, dev, kWsIn, proj, nray3(g), &dp));
     TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
     BpParams P = cone_bp_params(ctx, g, flags);
-    TRY(check_launch(ctx, launch_adjoint(P, dp, dv, nullptr, 0, ctx->stream), "cone adjoint"));
+    TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+    TRY(check_launch(ctx,
+                     launch_adjoint(P, ctx->foot_adj.plan, (const int4 *)ctx->foot_adj.buf.ptr, dp, dv, nullptr, 0,
+                                    ctx->stream),
+                     "cone adjoint"));
     return finish_out(ctx, dev, dv, vol, nvox3(g));
 }
 
@@ -835,7 +865,9 @@ ctr_status ctr_fdk_bp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *filtere
     TRY(stage_in(ctx, dev, kWsIn, filtered, nray3(g), &dp));
     TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
     BpParams P = cone_bp_params(ctx, g, flags);
-    TRY(check_launch(ctx, launch_bp_tiled(kBpFdk, P, dp, dv, ctx->stream), "FDK backprojection"));
+    TRY(ensure_footprints(ctx, P, ctx->foot_bp, false));
+    TRY(check_launch(ctx, launch_bp_tiled(kBpFdk, P, ctx->foot_bp.plan, (const int4 *)ctx->foot_bp.buf.ptr, dp, dv, ctx->stream),
+                     "FDK backprojection"));
     return finish_out(ctx, dev, dv, vol, nvox3(g));
 }
 
@@ -855,7 +887,9 @@ ctr_status ctr_fdk(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float
     float *filt = (float *)ctx->ws[kWsTmp].ptr;
     TRY(cone_filter_impl(ctx, g, filter, dp, filt));
     BpParams P = cone_bp_params(ctx, g, flags);
-    TRY(check_launch(ctx, launch_bp_tiled(kBpFdk, P, filt, dv, ctx->stream), "FDK backprojection"));
+    TRY(ensure_footprints(ctx, P, ctx->foot_bp, false));
+    TRY(check_launch(ctx, launch_bp_tiled(kBpFdk, P, ctx->foot_bp.plan, (const int4 *)ctx->foot_bp.buf.ptr, filt, dv, ctx->stream),
+                     "FDK backprojection"));
     if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
     return finish_out(ctx, dev, dv, vol, nvox3(g));
 }
@@ -1035,14 +1069,17 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
     TRY(cone_tables(ctx, g));
     const RayParams R = cone_ray_params(ctx, g);
     const BpParams P = cone_bp_params(ctx, g, 0);
+    TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+    const BpPlan AL = ctx->foot_adj.plan;
+    const int4 *AF = (const int4 *)ctx->foot_adj.buf.ptr;
     auto fp = [&](const float *xx, const float *bb, float *rr, double *part, int *nblk) {
         return check_launch(ctx, launch_ray_fp_residual(R, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
     };
     auto ones = [&](float *vinv) {
-        return check_launch(ctx, launch_adjoint(P, nullptr, vinv, nullptr, 1, ctx->stream), "SIRT column sums");
+        return check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, ctx->stream), "SIRT column sums");
     };
     auto upd = [&](const float *rr, SirtUpdate *U) {
-        return check_launch(ctx, launch_adjoint(P, rr, nullptr, U, 0, ctx->stream), "SIRT update");
+        return check_launch(ctx, launch_adjoint(P, AL, AF, rr, nullptr, U, 0, ctx->stream), "SIRT update");
     };
     return sirt_loop(ctx, nvox3(g), nray3(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd);
 }

@@ -98,9 +98,21 @@ hipError_t launch_ray_fp_residual(const RayParams &g, const float *vol, const fl
                                   double *partials, int *nblocks, hipStream_t st);
 int ray_fp_blocks(const RayParams &g);
 
-hipError_t launch_bp_tiled(int kind, const BpParams &P, const float *proj, float *vol, hipStream_t st);
-hipError_t launch_adjoint(const BpParams &P, const float *proj, float *vol, const SirtUpdate *upd,
-                          int ones_input, hipStream_t st);
+// Tiling of the voxel-driven backprojection and its per-(tile, view) detector
+// footprint table (int4 per entry, nfoot entries), computed once per geometry.
+struct BpPlan {
+    int faces;              // footprints of voxel boxes (exact adjoint) instead of voxel centres
+    int tz, wp, capf, hmax;
+    int tiles_t, tiles_s, tiles_z;
+    size_t nfoot;
+};
+BpPlan bp_plan(const BpParams &P);
+BpPlan adjoint_plan(const BpParams &P);
+hipError_t launch_bp_footprints(const BpParams &P, const BpPlan &L, int4 *foot, hipStream_t st);
+hipError_t launch_bp_tiled(int kind, const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj,
+                           float *vol, hipStream_t st);
+hipError_t launch_adjoint(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
+                          const SirtUpdate *upd, int ones_input, hipStream_t st);
 
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
 

@@ -16,6 +16,8 @@
 // different XCDs round-robin, so tile = (bid % 8) * (nwg / 8) + bid / 8
 // gives each XCD a contiguous z-slab of tiles whose footprints share rows
 // of the detector in that XCD's L2.
+#include <cmath>
+
 #include "ctr_internal.h"
 
 namespace ctr {
@@ -31,49 +33,71 @@ __device__ __forceinline__ int xcd_tile(int bid, int nwg)
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-struct Footprint {
-    int c0, c1, r0, r1;     // staged sample rectangle (inclusive)
-    bool empty, interior;
-    float mmin, mmax;
-};
+enum { kFootEmpty = 1, kFootInterior = 2, kFootFits = 4 };
 
-// Detector footprint of the voxel centres of a tile for one view.
-// u is a linear-fractional function of (i, j) and v is bilinear in
-// (z, magnification), so the extremes sit at the tile's corners.
-__device__ __forceinline__ Footprint tile_footprint(const BpParams &P, const BpView &vw, int i_lo, int i_hi, int j_lo,
-                                                    int j_hi, float zlo, float zhi)
+// ---------------------------------------------------------------------------
+// Per-(tile, view) detector footprints.  The rectangle of detector samples a
+// tile's voxel centres interpolate from, with a safety margin, computed once
+// per geometry in FP64 and read by the backprojection with one scalar load
+// per view.  flags: kFootEmpty (no sample of the tile is on the detector),
+// kFootInterior (every sample of the tile is inside the detector, so the
+// inner loop needs no masks), kFootFits (the rectangle fits the LDS tile).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bp_footprints(BpParams P, int tz, int wp, int hmax, int faces,
+                                                     int4 *__restrict__ out)
 {
-    float umin = kInf, umax = -kInf, mmin = kInf, mmax = -kInf;
-#pragma unroll
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    const long ntiles = (long)P.tiles_t * P.tiles_s * P.tiles_z;
+    if (idx >= ntiles * P.nb) return;
+    const int tile = (int)(idx / P.nb), b = (int)(idx % P.nb);
+    const int tt = tile % P.tiles_t, rest = tile / P.tiles_t;
+    const int ts = rest % P.tiles_s, tzi = rest / P.tiles_s;
+    const int i_lo = tt * kTX, i_hi = min(i_lo + kTX - 1, P.nt - 1);
+    const int j_lo = ts * kTS, j_hi = min(j_lo + kTS - 1, P.ns - 1);
+    const BpView vw = P.views[b];
+    // corners of the voxel centres (interpolating backprojection) or of the
+    // voxel boxes (exact adjoint: every ray through a voxel)
+    const double h = faces ? 0.5 : 0.0;
+    double umin = 1e300, umax = -1e300, mmin = 1e300, mmax = -1e300;
     for (int c = 0; c < 4; ++c) {
-        const float fi = (float)((c & 1) ? i_hi : i_lo), fj = (float)((c & 2) ? j_hi : j_lo);
-        const float dt = fmaf(vw.at, fi, fmaf(vw.as, fj, vw.a0));
-        const float ds = fmaf(vw.bt, fi, fmaf(vw.bs, fj, vw.b0));
-        const float m = P.D / (P.D - ds);
-        const float u = fmaf(dt * m, P.u_scale, P.u_off);
-        umin = fminf(umin, u);
-        umax = fmaxf(umax, u);
-        mmin = fminf(mmin, m);
-        mmax = fmaxf(mmax, m);
+        const double fi = (c & 1) ? i_hi + h : i_lo - h, fj = (c & 2) ? j_hi + h : j_lo - h;
+        const double dt = (double)vw.at * fi + (double)vw.as * fj + (double)vw.a0;
+        const double ds = (double)vw.bt * fi + (double)vw.bs * fj + (double)vw.b0;
+        const double m = (double)P.D / ((double)P.D - ds);
+        const double u = dt * m * (double)P.u_scale + (double)P.u_off;
+        umin = fmin(umin, u);
+        umax = fmax(umax, u);
+        mmin = fmin(mmin, m);
+        mmax = fmax(mmax, m);
     }
-    float vmin = kInf, vmax = -kInf;
-#pragma unroll
+    const double zlo = (double)tzi * tz * P.rz + P.zc0 - h * P.rz, zhi = zlo + (double)(tz - 1 + 2 * h) * P.rz;
+    double vmin = 1e300, vmax = -1e300;
     for (int c = 0; c < 4; ++c) {
-        const float z = (c & 1) ? zhi : zlo, m = (c & 2) ? mmax : mmin;
-        const float v = fmaf(z * m, P.v_scale, P.v_off);
-        vmin = fminf(vmin, v);
-        vmax = fmaxf(vmax, v);
+        const double v = ((c & 1) ? zhi : zlo) * ((c & 2) ? mmax : mmin) * (double)P.v_scale + (double)P.v_off;
+        vmin = fmin(vmin, v);
+        vmax = fmax(vmax, v);
     }
-    const float eu = 2e-3f + 4e-6f * fmaxf(fabsf(umin), fabsf(umax));
-    const float ev = 2e-3f + 4e-6f * fmaxf(fabsf(vmin), fabsf(vmax));
-    Footprint f;
-    f.mmin = mmin;
-    f.mmax = mmax;
-    f.interior = (umin - eu >= 0.0f) && (umax + eu < P.umax) && (vmin - ev >= 0.0f) && (vmax + ev < P.vmax);
-    // whole tile off the detector?
-    f.empty = !(umax + eu >= 0.0f && umin - eu < P.umax && vmax + ev >= 0.0f && vmin - ev < P.vmax);
-    int c0 = (int)floorf(fmaxf(umin - eu, -1.0f)), c1 = (int)floorf(fminf(umax + eu, (float)P.np)) + 1;
-    int r0 = (int)floorf(fmaxf(vmin - ev, -1.0f)), r1 = (int)floorf(fminf(vmax + ev, (float)P.nx)) + 1;
+    // float evaluation of the same maps in the backprojection differs by far less than this margin
+    const double eu = 2e-3 + 4e-6 * fmax(fabs(umin), fabs(umax));
+    const double ev = 2e-3 + 4e-6 * fmax(fabs(vmin), fabs(vmax));
+    int flags = 0;
+    if (!(umax + eu >= 0.0 && umin - eu < P.umax && vmax + ev >= 0.0 && vmin - ev < P.vmax)) flags |= kFootEmpty;
+    if (umin - eu >= 0.0 && umax + eu < P.umax && vmin - ev >= 0.0 && vmax + ev < P.vmax) flags |= kFootInterior;
+    int c0, c1, r0, r1;
+    if (faces) {
+        // the detector samples (integer positions) inside the projected boxes
+        c0 = (int)ceil(fmax(umin - eu, -1.0));
+        c1 = (int)floor(fmin(umax + eu, (double)P.np));
+        r0 = (int)ceil(fmax(vmin - ev, -1.0));
+        r1 = (int)floor(fmin(vmax + ev, (double)P.nx));
+        if (c1 < 0 || c0 > P.np - 1 || r1 < 0 || r0 > P.nx - 1 || c1 < c0 || r1 < r0) flags |= kFootEmpty;
+    } else {
+        // both neighbours of every bilinear sample
+        c0 = (int)floor(fmax(umin - eu, -1.0));
+        c1 = (int)floor(fmin(umax + eu, (double)P.np)) + 1;
+        r0 = (int)floor(fmax(vmin - ev, -1.0));
+        r1 = (int)floor(fmin(vmax + ev, (double)P.nx)) + 1;
+    }
     c0 = max(c0, 0);
     c1 = min(c1, P.np - 1);
     r0 = max(r0, 0);
@@ -85,61 +109,79 @@ __device__ __forceinline__ Footprint tile_footprint(const BpParams &P, const BpV
     if (r1 <= r0) {
         if (r0 > 0) r0 = r1 - 1; else r1 = r0 + 1;
     }
-    f.c0 = c0;
-    f.c1 = c1;
-    f.r0 = r0;
-    f.r1 = r1;
-    return f;
+    const int W = c1 - c0 + 1, H = r1 - r0 + 1;
+    if (W <= wp && H <= hmax) flags |= kFootFits;
+    out[idx] = make_int4(c0, r0, W | (H << 16), flags);
 }
 
-// Coalesced copy of rows r0..r1, columns c0..c1 of one view into LDS.
-__device__ __forceinline__ void stage(float *lds, const float *__restrict__ src, int np, int r0, int H, int c0, int W)
+// Chord of the source-to-voxel-centre line through the voxel
+// (Backprojection3D.cu:243-288): 2 |d| min_a(h_a / |d_a|).
+__device__ __forceinline__ float chord_len(float rho2, float mts, float zs, float hz)
 {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int r = wave; r < H; r += kThreads / 64) {
-        const float *row = src + (size_t)(r0 + r) * np + c0;
-        float *dst = lds + r * W;
-        for (int c = lane; c < W; c += 64) dst[c] = row[c];
-    }
+    return 2.0f * sqrtf(fmaf(zs, zs, rho2)) * fminf(mts, hz / fabsf(zs));
 }
 
-template <int KIND, int TZ, int CAP>
+// Direct global -> LDS copy of one footprint row: lane l loads column c0 + l
+// (clamped into the detector row) into lds_row[l].  The copy completes
+// asynchronously; a vmcnt wait and a barrier publish it.
+__device__ __forceinline__ void dma_row(const float *__restrict__ row, int c0, int np, float *lds_row)
+{
+    const int lane = threadIdx.x & 63;
+    const float *g = row + min(c0 + lane, np - 1);
+    __builtin_amdgcn_global_load_lds(g, lds_row, 4, 0, 0);
+}
+
+__device__ __forceinline__ void issue_view(const float *__restrict__ proj, const BpParams &P, int b, int4 f, float *buf)
+{
+    if ((f.w & kFootEmpty) || !(f.w & kFootFits)) return;
+    const int H = f.z >> 16;
+    const float *src = proj + ((size_t)b * P.nx + f.y) * P.np;
+    for (int r = threadIdx.x >> 6; r < H; r += kThreads / 64) dma_row(src + (size_t)r * P.np, f.x, P.np, buf + r * 64);
+}
+
+// Voxel-driven backprojection of all views in one pass.  LDS holds two
+// footprint buffers (row pitch 64 floats): while the tile interpolates view b
+// from one buffer, the DMA engine fills the other with view b+1, so one
+// barrier per view publishes the next footprint and retires the previous one.
+template <int KIND, int TZ, int HMAX>
 __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__ proj, float *__restrict__ vol,
-                                                       BpParams P)
+                                                       BpParams P, const int4 *__restrict__ foot)
 {
-    __shared__ float lds[CAP];
+    constexpr int WP = 64;
+    __shared__ float lds[2][HMAX * WP];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tt = tile % P.tiles_t, rest = tile / P.tiles_t;
     const int ts = rest % P.tiles_s, tz = rest / P.tiles_s;
     const int tx = threadIdx.x & (kTX - 1), ty = threadIdx.x / kTX;
     const int i = tt * kTX + tx, j = ts * kTS + ty, z0 = tz * TZ;
     const bool col_in = i < P.nt && j < P.ns;
-    const int i_lo = tt * kTX, i_hi = min(i_lo + kTX - 1, P.nt - 1);
-    const int j_lo = ts * kTS, j_hi = min(j_lo + kTS - 1, P.ns - 1);
     const int nzt = min(TZ, P.nz - z0);
-    const float zA = fmaf((float)z0, P.rz, P.zc0);              // centre of slice z0 relative to the source plane
-    const float zlo = zA, zhi = fmaf((float)(TZ - 1), P.rz, zA);
+    const float zA = fmaf((float)z0, P.rz, P.zc0);   // centre of slice z0 relative to the source plane
     const float fi = (float)i, fj = (float)j;
+    const int4 *ft = foot + (size_t)tile * P.nb;
 
     float acc[TZ];
 #pragma unroll
     for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
 
+    int4 fn = ft[0];
+    issue_view(proj, P, 0, fn, lds[0]);
     for (int b = 0; b < P.nb; ++b) {
-        const BpView vw = P.views[b];
-        const Footprint f = tile_footprint(P, vw, i_lo, i_hi, j_lo, j_hi, zlo, zhi);
-        if (f.empty) continue;
-        const int W = f.c1 - f.c0 + 1, H = f.r1 - f.r0 + 1;
-        const bool fits = W * H <= CAP;
-        const float *src = proj + (size_t)b * P.nx * P.np;
-        if (fits) {
-            __syncthreads();
-            stage(lds, src, P.np, f.r0, H, f.c0, W);
-            __syncthreads();
+        const int4 f = fn;
+        const float *buf = lds[b & 1];
+        // this wave's copies of view b are complete; after the barrier every
+        // wave's are, and no wave still reads the other buffer (view b-1)
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (b + 1 < P.nb) {
+            fn = ft[b + 1];
+            issue_view(proj, P, b + 1, fn, lds[(b + 1) & 1]);
         }
-        if (!col_in) continue;
+        if ((f.w & kFootEmpty) || !col_in) continue;
+        const bool fits = (f.w & kFootFits) != 0;
 
         // per-column projection of this view
+        const BpView vw = P.views[b];
         const float dt = fmaf(vw.at, fi, fmaf(vw.as, fj, vw.a0));
         const float ds = fmaf(vw.bt, fi, fmaf(vw.bs, fj, vw.b0));
         const float m = P.D * fast_rcp(P.D - ds);
@@ -153,54 +195,48 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
             rho2 = xs * xs + ys * ys;
             mts = fminf(P.ht / fabsf(xs), P.hs / fabsf(ys));
         }
-        const bool uok = u >= 0.0f && u < P.umax;
         const int iu_raw = (int)floorf(u);
         const float fu = u - (float)iu_raw;
 
-        if (fits && f.interior && nzt == TZ) {
-            // every sample of the tile lies inside the detector: unmasked loop
-            const int iu = min(max(iu_raw, f.c0), f.c1 - 1) - f.c0;
+        if (fits && (f.w & kFootInterior)) {
+            // every sample of the tile is on the detector and inside the staged
+            // rectangle: no masks, no clamps; the stencil is two ds_read2_b32
+            const int base4 = ((iu_raw - f.x) - f.y * WP) * 4;
+            const char *l8 = reinterpret_cast<const char *>(buf);
 #pragma unroll
             for (int k = 0; k < TZ; ++k) {
                 const float v = fmaf((float)k, vB, vA);
-                const int ivr = (int)v;
-                const float fv = v - (float)ivr;
-                const int iv = min(max(ivr, f.r0), f.r1 - 1) - f.r0;
-                const float *s = lds + iv * W + iu;
-                const float a = s[0], bq = s[1], c = s[W], d = s[W + 1];
+                const int iv = (int)v;
+                const float fv = __builtin_amdgcn_fractf(v);
+                const float *s = reinterpret_cast<const float *>(l8 + ((iv << 8) + base4));
+                const float a = s[0], bq = s[1], c = s[WP], d = s[WP + 1];
                 const float top = fmaf(fu, bq - a, a), bot = fmaf(fu, d - c, c);
                 const float val = fmaf(fv, bot - top, top);
-                if (KIND == kBpFdk) {
-                    acc[k] = fmaf(w, val, acc[k]);
-                } else {
-                    const float zs = fmaf((float)k, P.rz, zA);
-                    const float len = 2.0f * sqrtf(fmaf(zs, zs, rho2)) * fminf(mts, P.hz / fabsf(zs));
-                    acc[k] = fmaf(len, val, acc[k]);
-                }
+                if (KIND == kBpFdk) acc[k] = fmaf(w, val, acc[k]);
+                else acc[k] = fmaf(chord_len(rho2, mts, fmaf((float)k, P.rz, zA), P.hz), val, acc[k]);
             }
         } else if (fits) {
-            const int iu = min(max(iu_raw, f.c0), f.c1 - 1) - f.c0;
+            const int W = f.z & 0xffff, H = f.z >> 16;
+            const bool uok = u >= 0.0f && u < P.umax;
+            const int iu = min(max(iu_raw, f.x), f.x + W - 2) - f.x;
 #pragma unroll
             for (int k = 0; k < TZ; ++k) {
                 const float v = fmaf((float)k, vB, vA);
                 const bool ok = uok && v >= 0.0f && v < P.vmax;
                 const int ivr = (int)floorf(v);
                 const float fv = v - (float)ivr;
-                const int iv = min(max(ivr, f.r0), f.r1 - 1) - f.r0;
-                const float *s = lds + iv * W + iu;
-                const float a = s[0], bq = s[1], c = s[W], d = s[W + 1];
+                const int iv = min(max(ivr, f.y), f.y + H - 2) - f.y;
+                const float *s = buf + iv * WP + iu;
+                const float a = s[0], bq = s[1], c = s[WP], d = s[WP + 1];
                 const float top = fmaf(fu, bq - a, a), bot = fmaf(fu, d - c, c);
                 const float val = ok ? fmaf(fv, bot - top, top) : 0.0f;
-                if (KIND == kBpFdk) {
-                    acc[k] = fmaf(w, val, acc[k]);
-                } else {
-                    const float zs = fmaf((float)k, P.rz, zA);
-                    const float len = 2.0f * sqrtf(fmaf(zs, zs, rho2)) * fminf(mts, P.hz / fabsf(zs));
-                    acc[k] = fmaf(len, val, acc[k]);
-                }
+                if (KIND == kBpFdk) acc[k] = fmaf(w, val, acc[k]);
+                else acc[k] = fmaf(chord_len(rho2, mts, fmaf((float)k, P.rz, zA), P.hz), val, acc[k]);
             }
         } else {
-            // footprint larger than the LDS buffer: gather from global memory
+            // footprint larger than an LDS buffer: gather from global memory
+            const float *src = proj + (size_t)b * P.nx * P.np;
+            const bool uok = u >= 0.0f && u < P.umax;
             const int iu = min(max(iu_raw, 0), P.np - 2);
 #pragma unroll 4
             for (int k = 0; k < TZ; ++k) {
@@ -213,13 +249,8 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
                 const float a = s[0], bq = s[1], c = s[P.np], d = s[P.np + 1];
                 const float top = fmaf(fu, bq - a, a), bot = fmaf(fu, d - c, c);
                 const float val = ok ? fmaf(fv, bot - top, top) : 0.0f;
-                if (KIND == kBpFdk) {
-                    acc[k] = fmaf(w, val, acc[k]);
-                } else {
-                    const float zs = fmaf((float)k, P.rz, zA);
-                    const float len = 2.0f * sqrtf(fmaf(zs, zs, rho2)) * fminf(mts, P.hz / fabsf(zs));
-                    acc[k] = fmaf(len, val, acc[k]);
-                }
+                if (KIND == kBpFdk) acc[k] = fmaf(w, val, acc[k]);
+                else acc[k] = fmaf(chord_len(rho2, mts, fmaf((float)k, P.rz, zA), P.hz), val, acc[k]);
             }
         }
     }
@@ -281,39 +312,41 @@ __global__ __launch_bounds__(256) void k_fdk_quirk(const float *__restrict__ pro
 // of the column.  Segment length = (lam_hi - lam_lo) * |(a, b, xi)|; the
 // norm sqrt(p^2 + D^2 + xi^2) is folded into the staged samples.
 // ---------------------------------------------------------------------------
-constexpr int kCMax = 8;
+constexpr int kCMax = 6;
 
-struct AdjCol {
-    float lo[kCMax], hi[kCMax];
-    int cfirst, ccount;
-};
-
-__device__ __forceinline__ void slab(float pos_lo, float pitch, float src, float dirc, float &lo, float &hi)
+// Slab interval of one axis for a ray parameterised S + lam*dir: the voxel
+// [lo_rel, lo_rel + pitch] (relative to S) holds lam in [l1, l2] sorted;
+// inv = 1/dir, with inv == 0 marking a ray parallel to the slab (inside iff
+// the source coordinate lies in [lo, lo + pitch)).
+__device__ __forceinline__ void slab_interval(float lo_rel, float pitch, float inv, float &lo, float &hi)
 {
-    if (dirc != 0.0f) {
-        const float inv = 1.0f / dirc;
-        const float l1 = (pos_lo - src) * inv, l2 = (pos_lo + pitch - src) * inv;
-        lo = fmaxf(lo, fminf(l1, l2));
-        hi = fminf(hi, fmaxf(l1, l2));
-    } else if (!(src >= pos_lo && src < pos_lo + pitch)) {
-        lo = kInf;
-        hi = -kInf;
+    if (inv != 0.0f) {
+        const float l1 = lo_rel * inv, l2 = fmaf(pitch, inv, l1);
+        lo = fminf(l1, l2);
+        hi = fmaxf(l1, l2);
+    } else {
+        const bool in = lo_rel <= 0.0f && 0.0f < lo_rel + pitch;
+        lo = in ? -kInf : kInf;
+        hi = in ? kInf : -kInf;
     }
 }
 
-template <bool FROM_LDS, int TZ>
-__device__ __forceinline__ void adjoint_view(const BpParams &P, const BpView &vw, const float *lds, const float *src,
-                                             const Footprint &f, int W, int i, int j, int z0, float *acc, int ones)
+__device__ __forceinline__ float safe_rcp(float x) { return x != 0.0f ? 1.0f / x : 0.0f; }
+
+// Column footprint of one voxel column (box corners): candidate detector
+// columns [cu0, cu1] and the magnification range of the column box.
+__device__ __forceinline__ void column_box(const BpParams &P, const BpView &vw, float fi, float fj, int &cu0, int &cu1,
+                                           float &mmin, float &mmax)
 {
-    const float fi = (float)i, fj = (float)j;
-    // the column's own footprint (voxel box corners, indices +-0.5)
-    float umin = kInf, umax = -kInf, mmin = kInf, mmax = -kInf;
+    float umin = kInf, umax = -kInf;
+    mmin = kInf;
+    mmax = -kInf;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const float ci = fi + ((c & 1) ? 0.5f : -0.5f), cj = fj + ((c & 2) ? 0.5f : -0.5f);
         const float dt = fmaf(vw.at, ci, fmaf(vw.as, cj, vw.a0));
         const float ds = fmaf(vw.bt, ci, fmaf(vw.bs, cj, vw.b0));
-        const float m = P.D / (P.D - ds);
+        const float m = P.D * fast_rcp(P.D - ds);
         const float u = fmaf(dt * m, P.u_scale, P.u_off);
         umin = fminf(umin, u);
         umax = fmaxf(umax, u);
@@ -321,140 +354,154 @@ __device__ __forceinline__ void adjoint_view(const BpParams &P, const BpView &vw
         mmax = fmaxf(mmax, m);
     }
     const float eu = 1e-3f + 4e-6f * fmaxf(fabsf(umin), fabsf(umax));
-    int cu0 = (int)ceilf(fmaxf(umin - eu, -1.0f)), cu1 = (int)floorf(fminf(umax + eu, (float)P.np));
-    cu0 = max(cu0, FROM_LDS ? f.c0 : 0);
-    cu1 = min(cu1, FROM_LDS ? f.c1 : P.np - 1);
-    const float tlo = fi * P.rt, slo = fj * P.rs;
-    for (int cb = cu0; cb <= cu1; cb += kCMax) {
-        float clo[kCMax], chi[kCMax];
-#pragma unroll
-        for (int q = 0; q < kCMax; ++q) {
-            const int c = min(cb + q, P.np - 1);
-            const float p = P.pd[c];
-            const float a = fmaf(p, vw.cb, P.D * vw.sb), bq = fmaf(p, vw.sb, -P.D * vw.cb);
-            float lo = -kInf, hi = kInf;
-            slab(tlo, P.rt, vw.st, a, lo, hi);
-            slab(slo, P.rs, vw.ss, bq, lo, hi);
-            if (cb + q > cu1) {
-                lo = kInf;
-                hi = -kInf;
-            }
-            clo[q] = lo;
-            chi[q] = hi;
-        }
-#pragma unroll
-        for (int k = 0; k < TZ; ++k) {
-            const float zb = fmaf((float)(z0 + k), P.rz, P.zc0 - 0.5f * P.rz);  // low face of the voxel, relative to Cz
-            const float zt = zb + P.rz;
-            float vmin = fminf(fminf(zb * mmin, zb * mmax), fminf(zt * mmin, zt * mmax));
-            float vmax = fmaxf(fmaxf(zb * mmin, zb * mmax), fmaxf(zt * mmin, zt * mmax));
-            float v0 = fmaf(vmin, P.v_scale, P.v_off), v1 = fmaf(vmax, P.v_scale, P.v_off);
-            if (v0 > v1) {
-                const float tmp = v0;
-                v0 = v1;
-                v1 = tmp;
-            }
-            const float ev = 1e-3f + 4e-6f * fmaxf(fabsf(v0), fabsf(v1));
-            int rv0 = (int)ceilf(fmaxf(v0 - ev, -1.0f)), rv1 = (int)floorf(fminf(v1 + ev, (float)P.nx));
-            rv0 = max(rv0, FROM_LDS ? f.r0 : 0);
-            rv1 = min(rv1, FROM_LDS ? f.r1 : P.nx - 1);
-            float a_k = 0.0f;
-            for (int r = rv0; r <= rv1; ++r) {
-                const float xi = P.xd[r];
-                float zl = -kInf, zh = kInf;
-                if (xi != 0.0f) {
-                    const float inv = 1.0f / xi;
-                    const float l1 = zb * inv, l2 = zt * inv;
-                    zl = fminf(l1, l2);
-                    zh = fmaxf(l1, l2);
-                } else if (!(zb <= 0.0f && 0.0f < zt)) {
-                    zl = kInf;
-                    zh = -kInf;
-                }
-                float rowacc = 0.0f;
-                if (FROM_LDS) {
-                    const float *s = lds + (r - f.r0) * W + (cb - f.c0);
-#pragma unroll
-                    for (int q = 0; q < kCMax; ++q) {
-                        const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
-                        if (len > 0.0f) rowacc = fmaf(len, s[q], rowacc);
-                    }
-                } else {
-                    const float xi2 = xi * xi + P.D * P.D;
-#pragma unroll
-                    for (int q = 0; q < kCMax; ++q) {
-                        const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
-                        if (len > 0.0f) {
-                            const int c = min(cb + q, P.np - 1);
-                            const float p = P.pd[c];
-                            const float nrm = sqrtf(fmaf(p, p, xi2));
-                            const float y = ones ? 1.0f : src[(size_t)r * P.np + c];
-                            rowacc = fmaf(len, y * nrm, rowacc);
-                        }
-                    }
-                }
-                a_k += rowacc;
-            }
-            acc[k] += a_k;
-        }
-    }
+    cu0 = (int)ceilf(fmaxf(umin - eu, -1.0f));
+    cu1 = (int)floorf(fminf(umax + eu, (float)P.np));
 }
 
-template <int TZ, int CAP>
-__global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
-                                                      SirtUpdate U, int sirt, int ones)
+// Candidate detector rows of the voxel slice [zb, zt] (relative to the
+// source plane) seen through a column with magnification in [mmin, mmax].
+__device__ __forceinline__ void slice_rows(const BpParams &P, float zb, float zt, float mmin, float mmax, int &rv0,
+                                           int &rv1)
 {
-    // kCMax floats of slack: the unrolled candidate loop may address up to
-    // kCMax - 1 entries past the staged rectangle (their segment length is 0)
-    __shared__ float lds[CAP + kCMax];
+    const float a = zb * mmin, b = zb * mmax, c = zt * mmin, d = zt * mmax;
+    float v0 = fmaf(fminf(fminf(a, b), fminf(c, d)), P.v_scale, P.v_off);
+    float v1 = fmaf(fmaxf(fmaxf(a, b), fmaxf(c, d)), P.v_scale, P.v_off);
+    if (v0 > v1) {
+        const float t = v0;
+        v0 = v1;
+        v1 = t;
+    }
+    const float ev = 1e-3f + 4e-6f * fmaxf(fabsf(v0), fabsf(v1));
+    rv0 = (int)ceilf(fmaxf(v0 - ev, -1.0f));
+    rv1 = (int)floorf(fminf(v1 + ev, (float)P.nx));
+}
+
+// Exact adjoint (voxel-driven gather of Siddon segment lengths).  LDS holds,
+// for the view's footprint rectangle: y*|d| per sample (|d| = sqrt(p^2 +
+// D^2 + xi^2)), 1/a and 1/b per column and 1/xi per row, so a candidate
+// (ray, voxel) pair costs 5 VALU and one immediate-offset LDS read.
+template <int TZ, int WP, int CAPF>
+__global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
+                                                      const int4 *__restrict__ foot, SirtUpdate U, int sirt, int ones)
+{
+    constexpr int HMAX = (CAPF - 2 * WP - 2 * kCMax) / (WP + 1);
+    __shared__ float lds[CAPF];
+    float *yd = lds;                                // [HMAX][WP] (+ kCMax zero slack)
+    float *inva = lds + HMAX * WP + kCMax;          // [WP + kCMax]
+    float *invb = inva + WP + kCMax;                // [WP]
+    float *invxi = invb + WP;                       // [HMAX]
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tt = tile % P.tiles_t, rest = tile / P.tiles_t;
     const int ts = rest % P.tiles_s, tz = rest / P.tiles_s;
     const int tx = threadIdx.x & (kTX - 1), ty = threadIdx.x / kTX;
     const int i = tt * kTX + tx, j = ts * kTS + ty, z0 = tz * TZ;
     const bool col_in = i < P.nt && j < P.ns;
-    const int i_lo = tt * kTX, i_hi = min(i_lo + kTX - 1, P.nt - 1);
-    const int j_lo = ts * kTS, j_hi = min(j_lo + kTS - 1, P.ns - 1);
     const int nzt = min(TZ, P.nz - z0);
-    // box of the tile (faces, not centres), z relative to the source plane
-    const float zlo = fmaf((float)z0, P.rz, P.zc0 - 0.5f * P.rz);
-    const float zhi = fmaf((float)(z0 + TZ), P.rz, P.zc0 - 0.5f * P.rz);
+    const float zbase = fmaf((float)z0, P.rz, P.zc0 - 0.5f * P.rz);   // low face of slice z0, relative to the source
+    const float fi = (float)i, fj = (float)j;
+    const int4 *ft = foot + (size_t)tile * P.nb;
+    // zero the whole LDS once: the unrolled candidate reads past a staged row
+    // then only ever see finite values (their segment length is 0)
+    for (int e = threadIdx.x; e < CAPF; e += kThreads) lds[e] = 0.0f;
 
     float acc[TZ];
 #pragma unroll
     for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
 
     for (int b = 0; b < P.nb; ++b) {
+        const int4 f = ft[b];
+        if (f.w & kFootEmpty) continue;
+        const int W = f.z & 0xffff, H = f.z >> 16;
+        const bool fits = (f.w & kFootFits) != 0;
         const BpView vw = P.views[b];
-        // footprint of the tile box: centres at indices +-0.5 are the faces
-        BpView fb = vw;
-        fb.a0 = vw.a0 - 0.5f * (vw.at + vw.as);
-        fb.b0 = vw.b0 - 0.5f * (vw.bt + vw.bs);
-        Footprint f = tile_footprint(P, fb, i_lo, i_hi + 1, j_lo, j_hi + 1, zlo, zhi);
-        // rays sit on integer sample positions: the staged rectangle must hold
-        // every sample inside [umin, umax]; tile_footprint already pads by one.
-        if (f.empty) continue;
-        const int W = f.c1 - f.c0 + 1, H = f.r1 - f.r0 + 1;
-        const bool fits = W * H <= CAP;
         const float *src = proj + (size_t)b * P.nx * P.np;
         if (fits) {
             __syncthreads();
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
             for (int r = wave; r < H; r += kThreads / 64) {
-                const float xi = P.xd[f.r0 + r];
+                const float xi = P.xd[f.y + r];
                 const float xi2 = fmaf(xi, xi, P.D * P.D);
-                const float *row = src + (size_t)(f.r0 + r) * P.np + f.c0;
+                const float *row = src + (size_t)(f.y + r) * P.np + f.x;
                 for (int c = lane; c < W; c += 64) {
-                    const float p = P.pd[f.c0 + c];
+                    const float p = P.pd[f.x + c];
                     const float nrm = sqrtf(fmaf(p, p, xi2));
-                    lds[r * W + c] = ones ? nrm : row[c] * nrm;
+                    yd[r * WP + c] = ones ? nrm : row[c] * nrm;
                 }
             }
+            for (int c = threadIdx.x; c < W; c += kThreads) {
+                const float p = P.pd[f.x + c];
+                inva[c] = safe_rcp(fmaf(p, vw.cb, P.D * vw.sb));
+                invb[c] = safe_rcp(fmaf(p, vw.sb, -P.D * vw.cb));
+            }
+            for (int r = threadIdx.x; r < H; r += kThreads) invxi[r] = safe_rcp(P.xd[f.y + r]);
             __syncthreads();
         }
         if (!col_in) continue;
-        if (fits) adjoint_view<true, TZ>(P, vw, lds, src, f, W, i, j, z0, acc, ones);
-        else adjoint_view<false, TZ>(P, vw, lds, src, f, W, i, j, z0, acc, ones);
+
+        int cu0, cu1;
+        float mmin, mmax;
+        column_box(P, vw, fi, fj, cu0, cu1, mmin, mmax);
+        const int clo_lim = fits ? f.x : 0, chi_lim = fits ? f.x + W - 1 : P.np - 1;
+        const int rlo_lim = fits ? f.y : 0, rhi_lim = fits ? f.y + H - 1 : P.nx - 1;
+        cu0 = max(cu0, clo_lim);
+        cu1 = min(cu1, chi_lim);
+        const float tlo = fmaf(fi, P.rt, -vw.st), slo = fmaf(fj, P.rs, -vw.ss);   // voxel low faces relative to S
+        for (int cb = cu0; cb <= cu1; cb += kCMax) {
+            float clo[kCMax], chi[kCMax];
+#pragma unroll
+            for (int q = 0; q < kCMax; ++q) {
+                float ia, ib;
+                if (fits) {
+                    ia = inva[cb - f.x + q];
+                    ib = invb[cb - f.x + q];
+                } else {
+                    const float p = P.pd[min(cb + q, P.np - 1)];
+                    ia = safe_rcp(fmaf(p, vw.cb, P.D * vw.sb));
+                    ib = safe_rcp(fmaf(p, vw.sb, -P.D * vw.cb));
+                }
+                float l0, h0, l1, h1;
+                slab_interval(tlo, P.rt, ia, l0, h0);
+                slab_interval(slo, P.rs, ib, l1, h1);
+                const bool valid = cb + q <= cu1;
+                clo[q] = valid ? fmaxf(l0, l1) : kInf;
+                chi[q] = valid ? fminf(h0, h1) : -kInf;
+            }
+#pragma unroll
+            for (int k = 0; k < TZ; ++k) {
+                const float zb = fmaf((float)k, P.rz, zbase), zt = zb + P.rz;
+                int rv0, rv1;
+                slice_rows(P, zb, zt, mmin, mmax, rv0, rv1);
+                rv0 = max(rv0, rlo_lim);
+                rv1 = min(rv1, rhi_lim);
+                float a_k = 0.0f;
+                for (int r = rv0; r <= rv1; ++r) {
+                    const float ix = fits ? invxi[r - f.y] : safe_rcp(P.xd[r]);
+                    float zl, zh;
+                    slab_interval(zb, P.rz, ix, zl, zh);
+                    if (fits) {
+                        const float *sy = yd + (r - f.y) * WP + (cb - f.x);
+#pragma unroll
+                        for (int q = 0; q < kCMax; ++q) {
+                            const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
+                            a_k = fmaf(len, sy[q], a_k);
+                        }
+                    } else {
+                        const float xi = P.xd[r], xi2 = fmaf(xi, xi, P.D * P.D);
+#pragma unroll
+                        for (int q = 0; q < kCMax; ++q) {
+                            const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
+                            if (len > 0.0f) {
+                                const int c = min(cb + q, P.np - 1);
+                                const float p = P.pd[c];
+                                const float y = ones ? 1.0f : src[(size_t)r * P.np + c];
+                                a_k = fmaf(len, y * sqrtf(fmaf(p, p, xi2)), a_k);
+                            }
+                        }
+                    }
+                }
+                acc[k] += a_k;
+            }
+        }
     }
 
     if (!col_in) return;
@@ -472,12 +519,77 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ 
     }
 }
 
-constexpr int kBpTZ = 16, kBpCap = 6144;
+constexpr int kBpTZ = 16, kBpHmax = 64;
 constexpr int kAdjTZ = 8, kAdjCap = 6144;
 
 }  // namespace
 
-hipError_t launch_bp_tiled(int kind, const BpParams &Pin, const float *proj, float *vol, hipStream_t st)
+// Detector-column width bound of a tile footprint: the tile's diagonal (plus
+// one voxel diagonal for box faces) magnified at the volume edge nearest the
+// source, in detector columns, plus the bilinear neighbours.
+static int footprint_pitch(const BpParams &P, bool faces)
+{
+    const double rt = P.rt, rs = P.rs;
+    const double diag = std::sqrt((kTX * rt) * (kTX * rt) + (kTS * rs) * (kTS * rs)) +
+                        (faces ? std::sqrt(rt * rt + rs * rs) : 0.0);
+    const double half = 0.5 * std::sqrt((double)P.nt * rt * P.nt * rt + (double)P.ns * rs * P.ns * rs);
+    const double wmax = diag * (P.D / (P.D - half)) * std::fabs((double)P.u_scale) + 4.0;
+    return wmax <= 40 ? 40 : (wmax <= 72 ? 72 : 136);
+}
+
+BpPlan bp_plan(const BpParams &P)
+{
+    BpPlan L{};
+    L.faces = 0;
+    L.tz = kBpTZ;
+    L.tiles_t = (P.nt + kTX - 1) / kTX;
+    L.tiles_s = (P.ns + kTS - 1) / kTS;
+    L.tiles_z = (P.nz + L.tz - 1) / L.tz;
+    L.wp = 64;                 // one DMA row per wave instruction
+    L.capf = 2 * kBpHmax * 64;  // two buffers
+    L.hmax = kBpHmax;
+    L.nfoot = (size_t)L.tiles_t * L.tiles_s * L.tiles_z * P.nb;
+    return L;
+}
+
+BpPlan adjoint_plan(const BpParams &P)
+{
+    BpPlan L{};
+    L.faces = 1;
+    L.tz = kAdjTZ;
+    L.tiles_t = (P.nt + kTX - 1) / kTX;
+    L.tiles_s = (P.ns + kTS - 1) / kTS;
+    L.tiles_z = (P.nz + L.tz - 1) / L.tz;
+    L.wp = footprint_pitch(P, true);
+    L.capf = kAdjCap;
+    L.hmax = (L.capf - 2 * L.wp - 2 * kCMax) / (L.wp + 1);
+    L.nfoot = (size_t)L.tiles_t * L.tiles_s * L.tiles_z * P.nb;
+    return L;
+}
+
+hipError_t launch_bp_footprints(const BpParams &Pin, const BpPlan &L, int4 *foot, hipStream_t st)
+{
+    BpParams P = Pin;
+    P.tiles_t = L.tiles_t;
+    P.tiles_s = L.tiles_s;
+    P.tiles_z = L.tiles_z;
+    const long n = (long)L.nfoot;
+    hipLaunchKernelGGL(k_bp_footprints, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, L.tz, L.wp, L.hmax,
+                       L.faces, foot);
+    return hipGetLastError();
+}
+
+template <int KIND>
+static hipError_t launch_bp_kind(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
+                                 hipStream_t st)
+{
+    const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
+    hipLaunchKernelGGL((k_bp_tiled<KIND, kBpTZ, kBpHmax>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_tiled(int kind, const BpParams &Pin, const BpPlan &L, const int4 *foot, const float *proj,
+                           float *vol, hipStream_t st)
 {
     BpParams P = Pin;
     if (kind == kBpFdk && P.quirks) {
@@ -488,33 +600,36 @@ hipError_t launch_bp_tiled(int kind, const BpParams &Pin, const float *proj, flo
         hipLaunchKernelGGL(k_fdk_quirk, dim3((P.nt + 15) / 16, (P.ns + 15) / 16), dim3(256), 0, st, proj, vol, P);
         return hipGetLastError();
     }
-    P.tiles_t = (P.nt + kTX - 1) / kTX;
-    P.tiles_s = (P.ns + kTS - 1) / kTS;
-    P.tiles_z = (P.nz + kBpTZ - 1) / kBpTZ;
-    const int nwg = P.tiles_t * P.tiles_s * P.tiles_z;
-    if (kind == kBpFdk)
-        hipLaunchKernelGGL((k_bp_tiled<kBpFdk, kBpTZ, kBpCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P);
-    else
-        hipLaunchKernelGGL((k_bp_tiled<kBpRef, kBpTZ, kBpCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P);
-    return hipGetLastError();
+    P.tiles_t = L.tiles_t;
+    P.tiles_s = L.tiles_s;
+    P.tiles_z = L.tiles_z;
+    return kind == kBpFdk ? launch_bp_kind<kBpFdk>(P, L, foot, proj, vol, st)
+                          : launch_bp_kind<kBpRef>(P, L, foot, proj, vol, st);
 }
 
-hipError_t launch_adjoint(const BpParams &Pin, const float *proj, float *vol, const SirtUpdate *upd, int ones_input,
-                          hipStream_t st)
+hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
+                          const SirtUpdate *upd, int ones_input, hipStream_t st)
 {
     BpParams P = Pin;
-    P.tiles_t = (P.nt + kTX - 1) / kTX;
-    P.tiles_s = (P.ns + kTS - 1) / kTS;
-    P.tiles_z = (P.nz + kAdjTZ - 1) / kAdjTZ;
-    const int nwg = P.tiles_t * P.tiles_s * P.tiles_z;
+    P.tiles_t = L.tiles_t;
+    P.tiles_s = L.tiles_s;
+    P.tiles_z = L.tiles_z;
+    const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
     SirtUpdate U = {};
     int sirt = 0;
     if (upd) {
         U = *upd;
         sirt = 1;
     }
-    hipLaunchKernelGGL((k_adjoint<kAdjTZ, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, U, sirt,
-                       ones_input);
+    if (L.wp == 40)
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot, U,
+                           sirt, ones_input);
+    else if (L.wp == 72)
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot, U,
+                           sirt, ones_input);
+    else
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot, U,
+                           sirt, ones_input);
     return hipGetLastError();
 }
 
