@@ -35,7 +35,8 @@ struct ConeTables {
     std::vector<double> h_pd, h_xd; // positions (double)
     std::vector<BpView> h_bp;
     std::vector<float> h_pf, h_xf;
-    DevBuf d_view, d_p, d_xi, d_bp, d_pf, d_xf;
+    std::vector<float> h_zq;        // [nz + 32][2]: zs^2, hz/|zs| per slice (zs = slice centre - source z)
+    DevBuf d_view, d_p, d_xi, d_bp, d_pf, d_xf, d_zq;
 };
 
 struct FanTables {
@@ -284,6 +285,14 @@ ctr_status cone_tables(ctr_ctx *ctx, const ctr_cone_geom *g)
         v.cb = (float)cb;
         T.h_bp[b] = v;
     }
+    // per-slice chord factors, padded by 32 slices so that a partial z tile reads in bounds
+    const double rz = g->size[2] / g->nz, Cz = g->size[2] / 2;
+    T.h_zq.resize(2 * ((size_t)g->nz + 32));
+    for (int z = 0; z < g->nz + 32; ++z) {
+        const double zs = (z + 0.5) * rz - Cz;
+        T.h_zq[2 * z] = (float)(zs * zs);
+        T.h_zq[2 * z + 1] = (float)(0.5 * rz / std::fabs(zs));
+    }
     T.h_pd.assign(g->p, g->p + g->n_p);
     T.h_xd.assign(g->xi, g->xi + g->n_xi);
     T.h_pf.assign(g->p, g->p + g->n_p);
@@ -295,6 +304,7 @@ ctr_status cone_tables(ctr_ctx *ctx, const ctr_cone_geom *g)
     if ((s = upload(ctx, T.d_bp, T.h_bp.data(), sizeof(BpView) * T.h_bp.size())) != CTR_OK) return s;
     if ((s = upload(ctx, T.d_pf, T.h_pf.data(), sizeof(float) * T.h_pf.size())) != CTR_OK) return s;
     if ((s = upload(ctx, T.d_xf, T.h_xf.data(), sizeof(float) * T.h_xf.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_zq, T.h_zq.data(), sizeof(float) * T.h_zq.size())) != CTR_OK) return s;
     T.key = key;
     T.valid = true;
     return CTR_OK;
@@ -356,6 +366,7 @@ BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
     P.views = (const BpView *)T.d_bp.ptr;
     P.pd = (const float *)T.d_pf.ptr;
     P.xd = (const float *)T.d_xf.ptr;
+    P.zq = (const float2 *)T.d_zq.ptr;
     P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
     P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
     return P;
@@ -713,7 +724,7 @@ void ctr_ctx_destroy(ctr_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->ws) buf_free(b);
     for (DevBuf *b : {&ctx->cone.d_view, &ctx->cone.d_p, &ctx->cone.d_xi, &ctx->cone.d_bp, &ctx->cone.d_pf,
-                      &ctx->cone.d_xf, &ctx->fan.d_view, &ctx->fan.d_p, &ctx->fan.d_zero, &ctx->fan.d_pf,
+                      &ctx->cone.d_xf, &ctx->cone.d_zq, &ctx->fan.d_view, &ctx->fan.d_p, &ctx->fan.d_zero, &ctx->fan.d_pf,
                       &ctx->fan.d_gam, &ctx->fan.d_colw, &ctx->fan.d_xf, &ctx->taps.d, &ctx->foot_bp.buf, &ctx->foot_adj.buf})
         buf_free(*b);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);

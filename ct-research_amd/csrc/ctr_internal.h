@@ -49,6 +49,7 @@ struct BpParams {
     const BpView *views;
     const float *pd;        // [np] detector columns (float, device)
     const float *xd;        // [nx] detector rows (float, device)
+    const float2 *zq;       // [nz + 32]: (zs^2, hz/|zs|) per slice, zs = slice centre relative to the source
     int accumulate;
     int quirks;
 };

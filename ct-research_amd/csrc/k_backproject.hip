@@ -17,6 +17,7 @@
 // gives each XCD a contiguous z-slab of tiles whose footprints share rows
 // of the detector in that XCD's L2.
 #include <cmath>
+#include <cstdlib>
 
 #include "ctr_internal.h"
 
@@ -115,10 +116,11 @@ __global__ __launch_bounds__(256) void k_bp_footprints(BpParams P, int tz, int w
 }
 
 // Chord of the source-to-voxel-centre line through the voxel
-// (Backprojection3D.cu:243-288): 2 |d| min_a(h_a / |d_a|).
-__device__ __forceinline__ float chord_len(float rho2, float mts, float zs, float hz)
+// (Backprojection3D.cu:243-288): 2 |d| min_a(h_a / |d_a|), with
+// mts = min(ht/|dt|, hs/|ds|) per column and (zs^2, hz/|zs|) per slice.
+__device__ __forceinline__ float chord_len(float rho2, float mts, float2 zq)
 {
-    return 2.0f * sqrtf(fmaf(zs, zs, rho2)) * fminf(mts, hz / fabsf(zs));
+    return 2.0f * sqrtf(zq.x + rho2) * fminf(mts, zq.y);
 }
 
 // Direct global -> LDS copy of one footprint row: lane l loads column c0 + l
@@ -131,23 +133,28 @@ __device__ __forceinline__ void dma_row(const float *__restrict__ row, int c0, i
     __builtin_amdgcn_global_load_lds(g, lds_row, 4, 0, 0);
 }
 
+template <int WP>
 __device__ __forceinline__ void issue_view(const float *__restrict__ proj, const BpParams &P, int b, int4 f, float *buf)
 {
     if ((f.w & kFootEmpty) || !(f.w & kFootFits)) return;
     const int H = f.z >> 16;
     const float *src = proj + ((size_t)b * P.nx + f.y) * P.np;
-    for (int r = threadIdx.x >> 6; r < H; r += kThreads / 64) dma_row(src + (size_t)r * P.np, f.x, P.np, buf + r * 64);
+    for (int r = threadIdx.x >> 6; r < H; r += kThreads / 64) dma_row(src + (size_t)r * P.np, f.x, P.np, buf + r * WP);
 }
 
 // Voxel-driven backprojection of all views in one pass.  LDS holds two
-// footprint buffers (row pitch 64 floats): while the tile interpolates view b
-// from one buffer, the DMA engine fills the other with view b+1, so one
-// barrier per view publishes the next footprint and retires the previous one.
-template <int KIND, int TZ, int HMAX>
-__global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__ proj, float *__restrict__ vol,
-                                                       BpParams P, const int4 *__restrict__ foot)
+// footprint buffers (HMAX rows of pitch WP floats, 64 staged columns per
+// row): while the tile interpolates view b from one buffer, the DMA engine
+// fills the other with view b+1, so one barrier per view publishes the next
+// footprint and retires the previous one.  The view table is read through
+// the scalar cache (its own restrict argument), so no vector-memory wait in
+// the compute phase drains the prefetch.
+template <int KIND, int TZ, int WP, int HMAX>
+__global__ __launch_bounds__(kThreads, 5) void k_bp_tiled(const float *__restrict__ proj, float *__restrict__ vol,
+                                                       BpParams P, const int4 *__restrict__ foot,
+                                                       const BpView *__restrict__ views,
+                                                       const float2 *__restrict__ zq)
 {
-    constexpr int WP = 64;
     __shared__ float lds[2][HMAX * WP];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tt = tile % P.tiles_t, rest = tile / P.tiles_t;
@@ -158,6 +165,8 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
     const int nzt = min(TZ, P.nz - z0);
     const float zA = fmaf((float)z0, P.rz, P.zc0);   // centre of slice z0 relative to the source plane
     const float fi = (float)i, fj = (float)j;
+    // largest float below vmax: 0 <= v < vmax  <=>  med3(v, 0, vlast) == v
+    const float vlast = __builtin_bit_cast(float, __builtin_bit_cast(int, P.vmax) - 1);
     const int4 *ft = foot + (size_t)tile * P.nb;
 
     float acc[TZ];
@@ -165,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
     for (int k = 0; k < TZ; ++k) acc[k] = 0.0f;
 
     int4 fn = ft[0];
-    issue_view(proj, P, 0, fn, lds[0]);
+    issue_view<WP>(proj, P, 0, fn, lds[0]);
     for (int b = 0; b < P.nb; ++b) {
         const int4 f = fn;
         const float *buf = lds[b & 1];
@@ -175,13 +184,13 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
         __syncthreads();
         if (b + 1 < P.nb) {
             fn = ft[b + 1];
-            issue_view(proj, P, b + 1, fn, lds[(b + 1) & 1]);
+            issue_view<WP>(proj, P, b + 1, fn, lds[(b + 1) & 1]);
         }
         if ((f.w & kFootEmpty) || !col_in) continue;
         const bool fits = (f.w & kFootFits) != 0;
 
         // per-column projection of this view
-        const BpView vw = P.views[b];
+        const BpView vw = views[b];
         const float dt = fmaf(vw.at, fi, fmaf(vw.as, fj, vw.a0));
         const float ds = fmaf(vw.bt, fi, fmaf(vw.bs, fj, vw.b0));
         const float m = P.D * fast_rcp(P.D - ds);
@@ -208,30 +217,35 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
                 const float v = fmaf((float)k, vB, vA);
                 const int iv = (int)v;
                 const float fv = __builtin_amdgcn_fractf(v);
-                const float *s = reinterpret_cast<const float *>(l8 + ((iv << 8) + base4));
+                const float *s = reinterpret_cast<const float *>(l8 + (__mul24(iv, WP * 4) + base4));
                 const float a = s[0], bq = s[1], c = s[WP], d = s[WP + 1];
                 const float top = fmaf(fu, bq - a, a), bot = fmaf(fu, d - c, c);
                 const float val = fmaf(fv, bot - top, top);
                 if (KIND == kBpFdk) acc[k] = fmaf(w, val, acc[k]);
-                else acc[k] = fmaf(chord_len(rho2, mts, fmaf((float)k, P.rz, zA), P.hz), val, acc[k]);
+                else acc[k] = fmaf(chord_len(rho2, mts, zq[z0 + k]), val, acc[k]);
             }
         } else if (fits) {
-            const int W = f.z & 0xffff, H = f.z >> 16;
+            // part of the tile projects off the detector.  v is clamped into
+            // [0, vlast] (the clamped stencil lies in the staged rectangle, whose
+            // rows reach the detector edge whenever a sample leaves it) and the
+            // weight is zeroed where the clamp moved v or u is off the detector.
+            const int W = f.z & 0xffff;
             const bool uok = u >= 0.0f && u < P.umax;
             const int iu = min(max(iu_raw, f.x), f.x + W - 2) - f.x;
+            const float *col = buf + iu - f.y * WP;
+            const float wu = uok ? w : 0.0f;
 #pragma unroll
             for (int k = 0; k < TZ; ++k) {
                 const float v = fmaf((float)k, vB, vA);
-                const bool ok = uok && v >= 0.0f && v < P.vmax;
-                const int ivr = (int)floorf(v);
-                const float fv = v - (float)ivr;
-                const int iv = min(max(ivr, f.y), f.y + H - 2) - f.y;
-                const float *s = buf + iv * WP + iu;
+                const float vc = __builtin_amdgcn_fmed3f(v, 0.0f, vlast);
+                const int iv = (int)vc;
+                const float fv = __builtin_amdgcn_fractf(vc);
+                const float *s = col + __mul24(iv, WP);
                 const float a = s[0], bq = s[1], c = s[WP], d = s[WP + 1];
                 const float top = fmaf(fu, bq - a, a), bot = fmaf(fu, d - c, c);
-                const float val = ok ? fmaf(fv, bot - top, top) : 0.0f;
-                if (KIND == kBpFdk) acc[k] = fmaf(w, val, acc[k]);
-                else acc[k] = fmaf(chord_len(rho2, mts, fmaf((float)k, P.rz, zA), P.hz), val, acc[k]);
+                const float val = fmaf(fv, bot - top, top);
+                const float wk = KIND == kBpFdk ? wu : chord_len(rho2, mts, zq[z0 + k]);
+                acc[k] = fmaf((KIND == kBpFdk || uok) && v == vc ? wk : 0.0f, val, acc[k]);
             }
         } else {
             // footprint larger than an LDS buffer: gather from global memory
@@ -250,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_bp_tiled(const float *__restrict__
                 const float top = fmaf(fu, bq - a, a), bot = fmaf(fu, d - c, c);
                 const float val = ok ? fmaf(fv, bot - top, top) : 0.0f;
                 if (KIND == kBpFdk) acc[k] = fmaf(w, val, acc[k]);
-                else acc[k] = fmaf(chord_len(rho2, mts, fmaf((float)k, P.rz, zA), P.hz), val, acc[k]);
+                else acc[k] = fmaf(chord_len(rho2, mts, zq[z0 + k]), val, acc[k]);
             }
         }
     }
@@ -519,7 +533,9 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ 
     }
 }
 
-constexpr int kBpTZ = 16, kBpHmax = 64;
+constexpr int kBpTZ = 16;
+// (row pitch, rows) of the two FDK/BP_REF footprint buffers, 2 x 32 KB at most
+constexpr int kBpShapes[3][2] = {{64, 64}, {72, 56}, {80, 50}};
 constexpr int kAdjTZ = 8, kAdjCap = 6144;
 
 }  // namespace
@@ -545,9 +561,18 @@ BpPlan bp_plan(const BpParams &P)
     L.tiles_t = (P.nt + kTX - 1) / kTX;
     L.tiles_s = (P.ns + kTS - 1) / kTS;
     L.tiles_z = (P.nz + L.tz - 1) / L.tz;
-    L.wp = 64;                 // one DMA row per wave instruction
-    L.capf = 2 * kBpHmax * 64;  // two buffers
-    L.hmax = kBpHmax;
+    // row pitch of the staged footprint: 80 floats puts neighbouring rows 16
+    // banks apart (fewest bank conflicts measured at 256^3 / 512^2);
+    // tuning knob CTR_BP_PITCH (64, 72 or 80)
+    int shape = 2;
+    if (const char *e = std::getenv("CTR_BP_PITCH")) {
+        const int want = std::atoi(e);
+        for (int q = 0; q < 3; ++q)
+            if (kBpShapes[q][0] == want) shape = q;
+    }
+    L.wp = kBpShapes[shape][0];
+    L.hmax = kBpShapes[shape][1];
+    L.capf = 2 * L.wp * L.hmax;
     L.nfoot = (size_t)L.tiles_t * L.tiles_s * L.tiles_z * P.nb;
     return L;
 }
@@ -574,7 +599,10 @@ hipError_t launch_bp_footprints(const BpParams &Pin, const BpPlan &L, int4 *foot
     P.tiles_s = L.tiles_s;
     P.tiles_z = L.tiles_z;
     const long n = (long)L.nfoot;
-    hipLaunchKernelGGL(k_bp_footprints, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, L.tz, L.wp, L.hmax,
+    // the interpolating kernels stage 64 columns per row (one DMA lane each)
+    // whatever the row pitch; the adjoint stages up to its pitch
+    const int wcap = L.faces ? L.wp : 64;
+    hipLaunchKernelGGL(k_bp_footprints, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, L.tz, wcap, L.hmax,
                        L.faces, foot);
     return hipGetLastError();
 }
@@ -584,7 +612,15 @@ static hipError_t launch_bp_kind(const BpParams &P, const BpPlan &L, const int4 
                                  hipStream_t st)
 {
     const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
-    hipLaunchKernelGGL((k_bp_tiled<KIND, kBpTZ, kBpHmax>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot);
+    if (L.wp == 64)
+        hipLaunchKernelGGL((k_bp_tiled<KIND, kBpTZ, 64, 64>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, P.zq);
+    else if (L.wp == 72)
+        hipLaunchKernelGGL((k_bp_tiled<KIND, kBpTZ, 72, 56>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, P.zq);
+    else
+        hipLaunchKernelGGL((k_bp_tiled<KIND, kBpTZ, 80, 50>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, P.zq);
     return hipGetLastError();
 }
 
