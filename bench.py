@@ -29,8 +29,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=360, help="views in the CPU-baseline sample")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="before the timed region, copy this many MiB device-to-device (a plain copy kernel of "
-                         "known size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+                    help="before the timed region, fill and then copy (elementwise multiply by 1) a buffer of this "
+                         "many MiB: kernels of known traffic that calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -91,9 +91,11 @@ def main():
     filt = ctx.cone_filter(g, proj, filt=ctr.FILTER_SL_HALF)
     out = torch.empty(nvox, dtype=torch.float32, device=dev)
     if args.calib_copy_mb:
+        # fill (writes n bytes) and an elementwise multiply (reads n, writes n):
+        # plain kernels of known traffic, unlike copy_, which may use the DMA engine
         a = torch.empty(args.calib_copy_mb << 18, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
-        b.copy_(a)
+        torch.mul(a, 1.0, out=b)
         del a, b
     torch.cuda.synchronize()
 

@@ -58,6 +58,9 @@ struct TapTable {
     bool valid = false;
     std::vector<double> h;
     DevBuf d;
+    int fft_len = 0;                // FFT path length (0: direct convolution)
+    std::vector<double> spec;       // [fft_len] H[k] / fft_len
+    DevBuf d_spec;
 };
 
 }  // namespace
@@ -194,6 +197,9 @@ ctr_status check_cone(const ctr_cone_geom *g, int need_uniform)
         return fail(nullptr, CTR_EINVAL, "backprojection needs at least 2 detector columns and rows");
     if ((double)g->n_beta * g->n_p * g->n_xi > (double)std::numeric_limits<int>::max())
         return fail(nullptr, CTR_ETOOBIG, "more than 2^31-1 detector samples");
+    // a view is addressed through one buffer descriptor with 32-bit byte offsets
+    if ((double)g->n_p * g->n_xi > (double)(1 << 28))
+        return fail(nullptr, CTR_ETOOBIG, "a view has more than 2^28 detector samples");
     const double half = 0.5 * std::sqrt(g->size[0] * g->size[0] + g->size[1] * g->size[1]);
     if (!(g->distance > half * (1.0 + 1e-6)))
         return fail(nullptr, CTR_EINVAL, "source orbit (distance %g) intersects the volume (half diagonal %g)",
@@ -543,6 +549,22 @@ ctr_status tap_table(ctr_ctx *ctx, int filter, int equiangular, double d, int n)
     for (int k = 0; k < 2 * n - 1; ++k) T.h[k] = h[std::abs(k - (n - 1))];
     ctr_status s = upload(ctx, T.d, T.h.data(), sizeof(double) * T.h.size());
     if (s != CTR_OK) return s;
+    // spectrum of the circular kernel c[m] = c[L-m] = h[m] (m < n) for the FFT
+    // path: H[k] = h[0] + 2 sum_m h[m] cos(2 pi k m / L), real because c is even
+    T.fft_len = fft_filter_length(n);
+    if (T.fft_len) {
+        const int L = T.fft_len;
+        std::vector<double> cosL(L);
+        for (int i = 0; i < L; ++i) cosL[i] = std::cos(2.0 * pi * i / L);
+        T.spec.assign(L, 0.0);
+        for (int k = 0; k < L; ++k) {
+            long double acc = h[0];
+            for (int m = 1; m < n; ++m) acc += 2.0L * h[m] * cosL[((long)k * m) % L];
+            T.spec[k] = (double)(acc / L);
+        }
+        s = upload(ctx, T.d_spec, T.spec.data(), sizeof(double) * L);
+        if (s != CTR_OK) return s;
+    }
     T.key = key;
     T.valid = true;
     return CTR_OK;
@@ -622,6 +644,8 @@ ctr_status cone_filter_impl(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, co
     F.pd = (const float *)ctx->cone.d_pf.ptr;
     F.xd = (const float *)ctx->cone.d_xf.ptr;
     F.taps = (const double *)ctx->taps.d.ptr;
+    F.spec = ctx->taps.fft_len ? (const double *)ctx->taps.d_spec.ptr : nullptr;
+    F.fft_len = ctx->taps.fft_len;
     return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "ramp filter");
 }
 
@@ -645,6 +669,8 @@ ctr_status fan_filter_impl(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, in
     F.xd = (const float *)ctx->fan.d_xf.ptr;
     F.colw = (const float *)ctx->fan.d_colw.ptr;
     F.taps = (const double *)ctx->taps.d.ptr;
+    F.spec = ctx->taps.fft_len ? (const double *)ctx->taps.d_spec.ptr : nullptr;
+    F.fft_len = ctx->taps.fft_len;
     return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "fan ramp filter");
 }
 
@@ -725,7 +751,7 @@ void ctr_ctx_destroy(ctr_ctx *ctx)
     for (auto &b : ctx->ws) buf_free(b);
     for (DevBuf *b : {&ctx->cone.d_view, &ctx->cone.d_p, &ctx->cone.d_xi, &ctx->cone.d_bp, &ctx->cone.d_pf,
                       &ctx->cone.d_xf, &ctx->cone.d_zq, &ctx->fan.d_view, &ctx->fan.d_p, &ctx->fan.d_zero, &ctx->fan.d_pf,
-                      &ctx->fan.d_gam, &ctx->fan.d_colw, &ctx->fan.d_xf, &ctx->taps.d, &ctx->foot_bp.buf, &ctx->foot_adj.buf})
+                      &ctx->fan.d_gam, &ctx->fan.d_colw, &ctx->fan.d_xf, &ctx->taps.d, &ctx->taps.d_spec, &ctx->foot_bp.buf, &ctx->foot_adj.buf})
         buf_free(*b);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;

@@ -72,6 +72,8 @@ struct FilterParams {
     const float *xd;        // [nx] row positions
     const float *colw;      // [np] column weights (mode 1)
     const double *taps;     // [2np-1] two-sided taps times the pitch: h2[k] = d*h[|k-(np-1)|]
+    const double *spec;     // [fft_len] real DFT of the circular taps divided by fft_len, or null (direct path)
+    int fft_len;
 };
 
 // Fan-beam pixel-driven kernels.
@@ -116,6 +118,8 @@ hipError_t launch_adjoint(const BpParams &P, const BpPlan &L, const int4 *foot, 
                           const SirtUpdate *upd, int ones_input, hipStream_t st);
 
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
+// transform length of the FFT filter path for np columns (0: direct convolution)
+int fft_filter_length(int np);
 
 hipError_t launch_fan_bp_ref(const FanParams &P, const float *proj, float *img, hipStream_t st);
 hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img, const SirtUpdate *upd,

@@ -123,23 +123,26 @@ __device__ __forceinline__ float chord_len(float rho2, float mts, float2 zq)
     return 2.0f * sqrtf(zq.x + rho2) * fminf(mts, zq.y);
 }
 
-// Direct global -> LDS copy of one footprint row: lane l loads column c0 + l
-// (clamped into the detector row) into lds_row[l].  The copy completes
-// asynchronously; a vmcnt wait and a barrier publish it.
-__device__ __forceinline__ void dma_row(const float *__restrict__ row, int c0, int np, float *lds_row)
-{
-    const int lane = threadIdx.x & 63;
-    const float *g = row + min(c0 + lane, np - 1);
-    __builtin_amdgcn_global_load_lds(g, lds_row, 4, 0, 0);
-}
+// Direct global -> LDS copy of a view's footprint rows: lane l of a wave
+// copies column c0 + l (clamped into the row) of row r into buf[r * WP + l]
+// through a buffer descriptor of the view, the row selected by the scalar
+// offset, so the issue loop costs no vector instructions.  The copies
+// complete asynchronously; a vmcnt wait and a barrier publish them.  Offsets
+// past the view read as zero (hardware range check).
+constexpr int kBufDword3 = 0x00020000;   // raw 32-bit buffer descriptor word 3 on gfx9
 
 template <int WP>
 __device__ __forceinline__ void issue_view(const float *__restrict__ proj, const BpParams &P, int b, int4 f, float *buf)
 {
     if ((f.w & kFootEmpty) || !(f.w & kFootFits)) return;
     const int H = f.z >> 16;
-    const float *src = proj + ((size_t)b * P.nx + f.y) * P.np;
-    for (int r = threadIdx.x >> 6; r < H; r += kThreads / 64) dma_row(src + (size_t)r * P.np, f.x, P.np, buf + r * WP);
+    const size_t view = (size_t)P.nx * P.np;
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(proj + (size_t)b * view), 0,
+                                                                  (int)(view * sizeof(float)), kBufDword3);
+    const unsigned voff = (unsigned)min(f.x + (int)(threadIdx.x & 63), P.np - 1) * 4u;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int r = wave; r < H; r += kThreads / 64)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, buf + r * WP, 4, voff, (unsigned)((f.y + r) * P.np) * 4u, 0, 0);
 }
 
 // Voxel-driven backprojection of all views in one pass.  LDS holds two

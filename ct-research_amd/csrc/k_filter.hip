@@ -94,6 +94,263 @@ __global__ __launch_bounds__(64 *RW) void k_row_filter(const float *__restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// FFT path.  The linear convolution of a row (np samples) with the two-sided
+// taps h2 equals a circular convolution of length L >= 2np - 1 with
+// c[m] = c[L-m] = h2[np-1+m] (m < np), zero elsewhere.  c is real and even,
+// so its DFT H is real: two real rows packed as z = x0 + i x1 are filtered by
+// one complex transform pair, z' = IDFT(DFT(z) H) = y0 + i y1, computed as
+// conj(DFT(conj(DFT(z)) H / L)).  FP64 throughout; H / L comes from the host.
+//
+// Transforms are Stockham radix-16, radix-16, radix-(L/256) passes over
+// G = L/16 threads that each hold 16 points; after the last pass thread j
+// holds X[j + G t] (t = 0..15), which is exactly the input distribution of
+// the first pass, so the spectrum multiply and the second transform follow
+// in registers.  LDS holds one padded transform (index i at i + i/16) per
+// group of G threads.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }   // a * (-i)
+
+// forward DFT (exponent -2 pi i n k / R), natural order in and out
+__device__ __forceinline__ void dft2(double2 &a, double2 &b)
+{
+    const double2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3)
+{
+    const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_mi(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+constexpr double kC8 = 0.92387953251128675613;   // cos(pi/8)
+constexpr double kS8 = 0.38268343236508977173;   // sin(pi/8)
+constexpr double kR2 = 0.70710678118654752440;   // sqrt(1/2)
+
+// exp(-2 pi i m / 16) for the products m = n1 * k1 of the 4 x 4 split
+__device__ __forceinline__ double2 w16(int m)
+{
+    switch (m & 15) {
+    case 0: return make_double2(1.0, 0.0);
+    case 1: return make_double2(kC8, -kS8);
+    case 2: return make_double2(kR2, -kR2);
+    case 3: return make_double2(kS8, -kC8);
+    case 4: return make_double2(0.0, -1.0);
+    case 6: return make_double2(-kR2, -kR2);
+    case 9: return make_double2(-kC8, kS8);
+    default: return make_double2(0.0, 0.0);   // not used by dft16
+    }
+}
+
+__device__ __forceinline__ void dft8(double2 *v)
+{
+    // even/odd split: X[k] = E[k] + w8^k O[k], X[k+4] = E[k] - w8^k O[k]
+    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    o1 = cmul(o1, make_double2(kR2, -kR2));
+    o2 = mul_mi(o2);
+    o3 = cmul(o3, make_double2(-kR2, -kR2));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+__device__ __forceinline__ void dft16(double2 *v)
+{
+    // x[n1 + 4 n2]: 4-point DFTs over n2, twiddle w16^(n1 k1), 4-point DFTs over n1 -> X[k1 + 4 k2]
+    double2 a[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        a[n1][0] = v[n1];
+        a[n1][1] = v[n1 + 4];
+        a[n1][2] = v[n1 + 8];
+        a[n1][3] = v[n1 + 12];
+        dft4(a[n1][0], a[n1][1], a[n1][2], a[n1][3]);
+    }
+#pragma unroll
+    for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+        for (int k1 = 1; k1 < 4; ++k1) a[n1][k1] = n1 * k1 == 4 ? mul_mi(a[n1][k1]) : cmul(a[n1][k1], w16(n1 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        dft4(a[0][k1], a[1][k1], a[2][k1], a[3][k1]);
+        v[k1] = a[0][k1];
+        v[k1 + 4] = a[1][k1];
+        v[k1 + 8] = a[2][k1];
+        v[k1 + 12] = a[3][k1];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_small(double2 *u)
+{
+    if constexpr (R == 2) dft2(u[0], u[1]);
+    else if constexpr (R == 4) dft4(u[0], u[1], u[2], u[3]);
+    else dft8(u);
+}
+
+__device__ __forceinline__ int fft_pad(int i) { return i + (i >> 4); }
+
+// w = exp(-2 pi i num / den)
+__device__ __forceinline__ double2 twiddle(int num, int den)
+{
+    double s, c;
+    sincospi(-2.0 * (double)num / (double)den, &s, &c);
+    return make_double2(c, s);
+}
+
+constexpr int kFftThreads = 128;
+
+// Per-thread twiddle bases; the powers w^r are formed by recurrence in each
+// pass (relative error below 16 ulp), which keeps the registers for the data.
+template <int L>
+struct FftTwiddles {
+    static constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1;
+    double2 b;          // pass 2: w256^(j % 16)
+    double2 c[NBF];     // pass 3: wL^jj, jj = j + G m
+};
+
+// One forward transform of the 16 points v[t] = x[j + G t] of this thread's
+// group; on return v[t] = X[j + G t].  buf is the group's padded LDS slot.
+template <int L>
+__device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const FftTwiddles<L> &tw)
+{
+    constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1;
+    // pass 1: radix 16, Ns = 1 -> X1[16 j + r]
+    dft16(v);
+    __syncthreads();   // the group's previous readers of buf are done
+#pragma unroll
+    for (int r = 0; r < 16; ++r) buf[fft_pad(16 * j + r)] = v[r];
+    __syncthreads();
+    // pass 2: radix 16, Ns = 16
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = buf[fft_pad(j + G * r)];
+    {
+        double2 w = tw.b;
+#pragma unroll
+        for (int r = 1; r < 16; ++r) {
+            v[r] = cmul(v[r], w);
+            if (r < 15) w = cmul(w, tw.b);
+        }
+    }
+    dft16(v);
+    if constexpr (RL > 1) {
+        __syncthreads();   // every thread has read its pass-2 inputs
+        const int d0 = (j >> 4) * 256 + (j & 15);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) buf[fft_pad(d0 + 16 * r)] = v[r];
+        __syncthreads();
+        // pass 3: radix RL, Ns = 256, butterflies jj = j + G m -> X[jj + 256 r] = X[j + G (m + NBF r)]
+#pragma unroll
+        for (int m = 0; m < NBF; ++m) {
+            const int jj = j + G * m;
+            double2 u[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r) u[r] = buf[fft_pad(jj + 256 * r)];
+            double2 w = tw.c[m];
+#pragma unroll
+            for (int r = 1; r < RL; ++r) {
+                u[r] = cmul(u[r], w);
+                if (r + 1 < RL) w = cmul(w, tw.c[m]);
+            }
+            dft_small<RL>(u);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) v[m + NBF * r] = u[r];
+        }
+    }
+    // L == 256: thread j < 16 holds X[j + 16 r] = v[r] after pass 2 already
+}
+
+template <int L>
+__global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restrict__ in, float *__restrict__ out,
+                                                          FilterParams P)
+{
+    constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1, GPB = kFftThreads / G;
+    constexpr int SLOT = L + L / 16;
+    __shared__ double2 lds[GPB * SLOT];
+    const int grp = threadIdx.x / G, j = threadIdx.x % G;
+    double2 *buf = lds + grp * SLOT;
+    const int np = P.np;
+
+    FftTwiddles<L> tw;
+    tw.b = twiddle(j & 15, 256);
+#pragma unroll
+    for (int m = 0; m < NBF; ++m) tw.c[m] = twiddle(j + G * m, L);
+
+    const int npairs = (P.rows + 1) / 2;
+    for (int base = blockIdx.x * GPB; base < npairs; base += gridDim.x * GPB) {
+        const int pair = base + grp;   // every group takes part in the barriers, active or not
+        const int row0 = 2 * pair, row1 = row0 + 1;
+        const bool has0 = pair < npairs, has1 = has0 && row1 < P.rows;
+        const float xi0 = P.mode == 0 && has0 ? P.xd[row0 % P.nx] : 0.0f;
+        const float xi1 = P.mode == 0 && has1 ? P.xd[row1 % P.nx] : 0.0f;
+        const float *src0 = in + (size_t)row0 * np, *src1 = in + (size_t)row1 * np;
+        double2 v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int q = j + G * t;
+            double a = 0.0, b = 0.0;
+            if (q < np) {
+                float w0, w1;
+                if (P.mode == 0) {
+                    const float pq = P.pd[q];
+                    w0 = P.D / sqrtf(P.D * P.D + pq * pq + xi0 * xi0);
+                    w1 = P.D / sqrtf(P.D * P.D + pq * pq + xi1 * xi1);
+                } else {
+                    w0 = w1 = P.colw[q];
+                }
+                // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
+                if (has0) a = (double)(src0[q] * w0);
+                if (has1) b = (double)(src1[q] * w1);
+            }
+            v[t] = make_double2(a, b);
+        }
+        fft_group<L>(v, buf, j, tw);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const double h = P.spec[j + G * t];   // H[k] / L, real
+            v[t] = make_double2(v[t].x * h, -v[t].y * h);
+        }
+        fft_group<L>(v, buf, j, tw);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int q = j + G * t;
+            if (q < np) {
+                if (has0) out[(size_t)row0 * np + q] = (float)v[t].x;
+                if (has1) out[(size_t)row1 * np + q] = (float)(-v[t].y);
+            }
+        }
+    }
+}
+
+template <int L>
+hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStream_t st)
+{
+    constexpr int GPB = kFftThreads / (L / 16);
+    const int npairs = (P.rows + 1) / 2;
+    const int need = (npairs + GPB - 1) / GPB;
+    const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
+    hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+    return hipGetLastError();
+}
+
 template <int K, int RW>
 hipError_t launch_k(const FilterParams &P, const float *in, float *out, hipStream_t st)
 {
@@ -106,8 +363,24 @@ hipError_t launch_k(const FilterParams &P, const float *in, float *out, hipStrea
 
 }  // namespace
 
+int fft_filter_length(int np)
+{
+    int L = 256;
+    while (L < 2 * np - 1) L *= 2;
+    return L <= 2048 ? L : 0;
+}
+
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st)
 {
+    if (P.spec) {
+        switch (P.fft_len) {
+        case 256: return launch_fft<256>(P, in, out, st);
+        case 512: return launch_fft<512>(P, in, out, st);
+        case 1024: return launch_fft<1024>(P, in, out, st);
+        case 2048: return launch_fft<2048>(P, in, out, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     const int need = (P.np + 63) / 64;
     if (need <= 1) return launch_k<1, 4>(P, in, out, st);
     if (need <= 2) return launch_k<2, 4>(P, in, out, st);
