@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libctr.so")
 
-SOURCES = ["ctr_api.cpp", "k_forward.hip", "k_backproject.hip", "k_filter.hip", "k_fan.hip", "k_util.hip"]
+SOURCES = ["ctr_api.cpp", "k_forward.hip", "k_backproject.hip", "k_filter.hip", "k_fan.hip", "k_util.hip", "k_cnn.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -41,7 +41,7 @@ def _compile(src):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 6)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 7)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

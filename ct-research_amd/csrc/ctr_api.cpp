@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "ctr_cnn.h"
 #include "ctr_internal.h"
 
 using namespace ctr;
@@ -21,7 +22,7 @@ namespace {
 
 thread_local std::string g_last_error;
 
-enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsCount };
+enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsCount };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -76,7 +77,12 @@ struct ctr_ctx {
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
+    std::vector<ctr_cnn *> nets;   // networks created on this context (released with it)
 };
+
+namespace {
+void cnn_release(ctr_cnn *net);   // frees a network's device buffers and detaches it from its context
+}
 
 namespace {
 
@@ -748,6 +754,8 @@ void ctr_ctx_destroy(ctr_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    // networks still alive become detached handles: ctr_cnn_destroy only frees the struct
+    while (!ctx->nets.empty()) cnn_release(ctx->nets.back());
     for (auto &b : ctx->ws) buf_free(b);
     for (DevBuf *b : {&ctx->cone.d_view, &ctx->cone.d_p, &ctx->cone.d_xi, &ctx->cone.d_bp, &ctx->cone.d_pf,
                       &ctx->cone.d_xf, &ctx->cone.d_zq, &ctx->fan.d_view, &ctx->fan.d_p, &ctx->fan.d_zero, &ctx->fan.d_pf,
@@ -1140,6 +1148,467 @@ ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float
         return check_launch(ctx, launch_fan_adjoint(P, rr, nullptr, U, 0, ctx->stream), "SIRT update");
     };
     return sirt_loop(ctx, nvox2(g), nray2(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Learned SIRT update: the 3-D CNN and the residual-judged loop
+// ===========================================================================
+namespace {
+
+// Layer table: real channels, padded channel counts of the kernels and the
+// offsets of the layer's weights (followed by its bias) in the flat
+// FP32 parameter vector.
+struct CnnLayer {
+    int ci, co;             // real input / output channels
+    int ci_pad;             // forward input channels (16 or 32)
+    int co_pad_w;           // output channels of the weight-gradient partials (16 or 32)
+    int bwd_cin;            // input channels of the input-gradient convolution (0: not needed)
+    size_t woff;            // weights [co][ci][27]; bias [co] follows at woff + co*ci*27
+};
+constexpr CnnLayer kLayers[4] = {
+    {2, 32, 16, 32, 0, 0},
+    {32, 32, 32, 32, 32, 1760},
+    {32, 32, 32, 32, 32, 29440},
+    {32, 1, 32, 16, 16, 57120},
+};
+constexpr size_t kCnnParams = CTR_CNN_NUM_PARAMS;
+static_assert(57120 + 32 * 27 + 1 == CTR_CNN_NUM_PARAMS, "parameter layout");
+
+}  // namespace
+
+struct ctr_cnn {
+    ctr_ctx *ctx = nullptr;
+    DevBuf params, m, v, grad;
+    DevBuf wf[4], wb[4];                    // packed BF16 forward / input-gradient weights
+    DevBuf a0, a1, a2, a3, out, d4, da, db; // activations and deltas (one patch or volume)
+    DevBuf partial, lsum, xin, gin, tin;
+    int step = 0;
+};
+
+namespace {
+
+ctr_status cnn_pack(ctr_cnn *net)
+{
+    ctr_ctx *ctx = net->ctx;
+    for (int l = 0; l < 4; ++l) {
+        const CnnLayer &L = kLayers[l];
+        const float *w = (const float *)net->params.ptr + L.woff;
+        TRY(check_launch(ctx,
+                         launch_pack_weights(w, L.ci, L.co, L.ci_pad, (__bf16 *)net->wf[l].ptr, L.bwd_cin,
+                                             L.bwd_cin ? (__bf16 *)net->wb[l].ptr : nullptr, ctx->stream),
+                         "pack weights"));
+    }
+    return CTR_OK;
+}
+
+const float *layer_bias(const ctr_cnn *net, int l)
+{
+    const CnnLayer &L = kLayers[l];
+    return (const float *)net->params.ptr + L.woff + (size_t)L.co * L.ci * 27;
+}
+
+ctr_status cnn_reserve(ctr_cnn *net, size_t nvox, bool train)
+{
+    ctr_ctx *ctx = net->ctx;
+    TRY(buf_reserve(ctx, net->a0, nvox * 16 * sizeof(__bf16)));
+    TRY(buf_reserve(ctx, net->a1, nvox * 32 * sizeof(__bf16)));
+    TRY(buf_reserve(ctx, net->a2, nvox * 32 * sizeof(__bf16)));
+    TRY(buf_reserve(ctx, net->out, nvox * sizeof(float)));
+    if (train) {
+        TRY(buf_reserve(ctx, net->a3, nvox * 32 * sizeof(__bf16)));
+        TRY(buf_reserve(ctx, net->d4, nvox * 16 * sizeof(__bf16)));
+        TRY(buf_reserve(ctx, net->da, nvox * 32 * sizeof(__bf16)));
+        TRY(buf_reserve(ctx, net->db, nvox * 32 * sizeof(__bf16)));
+    }
+    return CTR_OK;
+}
+
+// Forward pass over one volume; with keep the four activations stay for the
+// backward pass (a0..a3), otherwise a1/a2 are reused.  Result in net->out.
+ctr_status cnn_forward_dev(ctr_cnn *net, const CnnShape &S, const float *x, const float *g, float g_scale, bool keep)
+{
+    ctr_ctx *ctx = net->ctx;
+    const size_t n = (size_t)S.nx * S.ny * S.nz;
+    __bf16 *a0 = (__bf16 *)net->a0.ptr, *a1 = (__bf16 *)net->a1.ptr, *a2 = (__bf16 *)net->a2.ptr;
+    __bf16 *a3 = keep ? (__bf16 *)net->a3.ptr : a1;
+    TRY(check_launch(ctx, launch_pack_input(x, g, g_scale, n, a0, ctx->stream), "CNN input"));
+    TRY(check_launch(ctx, launch_conv3d(16, kEpiBiasRelu, a0, (const __bf16 *)net->wf[0].ptr, layer_bias(net, 0),
+                                        nullptr, a1, S, ctx->stream), "conv 1"));
+    TRY(check_launch(ctx, launch_conv3d(32, kEpiBiasRelu, a1, (const __bf16 *)net->wf[1].ptr, layer_bias(net, 1),
+                                        nullptr, a2, S, ctx->stream), "conv 2"));
+    TRY(check_launch(ctx, launch_conv3d(32, kEpiBiasRelu, a2, (const __bf16 *)net->wf[2].ptr, layer_bias(net, 2),
+                                        nullptr, a3, S, ctx->stream), "conv 3"));
+    TRY(check_launch(ctx, launch_conv3d(32, kEpiLinearF32, a3, (const __bf16 *)net->wf[3].ptr, layer_bias(net, 3),
+                                        nullptr, net->out.ptr, S, ctx->stream), "conv 4"));
+    return CTR_OK;
+}
+
+// Backward pass after cnn_forward_dev(keep): accumulates dloss/dparams into
+// net->grad and the patch's sum of squared errors into lsum[slot].
+ctr_status cnn_backward_dev(ctr_cnn *net, const CnnShape &S, const float *target, int border, float inv_count,
+                            int slot)
+{
+    ctr_ctx *ctx = net->ctx;
+    hipStream_t st = ctx->stream;
+    __bf16 *a0 = (__bf16 *)net->a0.ptr, *a1 = (__bf16 *)net->a1.ptr, *a2 = (__bf16 *)net->a2.ptr,
+           *a3 = (__bf16 *)net->a3.ptr, *d4 = (__bf16 *)net->d4.ptr, *da = (__bf16 *)net->da.ptr,
+           *db = (__bf16 *)net->db.ptr;
+    float *grad = (float *)net->grad.ptr, *part = (float *)net->partial.ptr;
+    double *lpart = (double *)ctx->ws[kWsPart].ptr, *lsum = (double *)net->lsum.ptr;
+    int nb = 0;
+    TRY(check_launch(ctx, launch_mse_grad((const float *)net->out.ptr, target, S, border, inv_count, d4, lpart, &nb, st),
+                     "loss"));
+    TRY(check_launch(ctx, launch_reduce_partials(lpart, nb, lsum + 2 * slot, st), "loss sum"));
+    const int nblk = wgrad_blocks(S);
+    auto wgrad = [&](int l, const __bf16 *act, const __bf16 *delta) {
+        const CnnLayer &L = kLayers[l];
+        TRY(check_launch(ctx, launch_conv3d_wgrad(L.ci_pad, L.co_pad_w, act, delta, part, nblk, S, st), "weight grad"));
+        return check_launch(ctx, launch_reduce_grad(part, nblk, L.ci_pad, L.co_pad_w, L.ci, L.co, grad + L.woff, 1, st),
+                            "grad sum");
+    };
+    TRY(wgrad(3, a3, d4));
+    TRY(check_launch(ctx, launch_conv3d(16, kEpiMask, d4, (const __bf16 *)net->wb[3].ptr, nullptr, a3, da, S, st),
+                     "input grad 4"));
+    TRY(wgrad(2, a2, da));
+    TRY(check_launch(ctx, launch_conv3d(32, kEpiMask, da, (const __bf16 *)net->wb[2].ptr, nullptr, a2, db, S, st),
+                     "input grad 3"));
+    TRY(wgrad(1, a1, db));
+    TRY(check_launch(ctx, launch_conv3d(32, kEpiMask, db, (const __bf16 *)net->wb[1].ptr, nullptr, a1, da, S, st),
+                     "input grad 2"));
+    TRY(wgrad(0, a0, da));
+    return CTR_OK;
+}
+
+ctr_status need_net(ctr_cnn *net)
+{
+    if (!net) return fail(nullptr, CTR_EINVAL, "network is NULL");
+    if (!net->ctx) return fail(nullptr, CTR_EINVAL, "network's context was destroyed");
+    return CTR_OK;
+}
+
+void cnn_release(ctr_cnn *net)
+{
+    ctr_ctx *ctx = net->ctx;
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : {&net->params, &net->m, &net->v, &net->grad, &net->a0, &net->a1, &net->a2, &net->a3, &net->out,
+                      &net->d4, &net->da, &net->db, &net->partial, &net->lsum, &net->xin, &net->gin, &net->tin})
+        buf_free(*b);
+    for (int l = 0; l < 4; ++l) {
+        buf_free(net->wf[l]);
+        buf_free(net->wb[l]);
+    }
+    ctx->nets.erase(std::remove(ctx->nets.begin(), ctx->nets.end(), net), ctx->nets.end());
+    net->ctx = nullptr;
+}
+
+ctr_status check_cnn_shape(ctr_ctx *ctx, int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return fail(ctx, CTR_EINVAL, "volume dimensions must be positive");
+    if ((double)nx * ny * nz * 32 > (double)std::numeric_limits<int>::max() * 8.0)
+        return fail(ctx, CTR_ETOOBIG, "volume too large for the CNN workspaces");
+    return CTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ctr_status ctr_cnn_create(ctr_ctx *ctx, ctr_cnn **out)
+{
+    if (!out) return fail(ctx, CTR_EINVAL, "output pointer is NULL");
+    *out = nullptr;
+    TRY(need_ctx(ctx));
+    ctr_cnn *net = new ctr_cnn;
+    net->ctx = ctx;
+    ctx->nets.push_back(net);
+    ctr_status s = CTR_OK;
+    for (DevBuf *b : {&net->params, &net->m, &net->v, &net->grad})
+        if (s == CTR_OK) s = buf_reserve(ctx, *b, kCnnParams * sizeof(float));
+    for (int l = 0; l < 4 && s == CTR_OK; ++l) {
+        s = buf_reserve(ctx, net->wf[l], (size_t)27 * kLayers[l].ci_pad * 32 * sizeof(__bf16));
+        if (s == CTR_OK && kLayers[l].bwd_cin)
+            s = buf_reserve(ctx, net->wb[l], (size_t)27 * kLayers[l].bwd_cin * 32 * sizeof(__bf16));
+    }
+    if (s == CTR_OK) s = buf_reserve(ctx, net->lsum, sizeof(double) * 2 * 4096);
+    if (s == CTR_OK) {
+        for (DevBuf *b : {&net->params, &net->m, &net->v, &net->grad})
+            if (s == CTR_OK) s = check_launch(ctx, launch_fill((float *)b->ptr, 0.0f, kCnnParams, ctx->stream), "fill");
+    }
+    if (s == CTR_OK) s = cnn_pack(net);
+    if (s != CTR_OK) {
+        ctr_cnn_destroy(net);
+        return s;
+    }
+    *out = net;
+    return CTR_OK;
+}
+
+void ctr_cnn_destroy(ctr_cnn *net)
+{
+    if (!net) return;
+    cnn_release(net);
+    delete net;
+}
+
+ctr_status ctr_cnn_set_params(ctr_cnn *net, const float *params, size_t n)
+{
+    TRY(need_net(net));
+    ctr_ctx *ctx = net->ctx;
+    if (!params || n != kCnnParams) return fail(ctx, CTR_EINVAL, "expected %zu parameters", kCnnParams);
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return fail(ctx, CTR_EINVAL, "parameter %zu is not finite", i);
+    TRY(upload(ctx, net->params, params, n * sizeof(float)));
+    for (DevBuf *b : {&net->m, &net->v})
+        TRY(check_launch(ctx, launch_fill((float *)b->ptr, 0.0f, kCnnParams, ctx->stream), "fill"));
+    net->step = 0;
+    TRY(cnn_pack(net));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status ctr_cnn_get_params(ctr_cnn *net, float *params, size_t n)
+{
+    TRY(need_net(net));
+    ctr_ctx *ctx = net->ctx;
+    if (!params || n != kCnnParams) return fail(ctx, CTR_EINVAL, "expected %zu parameters", kCnnParams);
+    HIP_TRY(ctx, hipMemcpyAsync(params, net->params.ptr, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status ctr_cnn_forward(ctr_cnn *net, int nx, int ny, int nz, const float *x, const float *g, float g_scale,
+                           float *out, unsigned flags)
+{
+    TRY(need_net(net));
+    ctr_ctx *ctx = net->ctx;
+    TRY(check_cnn_shape(ctx, nx, ny, nz));
+    if (!std::isfinite(g_scale)) return fail(ctx, CTR_EINVAL, "g_scale must be finite");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const size_t n = (size_t)nx * ny * nz;
+    const float *dx, *dg;
+    TRY(stage_in(ctx, dev, kWsIn, x, n, &dx));
+    TRY(stage_in(ctx, dev, kWsTmp, g, n, &dg));
+    if (!out) return fail(ctx, CTR_EINVAL, "output data pointer is NULL");
+    TRY(cnn_reserve(net, n, false));
+    const CnnShape S{nx, ny, nz};
+    TRY(cnn_forward_dev(net, S, dx, dg, g_scale, false));
+    if (dev) {
+        HIP_TRY(ctx, hipMemcpyAsync(out, net->out.ptr, n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        return CTR_OK;
+    }
+    return finish_out(ctx, false, (const float *)net->out.ptr, out, n);
+}
+
+ctr_status ctr_cnn_train_step(ctr_cnn *net, int n_patch, int nx, int ny, int nz, const float *x, const float *g,
+                              const float *g_scale, const float *target, int border, float step_lr, double *loss,
+                              float *grad, unsigned flags)
+{
+    TRY(need_net(net));
+    ctr_ctx *ctx = net->ctx;
+    hipStream_t st = ctx->stream;
+    TRY(check_cnn_shape(ctx, nx, ny, nz));
+    if (n_patch < 1 || n_patch > 4096) return fail(ctx, CTR_EINVAL, "n_patch must be in 1..4096");
+    if (!x || !g || !g_scale || !target) return fail(ctx, CTR_EINVAL, "input data pointer is NULL");
+    if (border < 0 || 2 * border >= nx || 2 * border >= ny || 2 * border >= nz)
+        return fail(ctx, CTR_EINVAL, "border leaves no interior voxels");
+    if (!(step_lr >= 0.0f) || !std::isfinite(step_lr)) return fail(ctx, CTR_EINVAL, "step_lr must be >= 0");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const size_t n = (size_t)nx * ny * nz;
+    const CnnShape S{nx, ny, nz};
+    TRY(cnn_reserve(net, n, true));
+    const int nblk = wgrad_blocks(S);
+    TRY(buf_reserve(ctx, net->partial, (size_t)nblk * (32 * 32 * 27 + 32) * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((n + 255) / 256 + 1)));
+    if (!dev) {
+        TRY(buf_reserve(ctx, net->xin, n * sizeof(float)));
+        TRY(buf_reserve(ctx, net->gin, n * sizeof(float)));
+        TRY(buf_reserve(ctx, net->tin, n * sizeof(float)));
+    }
+    TRY(check_launch(ctx, launch_fill((float *)net->grad.ptr, 0.0f, kCnnParams, st), "fill"));
+    const double interior = (double)(nx - 2 * border) * (ny - 2 * border) * (nz - 2 * border) * n_patch;
+    const float inv_count = (float)(1.0 / interior);
+    for (int i = 0; i < n_patch; ++i) {
+        const float *px = x + (size_t)i * n, *pg = g + (size_t)i * n, *pt = target + (size_t)i * n;
+        if (!dev) {
+            HIP_TRY(ctx, hipMemcpyAsync(net->xin.ptr, px, n * sizeof(float), hipMemcpyHostToDevice, st));
+            HIP_TRY(ctx, hipMemcpyAsync(net->gin.ptr, pg, n * sizeof(float), hipMemcpyHostToDevice, st));
+            HIP_TRY(ctx, hipMemcpyAsync(net->tin.ptr, pt, n * sizeof(float), hipMemcpyHostToDevice, st));
+            px = (const float *)net->xin.ptr;
+            pg = (const float *)net->gin.ptr;
+            pt = (const float *)net->tin.ptr;
+        }
+        if (!std::isfinite(g_scale[i])) return fail(ctx, CTR_EINVAL, "g_scale[%d] is not finite", i);
+        TRY(cnn_forward_dev(net, S, px, pg, g_scale[i], true));
+        TRY(cnn_backward_dev(net, S, pt, border, inv_count, i));
+        if (!dev) HIP_TRY(ctx, hipStreamSynchronize(st));   // the staging buffers are reused by the next patch
+    }
+    std::vector<double> ls(2 * (size_t)n_patch);
+    HIP_TRY(ctx, hipMemcpyAsync(ls.data(), net->lsum.ptr, sizeof(double) * ls.size(), hipMemcpyDeviceToHost, st));
+    if (grad) HIP_TRY(ctx, hipMemcpyAsync(grad, net->grad.ptr, kCnnParams * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    double l = 0.0;
+    for (int i = 0; i < n_patch; ++i) l += ls[2 * i];
+    if (loss) *loss = l / interior;
+    if (step_lr > 0.0f) {
+        const float b1 = 0.9f, b2 = 0.999f;
+        ++net->step;
+        const double lr_t = step_lr * std::sqrt(1.0 - std::pow(b2, net->step)) / (1.0 - std::pow(b1, net->step));
+        TRY(check_launch(ctx, launch_adam((float *)net->params.ptr, (float *)net->m.ptr, (float *)net->v.ptr,
+                                          (const float *)net->grad.ptr, (int)kCnnParams, (float)lr_t, b1, b2, 1e-8f, st),
+                         "adam"));
+        TRY(cnn_pack(net));
+    }
+    return CTR_OK;
+}
+
+ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_in, const float *x_in, float *s_out,
+                           float lambda, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    if (!std::isfinite(lambda)) return fail(ctx, CTR_EINVAL, "lambda must be finite");
+    TRY(cone_tables(ctx, g));
+    hipStream_t st = ctx->stream;
+    const RayParams R = cone_ray_params(ctx, g);
+    const BpParams P = cone_bp_params(ctx, g, 0);
+    TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+    const BpPlan AL = ctx->foot_adj.plan;
+    const int4 *AF = (const int4 *)ctx->foot_adj.buf.ptr;
+    const size_t nvox = nvox3(g), nray = nray3(g);
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const float *b, *x;
+    float *s;
+    TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
+    TRY(stage_in(ctx, dev, kWsTmp, x_in, nvox, &x));
+    TRY(stage_out(ctx, dev, false, kWsOut, s_out, nvox, &s));
+    TRY(buf_reserve(ctx, ctx->ws[kWsVinv], nvox * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsG], nvox * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
+    float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr, *r = (float *)ctx->ws[kWsR].ptr;
+    double *part = (double *)ctx->ws[kWsPart].ptr;
+    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, st), "SIRT column sums"));
+    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
+    int nblk = 0;
+    TRY(check_launch(ctx, launch_ray_fp_residual(R, x, b, r, part, &nblk, st), "SIRT residual"));
+    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
+    TRY(check_launch(ctx, launch_sirt_step(vinv, gb, lambda, nvox, s, st), "SIRT step"));
+    return finish_out(ctx, dev, s, s_out, nvox);
+}
+
+ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_in, float *x_in, ctr_cnn *net,
+                              const ctr_learned_opts *o, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(need_net(net));
+    if (net->ctx != ctx) return fail(ctx, CTR_EINVAL, "network belongs to another context");
+    if (!o) return fail(ctx, CTR_EINVAL, "options are NULL");
+    if (o->n_steps < 0) return fail(ctx, CTR_EINVAL, "n_steps must be >= 0");
+    if (!std::isfinite(o->lambda)) return fail(ctx, CTR_EINVAL, "lambda must be finite");
+    if (std::isnan(o->min_val) || std::isnan(o->max_val) || o->min_val > o->max_val)
+        return fail(ctx, CTR_EINVAL, "clamp bounds must satisfy min_val <= max_val");
+    TRY(cone_tables(ctx, g));
+    hipStream_t st = ctx->stream;
+    const RayParams R = cone_ray_params(ctx, g);
+    const BpParams P = cone_bp_params(ctx, g, 0);
+    TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+    const BpPlan AL = ctx->foot_adj.plan;
+    const int4 *AF = (const int4 *)ctx->foot_adj.buf.ptr;
+    const size_t nvox = nvox3(g), nray = nray3(g);
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const float *b;
+    float *x;
+    TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
+    TRY(stage_out(ctx, dev, o->use_x0 != 0, kWsOut, x_in, nvox, &x));
+    for (int s : {kWsVinv, kWsG, kWsS, kWsD, kWsX2}) TRY(buf_reserve(ctx, ctx->ws[s], nvox * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * 4));
+    TRY(cnn_reserve(net, nvox, false));
+    float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr, *sb = (float *)ctx->ws[kWsS].ptr;
+    float *r = (float *)ctx->ws[kWsR].ptr, *alt = (float *)ctx->ws[kWsX2].ptr;
+    double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
+    if (!o->use_x0) TRY(check_launch(ctx, launch_fill(x, 0.0f, nvox, st), "fill"));
+    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, st), "SIRT column sums"));
+    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
+    // reduction of a vector to (sum of squares, sum) read back to the host
+    auto host_sums = [&](double *out2) -> ctr_status {
+        HIP_TRY(ctx, hipMemcpyAsync(out2, sums, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        return CTR_OK;
+    };
+    double h[2];
+    int nb = 0;
+    TRY(check_launch(ctx, launch_sum_sq_sum(b, nray, part, &nb, st), "norm"));
+    TRY(check_launch(ctx, launch_reduce_partials(part, nb, sums, st), "reduce"));
+    TRY(host_sums(h));
+    const double bn = std::sqrt(h[0]) > 0 ? std::sqrt(h[0]) : 1.0;
+    // residual pass: r = W (b - A xx); returns ||b - A xx|| / ||b||
+    auto residual = [&](const float *xx, double *res) -> ctr_status {
+        int nblk = 0;
+        TRY(check_launch(ctx, launch_ray_fp_residual(R, xx, b, r, part, &nblk, st), "SIRT residual"));
+        TRY(check_launch(ctx, launch_reduce_partials(part, nblk, sums, st), "reduce"));
+        double hh[2];
+        TRY(host_sums(hh));
+        *res = std::sqrt(hh[0]) / bn;
+        return CTR_OK;
+    };
+    float *cur = x;   // buffer holding x_k
+    double res_k = 0.0;
+    TRY(residual(cur, &res_k));
+    if (o->residual_l2) o->residual_l2[0] = res_k;
+    const CnnShape S{g->nt, g->ns, g->nz};
+    for (int k = 0; k < o->n_steps; ++k) {
+        // plain step s = lambda V A' r
+        TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
+        TRY(check_launch(ctx, launch_sirt_step(vinv, gb, o->lambda, nvox, sb, st), "SIRT step"));
+        TRY(check_launch(ctx, launch_sum_sq_sum(sb, nvox, part, &nb, st), "step norm"));
+        TRY(check_launch(ctx, launch_reduce_partials(part, nb, sums, st), "reduce"));
+        TRY(host_sums(h));
+        const double rms = std::sqrt(h[0] / (double)nvox);
+        float *cand = (cur == x) ? alt : x;
+        float scale = 0.0f;
+        if (rms > 0.0 && std::isfinite(rms)) {
+            TRY(cnn_forward_dev(net, S, cur, sb, (float)(1.0 / rms), false));
+            scale = (float)rms;
+        }
+        TRY(check_launch(ctx, launch_learned_candidate(cur, sb, (const float *)net->out.ptr, scale, o->min_val,
+                                                       o->max_val, nvox, cand, st), "learned step"));
+        double res_c = 0.0;
+        TRY(residual(cand, &res_c));
+        int acc = 1;
+        if (o->judge_vs_plain && scale > 0.0f) {
+            // the plain step's residual (into the spare volume, then r is recomputed for the winner)
+            float *plain = (float *)ctx->ws[kWsD].ptr;
+            TRY(check_launch(ctx, launch_learned_candidate(cur, sb, sb, 0.0f, o->min_val, o->max_val, nvox, plain, st),
+                             "plain step"));
+            double res_p = 0.0;
+            TRY(residual(plain, &res_p));
+            if (res_c < res_p) {
+                TRY(residual(cand, &res_c));   // r of the kept iterate for the next step
+            } else {
+                acc = 0;
+                HIP_TRY(ctx, hipMemcpyAsync(cand, plain, nvox * sizeof(float), hipMemcpyDeviceToDevice, st));
+                res_c = res_p;                  // r already belongs to the plain iterate
+            }
+        } else if (!(res_c < res_k)) {
+            // fall back to the plain step
+            acc = 0;
+            TRY(check_launch(ctx, launch_learned_candidate(cur, sb, sb, 0.0f, o->min_val, o->max_val, nvox, cand, st),
+                             "plain step"));
+            TRY(residual(cand, &res_c));
+        }
+        cur = cand;
+        res_k = res_c;
+        if (o->residual_l2) o->residual_l2[k + 1] = res_k;
+        if (o->accepted) o->accepted[k] = acc;
+    }
+    if (cur != x) HIP_TRY(ctx, hipMemcpyAsync(x, cur, nvox * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return finish_out(ctx, dev, x, x_in, nvox);
 }
 
 }  // extern "C"

@@ -1,0 +1,457 @@
+// k_cnn.hip - 3-D CNN of the learned SIRT update on gfx950.
+//
+// Network: conv3d 2->32, ReLU, 32->32, ReLU, 32->32, ReLU, 32->1; 3x3x3
+// kernels, zero padding 1.  Activations are BF16, channels-last
+// ([z][y][x][c], c fastest), weights BF16, accumulation FP32.
+//
+// Convolution = implicit GEMM on v_mfma_f32_32x32x16_bf16:
+//   D[co][voxel] += W[co][ci] * act[ci][voxel]   (M = 32 output channels,
+//   N = 32 voxels along x, K = 16 input channels of one tap)
+// A (weights, 32 x 16 per tap and channel chunk) is staged in LDS once per
+// block; B (activations) is read straight from global memory: lane l takes
+// 8 consecutive channels (16 bytes) of voxel x0 + (l & 31).  A wave owns
+// 32 x 4 voxels (4 rows in y) and loads the 6 rows y0-1 .. y0+4 once per
+// (dz, dx, chunk), reusing them for the 3 dy taps.
+//
+// The backward pass w.r.t. the input is the same convolution with the taps
+// flipped and the channel roles swapped; its epilogue applies the ReLU
+// derivative of the layer below.  The weight gradient is a VALU reduction
+// over LDS-staged tiles (per-block partial sums, reduced deterministically).
+#include "ctr_cnn.h"
+
+namespace ctr {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTaps = 27;
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+
+// conv3d, CIN = 16 or 32 input channels (zero padded), 32 output channels
+// (zero padded).  Epilogues: see CnnEpi.
+template <int CIN, int EPI>
+__global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
+                                                const float *__restrict__ bias, const __bf16 *__restrict__ mask,
+                                                void *__restrict__ out, CnnShape S)
+{
+    constexpr int NQ = CIN / 16;
+    constexpr int NW = kTaps * NQ * 32 * 2;   // bf16x8 vectors of the A layout [tap][q][co][16]
+    __shared__ bf16x8 wl[NW];
+    for (int i = threadIdx.x; i < NW; i += 256) wl[i] = reinterpret_cast<const bf16x8 *>(wts)[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 31, kh = lane >> 5;
+    const int tiles_x = (S.nx + 31) / 32, tiles_y = (S.ny + 15) / 16;
+    const int ntiles = tiles_x * tiles_y * S.nz;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int tx = tile % tiles_x, rest = tile / tiles_x;
+        const int ty = rest % tiles_y, z = rest / tiles_y;
+        const int x = tx * 32 + col, y0 = ty * 16 + wave * 4;
+        f32x16 acc[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[g][r] = 0.0f;
+
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int zz = z + dz;
+            const bool zok = zz >= 0 && zz < S.nz;
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx;
+                const bool zxok = zok && xx >= 0 && xx < S.nx;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    bf16x8 rows[6];
+#pragma unroll
+                    for (int rr = 0; rr < 6; ++rr) {
+                        const int yy = y0 - 1 + rr;
+                        bf16x8 v;
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+                        if (zxok && yy >= 0 && yy < S.ny) {
+                            const size_t vox = ((size_t)zz * S.ny + yy) * S.nx + xx;
+                            v = *reinterpret_cast<const bf16x8 *>(in + vox * CIN + q * 16 + kh * 8);
+                        }
+                        rows[rr] = v;
+                    }
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                        const bf16x8 a = wl[((t * NQ + q) * 32 + col) * 2 + kh];
+#pragma unroll
+                        for (int g = 0; g < 4; ++g)
+                            acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, rows[g + dy + 1], acc[g], 0, 0, 0);
+                    }
+                }
+            }
+        }
+
+        // lane holds D[co][voxel col] for co = 8 s + 4 kh + u (s, u = 0..3)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int y = y0 + g;
+            if (x >= S.nx || y >= S.ny) continue;
+            const size_t vox = ((size_t)z * S.ny + y) * S.nx + x;
+            if (EPI == kEpiLinearF32) {
+                if (kh == 0) reinterpret_cast<float *>(out)[vox] = acc[g][0] + bias[0];
+                continue;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c0 = 8 * s + 4 * kh;
+                bf16x4 o;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float v = acc[g][4 * s + u];
+                    if (EPI == kEpiBiasRelu) {
+                        v = fmaxf(v + bias[c0 + u], 0.0f);
+                    } else if (EPI == kEpiMask) {
+                        v = bf2f(mask[vox * 32 + c0 + u]) > 0.0f ? v : 0.0f;
+                    }
+                    o[u] = (__bf16)v;
+                }
+                *reinterpret_cast<bf16x4 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+            }
+        }
+    }
+}
+
+// Weight and bias gradient of one layer over a volume:
+//   dW[co][ci][t] += sum_v delta[v][co] * act[v + off(t)][ci],  db[co] += sum_v delta[v][co]
+// Tiles of 32 x 8 voxels of one slice; the activations with their halo
+// (34 x 10 x 3) and the deltas are staged in LDS.  Thread (co, ci block of
+// 4) accumulates 4 x 27 weights over every tile it visits; each block
+// writes its partial sums, reduced afterwards by k_reduce_grad.
+template <int CI, int CO>
+__global__ __launch_bounds__(256) void k_conv3d_wgrad(const __bf16 *__restrict__ act, const __bf16 *__restrict__ delta,
+                                                      float *__restrict__ partial, CnnShape S)
+{
+    constexpr int HX = 34, HY = 10, HZ = 3, HV = HX * HY * HZ;
+    constexpr int CB = CI / 4;                       // ci blocks
+    constexpr int NT = CO * CB;                      // active threads
+    static_assert(NT <= 256, "too many weight blocks");
+    __shared__ __bf16 sa[HV * CI];
+    __shared__ __bf16 sd[256 * CO];
+    const int tid = threadIdx.x;
+    const int co = tid % CO, cb = tid / CO;
+    float acc[4][kTaps], accb = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int t = 0; t < kTaps; ++t) acc[c][t] = 0.0f;
+
+    const int tiles_x = (S.nx + 31) / 32, tiles_y = (S.ny + 7) / 8;
+    const int ntiles = tiles_x * tiles_y * S.nz;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int tx = tile % tiles_x, rest = tile / tiles_x;
+        const int ty = rest % tiles_y, z = rest / tiles_y;
+        const int x0 = tx * 32, y0 = ty * 8;
+        __syncthreads();
+        // stage activations with halo (zero outside the volume), 8 channels per step
+        for (int e = tid; e < HV * (CI / 8); e += 256) {
+            const int hv = e / (CI / 8), c8 = e % (CI / 8);
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int xx = x0 - 1 + hx, yy = y0 - 1 + hy, zz = z - 1 + hz;
+            bf16x8 v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (__bf16)0.0f;
+            if (xx >= 0 && xx < S.nx && yy >= 0 && yy < S.ny && zz >= 0 && zz < S.nz)
+                v = *reinterpret_cast<const bf16x8 *>(act + (((size_t)zz * S.ny + yy) * S.nx + xx) * CI + c8 * 8);
+            *reinterpret_cast<bf16x8 *>(sa + hv * CI + c8 * 8) = v;
+        }
+        // stage deltas of the 32 x 8 tile (zero outside)
+        for (int e = tid; e < 256 * (CO / 8); e += 256) {
+            const int v = e / (CO / 8), c8 = e % (CO / 8);
+            const int xx = x0 + (v & 31), yy = y0 + (v >> 5);
+            bf16x8 d;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) d[k] = (__bf16)0.0f;
+            if (xx < S.nx && yy < S.ny)
+                d = *reinterpret_cast<const bf16x8 *>(delta + (((size_t)z * S.ny + yy) * S.nx + xx) * CO + c8 * 8);
+            *reinterpret_cast<bf16x8 *>(sd + v * CO + c8 * 8) = d;
+        }
+        __syncthreads();
+        if (tid < NT) {
+            for (int v = 0; v < 256; ++v) {
+                const float dv = bf2f(sd[v * CO + co]);
+                accb += dv;
+                const int vx = v & 31, vy = v >> 5;
+#pragma unroll
+                for (int t = 0; t < kTaps; ++t) {
+                    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;   // halo offsets 0..2
+                    const int hv = (dz * HY + vy + dy) * HX + vx + dx;
+                    const bf16x4 a = *reinterpret_cast<const bf16x4 *>(sa + hv * CI + cb * 4);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[c][t] = fmaf(dv, bf2f(a[c]), acc[c][t]);
+                }
+            }
+        }
+    }
+    // partial layout per block: [CO][CI][27] weights then [CO] bias
+    constexpr int NP = CO * CI * kTaps + CO;
+    float *P = partial + (size_t)blockIdx.x * NP;
+    if (tid < NT) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int t = 0; t < kTaps; ++t) P[(co * CI + cb * 4 + c) * kTaps + t] = acc[c][t];
+        if (cb == 0) P[CO * CI * kTaps + co] = accb;
+    }
+}
+
+// Deterministic reduction of per-block gradient partials into the flat
+// parameter layout of one layer ([co][ci][27] weights, then [co] bias):
+// partials use padded channel counts ([co_pad][ci_pad][27] + [co_pad]).
+__global__ __launch_bounds__(256) void k_reduce_grad(const float *__restrict__ partial, int nblk, int ci_pad,
+                                                     int co_pad, int ci_real, int co_real, float *__restrict__ out,
+                                                     int accumulate)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int nw = co_real * ci_real * kTaps;
+    if (i >= nw + co_real) return;
+    int src;
+    if (i < nw) {
+        const int co = i / (ci_real * kTaps), rem = i % (ci_real * kTaps);
+        src = (co * ci_pad + rem / kTaps) * kTaps + rem % kTaps;
+    } else {
+        src = co_pad * ci_pad * kTaps + (i - nw);
+    }
+    const size_t np = (size_t)co_pad * ci_pad * kTaps + co_pad;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * np + src];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+// SIRT step s = lambda * vinv * g.
+__global__ __launch_bounds__(256) void k_sirt_step(const float *__restrict__ vinv, const float *__restrict__ g,
+                                                   float lambda, size_t n, float *__restrict__ s)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) s[i] = lambda * vinv[i] * g[i];
+}
+
+// FP32 parameters (PyTorch conv3d layout [co][ci][kz][ky][kx] + bias) ->
+// BF16 MFMA A layouts: forward [tap][q][32 co][16 ci] and, for the input
+// gradient, the flipped/transposed convolution [tap'][q'][32 ci][16 co].
+__global__ __launch_bounds__(256) void k_pack_weights(const float *__restrict__ w, int ci_real, int co_real, int cin_pad,
+                                                      __bf16 *__restrict__ fwd, int cout_pad_t, __bf16 *__restrict__ bwd)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    // forward layout
+    const int nq = cin_pad / 16, nf = kTaps * nq * 32 * 16;
+    if (i < nf) {
+        const int k = i % 16, co = (i / 16) % 32, q = (i / 512) % nq, t = i / (512 * nq);
+        const int ci = q * 16 + k;
+        float v = 0.0f;
+        if (co < co_real && ci < ci_real) v = w[((size_t)co * ci_real + ci) * kTaps + t];
+        fwd[i] = (__bf16)v;
+    }
+    if (!bwd) return;
+    // backward-data layout: input channels = forward outputs (padded to cout_pad_t)
+    const int nqb = cout_pad_t / 16, nb = kTaps * nqb * 32 * 16;
+    if (i < nb) {
+        const int k = i % 16, m = (i / 16) % 32, q = (i / 512) % nqb, t = i / (512 * nqb);
+        const int co = q * 16 + k, ci = m;   // forward output channel, forward input channel
+        float v = 0.0f;
+        if (co < co_real && ci < ci_real) v = w[((size_t)co * ci_real + ci) * kTaps + (kTaps - 1 - t)];
+        bwd[i] = (__bf16)v;
+    }
+}
+
+// Adam (decoupled from the gradient scale; bias-corrected step size given).
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                                              const float *__restrict__ g, int n, float lr_t, float b1, float b2,
+                                              float eps)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
+// Network input: channel 0 = x, channel 1 = g * g_scale, channels 2..15 = 0.
+__global__ __launch_bounds__(256) void k_pack_input(const float *__restrict__ x, const float *__restrict__ g,
+                                                    float g_scale, size_t n, __bf16 *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    bf16x8 a, b;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        a[k] = (__bf16)0.0f;
+        b[k] = (__bf16)0.0f;
+    }
+    a[0] = (__bf16)x[i];
+    a[1] = (__bf16)(g[i] * g_scale);
+    reinterpret_cast<bf16x8 *>(out)[2 * i] = a;
+    reinterpret_cast<bf16x8 *>(out)[2 * i + 1] = b;
+}
+
+// Masked mean-squared error of pred against target over the interior of
+// the volume (voxels at least `border` from every face) and its gradient,
+// written as the BF16 delta of the last layer (channel 0 of 16).
+// partial[block] = (sum of squared errors, 0); inv_count = 1 / (interior voxels of all patches).
+__global__ __launch_bounds__(256) void k_mse_grad(const float *__restrict__ pred, const float *__restrict__ target,
+                                                  CnnShape S, int border, float inv_count,
+                                                  __bf16 *__restrict__ delta, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const size_t n = (size_t)S.nx * S.ny * S.nz;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double e2 = 0.0;
+    if (i < n) {
+        const int x = (int)(i % S.nx), y = (int)((i / S.nx) % S.ny), z = (int)(i / ((size_t)S.nx * S.ny));
+        const bool in = x >= border && x < S.nx - border && y >= border && y < S.ny - border && z >= border &&
+                        z < S.nz - border;
+        const float e = in ? pred[i] - target[i] : 0.0f;
+        e2 = (double)e * e;
+        bf16x8 d0, d1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            d0[k] = (__bf16)0.0f;
+            d1[k] = (__bf16)0.0f;
+        }
+        d0[0] = (__bf16)(2.0f * e * inv_count);
+        reinterpret_cast<bf16x8 *>(delta)[2 * i] = d0;
+        reinterpret_cast<bf16x8 *>(delta)[2 * i + 1] = d1;
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = red[0];
+        partial[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+// Learned-step candidate: x' = clamp(x + step + out_scale * d, lo, hi).
+__global__ __launch_bounds__(256) void k_learned_candidate(const float *__restrict__ x, const float *__restrict__ step,
+                                                           const float *__restrict__ d, float out_scale, float lo,
+                                                           float hi, size_t n, float *__restrict__ xn)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    xn[i] = fminf(fmaxf(x[i] + step[i] + out_scale * d[i], lo), hi);
+}
+
+int conv_grid(const CnnShape &S)
+{
+    const long tiles = (long)((S.nx + 31) / 32) * ((S.ny + 15) / 16) * S.nz;
+    return (int)(tiles < 2048 ? tiles : 2048);
+}
+
+}  // namespace
+
+hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w, const float *bias, const __bf16 *mask,
+                         void *out, const CnnShape &S, hipStream_t st)
+{
+    const int grid = conv_grid(S);
+#define CTR_CONV(CIN, EPI)                                                                                          \
+    hipLaunchKernelGGL((k_conv3d<CIN, EPI>), dim3(grid), dim3(256), 0, st, in, w, bias, mask, out, S)
+    if (cin_pad == 16) {
+        if (epi == kEpiBiasRelu) CTR_CONV(16, kEpiBiasRelu);
+        else if (epi == kEpiLinearF32) CTR_CONV(16, kEpiLinearF32);
+        else if (epi == kEpiMask) CTR_CONV(16, kEpiMask);
+        else return hipErrorInvalidValue;
+    } else if (cin_pad == 32) {
+        if (epi == kEpiBiasRelu) CTR_CONV(32, kEpiBiasRelu);
+        else if (epi == kEpiLinearF32) CTR_CONV(32, kEpiLinearF32);
+        else if (epi == kEpiMask) CTR_CONV(32, kEpiMask);
+        else return hipErrorInvalidValue;
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef CTR_CONV
+    return hipGetLastError();
+}
+
+int wgrad_blocks(const CnnShape &S)
+{
+    const long tiles = (long)((S.nx + 31) / 32) * ((S.ny + 7) / 8) * S.nz;
+    return (int)(tiles < 512 ? tiles : 512);
+}
+
+hipError_t launch_conv3d_wgrad(int ci, int co, const __bf16 *act, const __bf16 *delta, float *partial, int nblk,
+                               const CnnShape &S, hipStream_t st)
+{
+    if (ci == 16 && co == 32)
+        hipLaunchKernelGGL((k_conv3d_wgrad<16, 32>), dim3(nblk), dim3(256), 0, st, act, delta, partial, S);
+    else if (ci == 32 && co == 32)
+        hipLaunchKernelGGL((k_conv3d_wgrad<32, 32>), dim3(nblk), dim3(256), 0, st, act, delta, partial, S);
+    else if (ci == 32 && co == 16)
+        hipLaunchKernelGGL((k_conv3d_wgrad<32, 16>), dim3(nblk), dim3(256), 0, st, act, delta, partial, S);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_grad(const float *partial, int nblk, int ci_pad, int co_pad, int ci_real, int co_real,
+                              float *out, int accumulate, hipStream_t st)
+{
+    const int n = co_real * ci_real * kTaps + co_real;
+    hipLaunchKernelGGL(k_reduce_grad, dim3((n + 255) / 256), dim3(256), 0, st, partial, nblk, ci_pad, co_pad, ci_real,
+                       co_real, out, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_sirt_step(const float *vinv, const float *g, float lambda, size_t n, float *s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sirt_step, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vinv, g, lambda, n, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_weights(const float *w, int ci_real, int co_real, int cin_pad, __bf16 *fwd, int cout_pad_t,
+                               __bf16 *bwd, hipStream_t st)
+{
+    const int nf = kTaps * (cin_pad / 16) * 512, nb = bwd ? kTaps * (cout_pad_t / 16) * 512 : 0;
+    const int n = nf > nb ? nf : nb;
+    hipLaunchKernelGGL(k_pack_weights, dim3((n + 255) / 256), dim3(256), 0, st, w, ci_real, co_real, cin_pad, fwd,
+                       cout_pad_t, bwd);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, int n, float lr_t, float b1, float b2, float eps,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, m, v, g, n, lr_t, b1, b2, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_input(const float *x, const float *g, float g_scale, size_t n, __bf16 *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, g, g_scale, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mse_grad(const float *pred, const float *target, const CnnShape &S, int border, float inv_count,
+                           __bf16 *delta, double *partial, int *nblocks, hipStream_t st)
+{
+    const size_t n = (size_t)S.nx * S.ny * S.nz;
+    const int nb = (int)((n + 255) / 256);
+    *nblocks = nb;
+    hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, st, pred, target, S, border, inv_count, delta, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_learned_candidate(const float *x, const float *step, const float *d, float out_scale, float lo,
+                                    float hi, size_t n, float *xn, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_learned_candidate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, step, d,
+                       out_scale, lo, hi, n, xn);
+    return hipGetLastError();
+}
+
+}  // namespace ctr

@@ -27,7 +27,13 @@ EXPORTS = [
     "ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_cone_filter", "ctr_fdk_bp", "ctr_fdk",
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
     "ctr_sirt3d", "ctr_sirt2d",
+    "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
+    "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d",
 ]
+
+CNN_NUM_PARAMS = 57985
+# (out, in) channels of the four 3x3x3 layers; weights then bias per layer in the flat vector
+CNN_LAYERS = ((32, 2), (32, 32), (32, 32), (1, 32))
 
 
 class CtrError(RuntimeError):
@@ -53,6 +59,12 @@ class SirtOptsC(ctypes.Structure):
     _fields_ = [("n_iter", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
                 ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
                 ("residual_l2", ctypes.c_void_p), ("residual_l1", ctypes.c_void_p)]
+
+
+class LearnedOptsC(ctypes.Structure):
+    _fields_ = [("n_steps", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
+                ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
+                ("residual_l2", ctypes.c_void_p), ("accepted", ctypes.c_void_p), ("judge_vs_plain", ctypes.c_int)]
 
 
 _lib = None
@@ -92,6 +104,21 @@ def lib():
         for name in ("ctr_sirt3d", "ctr_sirt2d"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_uint]
+        L.ctr_cnn_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.ctr_cnn_destroy.argtypes = [ctypes.c_void_p]
+        L.ctr_cnn_destroy.restype = None
+        L.ctr_cnn_set_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.ctr_cnn_get_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.ctr_cnn_forward.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_uint]
+        L.ctr_cnn_train_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint]
+        L.ctr_sirt3d_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_float, ctypes.c_uint]
+        L.ctr_learned_sirt3d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
         _lib = L
     return _lib
 
@@ -294,6 +321,137 @@ class Context:
 
     def sirt2d(self, g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None, residuals=True, flags=0):
         return self._sirt(lib().ctr_sirt2d, g, b, n_iter, lam, lo, hi, x0, residuals, flags)
+
+    # -- learned SIRT -------------------------------------------------------------
+    def sirt3d_step(self, g, b, x, lam=1.0, out=None):
+        """The plain SIRT step s = lam V A'(W (b - A x)) at x."""
+        if _is_device(b):
+            import torch
+            self._bind_torch_stream()
+            s = out if out is not None else torch.empty(g.nvox, dtype=torch.float32, device=b.device)
+            self._check(lib().ctr_sirt3d_step(self.h, ctypes.byref(g.c), b.data_ptr(), x.data_ptr(), s.data_ptr(),
+                                              float(lam), DEVICE_PTRS))
+            return s
+        bb, xx = _f32(b), _f32(x)
+        s = np.zeros(g.nvox, dtype=np.float32)
+        self._check(lib().ctr_sirt3d_step(self.h, ctypes.byref(g.c), bb.ctypes.data, xx.ctypes.data, s.ctypes.data,
+                                          float(lam), 0))
+        return s
+
+    def learned_sirt3d(self, g, b, net, n_steps, lam=1.0, lo=0.0, hi=np.inf, x0=None, judge_vs_plain=False, flags=0):
+        """Residual-judged learned SIRT; returns (x, residual_l2[n_steps+1], accepted[n_steps])."""
+        r2 = np.zeros(n_steps + 1)
+        acc = np.zeros(max(n_steps, 1), dtype=np.int32)
+        opts = LearnedOptsC(int(n_steps), float(lam), float(lo), float(hi), int(x0 is not None),
+                            r2.ctypes.data, acc.ctypes.data, int(bool(judge_vs_plain)))
+        fn = lib().ctr_learned_sirt3d
+        if _is_device(b):
+            import torch
+            self._bind_torch_stream()
+            x = x0 if x0 is not None else torch.zeros(g.nvox, dtype=torch.float32, device=b.device)
+            self._check(fn(self.h, ctypes.byref(g.c), b.data_ptr(), x.data_ptr(), net.h, ctypes.byref(opts),
+                           DEVICE_PTRS | flags))
+        else:
+            bb = _f32(b)
+            x = _f32(x0).copy() if x0 is not None else np.zeros(g.nvox, dtype=np.float32)
+            self._check(fn(self.h, ctypes.byref(g.c), bb.ctypes.data, x.ctypes.data, net.h, ctypes.byref(opts),
+                           flags))
+        return x, r2, acc[:n_steps]
+
+
+def cnn_init_params(seed=0, last_scale=0.0):
+    """He-initialised parameters (zero biases); the last layer is scaled by
+    last_scale (0: the network starts as the plain SIRT step)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for li, (co, ci) in enumerate(CNN_LAYERS):
+        std = np.sqrt(2.0 / (ci * 27))
+        w = rng.standard_normal((co, ci, 3, 3, 3)) * std
+        if li == len(CNN_LAYERS) - 1:
+            w *= last_scale
+        out += [w.ravel(), np.zeros(co)]
+    p = np.concatenate(out).astype(np.float32)
+    assert p.size == CNN_NUM_PARAMS
+    return p
+
+
+def cnn_split_params(p):
+    """Flat parameters -> [(W[co][ci][3][3][3], b[co]), ...] for the four layers."""
+    out, o = [], 0
+    for co, ci in CNN_LAYERS:
+        nw = co * ci * 27
+        out.append((np.asarray(p[o:o + nw]).reshape(co, ci, 3, 3, 3), np.asarray(p[o + nw:o + nw + co])))
+        o += nw + co
+    return out
+
+
+class Cnn:
+    """The learned-update network on a context (ctr_cnn)."""
+
+    def __init__(self, ctx, params=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._check(lib().ctr_cnn_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if self.h:
+            lib().ctr_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.ctx._check(lib().ctr_cnn_set_params(self.h, p.ctypes.data, p.size))
+
+    def get_params(self):
+        p = np.zeros(CNN_NUM_PARAMS, dtype=np.float32)
+        self.ctx._check(lib().ctr_cnn_get_params(self.h, p.ctypes.data, p.size))
+        return p
+
+    def forward(self, shape, x, g, g_scale=1.0):
+        """shape = (nx, ny, nz); x, g: [nz][ny][nx] numpy or CUDA tensors."""
+        nx, ny, nz = shape
+        n = nx * ny * nz
+        if _is_device(x):
+            import torch
+            self.ctx._bind_torch_stream()
+            out = torch.empty(n, dtype=torch.float32, device=x.device)
+            self.ctx._check(lib().ctr_cnn_forward(self.h, nx, ny, nz, x.data_ptr(), g.data_ptr(), float(g_scale),
+                                                  out.data_ptr(), DEVICE_PTRS))
+            return out
+        xx, gg = _f32(x), _f32(g)
+        out = np.zeros(n, dtype=np.float32)
+        self.ctx._check(lib().ctr_cnn_forward(self.h, nx, ny, nz, xx.ctypes.data, gg.ctypes.data, float(g_scale),
+                                              out.ctypes.data, 0))
+        return out
+
+    def train_step(self, shape, x, g, g_scale, target, border=0, lr=0.0, want_grad=False):
+        """Loss (and gradient) of a batch of patches [n][nz][ny][nx]; one Adam step when lr > 0."""
+        nx, ny, nz = shape
+        gs = np.ascontiguousarray(np.atleast_1d(g_scale), dtype=np.float32)
+        npatch = gs.size
+        loss = ctypes.c_double()
+        grad = np.zeros(CNN_NUM_PARAMS, dtype=np.float32) if want_grad else None
+        gptr = grad.ctypes.data if want_grad else None
+        if _is_device(x):
+            self.ctx._bind_torch_stream()
+            self.ctx._check(lib().ctr_cnn_train_step(self.h, npatch, nx, ny, nz, x.data_ptr(), g.data_ptr(),
+                                                     gs.ctypes.data, target.data_ptr(), int(border), float(lr),
+                                                     ctypes.byref(loss), gptr, DEVICE_PTRS))
+        else:
+            xx, gg, tt = _f32(x), _f32(g), _f32(target)
+            self.ctx._check(lib().ctr_cnn_train_step(self.h, npatch, nx, ny, nz, xx.ctypes.data, gg.ctypes.data,
+                                                     gs.ctypes.data, tt.ctypes.data, int(border), float(lr),
+                                                     ctypes.byref(loss), gptr, 0))
+        return (loss.value, grad) if want_grad else loss.value
 
 
 _default_ctx = None

@@ -198,6 +198,59 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
 ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
                       unsigned flags);
 
+/* ---- learned SIRT update (no counterpart in the reference) -------------
+ *
+ * A 3-D CNN maps (x_k, s_k / rms(s_k)) to a correction d_k, where s_k is the
+ * plain SIRT step lambda V .* A'(W .* (b - A x_k)).  Network: conv3d
+ * 2->32->32->32->1, 3x3x3 kernels, zero padding, ReLU between layers, BF16
+ * weights and activations, FP32 accumulation.  Parameters are FP32 in
+ * PyTorch conv3d order: W1[32][2][3][3][3], b1[32], W2[32][32][3][3][3], b2[32],
+ * W3[32][32][3][3][3], b3[32], W4[1][32][3][3][3], b4[1]. */
+#define CTR_CNN_NUM_PARAMS 57985
+
+typedef struct ctr_cnn ctr_cnn;
+
+ctr_status ctr_cnn_create(ctr_ctx *ctx, ctr_cnn **out);   /* parameters start at zero */
+void ctr_cnn_destroy(ctr_cnn *net);
+ctr_status ctr_cnn_set_params(ctr_cnn *net, const float *params, size_t n);   /* host array, n = CTR_CNN_NUM_PARAMS */
+ctr_status ctr_cnn_get_params(ctr_cnn *net, float *params, size_t n);
+
+/* out = CNN(x, g * g_scale) over an nx x ny x nz volume (x fastest). */
+ctr_status ctr_cnn_forward(ctr_cnn *net, int nx, int ny, int nz, const float *x, const float *g, float g_scale,
+                           float *out, unsigned flags);
+
+/* Mean squared error of CNN(x_i, g_i * g_scale[i]) against target_i over the
+ * voxels at least `border` from every face, averaged over n_patch patches of
+ * nx x ny x nz (x, g, target: [n_patch][nz][ny][nx]; g_scale: host [n_patch]).
+ * Returns the loss and, when grad is not NULL, dloss/dparams (host
+ * [CTR_CNN_NUM_PARAMS]).  With step_lr > 0 it also takes one Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8) with that learning rate. */
+ctr_status ctr_cnn_train_step(ctr_cnn *net, int n_patch, int nx, int ny, int nz, const float *x, const float *g,
+                              const float *g_scale, const float *target, int border, float step_lr, double *loss,
+                              float *grad, unsigned flags);
+
+typedef struct ctr_learned_opts {
+    int n_steps;            /* learned steps */
+    float lambda;           /* relaxation of the plain step */
+    float min_val, max_val; /* clamp of every iterate */
+    int use_x0;             /* nonzero: x holds the starting iterate, else start from 0 */
+    double *residual_l2;    /* optional host [n_steps+1]: ||b - A x_k||_2 / ||b||_2 */
+    int *accepted;          /* optional host [n_steps]: 1 learned step kept, 0 fell back to the plain step */
+    int judge_vs_plain;     /* nonzero: keep the learned step only if its residual is below the plain
+                               step's (one more forward projection per step) */
+} ctr_learned_opts;
+
+/* The plain SIRT step at x: s = lambda * V .* A'(W .* (b - A x)) (the
+ * network's second input; ctr_sirt3d applies clamp(x + s) each iteration). */
+ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, const float *x, float *s,
+                           float lambda, unsigned flags);
+
+/* Learned SIRT: x_{k+1} = clamp(x_k + s_k + rms(s_k) d_k) when that lowers
+ * the data residual ||b - A x|| (below ||b - A x_k||, or below the plain
+ * step's residual with judge_vs_plain), else the plain step clamp(x_k + s_k). */
+ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, float *x, ctr_cnn *net,
+                              const ctr_learned_opts *opts, unsigned flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,184 @@
+"""The learned SIRT update: CNN kernels against a PyTorch fp32 reference of
+the same network, and the residual-judged loop against plain SIRT.
+
+The reference rounds inputs, weights and the ReLU activations to bfloat16
+exactly where the kernels store BF16, and accumulates in fp32 (CPU), so the
+remaining differences are FP32 summation order and BF16 rounding flips of
+individual activations.  The gradient reference differentiates the same
+graph with straight-through rounding; the kernels also round the deltas to
+BF16, hence the looser gradient tolerance.
+"""
+import numpy as np
+import pytest
+
+import ctr
+from ctr import geometry
+from conftest import report
+
+FWD_TOL = 5e-3     # rel-L2 of the network output
+GRAD_TOL = 3e-2    # rel-L2 of each layer's weight gradient
+
+
+def test_param_layout():
+    p = ctr.cnn_init_params(0, 1.0)
+    assert p.size == ctr.CNN_NUM_PARAMS == 57985
+    layers = ctr.cnn_split_params(p)
+    assert [w.shape for w, _ in layers] == [(32, 2, 3, 3, 3), (32, 32, 3, 3, 3), (32, 32, 3, 3, 3), (1, 32, 3, 3, 3)]
+    assert [b.shape for _, b in layers] == [(32,), (32,), (32,), (1,)]
+    assert np.all(ctr.cnn_init_params(0, 0.0)[-865:] == 0)   # last layer off: plain SIRT
+
+
+def _torch_net(p, shape, x, g, gs, ste=False):
+    import torch
+    import torch.nn.functional as F
+    nx, ny, nz = shape
+
+    def bf(t):
+        r = t.to(torch.bfloat16).float()
+        return t + (r - t).detach() if ste else r
+
+    inp = torch.stack([torch.from_numpy(np.asarray(x, np.float32).reshape(nz, ny, nx)),
+                       torch.from_numpy(np.asarray(g, np.float32).reshape(nz, ny, nx)) * gs])[None]
+    a = bf(inp)
+    params = []
+    for i, (w, b) in enumerate(ctr.cnn_split_params(p)):
+        wt = torch.tensor(w, dtype=torch.float32, requires_grad=ste)
+        bt = torch.tensor(b, dtype=torch.float32, requires_grad=ste)
+        params += [wt, bt]
+        z = F.conv3d(a, bf(wt), bt, padding=1)
+        a = bf(torch.relu(z)) if i < 3 else z
+    return a[0, 0], params
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(37, 19, 11), (64, 16, 4), (8, 40, 5)])
+def test_cnn_forward_matches_torch(ctx, shape):
+    rng = np.random.default_rng(1)
+    nx, ny, nz = shape
+    p = ctr.cnn_init_params(3, 1.0)
+    p[1728:1760] = rng.uniform(-0.1, 0.1, 32)            # nonzero biases
+    p[-1] = 0.05
+    x = rng.uniform(0, 1, nx * ny * nz).astype(np.float32)
+    g = rng.standard_normal(nx * ny * nz).astype(np.float32)
+    net = ctr.Cnn(ctx, p)
+    out = net.forward(shape, x, g, 0.7)
+    ref, _ = _torch_net(p, shape, x, g, 0.7)
+    e = _rel(out, ref.detach().numpy())
+    report(f"cnn_forward{shape}", rel_l2=e)
+    assert e < FWD_TOL
+    net.close()
+
+
+@pytest.mark.gpu
+def test_cnn_gradient_matches_torch(ctx):
+    import torch
+    rng = np.random.default_rng(2)
+    shape, border, npatch = (36, 20, 9), 2, 2
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    p = ctr.cnn_init_params(4, 1.0)
+    x = rng.uniform(0, 1, (npatch, n)).astype(np.float32)
+    g = rng.standard_normal((npatch, n)).astype(np.float32)
+    t = rng.standard_normal((npatch, n)).astype(np.float32) * 0.3
+    gs = np.array([0.5, 1.5], np.float32)
+    net = ctr.Cnn(ctx, p)
+    loss, grad = net.train_step(shape, x, g, gs, t, border=border, want_grad=True)
+
+    interior = (nx - 2 * border) * (ny - 2 * border) * (nz - 2 * border) * npatch
+    total, grads = 0.0, None
+    for i in range(npatch):
+        out, params = _torch_net(p, shape, x[i], g[i], float(gs[i]), ste=True)
+        tt = torch.from_numpy(t[i].reshape(nz, ny, nx))
+        d = (out - tt)[border:nz - border, border:ny - border, border:nx - border]
+        li = (d * d).sum() / interior
+        li.backward()
+        total += float(li)
+        gi = np.concatenate([q.grad.numpy().ravel() for q in params])
+        grads = gi if grads is None else grads + gi
+    el = abs(loss - total) / total
+    report("cnn_loss", rel=el)
+    assert el < FWD_TOL
+    o = 0
+    for li, (co, ci) in enumerate(ctr.CNN_LAYERS):
+        nw = co * ci * 27
+        ew = _rel(grad[o:o + nw], grads[o:o + nw])
+        eb = _rel(grad[o + nw:o + nw + co], grads[o + nw:o + nw + co])
+        report(f"cnn_grad[layer {li + 1}]", w_rel_l2=ew, b_rel_l2=eb)
+        assert ew < GRAD_TOL and eb < GRAD_TOL
+        o += nw + co
+    net.close()
+
+
+@pytest.mark.gpu
+def test_cnn_training_lowers_loss(ctx):
+    rng = np.random.default_rng(5)
+    shape = (24, 24, 12)
+    n = 24 * 24 * 12
+    x = rng.uniform(0, 1, (3, n)).astype(np.float32)
+    g = rng.standard_normal((3, n)).astype(np.float32)
+    t = (0.5 * g + 0.2 * x).astype(np.float32)
+    gs = np.ones(3, np.float32)
+    net = ctr.Cnn(ctx, ctr.cnn_init_params(6, 0.1))
+    try:
+        losses = [net.train_step(shape, x, g, gs, t, border=1, lr=3e-3) for _ in range(60)]
+    finally:
+        net.close()
+    report("cnn_train", first=losses[0], last=losses[-1])
+    assert losses[-1] < 0.6 * losses[0]
+    assert all(np.isfinite(losses))
+
+
+def _small_problem():
+    g = geometry.cone_circular(32, 1.0, 24, 48, 48, 1.0, 100.0, 200.0)
+    rng = np.random.default_rng(7)
+    vol = np.zeros((32, 32, 32), np.float32)
+    vol[8:24, 10:22, 6:26] = 1.0
+    vol += rng.uniform(0, 0.2, vol.shape).astype(np.float32)
+    return g, vol.ravel()
+
+
+@pytest.mark.gpu
+def test_learned_sirt_zero_network_is_plain_sirt(ctx):
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    net = ctr.Cnn(ctx, ctr.cnn_init_params(0, 0.0))
+    xl, rl, acc = ctx.learned_sirt3d(g, b, net, 6, lam=1.0)
+    xp, rp, _ = ctx.sirt3d(g, b, 6, lam=1.0)
+    e = _rel(xl, xp)
+    report("learned_sirt[zero net]", x_rel=e, res_rel=_rel(rl, rp))
+    assert e < 1e-5
+    assert _rel(rl, rp) < 1e-5
+    net.close()
+
+
+@pytest.mark.gpu
+def test_learned_sirt_residual_never_rises(ctx):
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    net = ctr.Cnn(ctx, ctr.cnn_init_params(11, 1.0))   # an untrained, large correction
+    x, r, acc = ctx.learned_sirt3d(g, b, net, 8, lam=1.0)
+    report("learned_sirt[random net]", residuals=np.round(r, 5).tolist(), accepted=acc.tolist())
+    assert np.all(np.diff(r) <= 1e-7)
+    assert set(acc.tolist()) <= {0, 1}
+    assert np.all(np.isfinite(x)) and x.min() >= 0.0
+    net.close()
+
+
+@pytest.mark.gpu
+def test_learned_sirt_judged_against_plain_step(ctx):
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    net = ctr.Cnn(ctx, ctr.cnn_init_params(12, 1.0))
+    try:
+        x, r, acc = ctx.learned_sirt3d(g, b, net, 6, lam=1.0, judge_vs_plain=True)
+        _, rp, _ = ctx.sirt3d(g, b, 1, lam=1.0)
+    finally:
+        net.close()
+    report("learned_sirt[judged vs plain]", residuals=np.round(r, 5).tolist(), accepted=acc.tolist())
+    assert r[1] <= rp[1] * (1 + 1e-6)          # step 1 starts from the same iterate as the plain step
+    assert np.all(np.diff(r) <= 1e-7)
