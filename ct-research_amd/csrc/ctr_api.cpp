@@ -22,7 +22,7 @@ namespace {
 
 thread_local std::string g_last_error;
 
-enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsCount };
+enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsCount };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -193,8 +193,9 @@ ctr_status check_cone(const ctr_cone_geom *g, int need_uniform)
 {
     if (!g) return fail(nullptr, CTR_EINVAL, "geometry is NULL");
     if (g->nt < 1 || g->ns < 1 || g->nz < 1) return fail(nullptr, CTR_EINVAL, "volume dimensions must be positive");
-    if ((double)g->nt * g->ns * g->nz > (double)std::numeric_limits<int>::max())
-        return fail(nullptr, CTR_ETOOBIG, "volume has more than 2^31-1 voxels");
+    // the projector reads the volume through one buffer descriptor with 32-bit byte offsets
+    if ((double)g->nt * g->ns * g->nz > (double)(1 << 30))
+        return fail(nullptr, CTR_ETOOBIG, "volume has more than 2^30 voxels");
     for (int a = 0; a < 3; ++a)
         if (!finite_pos(g->size[a])) return fail(nullptr, CTR_EINVAL, "size[%d] must be positive and finite", a);
     if (!finite_pos(g->distance)) return fail(nullptr, CTR_EINVAL, "distance must be positive and finite");
@@ -224,8 +225,8 @@ ctr_status check_fan(const ctr_fan_geom *g, int need_uniform)
 {
     if (!g) return fail(nullptr, CTR_EINVAL, "geometry is NULL");
     if (g->nt < 1 || g->ns < 1) return fail(nullptr, CTR_EINVAL, "image dimensions must be positive");
-    if ((double)g->nt * g->ns > (double)std::numeric_limits<int>::max())
-        return fail(nullptr, CTR_ETOOBIG, "image has more than 2^31-1 pixels");
+    if ((double)g->nt * g->ns > (double)(1 << 30))
+        return fail(nullptr, CTR_ETOOBIG, "image has more than 2^30 pixels");
     for (int a = 0; a < 2; ++a)
         if (!finite_pos(g->size[a])) return fail(nullptr, CTR_EINVAL, "size[%d] must be positive and finite", a);
     if (!finite_pos(g->distance)) return fail(nullptr, CTR_EINVAL, "distance must be positive and finite");
@@ -263,9 +264,8 @@ uint64_t cone_key(const ctr_cone_geom *g)
     return h;
 }
 
-ctr_status cone_tables(ctr_ctx *ctx, const ctr_cone_geom *g)
+ctr_status cone_tables_into(ctr_ctx *ctx, ConeTables &T, const ctr_cone_geom *g)
 {
-    ConeTables &T = ctx->cone;
     const uint64_t key = cone_key(g);
     if (T.valid && T.key == key) return CTR_OK;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // queued work may still read the old tables
@@ -322,9 +322,8 @@ ctr_status cone_tables(ctr_ctx *ctx, const ctr_cone_geom *g)
     return CTR_OK;
 }
 
-RayParams cone_ray_params(ctr_ctx *ctx, const ctr_cone_geom *g)
+RayParams cone_ray_params_from(const ConeTables &T, const ctr_cone_geom *g)
 {
-    const ConeTables &T = ctx->cone;
     RayParams r{};
     r.nt = g->nt;
     r.ns = g->ns;
@@ -346,9 +345,8 @@ RayParams cone_ray_params(ctr_ctx *ctx, const ctr_cone_geom *g)
     return r;
 }
 
-BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
+BpParams cone_bp_params_from(const ConeTables &T, const ctr_cone_geom *g, unsigned flags)
 {
-    const ConeTables &T = ctx->cone;
     BpParams P{};
     P.nt = g->nt;
     P.ns = g->ns;
@@ -386,10 +384,10 @@ BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
 
 // Footprint table of the tiled backprojection for the current cone geometry
 // (computed on the device once per geometry, reused by every later call).
-ctr_status ensure_footprints(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint)
+ctr_status ensure_footprints_key(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint, uint64_t base_key)
 {
     const BpPlan L = adjoint ? adjoint_plan(P) : bp_plan(P);
-    uint64_t key = ctx->cone.key;
+    uint64_t key = base_key;
     key = fnv(key, &L, sizeof L);
     if (fc.buf.ptr && fc.key == key) return CTR_OK;
     ctr_status s = buf_reserve(ctx, fc.buf, L.nfoot * sizeof(int4));
@@ -399,6 +397,17 @@ ctr_status ensure_footprints(ctr_ctx *ctx, const BpParams &P, FootCache &fc, boo
     fc.key = key;
     fc.plan = L;
     return CTR_OK;
+}
+
+ctr_status cone_tables(ctr_ctx *ctx, const ctr_cone_geom *g) { return cone_tables_into(ctx, ctx->cone, g); }
+RayParams cone_ray_params(ctr_ctx *ctx, const ctr_cone_geom *g) { return cone_ray_params_from(ctx->cone, g); }
+BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
+{
+    return cone_bp_params_from(ctx->cone, g, flags);
+}
+ctr_status ensure_footprints(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint)
+{
+    return ensure_footprints_key(ctx, P, fc, adjoint, ctx->cone.key);
 }
 
 uint64_t fan_key(const ctr_fan_geom *g)
@@ -1526,11 +1535,13 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     TRY(stage_out(ctx, dev, o->use_x0 != 0, kWsOut, x_in, nvox, &x));
     for (int s : {kWsVinv, kWsG, kWsS, kWsD, kWsX2}) TRY(buf_reserve(ctx, ctx->ws[s], nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
+    if (o->judge_vs_plain) TRY(buf_reserve(ctx, ctx->ws[kWsR2], nray * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * 4));
     TRY(cnn_reserve(net, nvox, false));
     float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr, *sb = (float *)ctx->ws[kWsS].ptr;
     float *r = (float *)ctx->ws[kWsR].ptr, *alt = (float *)ctx->ws[kWsX2].ptr;
+    float *r_spare = o->judge_vs_plain ? (float *)ctx->ws[kWsR2].ptr : nullptr;
     double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
     if (!o->use_x0) TRY(check_launch(ctx, launch_fill(x, 0.0f, nvox, st), "fill"));
     TRY(check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, st), "SIRT column sums"));
@@ -1547,16 +1558,17 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     TRY(check_launch(ctx, launch_reduce_partials(part, nb, sums, st), "reduce"));
     TRY(host_sums(h));
     const double bn = std::sqrt(h[0]) > 0 ? std::sqrt(h[0]) : 1.0;
-    // residual pass: r = W (b - A xx); returns ||b - A xx|| / ||b||
-    auto residual = [&](const float *xx, double *res) -> ctr_status {
+    // residual pass: rr = W (b - A xx); returns ||b - A xx|| / ||b||
+    auto residual_into = [&](const float *xx, float *rr, double *res) -> ctr_status {
         int nblk = 0;
-        TRY(check_launch(ctx, launch_ray_fp_residual(R, xx, b, r, part, &nblk, st), "SIRT residual"));
+        TRY(check_launch(ctx, launch_ray_fp_residual(R, xx, b, rr, part, &nblk, st), "SIRT residual"));
         TRY(check_launch(ctx, launch_reduce_partials(part, nblk, sums, st), "reduce"));
         double hh[2];
         TRY(host_sums(hh));
         *res = std::sqrt(hh[0]) / bn;
         return CTR_OK;
     };
+    auto residual = [&](const float *xx, double *res) { return residual_into(xx, r, res); };
     float *cur = x;   // buffer holding x_k
     double res_k = 0.0;
     TRY(residual(cur, &res_k));
@@ -1587,13 +1599,12 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
             TRY(check_launch(ctx, launch_learned_candidate(cur, sb, sb, 0.0f, o->min_val, o->max_val, nvox, plain, st),
                              "plain step"));
             double res_p = 0.0;
-            TRY(residual(plain, &res_p));
-            if (res_c < res_p) {
-                TRY(residual(cand, &res_c));   // r of the kept iterate for the next step
-            } else {
+            TRY(residual_into(plain, r_spare, &res_p));
+            if (!(res_c < res_p)) {
                 acc = 0;
                 HIP_TRY(ctx, hipMemcpyAsync(cand, plain, nvox * sizeof(float), hipMemcpyDeviceToDevice, st));
-                res_c = res_p;                  // r already belongs to the plain iterate
+                std::swap(r, r_spare);          // the plain iterate's weighted residual drives the next step
+                res_c = res_p;
             }
         } else if (!(res_c < res_k)) {
             // fall back to the plain step
@@ -1609,6 +1620,205 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     }
     if (cur != x) HIP_TRY(ctx, hipMemcpyAsync(x, cur, nvox * sizeof(float), hipMemcpyDeviceToDevice, st));
     return finish_out(ctx, dev, x, x_in, nvox);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched SIRT3D: independent problems advanced by shared launches
+// ===========================================================================
+namespace {
+
+struct BatchProblem {
+    ConeTables T;
+    FootCache F;
+    RayParams R{};
+    BpParams P{};
+    size_t nvox = 0, nray = 0;
+    DevBuf b, x, r, vinv, part;
+    const float *bp = nullptr;
+    float *xp = nullptr;
+    int fp_blocks = 0, gx = 0, gy = 0;
+};
+
+void free_batch_problem(BatchProblem &q)
+{
+    for (DevBuf *b : {&q.T.d_view, &q.T.d_p, &q.T.d_xi, &q.T.d_bp, &q.T.d_pf, &q.T.d_xf, &q.T.d_zq, &q.F.buf, &q.b, &q.x,
+                      &q.r, &q.vinv, &q.part})
+        buf_free(*b);
+}
+
+// Owns the batch's device memory, graph and stream; releases them on every exit path.
+struct BatchRun {
+    ctr_ctx *ctx;
+    std::vector<BatchProblem> probs;
+    DevBuf fp_jobs, red_jobs, adj_jobs, adj_ones_jobs, sums, norms, iter;
+    hipStream_t cs = nullptr;
+    hipEvent_t ev = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    explicit BatchRun(ctr_ctx *c, int n) : ctx(c), probs(n) {}
+    ~BatchRun()
+    {
+        if (cs) (void)hipStreamSynchronize(cs);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev) (void)hipEventDestroy(ev);
+        if (cs) (void)hipStreamDestroy(cs);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &q : probs) free_batch_problem(q);
+        for (DevBuf *b : {&fp_jobs, &red_jobs, &adj_jobs, &adj_ones_jobs, &sums, &norms, &iter}) buf_free(*b);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, const ctr_sirt_opts *o, unsigned flags)
+{
+    TRY(need_ctx(ctx));
+    if (!pr || n < 1 || n > 1024) return fail(ctx, CTR_EINVAL, "need 1..1024 problems");
+    TRY(sirt_opts_check(ctx, o));
+    for (int p = 0; p < n; ++p) {
+        TRY(check_cone(&pr[p].geom, 1));
+        if (!pr[p].b || !pr[p].x) return fail(ctx, CTR_EINVAL, "problem %d: data pointer is NULL", p);
+    }
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    BatchRun B(ctx, n);
+    hipStream_t st = ctx->stream;
+
+    // per-problem tables, footprints and workspaces
+    std::vector<FpJob> fj(n);
+    std::vector<ReduceJob> rj(n);
+    std::vector<std::vector<AdjointJob>> groups(3), ones_groups(3);   // by LDS pitch 40 / 72 / 136
+    std::vector<int> group_blocks(3, 0);
+    const int pitches[3] = {40, 72, 136};
+    int fp_total = 0;
+    for (int p = 0; p < n; ++p) {
+        BatchProblem &q = B.probs[p];
+        const ctr_cone_geom *g = &pr[p].geom;
+        TRY(cone_tables_into(ctx, q.T, g));
+        q.R = cone_ray_params_from(q.T, g);
+        q.P = cone_bp_params_from(q.T, g, 0);
+        TRY(ensure_footprints_key(ctx, q.P, q.F, true, q.T.key));
+        q.nvox = nvox3(g);
+        q.nray = nray3(g);
+        if (dev) {
+            q.bp = pr[p].b;
+            q.xp = pr[p].x;
+        } else {
+            TRY(upload(ctx, q.b, pr[p].b, q.nray * sizeof(float)));
+            TRY(buf_reserve(ctx, q.x, q.nvox * sizeof(float)));
+            if (o->use_x0)
+                HIP_TRY(ctx, hipMemcpyAsync(q.x.ptr, pr[p].x, q.nvox * sizeof(float), hipMemcpyHostToDevice, st));
+            q.bp = (const float *)q.b.ptr;
+            q.xp = (float *)q.x.ptr;
+        }
+        TRY(buf_reserve(ctx, q.r, q.nray * sizeof(float)));
+        TRY(buf_reserve(ctx, q.vinv, q.nvox * sizeof(float)));
+        q.fp_blocks = fp_job_blocks(q.R, &q.gx, &q.gy);
+        TRY(buf_reserve(ctx, q.part, sizeof(double) * 2 * (size_t)std::max(q.fp_blocks, 4096)));
+        fj[p] = FpJob{q.R, q.xp, q.bp, (float *)q.r.ptr, (double *)q.part.ptr, fp_total, q.gx, q.gy};
+        fp_total += q.fp_blocks;
+        rj[p] = ReduceJob{(const double *)q.part.ptr, q.fp_blocks};
+        const BpPlan &L = q.F.plan;
+        BpParams P = q.P;
+        P.tiles_t = L.tiles_t;
+        P.tiles_s = L.tiles_s;
+        P.tiles_z = L.tiles_z;
+        const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
+        int gi = 0;
+        while (gi < 2 && pitches[gi] != L.wp) ++gi;
+        if (pitches[gi] != L.wp) return fail(ctx, CTR_EINVAL, "problem %d: unsupported footprint pitch %d", p, L.wp);
+        const SirtUpdate U{(const float *)q.vinv.ptr, q.xp, o->lambda, o->min_val, o->max_val};
+        groups[gi].push_back(AdjointJob{P, (const int4 *)q.F.buf.ptr, (const float *)q.r.ptr, nullptr, U,
+                                        group_blocks[gi], nwg});
+        ones_groups[gi].push_back(AdjointJob{P, (const int4 *)q.F.buf.ptr, nullptr, (float *)q.vinv.ptr, U,
+                                             group_blocks[gi], nwg});
+        group_blocks[gi] += nwg;
+        if (!o->use_x0) TRY(check_launch(ctx, launch_fill(q.xp, 0.0f, q.nvox, st), "fill"));
+    }
+    // job tables in device memory (adjoint groups packed one after another)
+    size_t nadj = 0;
+    for (auto &gv : groups) nadj += gv.size();
+    std::vector<AdjointJob> adj_all, ones_all;
+    std::vector<size_t> gofs(3);
+    for (int gi = 0; gi < 3; ++gi) {
+        gofs[gi] = adj_all.size();
+        adj_all.insert(adj_all.end(), groups[gi].begin(), groups[gi].end());
+        ones_all.insert(ones_all.end(), ones_groups[gi].begin(), ones_groups[gi].end());
+    }
+    TRY(upload(ctx, B.fp_jobs, fj.data(), sizeof(FpJob) * n));
+    TRY(upload(ctx, B.red_jobs, rj.data(), sizeof(ReduceJob) * n));
+    TRY(upload(ctx, B.adj_jobs, adj_all.data(), sizeof(AdjointJob) * nadj));
+    TRY(upload(ctx, B.adj_ones_jobs, ones_all.data(), sizeof(AdjointJob) * nadj));
+    const size_t nslots = (size_t)o->n_iter + 1;
+    TRY(buf_reserve(ctx, B.sums, sizeof(double) * 2 * nslots * n));
+    TRY(buf_reserve(ctx, B.norms, sizeof(double) * 2 * n));
+    TRY(buf_reserve(ctx, B.iter, sizeof(int)));
+    const AdjointJob *dadj = (const AdjointJob *)B.adj_jobs.ptr, *dones = (const AdjointJob *)B.adj_ones_jobs.ptr;
+    const FpJob *dfp = (const FpJob *)B.fp_jobs.ptr;
+    const ReduceJob *drj = (const ReduceJob *)B.red_jobs.ptr;
+    int *diter = (int *)B.iter.ptr;
+    double *dsums = (double *)B.sums.ptr;
+
+    // column sums V = 1/(A'1 + 1e-9) and ||b|| per problem
+    for (int gi = 0; gi < 3; ++gi)
+        if (!groups[gi].empty())
+            TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], dones + gofs[gi], (int)groups[gi].size(),
+                                                         group_blocks[gi], 0, 1, st), "batched column sums"));
+    for (int p = 0; p < n; ++p) {
+        BatchProblem &q = B.probs[p];
+        TRY(check_launch(ctx, launch_reciprocal_eps((float *)q.vinv.ptr, q.nvox, 1e-9f, st), "reciprocal"));
+        int nb = 0;
+        TRY(check_launch(ctx, launch_sum_sq_sum(q.bp, q.nray, (double *)q.part.ptr, &nb, st), "norm"));
+        TRY(check_launch(ctx, launch_reduce_partials((const double *)q.part.ptr, nb, (double *)B.norms.ptr + 2 * p, st),
+                         "reduce"));
+    }
+    HIP_TRY(ctx, hipMemsetAsync(diter, 0, sizeof(int), st));   // before capture: not part of the graph
+
+    // one iteration, captured on a private stream that follows the context's stream
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&B.cs, hipStreamNonBlocking));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&B.ev, hipEventDisableTiming));
+    HIP_TRY(ctx, hipEventRecord(B.ev, st));
+    HIP_TRY(ctx, hipStreamWaitEvent(B.cs, B.ev, 0));
+    auto iteration = [&](hipStream_t s, bool update) -> ctr_status {
+        TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_total, s), "batched residual"));
+        TRY(check_launch(ctx, launch_reduce_jobs(drj, n, diter, dsums, s), "batched reduce"));
+        if (!update) return CTR_OK;
+        for (int gi = 0; gi < 3; ++gi)
+            if (!groups[gi].empty())
+                TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], dadj + gofs[gi], (int)groups[gi].size(),
+                                                             group_blocks[gi], 1, 0, s), "batched update"));
+        return check_launch(ctx, launch_iter_inc(diter, s), "iteration counter");
+    };
+    if (o->n_iter > 0) {
+        HIP_TRY(ctx, hipStreamBeginCapture(B.cs, hipStreamCaptureModeThreadLocal));
+        ctr_status cap = iteration(B.cs, true);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(B.cs, &graph);
+        B.graph = graph;
+        if (cap != CTR_OK) return cap;
+        HIP_TRY(ctx, e);
+        HIP_TRY(ctx, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        for (int k = 0; k < o->n_iter; ++k) HIP_TRY(ctx, hipGraphLaunch(B.exec, B.cs));
+    }
+    TRY(iteration(B.cs, false));   // residual of the final iterate
+    HIP_TRY(ctx, hipStreamSynchronize(B.cs));
+
+    std::vector<double> hs(2 * nslots * n), hn(2 * n);
+    HIP_TRY(ctx, hipMemcpy(hs.data(), dsums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(hn.data(), B.norms.ptr, sizeof(double) * hn.size(), hipMemcpyDeviceToHost));
+    for (int p = 0; p < n; ++p) {
+        if (pr[p].residual_l2) {
+            const double bn = std::sqrt(hn[2 * p]) > 0 ? std::sqrt(hn[2 * p]) : 1.0;
+            for (size_t k = 0; k < nslots; ++k) pr[p].residual_l2[k] = std::sqrt(hs[2 * (k * n + p)]) / bn;
+        }
+        if (!dev)
+            HIP_TRY(ctx, hipMemcpy(pr[p].x, B.probs[p].xp, B.probs[p].nvox * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return CTR_OK;
 }
 
 }  // extern "C"

@@ -117,6 +117,38 @@ hipError_t launch_bp_tiled(int kind, const BpParams &P, const BpPlan &L, const i
 hipError_t launch_adjoint(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
                           const SirtUpdate *upd, int ones_input, hipStream_t st);
 
+// Batched exact adjoint (independent problems in one launch).  P.tiles_* are
+// the problem's plan; jobs live in device memory, sorted by block0.
+struct AdjointJob {
+    BpParams P;
+    const int4 *foot;
+    const float *proj;      // weighted residual (ignored when ones_input)
+    float *vol;             // output (when not a SIRT update)
+    SirtUpdate U;
+    int block0, nblocks;    // this job's blocks in the launch
+};
+hipError_t launch_adjoint_batched(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt, int ones_input,
+                                  hipStream_t st);
+
+// Batched SIRT residual pass: r = (b - A x)/(A1 + 1e-9) and per-block (sum e^2, sum |e|) per job.
+struct FpJob {
+    RayParams g;
+    const float *vol;
+    const float *b;
+    float *r;
+    double *partials;
+    int block0, gx, gy;     // first block and the job's (columns, rows) block grid; views = z
+};
+int fp_job_blocks(const RayParams &g, int *gx, int *gy);
+hipError_t launch_ray_fp_residual_batched(const FpJob *jobs, int njobs, int nblocks, hipStream_t st);
+// sums[(*iter) * njobs + p] = sum of job p's partial pairs (one block per job)
+struct ReduceJob {
+    const double *partials;
+    int n;
+};
+hipError_t launch_reduce_jobs(const ReduceJob *jobs, int njobs, const int *iter, double *sums, hipStream_t st);
+hipError_t launch_iter_inc(int *iter, hipStream_t st);
+
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
 // transform length of the FFT filter path for np columns (0: direct convolution)
 int fft_filter_length(int np);

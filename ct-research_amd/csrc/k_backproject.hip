@@ -398,16 +398,15 @@ __device__ __forceinline__ void slice_rows(const BpParams &P, float zb, float zt
 // D^2 + xi^2)), 1/a and 1/b per column and 1/xi per row, so a candidate
 // (ray, voxel) pair costs 5 VALU and one immediate-offset LDS read.
 template <int TZ, int WP, int CAPF>
-__global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
-                                                      const int4 *__restrict__ foot, SirtUpdate U, int sirt, int ones)
+__device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, float *__restrict__ vol, const BpParams &P,
+                                             const int4 *__restrict__ foot, const BpView *__restrict__ views,
+                                             const SirtUpdate &U, int sirt, int ones, int tile, float *lds)
 {
     constexpr int HMAX = (CAPF - 2 * WP - 2 * kCMax) / (WP + 1);
-    __shared__ float lds[CAPF];
     float *yd = lds;                                // [HMAX][WP] (+ kCMax zero slack)
     float *inva = lds + HMAX * WP + kCMax;          // [WP + kCMax]
     float *invb = inva + WP + kCMax;                // [WP]
     float *invxi = invb + WP;                       // [HMAX]
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tt = tile % P.tiles_t, rest = tile / P.tiles_t;
     const int ts = rest % P.tiles_s, tz = rest / P.tiles_s;
     const int tx = threadIdx.x & (kTX - 1), ty = threadIdx.x / kTX;
@@ -430,7 +429,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ 
         if (f.w & kFootEmpty) continue;
         const int W = f.z & 0xffff, H = f.z >> 16;
         const bool fits = (f.w & kFootFits) != 0;
-        const BpView vw = P.views[b];
+        const BpView vw = views[b];
         const float *src = proj + (size_t)b * P.nx * P.np;
         if (fits) {
             __syncthreads();
@@ -534,6 +533,31 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ 
             }
         }
     }
+}
+
+template <int TZ, int WP, int CAPF>
+__global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
+                                                      const int4 *__restrict__ foot, const BpView *__restrict__ views,
+                                                      SirtUpdate U, int sirt, int ones)
+{
+    __shared__ float lds[CAPF];
+    adjoint_tile<TZ, WP, CAPF>(proj, vol, P, foot, views, U, sirt, ones, xcd_tile(blockIdx.x, gridDim.x), lds);
+}
+
+// Many independent problems in one launch: block b belongs to the job whose
+// block range holds it (a scalar scan of the small job table).
+template <int TZ, int WP, int CAPF>
+__global__ __launch_bounds__(kThreads) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
+                                                              int ones)
+{
+    __shared__ float lds[CAPF];
+    int p = 0;
+    while (p + 1 < njobs && (int)blockIdx.x >= jobs[p + 1].block0) ++p;
+    const BpParams P = jobs[p].P;
+    const SirtUpdate U = jobs[p].U;
+    const int local = (int)blockIdx.x - jobs[p].block0;
+    adjoint_tile<TZ, WP, CAPF>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
+                               xcd_tile(local, jobs[p].nblocks), lds);
 }
 
 constexpr int kBpTZ = 16;
@@ -661,14 +685,31 @@ hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot
         sirt = 1;
     }
     if (L.wp == 40)
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot, U,
-                           sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, U, sirt, ones_input);
     else if (L.wp == 72)
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot, U,
-                           sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, U, sirt, ones_input);
     else
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot, U,
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, U, sirt, ones_input);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_batched(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt, int ones_input,
+                                  hipStream_t st)
+{
+    if (wp == 40)
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, kAdjCap>), dim3(nblocks), dim3(kThreads), 0, st, jobs, njobs,
                            sirt, ones_input);
+    else if (wp == 72)
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, kAdjCap>), dim3(nblocks), dim3(kThreads), 0, st, jobs, njobs,
+                           sirt, ones_input);
+    else if (wp == 136)
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, kAdjCap>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
+                           njobs, sirt, ones_input);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -5,11 +5,14 @@
 // nearly parallel rays that walk neighbouring voxels (shared cache lines).
 //
 // Per ray the entry/exit of the volume box and the first crossing of each
-// axis are computed in FP64 (near-parallel rays need it); the traversal then
-// runs in FP32 relative to the entry point.  The number of plane crossings
-// of each axis is fixed by the FP64 setup, so the voxel index can only move
-// between the clamped entry and exit voxels: no bounds test per step and no
-// out-of-volume read whatever the rounding.
+// axis are computed in FP64 (near-parallel rays need it).  The traversal
+// keeps the three next-crossing parameters in FP64 too (FP64 adds issue at
+// the FP32 rate on gfx950), advancing the axis whose plane comes first with
+// selects rather than branches, so a wavefront never splits three ways.  The
+// loop runs the FP64 setup's crossing count; segments are clipped at the exit
+// parameter, and the volume is read through a range-checked buffer
+// descriptor, so a rounding-level extra step past the exit reads zero with
+// zero weight.
 //
 // Geometry: ProjectionCone_3D.cu:26-42 (source, end point at 2D along the
 // ray through (p, xi) of the virtual isocentre detector); segment weight is
@@ -23,8 +26,8 @@ constexpr int kBX = 64, kBY = 4;
 
 struct AxisSetup {
     int i0, n;        // start index, number of plane crossings
-    float next0;      // first crossing, relative to the entry parameter
-    float step;       // parameter step between crossings
+    double next0;     // first crossing, relative to the entry parameter
+    double step;      // parameter step between crossings
     int stride;       // signed index stride
 };
 
@@ -39,8 +42,8 @@ __device__ __forceinline__ AxisSetup setup_axis(double s, double d, double lam_i
         i1 = min(max(i1, i0), nvox - 1);
         a.i0 = i0;
         a.n = i1 - i0;
-        a.next0 = (float)(((i0 + 1) * r - s) / d - lam_in);
-        a.step = (float)(r / d);
+        a.next0 = ((i0 + 1) * r - s) / d - lam_in;
+        a.step = r / d;
         a.stride = stride;
     } else if (d < 0.0) {
         int i0 = (int)ceil(xin) - 1, i1 = (int)floor(xout);
@@ -48,18 +51,20 @@ __device__ __forceinline__ AxisSetup setup_axis(double s, double d, double lam_i
         i1 = max(min(i1, i0), 0);
         a.i0 = i0;
         a.n = i0 - i1;
-        a.next0 = (float)((i0 * r - s) / d - lam_in);
-        a.step = (float)(-r / d);
+        a.next0 = (i0 * r - s) / d - lam_in;
+        a.step = -r / d;
         a.stride = -stride;
     } else {
         a.i0 = min(max((int)floor(xin), 0), nvox - 1);
         a.n = 0;
-        a.next0 = kInf;
-        a.step = 0.0f;
+        a.next0 = __builtin_huge_val();
+        a.step = 0.0;
         a.stride = 0;
     }
     return a;
 }
+
+constexpr int kBufDword3 = 0x00020000;   // raw 32-bit buffer descriptor word 3 on gfx9
 
 // Slab clip of the ray s + lam*d against [0, b]; returns false on a miss.
 __device__ __forceinline__ bool clip_axis(double s, double d, double b, double &lo, double &hi)
@@ -95,37 +100,31 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     const AxisSetup at = setup_axis(st, dx, lo, hi, g.rt, g.nt, 1);
     const AxisSetup as = setup_axis(ss, dy, lo, hi, g.rs, g.ns, g.nt);
     const AxisSetup az = setup_axis(g.sz, dz, lo, hi, g.rz, g.nz, g.nt * g.ns);
-    const float lam_end = (float)(hi - lo);
-    chord = lam_end;
+    const double lam_end = hi - lo;
+    chord = (float)lam_end;
 
+    const unsigned nbytes = (unsigned)((size_t)g.nt * g.ns * g.nz * sizeof(float));
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)vol, 0, (int)nbytes, kBufDword3);
     int idx = (az.i0 * g.ns + as.i0) * g.nt + at.i0;
-    int kt = 0, ks = 0, kz = 0;
-    float nt = at.n > 0 ? at.next0 : kInf;
-    float ns = as.n > 0 ? as.next0 : kInf;
-    float nz = az.n > 0 ? az.next0 : kInf;
-    float lam = 0.0f, acc = 0.0f;
+    const double inf = __builtin_huge_val();
+    double nt = at.n > 0 ? at.next0 : inf;
+    double ns = as.n > 0 ? as.next0 : inf;
+    double nz = az.n > 0 ? az.next0 : inf;
+    double lam = 0.0;
+    float acc = 0.0f;
     const int total = at.n + as.n + az.n + 1;
     for (int k = 0; k < total; ++k) {
-        const float m = fminf(fminf(nt, ns), fminf(nz, lam_end));
-        const float v = vol[idx];
-        acc = fmaf(fmaxf(m - lam, 0.0f), v, acc);
-        lam = fmaxf(lam, m);
-        // step the axis whose plane comes first (exhausted axes sit at +inf)
-        const bool bt = (nt <= ns) && (nt <= nz);
-        const bool bs = !bt && (ns <= nz);
-        if (bt) {
-            idx += at.stride;
-            ++kt;
-            nt = kt < at.n ? fmaf((float)kt, at.step, at.next0) : kInf;
-        } else if (bs) {
-            idx += as.stride;
-            ++ks;
-            ns = ks < as.n ? fmaf((float)ks, as.step, as.next0) : kInf;
-        } else {
-            idx += az.stride;
-            ++kz;
-            nz = kz < az.n ? fmaf((float)kz, az.step, az.next0) : kInf;
-        }
+        const double m = fmin(fmin(nt, ns), fmin(nz, lam_end));
+        const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (unsigned)idx * 4u, 0, 0));
+        acc = fmaf((float)(m - lam), v, acc);
+        lam = m;
+        // step the axis whose plane comes first (t before s before z on ties)
+        const bool bt = nt <= ns && nt <= nz;
+        const bool bs = !bt && ns <= nz;
+        idx += bt ? at.stride : (bs ? as.stride : az.stride);
+        nt = bt ? nt + at.step : nt;
+        ns = bs ? ns + as.step : ns;
+        nz = (!bt && !bs) ? nz + az.step : nz;
     }
     return acc;
 }
@@ -148,15 +147,15 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // SIRT residual pass: r = (b - A x) / (A 1 + 1e-9) and per-block partial sums
-// of (b - A x)^2 and |b - A x| (SIRT3D.m:80-81, :87).
-__global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual(const float *__restrict__ vol, const float *__restrict__ b,
-                                                              float *__restrict__ r, double *__restrict__ partials,
-                                                              RayParams g)
+// of (b - A x)^2 and |b - A x| (SIRT3D.m:80-81, :87) for block (bx, by) of
+// view ib; the block's pair goes to partials[2 blk].
+__device__ __forceinline__ void fp_residual_block(const float *__restrict__ vol, const float *__restrict__ b,
+                                                  float *__restrict__ r, double *__restrict__ partials,
+                                                  const RayParams &g, int bx, int by, int ib, size_t blk)
 {
     __shared__ double red[2][kBY];
-    const int ip = blockIdx.x * kBX + threadIdx.x;
-    const int ix = blockIdx.y * kBY + threadIdx.y;
-    const int ib = blockIdx.z;
+    const int ip = bx * kBX + threadIdx.x;
+    const int ix = by * kBY + threadIdx.y;
     double e2 = 0.0, e1 = 0.0;
     if (ip < g.np && ix < g.nx) {
         float chord;
@@ -180,10 +179,28 @@ __global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual(const float *__res
             s2 += red[0][w];
             s1 += red[1][w];
         }
-        const size_t blk = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         partials[2 * blk + 0] = s2;
         partials[2 * blk + 1] = s1;
     }
+}
+
+__global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual(const float *__restrict__ vol, const float *__restrict__ b,
+                                                              float *__restrict__ r, double *__restrict__ partials,
+                                                              RayParams g)
+{
+    const size_t blk = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    fp_residual_block(vol, b, r, partials, g, blockIdx.x, blockIdx.y, blockIdx.z, blk);
+}
+
+// Independent problems in one launch: a flat grid over the jobs' block ranges.
+__global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual_batched(const FpJob *__restrict__ jobs, int njobs)
+{
+    int p = 0;
+    while (p + 1 < njobs && (int)blockIdx.x >= jobs[p + 1].block0) ++p;
+    const RayParams g = jobs[p].g;
+    const int local = (int)blockIdx.x - jobs[p].block0, gx = jobs[p].gx, gy = jobs[p].gy;
+    const int bx = local % gx, by = (local / gx) % gy, ib = local / (gx * gy);
+    fp_residual_block(jobs[p].vol, jobs[p].b, jobs[p].r, jobs[p].partials, g, bx, by, ib, (size_t)local);
 }
 
 dim3 fp_grid(const RayParams &g) { return dim3((g.np + kBX - 1) / kBX, (g.nx + kBY - 1) / kBY, g.nb); }
@@ -208,6 +225,20 @@ hipError_t launch_ray_fp_residual(const RayParams &g, const float *vol, const fl
     const dim3 grid = fp_grid(g);
     *nblocks = (int)(grid.x * grid.y * grid.z);
     hipLaunchKernelGGL(k_ray_fp_residual, grid, dim3(kBX, kBY), 0, st, vol, b, r, partials, g);
+    return hipGetLastError();
+}
+
+int fp_job_blocks(const RayParams &g, int *gx, int *gy)
+{
+    const dim3 grid = fp_grid(g);
+    *gx = (int)grid.x;
+    *gy = (int)grid.y;
+    return (int)(grid.x * grid.y * grid.z);
+}
+
+hipError_t launch_ray_fp_residual_batched(const FpJob *jobs, int njobs, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ray_fp_residual_batched, dim3(nblocks), dim3(kBX, kBY), 0, st, jobs, njobs);
     return hipGetLastError();
 }
 

@@ -44,6 +44,39 @@ __global__ __launch_bounds__(256) void k_reduce_pairs(const double *partials, in
     }
 }
 
+// One block per job: sums[(*iter) * njobs + p] = sum of job p's partial pairs.
+__global__ __launch_bounds__(256) void k_reduce_jobs(const ReduceJob *__restrict__ jobs, const int *__restrict__ iter,
+                                                     double *__restrict__ sums, int njobs)
+{
+    __shared__ double red[2][4];
+    const int p = blockIdx.x;
+    const double *partials = jobs[p].partials;
+    const int n = jobs[p].n;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double *o = sums + 2 * ((size_t)(*iter) * njobs + p);
+        o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// Advance the iteration counter of a captured SIRT iteration (vector store).
+__global__ void k_iter_inc(int *iter)
+{
+    if (threadIdx.x == 0) iter[0] = iter[0] + 1;
+}
+
 // Per-block (sum x^2, sum x).
 __global__ __launch_bounds__(256) void k_sum_sq_sum(const float *x, size_t n, double *partials)
 {
@@ -92,6 +125,18 @@ hipError_t launch_reciprocal_eps(float *v, size_t n, float eps, hipStream_t st)
 hipError_t launch_reduce_partials(const double *partials, int n, double *out2, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, st, partials, n, out2);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_jobs(const ReduceJob *jobs, int njobs, const int *iter, double *sums, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_jobs, dim3(njobs), dim3(256), 0, st, jobs, iter, sums, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_iter_inc(int *iter, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_iter_inc, dim3(1), dim3(64), 0, st, iter);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
-    "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d",
+    "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched",
 ]
 
 CNN_NUM_PARAMS = 57985
@@ -59,6 +59,11 @@ class SirtOptsC(ctypes.Structure):
     _fields_ = [("n_iter", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
                 ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
                 ("residual_l2", ctypes.c_void_p), ("residual_l1", ctypes.c_void_p)]
+
+
+class SirtProblemC(ctypes.Structure):
+    _fields_ = [("geom", ConeGeomC), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("residual_l2", ctypes.c_void_p)]
 
 
 class LearnedOptsC(ctypes.Structure):
@@ -114,6 +119,8 @@ def lib():
         L.ctr_cnn_train_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint]
+        L.ctr_sirt3d_batched.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_uint]
         L.ctr_sirt3d_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_float, ctypes.c_uint]
@@ -321,6 +328,36 @@ class Context:
 
     def sirt2d(self, g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None, residuals=True, flags=0):
         return self._sirt(lib().ctr_sirt2d, g, b, n_iter, lam, lo, hi, x0, residuals, flags)
+
+    def sirt3d_batched(self, problems, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0s=None, flags=0):
+        """problems: [(ConeGeom, b)], b numpy or CUDA tensors (all alike).
+        Returns ([x], [residual_l2]) with one graph-captured launch sequence per iteration."""
+        n = len(problems)
+        arr = (SirtProblemC * n)()
+        res = [np.zeros(n_iter + 1) for _ in range(n)]
+        xs, keep = [], []
+        device = _is_device(problems[0][1])
+        for i, (g, b) in enumerate(problems):
+            if device:
+                import torch
+                x = x0s[i] if x0s is not None else torch.zeros(g.nvox, dtype=torch.float32, device=b.device)
+                bp, xp = b.data_ptr(), x.data_ptr()
+            else:
+                bb = _f32(b)
+                x = _f32(x0s[i]).copy() if x0s is not None else np.zeros(g.nvox, dtype=np.float32)
+                keep.append(bb)
+                bp, xp = bb.ctypes.data, x.ctypes.data
+            xs.append(x)
+            arr[i].geom = g.c
+            arr[i].b = bp
+            arr[i].x = xp
+            arr[i].residual_l2 = res[i].ctypes.data
+        opts = SirtOptsC(int(n_iter), float(lam), float(lo), float(hi), int(x0s is not None), None, None)
+        if device:
+            self._bind_torch_stream()
+            flags |= DEVICE_PTRS
+        self._check(lib().ctr_sirt3d_batched(self.h, arr, n, ctypes.byref(opts), flags))
+        return xs, res
 
     # -- learned SIRT -------------------------------------------------------------
     def sirt3d_step(self, g, b, x, lam=1.0, out=None):

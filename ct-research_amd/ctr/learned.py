@@ -69,7 +69,7 @@ def collect(ctx, seeds, n=256, views=90, ahead=AHEAD, log=print):
     for seed in seeds:
         t0 = time.perf_counter()
         g, vol, b = make_problem(ctx, seed, n, views)
-        snaps, x200, res = plain_trajectory(ctx, g, b)
+        snaps, x200, res = plain_trajectory(ctx, g, b, snapshots=set(STARTS) | {k + ahead for k in STARTS})
         for k in STARTS:
             xk = snaps[k]
             s = ctx.sirt3d_step(g, b, xk)
@@ -79,6 +79,27 @@ def collect(ctx, seeds, n=256, views=90, ahead=AHEAD, log=print):
             recs.append((xk, s, rms, (snaps[k + ahead] - xk - s) / rms))
         torch.cuda.synchronize()
         log(f"  phantom {seed}: residual(200) {res[-1]:.5f}, {time.perf_counter() - t0:.1f} s", flush=True)
+    return recs
+
+
+def collect_on_policy(ctx, net, seeds, n=256, views=90, steps=10, ahead=AHEAD, log=print):
+    """Records at the iterates the learned loop visits (judged against the
+    plain step), labelled by `ahead` plain SIRT iterations from each."""
+    import torch
+    recs = []
+    for seed in seeds:
+        t0 = time.perf_counter()
+        g, vol, b = make_problem(ctx, seed, n, views)
+        x = torch.zeros(g.nvox, dtype=torch.float32, device=b.device)
+        for _ in range(steps):
+            s = ctx.sirt3d_step(g, b, x)
+            rms = float(torch.sqrt(torch.mean(s * s)))
+            if rms > 0:
+                ahead_x, _, _ = ctx.sirt3d(g, b, ahead, x0=x.clone(), residuals=False)
+                recs.append((x.clone(), s, rms, (ahead_x - x - s) / rms))
+            x, _, _ = ctx.learned_sirt3d(g, b, net, 1, x0=x, judge_vs_plain=True)
+        torch.cuda.synchronize()
+        log(f"  on-policy phantom {seed}: {time.perf_counter() - t0:.1f} s", flush=True)
     return recs
 
 
@@ -160,6 +181,8 @@ def main(argv=None):
     ap.add_argument("--phantoms", type=int, default=6)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--rounds", type=int, default=1, help="on-policy data rounds after the first training")
+    ap.add_argument("--ahead", type=int, default=AHEAD, help="plain iterations one learned step stands for")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2000)
     args = ap.parse_args(argv)
@@ -174,9 +197,14 @@ def main(argv=None):
         if not args.weights:
             net.set_params(_lib.cnn_init_params(0, 0.01))
         print(f"collecting {args.phantoms} training phantoms", flush=True)
-        recs = collect(ctx, range(1000, 1000 + args.phantoms), n=args.n)
+        recs = collect(ctx, range(1000, 1000 + args.phantoms), n=args.n, ahead=args.ahead)
         print(f"training {args.steps} steps on {len(recs)} iterates", flush=True)
         losses = train(net, recs, args.n, steps=args.steps, lr=args.lr)
+        for rnd in range(args.rounds):
+            # on-policy rounds: label the iterates the learned loop actually visits
+            recs += collect_on_policy(ctx, net, range(1000, 1000 + args.phantoms), n=args.n, ahead=args.ahead)
+            print(f"round {rnd + 1}: training {args.steps // 2} steps on {len(recs)} iterates", flush=True)
+            losses += train(net, recs, args.n, steps=args.steps // 2, lr=args.lr * 0.5, seed=rnd + 1)
         np.savez(args.out, params=net.get_params(), losses=np.array(losses))
         print(f"saved {args.out}", flush=True)
         del recs

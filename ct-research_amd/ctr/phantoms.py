@@ -79,6 +79,26 @@ def rasterize_3d(e, n, dtype=np.float32):
     return out.astype(dtype)
 
 
+def rasterize_3d_torch(e, n, device="cuda"):
+    """rasterize_3d on a torch device (same float64 arithmetic, same layout);
+    returns a float32 tensor [z][s][t]."""
+    import torch
+    r = torch.from_numpy(_axis(n)).to(device)
+    y = r[None, None, :]   # t axis
+    x = r[None, :, None]   # s axis
+    z = r[:, None, None]
+    out = torch.zeros((n, n, n), dtype=torch.float64, device=device)
+    for k in range(len(e)):
+        A, a, b, c, x0, y0, z0, phi = (float(v) for v in e[k])
+        ph = np.deg2rad(phi)
+        cp, sp = float(np.cos(ph)), float(np.sin(ph))
+        sx, sy = cp * x0 + sp * y0, -sp * x0 + cp * y0
+        u = (cp * x + sp * y - sx) ** 2 / a ** 2 + (-sp * x + cp * y - sy) ** 2 / b ** 2
+        w = (z - z0) ** 2 / c ** 2
+        out += A * ((u + w) <= 1.0)
+    return out.to(torch.float32)
+
+
 def phantom3d(n, kind="shepp-logan", n_ellipsoids=12, dtype=np.float32):
     return rasterize_3d(ellipsoids_3d(kind, n_ellipsoids), n, dtype)
 

@@ -198,6 +198,21 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
 ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
                       unsigned flags);
 
+/* Many independent SIRT3D problems (uneven volumes, views and detectors)
+ * advanced together: each iteration is one batched residual launch, one
+ * reduction and one batched update launch per LDS layout, captured once in
+ * a HIP graph and replayed n_iter times.  opts->residual_l2/l1 are ignored;
+ * each problem reports through its own residual_l2. */
+typedef struct ctr_sirt_problem {
+    ctr_cone_geom geom;
+    const float *b;         /* [n_beta*n_xi*n_p] projections */
+    float *x;               /* [nz*ns*nt] iterate (in when opts->use_x0, out) */
+    double *residual_l2;    /* optional host [n_iter+1]: ||b - A x_k||_2 / ||b||_2 */
+} ctr_sirt_problem;
+
+ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *problems, int n, const ctr_sirt_opts *opts,
+                              unsigned flags);
+
 /* ---- learned SIRT update (no counterpart in the reference) -------------
  *
  * A 3-D CNN maps (x_k, s_k / rms(s_k)) to a correction d_k, where s_k is the

@@ -88,3 +88,12 @@ def test_metrics():
     assert ctr.metrics.rrmse(x, x) == 0.0
     assert np.isclose(ctr.metrics.rrmse(2 * x, x), 1.0, rtol=1e-12)
     assert np.isclose(ctr.metrics.local_error(x + 1, x), np.sqrt(3) / np.sqrt(14))
+
+
+def test_torch_rasterizer_matches_numpy():
+    from ctr import phantoms
+    e = phantoms.random_ellipsoids(20261202)
+    a = phantoms.rasterize_3d(e, 24)
+    b = phantoms.rasterize_3d_torch(e, 24, device="cpu").numpy()
+    # same masks; the float64 amplitude sums may round differently in the last float32 bit
+    assert np.abs(a - b).max() <= 1e-6 and np.array_equal(a > 0, b > 0)
