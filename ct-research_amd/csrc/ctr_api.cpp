@@ -22,7 +22,7 @@ namespace {
 
 thread_local std::string g_last_error;
 
-enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsCount };
+enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsT, kWsCount };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -408,6 +408,17 @@ BpParams cone_bp_params(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
 ctr_status ensure_footprints(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint)
 {
     return ensure_footprints_key(ctx, P, fc, adjoint, ctx->cone.key);
+}
+
+// The [z][t][s] copy of a volume for the forward projector (its views along t read it).
+ctr_status fp_transposed(ctr_ctx *ctx, RayParams &R, const float *vol, DevBuf &buf, hipStream_t st)
+{
+    ctr_status s = buf_reserve(ctx, buf, (size_t)R.nt * R.ns * R.nz * sizeof(float));
+    if (s != CTR_OK) return s;
+    s = check_launch(ctx, launch_transpose_st(vol, (float *)buf.ptr, R.nt, R.ns, R.nz, st), "volume transpose");
+    if (s != CTR_OK) return s;
+    R.volT = (const float *)buf.ptr;
+    return CTR_OK;
 }
 
 uint64_t fan_key(const ctr_fan_geom *g)
@@ -851,6 +862,7 @@ ctr_status ctr_cone_fp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *vol, f
     TRY(stage_in(ctx, dev, kWsIn, vol, nvox3(g), &dv));
     TRY(stage_out(ctx, dev, false, kWsOut, proj, nray3(g), &dp));
     RayParams R = cone_ray_params(ctx, g);
+    TRY(fp_transposed(ctx, R, dv, ctx->ws[kWsT], ctx->stream));
     TRY(check_launch(ctx, launch_ray_fp(R, dv, dp, ctx->stream), "cone forward projection"));
     return finish_out(ctx, dev, dp, proj, nray3(g));
 }
@@ -961,6 +973,7 @@ ctr_status ctr_fan_fp(ctr_ctx *ctx, const ctr_fan_geom *g, const float *img, flo
     TRY(stage_in(ctx, dev, kWsIn, img, nvox2(g), &di));
     TRY(stage_out(ctx, dev, false, kWsOut, proj, nray2(g), &dp));
     RayParams R = fan_ray_params(ctx, g);
+    TRY(fp_transposed(ctx, R, di, ctx->ws[kWsT], ctx->stream));
     TRY(check_launch(ctx, launch_ray_fp(R, di, dp, ctx->stream), "fan forward projection"));
     return finish_out(ctx, dev, dp, proj, nray2(g));
 }
@@ -1127,7 +1140,9 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
     const BpPlan AL = ctx->foot_adj.plan;
     const int4 *AF = (const int4 *)ctx->foot_adj.buf.ptr;
     auto fp = [&](const float *xx, const float *bb, float *rr, double *part, int *nblk) {
-        return check_launch(ctx, launch_ray_fp_residual(R, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
+        RayParams Rx = R;
+        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], ctx->stream));
+        return check_launch(ctx, launch_ray_fp_residual(Rx, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
     };
     auto ones = [&](float *vinv) {
         return check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, ctx->stream), "SIRT column sums");
@@ -1148,7 +1163,9 @@ ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float
     const RayParams R = fan_ray_params(ctx, g);
     const FanParams P = fan_params(ctx, g, 0);
     auto fp = [&](const float *xx, const float *bb, float *rr, double *part, int *nblk) {
-        return check_launch(ctx, launch_ray_fp_residual(R, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
+        RayParams Rx = R;
+        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], ctx->stream));
+        return check_launch(ctx, launch_ray_fp_residual(Rx, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
     };
     auto ones = [&](float *vinv) {
         return check_launch(ctx, launch_fan_adjoint(P, nullptr, vinv, nullptr, 1, ctx->stream), "SIRT column sums");
@@ -1502,7 +1519,9 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_
     TRY(check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, st), "SIRT column sums"));
     TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
     int nblk = 0;
-    TRY(check_launch(ctx, launch_ray_fp_residual(R, x, b, r, part, &nblk, st), "SIRT residual"));
+    RayParams Rx = R;
+    TRY(fp_transposed(ctx, Rx, x, ctx->ws[kWsT], st));
+    TRY(check_launch(ctx, launch_ray_fp_residual(Rx, x, b, r, part, &nblk, st), "SIRT residual"));
     TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
     TRY(check_launch(ctx, launch_sirt_step(vinv, gb, lambda, nvox, s, st), "SIRT step"));
     return finish_out(ctx, dev, s, s_out, nvox);
@@ -1561,7 +1580,9 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     // residual pass: rr = W (b - A xx); returns ||b - A xx|| / ||b||
     auto residual_into = [&](const float *xx, float *rr, double *res) -> ctr_status {
         int nblk = 0;
-        TRY(check_launch(ctx, launch_ray_fp_residual(R, xx, b, rr, part, &nblk, st), "SIRT residual"));
+        RayParams Rx = R;
+        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], st));
+        TRY(check_launch(ctx, launch_ray_fp_residual(Rx, xx, b, rr, part, &nblk, st), "SIRT residual"));
         TRY(check_launch(ctx, launch_reduce_partials(part, nblk, sums, st), "reduce"));
         double hh[2];
         TRY(host_sums(hh));
@@ -1635,7 +1656,7 @@ struct BatchProblem {
     RayParams R{};
     BpParams P{};
     size_t nvox = 0, nray = 0;
-    DevBuf b, x, r, vinv, part;
+    DevBuf b, x, r, vinv, part, vt;
     const float *bp = nullptr;
     float *xp = nullptr;
     int fp_blocks = 0, gx = 0, gy = 0;
@@ -1644,7 +1665,7 @@ struct BatchProblem {
 void free_batch_problem(BatchProblem &q)
 {
     for (DevBuf *b : {&q.T.d_view, &q.T.d_p, &q.T.d_xi, &q.T.d_bp, &q.T.d_pf, &q.T.d_xf, &q.T.d_zq, &q.F.buf, &q.b, &q.x,
-                      &q.r, &q.vinv, &q.part})
+                      &q.r, &q.vinv, &q.part, &q.vt})
         buf_free(*b);
 }
 
@@ -1717,6 +1738,8 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         }
         TRY(buf_reserve(ctx, q.r, q.nray * sizeof(float)));
         TRY(buf_reserve(ctx, q.vinv, q.nvox * sizeof(float)));
+        TRY(buf_reserve(ctx, q.vt, q.nvox * sizeof(float)));
+        q.R.volT = (const float *)q.vt.ptr;   // refreshed from x before every residual pass
         q.fp_blocks = fp_job_blocks(q.R, &q.gx, &q.gy);
         TRY(buf_reserve(ctx, q.part, sizeof(double) * 2 * (size_t)std::max(q.fp_blocks, 4096)));
         fj[p] = FpJob{q.R, q.xp, q.bp, (float *)q.r.ptr, (double *)q.part.ptr, fp_total, q.gx, q.gy};
@@ -1784,6 +1807,11 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     HIP_TRY(ctx, hipEventRecord(B.ev, st));
     HIP_TRY(ctx, hipStreamWaitEvent(B.cs, B.ev, 0));
     auto iteration = [&](hipStream_t s, bool update) -> ctr_status {
+        for (int p = 0; p < n; ++p) {
+            const BatchProblem &q = B.probs[p];
+            TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, s),
+                             "volume transpose"));
+        }
         TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_total, s), "batched residual"));
         TRY(check_launch(ctx, launch_reduce_jobs(drj, n, diter, dsums, s), "batched reduce"));
         if (!update) return CTR_OK;

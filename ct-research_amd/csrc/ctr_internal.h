@@ -25,6 +25,8 @@ struct RayParams {
     const double *view;     // [nb][4]: sin b, cos b, source t, source s
     const double *p;        // [np] column positions at the isocentre
     const double *xi;       // [nx] row positions at the isocentre
+    const float *volT;      // optional [nz][nt][ns] copy of the volume, read by views whose
+                            // central ray runs mostly along t (neighbouring rays then differ in s)
 };
 
 // Per-view constants of the voxel-driven backprojectors, float:
@@ -160,6 +162,7 @@ hipError_t launch_fan_fbp_bp(const FanParams &P, const float *filt, float *img, 
 
 // Utilities.
 hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st);
+hipError_t launch_transpose_st(const float *in, float *out, int nt, int ns, int nz, hipStream_t st);   // [z][t][s] <- [z][s][t]
 hipError_t launch_reciprocal_eps(float *v, size_t n, float eps, hipStream_t st);   // v <- 1/(v+eps)
 hipError_t launch_reduce_partials(const double *partials, int n, double *out2, hipStream_t st);
 hipError_t launch_sum_sq_sum(const float *x, size_t n, double *partials, int *nblocks, hipStream_t st);  // (sum x^2, sum x)

@@ -97,15 +97,21 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     chord = 0.0f;
     if (!hit || !(hi > lo)) return 0.0f;
 
-    const AxisSetup at = setup_axis(st, dx, lo, hi, g.rt, g.nt, 1);
-    const AxisSetup as = setup_axis(ss, dy, lo, hi, g.rs, g.ns, g.nt);
+    // views whose central ray runs mostly along t read the [z][t][s] copy, so
+    // that the neighbouring rays of a wavefront (neighbouring p, offset in s)
+    // gather from neighbouring addresses
+    const bool tr = g.volT != nullptr && fabs(sb) > fabs(cb);
+    const int st_t = tr ? g.ns : 1, st_s = tr ? 1 : g.nt;
+    const AxisSetup at = setup_axis(st, dx, lo, hi, g.rt, g.nt, st_t);
+    const AxisSetup as = setup_axis(ss, dy, lo, hi, g.rs, g.ns, st_s);
     const AxisSetup az = setup_axis(g.sz, dz, lo, hi, g.rz, g.nz, g.nt * g.ns);
     const double lam_end = hi - lo;
     chord = (float)lam_end;
 
     const unsigned nbytes = (unsigned)((size_t)g.nt * g.ns * g.nz * sizeof(float));
-    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)vol, 0, (int)nbytes, kBufDword3);
-    int idx = (az.i0 * g.ns + as.i0) * g.nt + at.i0;
+    __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(tr ? g.volT : vol), 0, (int)nbytes, kBufDword3);
+    int idx = az.i0 * (g.nt * g.ns) + as.i0 * st_s + at.i0 * st_t;
     const double inf = __builtin_huge_val();
     double nt = at.n > 0 ? at.next0 : inf;
     double ns = as.n > 0 ? as.next0 : inf;

@@ -77,6 +77,25 @@ __global__ void k_iter_inc(int *iter)
     if (threadIdx.x == 0) iter[0] = iter[0] + 1;
 }
 
+// out[z][t][s] = in[z][s][t]: 32 x 32 tiles through LDS (padded row against bank conflicts).
+__global__ __launch_bounds__(256) void k_transpose_st(const float *__restrict__ in, float *__restrict__ out, int nt,
+                                                      int ns)
+{
+    __shared__ float tile[32][33];
+    const int t0 = blockIdx.x * 32, s0 = blockIdx.y * 32;
+    const size_t zo = (size_t)blockIdx.z * nt * ns;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int t = t0 + tx, s = s0 + k;
+        if (t < nt && s < ns) tile[k][tx] = in[zo + (size_t)s * nt + t];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int s = s0 + tx, t = t0 + k;
+        if (t < nt && s < ns) out[zo + (size_t)t * ns + s] = tile[tx][k];
+    }
+}
+
 // Per-block (sum x^2, sum x).
 __global__ __launch_bounds__(256) void k_sum_sq_sum(const float *x, size_t n, double *partials)
 {
@@ -131,6 +150,12 @@ hipError_t launch_reduce_partials(const double *partials, int n, double *out2, h
 hipError_t launch_reduce_jobs(const ReduceJob *jobs, int njobs, const int *iter, double *sums, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reduce_jobs, dim3(njobs), dim3(256), 0, st, jobs, iter, sums, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_st(const float *in, float *out, int nt, int ns, int nz, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_transpose_st, dim3((nt + 31) / 32, (ns + 31) / 32, nz), dim3(256), 0, st, in, out, nt, ns);
     return hipGetLastError();
 }
 
