@@ -1713,7 +1713,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     std::vector<FpJob> fj(n);
     std::vector<ReduceJob> rj(n);
     std::vector<std::vector<AdjointJob>> groups(3), ones_groups(3);   // by LDS pitch 40 / 72 / 136
-    std::vector<int> group_blocks(3, 0);
+    std::vector<int> group_blocks(3, 0), group_cm(3, 4);
     const int pitches[3] = {40, 72, 136};
     int fp_total = 0;
     for (int p = 0; p < n; ++p) {
@@ -1760,6 +1760,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         ones_groups[gi].push_back(AdjointJob{P, (const int4 *)q.F.buf.ptr, nullptr, (float *)q.vinv.ptr, U,
                                              group_blocks[gi], nwg});
         group_blocks[gi] += nwg;
+        group_cm[gi] = L.cm;
         if (!o->use_x0) TRY(check_launch(ctx, launch_fill(q.xp, 0.0f, q.nvox, st), "fill"));
     }
     // job tables in device memory (adjoint groups packed one after another)
@@ -1789,7 +1790,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     // column sums V = 1/(A'1 + 1e-9) and ||b|| per problem
     for (int gi = 0; gi < 3; ++gi)
         if (!groups[gi].empty())
-            TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], dones + gofs[gi], (int)groups[gi].size(),
+            TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], dones + gofs[gi], (int)groups[gi].size(),
                                                          group_blocks[gi], 0, 1, st), "batched column sums"));
     for (int p = 0; p < n; ++p) {
         BatchProblem &q = B.probs[p];
@@ -1817,7 +1818,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         if (!update) return CTR_OK;
         for (int gi = 0; gi < 3; ++gi)
             if (!groups[gi].empty())
-                TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], dadj + gofs[gi], (int)groups[gi].size(),
+                TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], dadj + gofs[gi], (int)groups[gi].size(),
                                                              group_blocks[gi], 1, 0, s), "batched update"));
         return check_launch(ctx, launch_iter_inc(diter, s), "iteration counter");
     };

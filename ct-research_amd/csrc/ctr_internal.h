@@ -108,6 +108,7 @@ int ray_fp_blocks(const RayParams &g);
 struct BpPlan {
     int faces;              // footprints of voxel boxes (exact adjoint) instead of voxel centres
     int tz, wp, capf, hmax;
+    int cm;                 // exact adjoint: candidate detector columns per chunk
     int tiles_t, tiles_s, tiles_z;
     size_t nfoot;
 };
@@ -129,8 +130,8 @@ struct AdjointJob {
     SirtUpdate U;
     int block0, nblocks;    // this job's blocks in the launch
 };
-hipError_t launch_adjoint_batched(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt, int ones_input,
-                                  hipStream_t st);
+hipError_t launch_adjoint_batched(int wp, int cm, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
+                                  int ones_input, hipStream_t st);
 
 // Batched SIRT residual pass: r = (b - A x)/(A1 + 1e-9) and per-block (sum e^2, sum |e|) per job.
 struct FpJob {

@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_fdk_quirk(const float *__restrict__ pro
 // of the column.  Segment length = (lam_hi - lam_lo) * |(a, b, xi)|; the
 // norm sqrt(p^2 + D^2 + xi^2) is folded into the staged samples.
 // ---------------------------------------------------------------------------
-constexpr int kCMax = 6;
+constexpr int kCMax = 6;   // LDS slack after each staged row and column table: the widest candidate chunk
 
 // Slab interval of one axis for a ray parameterised S + lam*dir: the voxel
 // [lo_rel, lo_rel + pitch] (relative to S) holds lam in [l1, l2] sorted;
@@ -397,7 +397,7 @@ __device__ __forceinline__ void slice_rows(const BpParams &P, float zb, float zt
 // for the view's footprint rectangle: y*|d| per sample (|d| = sqrt(p^2 +
 // D^2 + xi^2)), 1/a and 1/b per column and 1/xi per row, so a candidate
 // (ray, voxel) pair costs 5 VALU and one immediate-offset LDS read.
-template <int TZ, int WP, int CAPF>
+template <int TZ, int WP, int CAPF, int CM>
 __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, float *__restrict__ vol, const BpParams &P,
                                              const int4 *__restrict__ foot, const BpView *__restrict__ views,
                                              const SirtUpdate &U, int sirt, int ones, int tile, float *lds)
@@ -462,10 +462,10 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
         cu0 = max(cu0, clo_lim);
         cu1 = min(cu1, chi_lim);
         const float tlo = fmaf(fi, P.rt, -vw.st), slo = fmaf(fj, P.rs, -vw.ss);   // voxel low faces relative to S
-        for (int cb = cu0; cb <= cu1; cb += kCMax) {
-            float clo[kCMax], chi[kCMax];
+        for (int cb = cu0; cb <= cu1; cb += CM) {
+            float clo[CM], chi[CM];
 #pragma unroll
-            for (int q = 0; q < kCMax; ++q) {
+            for (int q = 0; q < CM; ++q) {
                 float ia, ib;
                 if (fits) {
                     ia = inva[cb - f.x + q];
@@ -497,14 +497,14 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                     if (fits) {
                         const float *sy = yd + (r - f.y) * WP + (cb - f.x);
 #pragma unroll
-                        for (int q = 0; q < kCMax; ++q) {
+                        for (int q = 0; q < CM; ++q) {
                             const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
                             a_k = fmaf(len, sy[q], a_k);
                         }
                     } else {
                         const float xi = P.xd[r], xi2 = fmaf(xi, xi, P.D * P.D);
 #pragma unroll
-                        for (int q = 0; q < kCMax; ++q) {
+                        for (int q = 0; q < CM; ++q) {
                             const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
                             if (len > 0.0f) {
                                 const int c = min(cb + q, P.np - 1);
@@ -535,18 +535,18 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
     }
 }
 
-template <int TZ, int WP, int CAPF>
+template <int TZ, int WP, int CAPF, int CM>
 __global__ __launch_bounds__(kThreads) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
                                                       const int4 *__restrict__ foot, const BpView *__restrict__ views,
                                                       SirtUpdate U, int sirt, int ones)
 {
     __shared__ float lds[CAPF];
-    adjoint_tile<TZ, WP, CAPF>(proj, vol, P, foot, views, U, sirt, ones, xcd_tile(blockIdx.x, gridDim.x), lds);
+    adjoint_tile<TZ, WP, CAPF, CM>(proj, vol, P, foot, views, U, sirt, ones, xcd_tile(blockIdx.x, gridDim.x), lds);
 }
 
 // Many independent problems in one launch: block b belongs to the job whose
 // block range holds it (a scalar scan of the small job table).
-template <int TZ, int WP, int CAPF>
+template <int TZ, int WP, int CAPF, int CM>
 __global__ __launch_bounds__(kThreads) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
                                                               int ones)
 {
@@ -556,8 +556,8 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_batched(const AdjointJob *
     const BpParams P = jobs[p].P;
     const SirtUpdate U = jobs[p].U;
     const int local = (int)blockIdx.x - jobs[p].block0;
-    adjoint_tile<TZ, WP, CAPF>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
-                               xcd_tile(local, jobs[p].nblocks), lds);
+    adjoint_tile<TZ, WP, CAPF, CM>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
+                                   xcd_tile(local, jobs[p].nblocks), lds);
 }
 
 constexpr int kBpTZ = 16;
@@ -613,6 +613,13 @@ BpPlan adjoint_plan(const BpParams &P)
     L.tiles_s = (P.ns + kTS - 1) / kTS;
     L.tiles_z = (P.nz + L.tz - 1) / L.tz;
     L.wp = footprint_pitch(P, true);
+    // candidate detector columns evaluated together per voxel (a voxel's shadow spans 2-4 at unit magnification);
+    // tuning knob CTR_ADJ_CM (3, 4 or 6)
+    L.cm = 4;
+    if (const char *e = std::getenv("CTR_ADJ_CM")) {
+        const int want = std::atoi(e);
+        if (want == 3 || want == 4 || want == 6) L.cm = want;
+    }
     L.capf = kAdjCap;
     L.hmax = (L.capf - 2 * L.wp - 2 * kCMax) / (L.wp + 1);
     L.nfoot = (size_t)L.tiles_t * L.tiles_s * L.tiles_z * P.nb;
@@ -670,6 +677,23 @@ hipError_t launch_bp_tiled(int kind, const BpParams &Pin, const BpPlan &L, const
                           : launch_bp_kind<kBpRef>(P, L, foot, proj, vol, st);
 }
 
+template <int CM>
+static hipError_t launch_adjoint_cm(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj,
+                                    float *vol, const SirtUpdate &U, int sirt, int ones_input, hipStream_t st)
+{
+    const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
+    if (L.wp == 40)
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, kAdjCap, CM>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, U, sirt, ones_input);
+    else if (L.wp == 72)
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, kAdjCap, CM>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
+                           P.views, U, sirt, ones_input);
+    else
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, kAdjCap, CM>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P,
+                           foot, P.views, U, sirt, ones_input);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
                           const SirtUpdate *upd, int ones_input, hipStream_t st)
 {
@@ -677,40 +701,41 @@ hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot
     P.tiles_t = L.tiles_t;
     P.tiles_s = L.tiles_s;
     P.tiles_z = L.tiles_z;
-    const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
     SirtUpdate U = {};
     int sirt = 0;
     if (upd) {
         U = *upd;
         sirt = 1;
     }
-    if (L.wp == 40)
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
-                           P.views, U, sirt, ones_input);
-    else if (L.wp == 72)
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
-                           P.views, U, sirt, ones_input);
-    else
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, kAdjCap>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
-                           P.views, U, sirt, ones_input);
-    return hipGetLastError();
+    if (L.cm == 3) return launch_adjoint_cm<3>(P, L, foot, proj, vol, U, sirt, ones_input, st);
+    if (L.cm == 4) return launch_adjoint_cm<4>(P, L, foot, proj, vol, U, sirt, ones_input, st);
+    return launch_adjoint_cm<6>(P, L, foot, proj, vol, U, sirt, ones_input, st);
 }
 
-hipError_t launch_adjoint_batched(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt, int ones_input,
-                                  hipStream_t st)
+template <int CM>
+static hipError_t launch_adjoint_batched_cm(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
+                                            int ones_input, hipStream_t st)
 {
     if (wp == 40)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, kAdjCap>), dim3(nblocks), dim3(kThreads), 0, st, jobs, njobs,
-                           sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, kAdjCap, CM>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
+                           njobs, sirt, ones_input);
     else if (wp == 72)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, kAdjCap>), dim3(nblocks), dim3(kThreads), 0, st, jobs, njobs,
-                           sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, kAdjCap, CM>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
+                           njobs, sirt, ones_input);
     else if (wp == 136)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, kAdjCap>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, kAdjCap, CM>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
                            njobs, sirt, ones_input);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+hipError_t launch_adjoint_batched(int wp, int cm, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
+                                  int ones_input, hipStream_t st)
+{
+    if (cm == 3) return launch_adjoint_batched_cm<3>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
+    if (cm == 4) return launch_adjoint_batched_cm<4>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
+    return launch_adjoint_batched_cm<6>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
 }
 
 }  // namespace ctr
