@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,14 @@ struct TapTable {
     DevBuf d_spec;
 };
 
+// Tile/layout choices of the cone-beam kernels for one geometry (defaults
+// until ctr_autotune_cone measures them).
+struct Tuning {
+    int bp_pitch = 80;      // row pitch of the FDK / BP_REF footprint buffers
+    int adj_cm = 4;         // candidate-column chunk of the exact adjoint
+    int fp_transpose = 1;   // forward projector reads the [z][t][s] copy for views along t
+};
+
 }  // namespace
 
 struct ctr_ctx {
@@ -78,6 +87,7 @@ struct ctr_ctx {
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
     std::vector<ctr_cnn *> nets;   // networks created on this context (released with it)
+    std::map<uint64_t, Tuning> tune;   // autotuned choices per cone geometry
 };
 
 namespace {
@@ -384,9 +394,16 @@ BpParams cone_bp_params_from(const ConeTables &T, const ctr_cone_geom *g, unsign
 
 // Footprint table of the tiled backprojection for the current cone geometry
 // (computed on the device once per geometry, reused by every later call).
+Tuning tuning_for(const ctr_ctx *ctx, uint64_t key)
+{
+    const auto it = ctx->tune.find(key);
+    return it == ctx->tune.end() ? Tuning{} : it->second;
+}
+
 ctr_status ensure_footprints_key(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint, uint64_t base_key)
 {
-    const BpPlan L = adjoint ? adjoint_plan(P) : bp_plan(P);
+    const Tuning tu = tuning_for(ctx, base_key);
+    const BpPlan L = adjoint ? adjoint_plan(P, tu.adj_cm) : bp_plan(P, tu.bp_pitch);
     uint64_t key = base_key;
     key = fnv(key, &L, sizeof L);
     if (fc.buf.ptr && fc.key == key) return CTR_OK;
@@ -410,9 +427,13 @@ ctr_status ensure_footprints(ctr_ctx *ctx, const BpParams &P, FootCache &fc, boo
     return ensure_footprints_key(ctx, P, fc, adjoint, ctx->cone.key);
 }
 
-// The [z][t][s] copy of a volume for the forward projector (its views along t read it).
-ctr_status fp_transposed(ctr_ctx *ctx, RayParams &R, const float *vol, DevBuf &buf, hipStream_t st)
+// The [z][t][s] copy of a volume for the forward projector (its views along t
+// read it), unless the geometry's tuning turned that layout off.
+ctr_status fp_transposed(ctr_ctx *ctx, RayParams &R, const float *vol, DevBuf &buf, hipStream_t st,
+                         bool enable = true)
 {
+    R.volT = nullptr;
+    if (!enable) return CTR_OK;
     ctr_status s = buf_reserve(ctx, buf, (size_t)R.nt * R.ns * R.nz * sizeof(float));
     if (s != CTR_OK) return s;
     s = check_launch(ctx, launch_transpose_st(vol, (float *)buf.ptr, R.nt, R.ns, R.nz, st), "volume transpose");
@@ -707,6 +728,15 @@ ctr_status check_filter_args(ctr_ctx *ctx, int filter)
     return CTR_OK;
 }
 
+// CTR_AUTOTUNE: tune a cone geometry on its first use in this context.
+ctr_status maybe_tune(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
+{
+    if (!(flags & CTR_AUTOTUNE)) return CTR_OK;
+    TRY(cone_tables(ctx, g));
+    if (ctx->tune.count(ctx->cone.key)) return CTR_OK;
+    return ctr_autotune_cone(ctx, g, nullptr);
+}
+
 ctr_status check_equiangular(ctr_ctx *ctx, const ctr_fan_geom *g)
 {
     const double g0 = std::atan(g->p[0] / g->distance), d0 = std::atan(g->p[1] / g->distance) - g0;
@@ -855,6 +885,7 @@ ctr_status ctr_cone_fp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *vol, f
 {
     TRY(check_cone(g, 0));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     const bool dev = flags & CTR_DEVICE_PTRS;
     TRY(cone_tables(ctx, g));
     const float *dv;
@@ -862,7 +893,7 @@ ctr_status ctr_cone_fp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *vol, f
     TRY(stage_in(ctx, dev, kWsIn, vol, nvox3(g), &dv));
     TRY(stage_out(ctx, dev, false, kWsOut, proj, nray3(g), &dp));
     RayParams R = cone_ray_params(ctx, g);
-    TRY(fp_transposed(ctx, R, dv, ctx->ws[kWsT], ctx->stream));
+    TRY(fp_transposed(ctx, R, dv, ctx->ws[kWsT], ctx->stream, tuning_for(ctx, ctx->cone.key).fp_transpose));
     TRY(check_launch(ctx, launch_ray_fp(R, dv, dp, ctx->stream), "cone forward projection"));
     return finish_out(ctx, dev, dp, proj, nray3(g));
 }
@@ -871,6 +902,7 @@ ctr_status ctr_cone_bp_ref(ctr_ctx *ctx, const ctr_cone_geom *g, const float *pr
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     const bool dev = flags & CTR_DEVICE_PTRS;
     TRY(cone_tables(ctx, g));
     const float *dp;
@@ -888,6 +920,7 @@ ctr_status ctr_cone_bp_adjoint(ctr_ctx *ctx, const ctr_cone_geom *g, const float
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     const bool dev = flags & CTR_DEVICE_PTRS;
     TRY(cone_tables(ctx, g));
     const float *dp;
@@ -923,6 +956,7 @@ ctr_status ctr_fdk_bp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *filtere
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     if (g->n_beta < 2) return fail(ctx, CTR_EINVAL, "FDK needs at least 2 views (dbeta = |beta[1]-beta[0]|)");
     const bool dev = flags & CTR_DEVICE_PTRS;
     TRY(cone_tables(ctx, g));
@@ -941,6 +975,7 @@ ctr_status ctr_fdk(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     TRY(check_filter_args(ctx, filter));
     if (g->n_beta < 2) return fail(ctx, CTR_EINVAL, "FDK needs at least 2 views (dbeta = |beta[1]-beta[0]|)");
     const bool dev = flags & CTR_DEVICE_PTRS;
@@ -1132,6 +1167,7 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     TRY(sirt_opts_check(ctx, opts));
     TRY(cone_tables(ctx, g));
     const RayParams R = cone_ray_params(ctx, g);
@@ -1139,9 +1175,10 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
     TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
     const BpPlan AL = ctx->foot_adj.plan;
     const int4 *AF = (const int4 *)ctx->foot_adj.buf.ptr;
+    const bool use_t = tuning_for(ctx, ctx->cone.key).fp_transpose;
     auto fp = [&](const float *xx, const float *bb, float *rr, double *part, int *nblk) {
         RayParams Rx = R;
-        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], ctx->stream));
+        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], ctx->stream, use_t));
         return check_launch(ctx, launch_ray_fp_residual(Rx, xx, bb, rr, part, nblk, ctx->stream), "SIRT residual");
     };
     auto ones = [&](float *vinv) {
@@ -1495,6 +1532,7 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     if (!std::isfinite(lambda)) return fail(ctx, CTR_EINVAL, "lambda must be finite");
     TRY(cone_tables(ctx, g));
     hipStream_t st = ctx->stream;
@@ -1520,7 +1558,7 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_
     TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
     int nblk = 0;
     RayParams Rx = R;
-    TRY(fp_transposed(ctx, Rx, x, ctx->ws[kWsT], st));
+    TRY(fp_transposed(ctx, Rx, x, ctx->ws[kWsT], st, tuning_for(ctx, ctx->cone.key).fp_transpose));
     TRY(check_launch(ctx, launch_ray_fp_residual(Rx, x, b, r, part, &nblk, st), "SIRT residual"));
     TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
     TRY(check_launch(ctx, launch_sirt_step(vinv, gb, lambda, nvox, s, st), "SIRT step"));
@@ -1532,6 +1570,7 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
 {
     TRY(check_cone(g, 1));
     TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
     TRY(need_net(net));
     if (net->ctx != ctx) return fail(ctx, CTR_EINVAL, "network belongs to another context");
     if (!o) return fail(ctx, CTR_EINVAL, "options are NULL");
@@ -1581,7 +1620,7 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     auto residual_into = [&](const float *xx, float *rr, double *res) -> ctr_status {
         int nblk = 0;
         RayParams Rx = R;
-        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], st));
+        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], st, tuning_for(ctx, ctx->cone.key).fp_transpose));
         TRY(check_launch(ctx, launch_ray_fp_residual(Rx, xx, b, rr, part, &nblk, st), "SIRT residual"));
         TRY(check_launch(ctx, launch_reduce_partials(part, nblk, sums, st), "reduce"));
         double hh[2];
@@ -1705,6 +1744,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         TRY(check_cone(&pr[p].geom, 1));
         if (!pr[p].b || !pr[p].x) return fail(ctx, CTR_EINVAL, "problem %d: data pointer is NULL", p);
     }
+    for (int p = 0; p < n; ++p) TRY(maybe_tune(ctx, &pr[p].geom, flags));
     const bool dev = flags & CTR_DEVICE_PTRS;
     BatchRun B(ctx, n);
     hipStream_t st = ctx->stream;
@@ -1738,8 +1778,10 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         }
         TRY(buf_reserve(ctx, q.r, q.nray * sizeof(float)));
         TRY(buf_reserve(ctx, q.vinv, q.nvox * sizeof(float)));
-        TRY(buf_reserve(ctx, q.vt, q.nvox * sizeof(float)));
-        q.R.volT = (const float *)q.vt.ptr;   // refreshed from x before every residual pass
+        if (tuning_for(ctx, q.T.key).fp_transpose) {
+            TRY(buf_reserve(ctx, q.vt, q.nvox * sizeof(float)));
+            q.R.volT = (const float *)q.vt.ptr;   // refreshed from x before every residual pass
+        }
         q.fp_blocks = fp_job_blocks(q.R, &q.gx, &q.gy);
         TRY(buf_reserve(ctx, q.part, sizeof(double) * 2 * (size_t)std::max(q.fp_blocks, 4096)));
         fj[p] = FpJob{q.R, q.xp, q.bp, (float *)q.r.ptr, (double *)q.part.ptr, fp_total, q.gx, q.gy};
@@ -1810,8 +1852,9 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     auto iteration = [&](hipStream_t s, bool update) -> ctr_status {
         for (int p = 0; p < n; ++p) {
             const BatchProblem &q = B.probs[p];
-            TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, s),
-                             "volume transpose"));
+            if (q.R.volT)
+                TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, s),
+                                 "volume transpose"));
         }
         TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_total, s), "batched residual"));
         TRY(check_launch(ctx, launch_reduce_jobs(drj, n, diter, dsums, s), "batched reduce"));
@@ -1847,6 +1890,105 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         if (!dev)
             HIP_TRY(ctx, hipMemcpy(pr[p].x, B.probs[p].xp, B.probs[p].nvox * sizeof(float), hipMemcpyDeviceToHost));
     }
+    return CTR_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Autotuner
+// ===========================================================================
+extern "C" {
+
+ctr_status ctr_autotune_cone(ctr_ctx *ctx, const ctr_cone_geom *g, ctr_tune_result *out)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(cone_tables(ctx, g));
+    const uint64_t key = ctx->cone.key;
+    hipStream_t st = ctx->stream;
+    const size_t nvox = nvox3(g), nray = nray3(g);
+    TRY(buf_reserve(ctx, ctx->ws[kWsIn], nvox * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsTmp], nray * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsOut], nvox * sizeof(float)));
+    float *vol = (float *)ctx->ws[kWsIn].ptr, *proj = (float *)ctx->ws[kWsTmp].ptr, *dst = (float *)ctx->ws[kWsOut].ptr;
+    // addresses and trip counts do not depend on the values: constant data suffices
+    TRY(check_launch(ctx, launch_fill(vol, 1.0f, nvox, st), "fill"));
+    TRY(check_launch(ctx, launch_fill(proj, 1.0f, nray, st), "fill"));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    hipError_t ec = hipEventCreate(&e1);
+    if (ec != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, CTR_EHIP, "hipEventCreate: %s", hipGetErrorString(ec));
+    }
+    struct Events {
+        hipEvent_t a, b;
+        ~Events()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{e0, e1};
+    // one warm run, then the mean of three
+    auto timed = [&](auto run, double *ms) -> ctr_status {
+        TRY(run());
+        HIP_TRY(ctx, hipEventRecord(ev.a, st));
+        for (int r = 0; r < 3; ++r) TRY(run());
+        HIP_TRY(ctx, hipEventRecord(ev.b, st));
+        HIP_TRY(ctx, hipEventSynchronize(ev.b));
+        float f = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&f, ev.a, ev.b));
+        *ms = f / 3.0;
+        return CTR_OK;
+    };
+    ctr_tune_result res{};
+    Tuning best;
+    const BpParams P = cone_bp_params(ctx, g, 0);
+    const int pitches[3] = {64, 72, 80}, cms[3] = {3, 4, 6};
+    double bestv = 1e300;
+    for (int i = 0; i < 3; ++i) {
+        Tuning t = best;
+        t.bp_pitch = pitches[i];
+        ctx->tune[key] = t;
+        TRY(ensure_footprints(ctx, P, ctx->foot_bp, false));
+        TRY(timed([&]() {
+            return check_launch(ctx, launch_bp_tiled(kBpFdk, P, ctx->foot_bp.plan, (const int4 *)ctx->foot_bp.buf.ptr,
+                                                     proj, dst, st), "FDK backprojection");
+        }, &res.bp_ms[i]));
+        if (res.bp_ms[i] < bestv) {
+            bestv = res.bp_ms[i];
+            res.bp_pitch = pitches[i];
+        }
+    }
+    best.bp_pitch = res.bp_pitch;
+    bestv = 1e300;
+    for (int i = 0; i < 3; ++i) {
+        Tuning t = best;
+        t.adj_cm = cms[i];
+        ctx->tune[key] = t;
+        TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+        TRY(timed([&]() {
+            return check_launch(ctx, launch_adjoint(P, ctx->foot_adj.plan, (const int4 *)ctx->foot_adj.buf.ptr, proj,
+                                                    dst, nullptr, 0, st), "adjoint");
+        }, &res.adj_ms[i]));
+        if (res.adj_ms[i] < bestv) {
+            bestv = res.adj_ms[i];
+            res.adj_cm = cms[i];
+        }
+    }
+    best.adj_cm = res.adj_cm;
+    for (int i = 0; i < 2; ++i) {
+        TRY(timed([&]() {
+            RayParams R = cone_ray_params(ctx, g);
+            TRY(fp_transposed(ctx, R, vol, ctx->ws[kWsT], st, i == 1));
+            return check_launch(ctx, launch_ray_fp(R, vol, proj, st), "forward projection");
+        }, &res.fp_ms[i]));
+    }
+    res.fp_transpose = res.fp_ms[1] < res.fp_ms[0] ? 1 : 0;
+    best.fp_transpose = res.fp_transpose;
+    ctx->tune[key] = best;
+    if (out) *out = res;
     return CTR_OK;
 }
 

@@ -112,8 +112,8 @@ struct BpPlan {
     int tiles_t, tiles_s, tiles_z;
     size_t nfoot;
 };
-BpPlan bp_plan(const BpParams &P);
-BpPlan adjoint_plan(const BpParams &P);
+BpPlan bp_plan(const BpParams &P, int pitch);        // pitch: 64, 72 or 80 floats per staged row
+BpPlan adjoint_plan(const BpParams &P, int cm);      // cm: 3, 4 or 6 candidate columns per chunk
 hipError_t launch_bp_footprints(const BpParams &P, const BpPlan &L, int4 *foot, hipStream_t st);
 hipError_t launch_bp_tiled(int kind, const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj,
                            float *vol, hipStream_t st);

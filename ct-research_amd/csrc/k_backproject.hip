@@ -580,7 +580,7 @@ static int footprint_pitch(const BpParams &P, bool faces)
     return wmax <= 40 ? 40 : (wmax <= 72 ? 72 : 136);
 }
 
-BpPlan bp_plan(const BpParams &P)
+BpPlan bp_plan(const BpParams &P, int pitch)
 {
     BpPlan L{};
     L.faces = 0;
@@ -588,15 +588,13 @@ BpPlan bp_plan(const BpParams &P)
     L.tiles_t = (P.nt + kTX - 1) / kTX;
     L.tiles_s = (P.ns + kTS - 1) / kTS;
     L.tiles_z = (P.nz + L.tz - 1) / L.tz;
-    // row pitch of the staged footprint: 80 floats puts neighbouring rows 16
-    // banks apart (fewest bank conflicts measured at 256^3 / 512^2);
-    // tuning knob CTR_BP_PITCH (64, 72 or 80)
+    // row pitch of the staged footprint (80 floats puts neighbouring rows 16
+    // banks apart); the autotuner's choice, overridden by CTR_BP_PITCH
+    int want = pitch;
+    if (const char *e = std::getenv("CTR_BP_PITCH")) want = std::atoi(e);
     int shape = 2;
-    if (const char *e = std::getenv("CTR_BP_PITCH")) {
-        const int want = std::atoi(e);
-        for (int q = 0; q < 3; ++q)
-            if (kBpShapes[q][0] == want) shape = q;
-    }
+    for (int q = 0; q < 3; ++q)
+        if (kBpShapes[q][0] == want) shape = q;
     L.wp = kBpShapes[shape][0];
     L.hmax = kBpShapes[shape][1];
     L.capf = 2 * L.wp * L.hmax;
@@ -604,7 +602,7 @@ BpPlan bp_plan(const BpParams &P)
     return L;
 }
 
-BpPlan adjoint_plan(const BpParams &P)
+BpPlan adjoint_plan(const BpParams &P, int cm)
 {
     BpPlan L{};
     L.faces = 1;
@@ -613,13 +611,12 @@ BpPlan adjoint_plan(const BpParams &P)
     L.tiles_s = (P.ns + kTS - 1) / kTS;
     L.tiles_z = (P.nz + L.tz - 1) / L.tz;
     L.wp = footprint_pitch(P, true);
-    // candidate detector columns evaluated together per voxel (a voxel's shadow spans 2-4 at unit magnification);
-    // tuning knob CTR_ADJ_CM (3, 4 or 6)
-    L.cm = 4;
-    if (const char *e = std::getenv("CTR_ADJ_CM")) {
-        const int want = std::atoi(e);
-        if (want == 3 || want == 4 || want == 6) L.cm = want;
-    }
+    // candidate detector columns evaluated together per voxel (a voxel's
+    // shadow spans 2-4 at unit magnification); the autotuner's choice,
+    // overridden by CTR_ADJ_CM (3, 4 or 6)
+    int want = cm;
+    if (const char *e = std::getenv("CTR_ADJ_CM")) want = std::atoi(e);
+    L.cm = (want == 3 || want == 6) ? want : 4;
     L.capf = kAdjCap;
     L.hmax = (L.capf - 2 * L.wp - 2 * kCMax) / (L.wp + 1);
     L.nfoot = (size_t)L.tiles_t * L.tiles_s * L.tiles_z * P.nb;

@@ -5,7 +5,7 @@ libctr.so (include/ctr/ctr.h).  Every operator runs in HIP kernels; a
 missing library raises instead of falling back to any CPU path.
 """
 from ._lib import (CNN_LAYERS, CNN_NUM_PARAMS, Cnn, cnn_init_params, cnn_split_params)
-from ._lib import (ACCUMULATE, DEVICE_PTRS, FAN_EQUIANGULAR, FAN_FLAT, FILTER_RL_HALF, FILTER_SL_HALF,
+from ._lib import (ACCUMULATE, AUTOTUNE, DEVICE_PTRS, FAN_EQUIANGULAR, FAN_FLAT, FILTER_RL_HALF, FILTER_SL_HALF,
                    REF_QUIRKS, ConeGeom, Context, CtrError, FanGeom, default_context, lib, version, LIB_PATH,
                    EXPORTS)
 from . import geometry, metrics, phantoms

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libctr.so")
 
 OK, EINVAL, ENOMEM, EHIP, ETOOBIG, EFAULT_INJECTED, ENODEV = range(7)
-DEVICE_PTRS, ACCUMULATE, REF_QUIRKS = 0x1, 0x2, 0x4
+DEVICE_PTRS, ACCUMULATE, REF_QUIRKS, AUTOTUNE = 0x1, 0x2, 0x4, 0x8
 FILTER_SL_HALF, FILTER_RL_HALF = 0, 1
 FAN_FLAT, FAN_EQUIANGULAR = 0, 1
 
@@ -28,7 +28,7 @@ EXPORTS = [
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
-    "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched",
+    "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
 ]
 
 CNN_NUM_PARAMS = 57985
@@ -59,6 +59,11 @@ class SirtOptsC(ctypes.Structure):
     _fields_ = [("n_iter", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
                 ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
                 ("residual_l2", ctypes.c_void_p), ("residual_l1", ctypes.c_void_p)]
+
+
+class TuneResultC(ctypes.Structure):
+    _fields_ = [("bp_pitch", ctypes.c_int), ("adj_cm", ctypes.c_int), ("fp_transpose", ctypes.c_int),
+                ("bp_ms", ctypes.c_double * 3), ("adj_ms", ctypes.c_double * 3), ("fp_ms", ctypes.c_double * 2)]
 
 
 class SirtProblemC(ctypes.Structure):
@@ -120,6 +125,7 @@ def lib():
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint]
+        L.ctr_autotune_cone.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.ctr_sirt3d_batched.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_uint]
         L.ctr_sirt3d_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -328,6 +334,13 @@ class Context:
 
     def sirt2d(self, g, b, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0=None, residuals=True, flags=0):
         return self._sirt(lib().ctr_sirt2d, g, b, n_iter, lam, lo, hi, x0, residuals, flags)
+
+    def autotune_cone(self, g):
+        """Time the tile/layout variants on geometry g; the fastest are kept for later calls."""
+        r = TuneResultC()
+        self._check(lib().ctr_autotune_cone(self.h, ctypes.byref(g.c), ctypes.byref(r)))
+        return {"bp_pitch": r.bp_pitch, "adj_cm": r.adj_cm, "fp_transpose": r.fp_transpose,
+                "bp_ms": list(r.bp_ms), "adj_ms": list(r.adj_ms), "fp_ms": list(r.fp_ms)}
 
     def sirt3d_batched(self, problems, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0s=None, flags=0):
         """problems: [(ConeGeom, b)], b numpy or CUDA tensors (all alike).

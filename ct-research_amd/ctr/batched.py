@@ -50,17 +50,19 @@ def main(argv=None):
     ap.add_argument("--problems", type=int, default=16)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="tune every problem's geometry on first use")
     args = ap.parse_args(argv)
     import torch
     ctx = _lib.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     probs = make_problems(ctx, args.problems)
     work = sum(2.0 * g.nvox * len(g.beta) * args.iters for g, *_ in probs)
-    # warm-up: tables, footprints, code objects
-    ctx.sirt3d_batched([(g, b) for g, b, *_ in probs], 1)
+    flags = _lib.AUTOTUNE if args.autotune else 0
+    # warm-up: tables, footprints, code objects (and the tuning, when asked)
+    ctx.sirt3d_batched([(g, b) for g, b, *_ in probs], 1, flags=flags)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xs, res = ctx.sirt3d_batched([(g, b) for g, b, *_ in probs], args.iters)
+    xs, res = ctx.sirt3d_batched([(g, b) for g, b, *_ in probs], args.iters, flags=flags)
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
     out = {
@@ -68,15 +70,16 @@ def main(argv=None):
         "unit": "Gvoxel-upd/s (2 Nvox Nviews per iteration)",
         "problems": [{"n": n, "views": v, "detector": d, "residual_final": round(float(r[-1]), 6)}
                      for (g, b, n, v, d), r in zip(probs, res)],
-        "iters": args.iters, "batched_s": round(t_batched, 4), "batched_gups": round(work / t_batched / 1e9, 2),
+        "iters": args.iters, "autotune": bool(args.autotune),
+        "batched_s": round(t_batched, 4), "batched_gups": round(work / t_batched / 1e9, 2),
     }
     if not args.no_sequential:
         for g, b, *_ in probs:
-            ctx.sirt3d(g, b, 1)
+            ctx.sirt3d(g, b, 1, flags=flags)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for (g, b, *_), xb in zip(probs, xs):
-            ctx.sirt3d(g, b, args.iters, residuals=False)
+            ctx.sirt3d(g, b, args.iters, residuals=False, flags=flags)
         torch.cuda.synchronize()
         t_seq = time.perf_counter() - t0
         out.update({"sequential_s": round(t_seq, 4), "sequential_gups": round(work / t_seq / 1e9, 2)})

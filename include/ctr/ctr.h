@@ -60,6 +60,7 @@ typedef enum ctr_status {
 #define CTR_ACCUMULATE 0x2u  /* backprojections add into the output instead of overwriting it */
 #define CTR_REF_QUIRKS 0x4u  /* reproduce the reference's documented arithmetic quirks
                                 (fan BP entry-face test, FDK stale sample above the detector) */
+#define CTR_AUTOTUNE 0x8u    /* cone beam: run ctr_autotune_cone for this geometry on its first use */
 
 /* Ramp filters (half-scaled: the 1/2 redundancy weight of a 360-degree scan is folded in). */
 #define CTR_FILTER_SL_HALF 0 /* Shepp-Logan  h[n] = -1/(pi^2 d^2 (4n^2-1)) */
@@ -197,6 +198,22 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
 /* 2-D fan-beam SIRT (Iterative_algorithm/SIRT.m:126-154). */
 ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
                       unsigned flags);
+
+/* ---- autotuner ----------------------------------------------------------
+ * Times the tile and layout variants of the cone-beam kernels on the given
+ * geometry (synthetic data, HIP events on the context's stream) and keeps the
+ * fastest for every later call on this geometry in this context.  No
+ * counterpart in the reference. */
+typedef struct ctr_tune_result {
+    int bp_pitch;           /* chosen row pitch (floats) of the FDK / BP_REF footprint buffers */
+    int adj_cm;             /* chosen candidate-column chunk of the exact adjoint */
+    int fp_transpose;       /* 1: views along t read a [z][t][s] copy of the volume */
+    double bp_ms[3];        /* FDK backprojection, pitch 64 / 72 / 80 */
+    double adj_ms[3];       /* exact adjoint, chunk 3 / 4 / 6 */
+    double fp_ms[2];        /* forward projection, without / with the transposed copy */
+} ctr_tune_result;
+
+ctr_status ctr_autotune_cone(ctr_ctx *ctx, const ctr_cone_geom *g, ctr_tune_result *out);
 
 /* Many independent SIRT3D problems (uneven volumes, views and detectors)
  * advanced together: each iteration is one batched residual launch, one
