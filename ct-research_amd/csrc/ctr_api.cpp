@@ -49,6 +49,14 @@ struct FanTables {
     DevBuf d_view, d_p, d_zero, d_pf, d_gam, d_colw, d_xf;
 };
 
+struct ParTables {
+    uint64_t key = 0;
+    bool valid = false;
+    std::vector<double> h_view, h_t, h_zero;   // projector: [nb][4] sin, cos of theta; ray offsets
+    std::vector<float> h_vf;                   // FBP: [nb][2] sin, cos of the single-precision theta
+    DevBuf d_view, d_t, d_zero, d_vf;
+};
+
 struct FootCache {
     DevBuf buf;
     uint64_t key = 0;
@@ -83,6 +91,7 @@ struct ctr_ctx {
     DevBuf ws[kWsCount];
     ConeTables cone;
     FanTables fan;
+    ParTables par;
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
@@ -250,6 +259,23 @@ ctr_status check_fan(const ctr_fan_geom *g, int need_uniform)
     for (int i = 0; i < g->n_beta; ++i)
         if (!std::isfinite(g->beta[i])) return fail(nullptr, CTR_EINVAL, "beta[%d] is not finite", i);
     return check_axis(g->p, g->n_p, need_uniform != 0, "p");
+}
+
+ctr_status check_par(const ctr_par_geom *g, int need_uniform)
+{
+    if (!g) return fail(nullptr, CTR_EINVAL, "geometry is NULL");
+    if (g->nt < 1 || g->ns < 1) return fail(nullptr, CTR_EINVAL, "image dimensions must be positive");
+    if ((double)g->nt * g->ns > (double)(1 << 30))
+        return fail(nullptr, CTR_ETOOBIG, "image has more than 2^30 pixels");
+    for (int a = 0; a < 2; ++a)
+        if (!finite_pos(g->size[a])) return fail(nullptr, CTR_EINVAL, "size[%d] must be positive and finite", a);
+    if (g->n_theta < 1 || g->n_t < 1) return fail(nullptr, CTR_EINVAL, "detector/view counts must be positive");
+    if (g->n_theta > 65535) return fail(nullptr, CTR_ETOOBIG, "at most 65535 views");
+    if (need_uniform && g->n_t < 2) return fail(nullptr, CTR_EINVAL, "backprojection needs at least 2 detector bins");
+    if (!g->theta) return fail(nullptr, CTR_EINVAL, "theta is NULL");
+    for (int i = 0; i < g->n_theta; ++i)
+        if (!std::isfinite(g->theta[i])) return fail(nullptr, CTR_EINVAL, "theta[%d] is not finite", i);
+    return check_axis(g->t, g->n_t, need_uniform != 0, "t");
 }
 
 ctr_status need_ctx(ctr_ctx *ctx)
@@ -561,6 +587,108 @@ FanParams fan_params(ctr_ctx *ctx, const ctr_fan_geom *g, unsigned flags)
     return P;
 }
 
+uint64_t par_key(const ctr_par_geom *g)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, &g->nt, sizeof(int) * 2);
+    h = fnv(h, g->size, sizeof(double) * 2);
+    h = fnv(h, &g->n_theta, sizeof(int) * 2);
+    h = fnv(h, g->theta, sizeof(double) * g->n_theta);
+    h = fnv(h, g->t, sizeof(double) * g->n_t);
+    return h;
+}
+
+ctr_status par_tables(ctr_ctx *ctx, const ctr_par_geom *g)
+{
+    ParTables &T = ctx->par;
+    const uint64_t key = par_key(g);
+    if (T.valid && T.key == key) return CTR_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    T.valid = false;
+    const int nb = g->n_theta, n = g->n_t;
+    T.h_view.assign(4 * (size_t)nb, 0.0);
+    T.h_vf.resize(2 * (size_t)nb);
+    for (int b = 0; b < nb; ++b) {
+        T.h_view[4 * b + 0] = std::sin(g->theta[b]);
+        T.h_view[4 * b + 1] = std::cos(g->theta[b]);
+        const double th = (double)(float)g->theta[b];   // FBPparallel receives single angles
+        T.h_vf[2 * b + 0] = (float)std::sin(th);
+        T.h_vf[2 * b + 1] = (float)std::cos(th);
+    }
+    T.h_t.assign(g->t, g->t + n);
+    T.h_zero.assign(1, 0.0);
+    ctr_status s;
+    if ((s = upload(ctx, T.d_view, T.h_view.data(), sizeof(double) * T.h_view.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_t, T.h_t.data(), sizeof(double) * n)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_zero, T.h_zero.data(), sizeof(double))) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_vf, T.h_vf.data(), sizeof(float) * T.h_vf.size())) != CTR_OK) return s;
+    T.key = key;
+    T.valid = true;
+    return CTR_OK;
+}
+
+// The parallel projector runs the shared Siddon kernel: one detector row,
+// rays through the foot points centre + t (cos, sin) along (-sin, cos).
+RayParams par_ray_params(ctr_ctx *ctx, const ctr_par_geom *g)
+{
+    const ParTables &T = ctx->par;
+    RayParams r{};
+    r.nt = g->nt;
+    r.ns = g->ns;
+    r.nz = 1;
+    r.np = g->n_t;
+    r.nx = 1;
+    r.nb = g->n_theta;
+    r.rt = g->size[0] / g->nt;
+    r.rs = g->size[1] / g->ns;
+    r.rz = 1.0;
+    r.bt = g->size[0];
+    r.bs = g->size[1];
+    r.bz = 1.0;
+    r.sz = 0.5;
+    r.parallel = 1;
+    r.view = (const double *)T.d_view.ptr;
+    r.p = (const double *)T.d_t.ptr;
+    r.xi = (const double *)T.d_zero.ptr;
+    return r;
+}
+
+// Pixel-driven parameters.  single_axis: the detector grid of the float
+// t positions (FBPparallel receives single arrays), else of the double ones.
+ParBpParams par_bp_params(ctr_ctx *ctx, const ctr_par_geom *g, bool single_axis, unsigned flags)
+{
+    const ParTables &T = ctx->par;
+    ParBpParams P{};
+    P.nt = g->nt;
+    P.ns = g->ns;
+    P.np = g->n_t;
+    P.nb = g->n_theta;
+    P.rt_d = g->size[0] / g->nt;
+    P.rs_d = g->size[1] / g->ns;
+    P.ct_d = g->size[0] / 2;
+    P.cs_d = g->size[1] / 2;
+    P.rt = (float)P.rt_d;
+    P.rs = (float)P.rs_d;
+    P.ct = (float)P.ct_d;
+    P.cs = (float)P.cs_d;
+    const int n = g->n_t;
+    if (n >= 2) {
+        // uniform grid through the end samples
+        const double t0 = single_axis ? (double)(float)g->t[0] : g->t[0];
+        const double t1 = single_axis ? (double)(float)g->t[n - 1] : g->t[n - 1];
+        const double dt = (t1 - t0) / (n - 1);
+        P.u_scale = (float)(1.0 / dt);
+        P.u_off = (float)(-t0 / dt);
+    }
+    P.umax = (float)(n - 1);
+    P.dtheta = g->n_theta >= 2 ? (float)std::fabs((double)(float)g->theta[1] - (double)(float)g->theta[0]) : 0.0f;
+    P.view = (const float *)T.d_vf.ptr;
+    P.view_d = (const double *)T.d_view.ptr;
+    P.t_d = (const double *)T.d_t.ptr;
+    P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
+    return P;
+}
+
 // Half-scaled ramp taps times the pitch, two-sided: h2[k] = d * h[|k-(n-1)|].
 ctr_status tap_table(ctr_ctx *ctx, int filter, int equiangular, double d, int n)
 {
@@ -670,6 +798,8 @@ size_t nvox3(const ctr_cone_geom *g) { return (size_t)g->nt * g->ns * g->nz; }
 size_t nray3(const ctr_cone_geom *g) { return (size_t)g->n_beta * g->n_p * g->n_xi; }
 size_t nvox2(const ctr_fan_geom *g) { return (size_t)g->nt * g->ns; }
 size_t nray2(const ctr_fan_geom *g) { return (size_t)g->n_beta * g->n_p; }
+size_t npix(const ctr_par_geom *g) { return (size_t)g->nt * g->ns; }
+size_t nrayp(const ctr_par_geom *g) { return (size_t)g->n_theta * g->n_t; }
 
 #define TRY(expr)                          \
     do {                                   \
@@ -735,6 +865,23 @@ ctr_status maybe_tune(ctr_ctx *ctx, const ctr_cone_geom *g, unsigned flags)
     TRY(cone_tables(ctx, g));
     if (ctx->tune.count(ctx->cone.key)) return CTR_OK;
     return ctr_autotune_cone(ctx, g, nullptr);
+}
+
+// FBPparallel.cu:6-39 (half-scaled taps, pitch |t[1]-t[0]| of the single t)
+ctr_status par_filter_impl(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *in, float *out)
+{
+    if (g->n_t > 2048) return fail(ctx, CTR_ETOOBIG, "ramp filter supports at most 2048 detector bins");
+    const double d = (double)std::fabs((float)g->t[1] - (float)g->t[0]);
+    TRY(tap_table(ctx, filter, 0, d, g->n_t));
+    FilterParams F{};
+    F.np = g->n_t;
+    F.nx = 1;
+    F.rows = g->n_theta;
+    F.mode = 2;
+    F.taps = (const double *)ctx->taps.d.ptr;
+    F.spec = ctx->taps.fft_len ? (const double *)ctx->taps.d_spec.ptr : nullptr;
+    F.fft_len = ctx->taps.fft_len;
+    return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "parallel ramp filter");
 }
 
 ctr_status check_equiangular(ctr_ctx *ctx, const ctr_fan_geom *g)
@@ -878,6 +1025,7 @@ ctr_status ctr_copy_to_host(ctr_ctx *ctx, void *dst, const void *src, size_t byt
 ctr_status ctr_cone_geom_check(const ctr_cone_geom *g, int need_uniform_pitch) { return check_cone(g, need_uniform_pitch); }
 
 ctr_status ctr_fan_geom_check(const ctr_fan_geom *g, int need_uniform_pitch) { return check_fan(g, need_uniform_pitch); }
+ctr_status ctr_par_geom_check(const ctr_par_geom *g, int need_uniform_pitch) { return check_par(g, need_uniform_pitch); }
 
 // ---- cone ------------------------------------------------------------------
 
@@ -1089,6 +1237,76 @@ ctr_status ctr_fan_fbp(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int fi
     TRY(check_launch(ctx, launch_fan_fbp_bp(P, filt, di, ctx->stream), "fan FBP backprojection"));
     if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return finish_out(ctx, dev, di, img, nvox2(g));
+}
+
+// ---- parallel beam ---------------------------------------------------------
+
+ctr_status ctr_par_fp(ctr_ctx *ctx, const ctr_par_geom *g, const float *img, float *proj, unsigned flags)
+{
+    TRY(check_par(g, 0));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(par_tables(ctx, g));
+    const float *di;
+    float *dp;
+    TRY(stage_in(ctx, dev, kWsIn, img, npix(g), &di));
+    TRY(stage_out(ctx, dev, false, kWsOut, proj, nrayp(g), &dp));
+    RayParams R = par_ray_params(ctx, g);
+    TRY(fp_transposed(ctx, R, di, ctx->ws[kWsT], ctx->stream));
+    TRY(check_launch(ctx, launch_ray_fp(R, di, dp, ctx->stream), "parallel forward projection"));
+    return finish_out(ctx, dev, dp, proj, nrayp(g));
+}
+
+ctr_status ctr_par_bp_adjoint(ctr_ctx *ctx, const ctr_par_geom *g, const float *proj, float *img, unsigned flags)
+{
+    TRY(check_par(g, 1));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(par_tables(ctx, g));
+    const float *dp;
+    float *di;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nrayp(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, img, npix(g), &di));
+    ParBpParams P = par_bp_params(ctx, g, false, flags);
+    TRY(check_launch(ctx, launch_par_adjoint(P, dp, di, nullptr, 0, ctx->stream), "parallel adjoint"));
+    return finish_out(ctx, dev, di, img, npix(g));
+}
+
+ctr_status ctr_par_filter(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *filtered,
+                          unsigned flags)
+{
+    TRY(check_par(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(par_tables(ctx, g));
+    const float *dp;
+    float *df;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nrayp(g), &dp));
+    TRY(stage_out(ctx, dev, false, kWsTmp, filtered, nrayp(g), &df));
+    TRY(par_filter_impl(ctx, g, filter, dp, df));
+    return finish_out(ctx, dev, df, filtered, nrayp(g));
+}
+
+ctr_status ctr_par_fbp(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *img, unsigned flags)
+{
+    TRY(check_par(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    if (g->n_theta < 2) return fail(ctx, CTR_EINVAL, "FBP needs at least 2 views (dtheta = |theta[1]-theta[0]|)");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(par_tables(ctx, g));
+    const float *dp;
+    float *di;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nrayp(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, img, npix(g), &di));
+    TRY(buf_reserve(ctx, ctx->ws[kWsTmp], nrayp(g) * sizeof(float)));
+    float *filt = (float *)ctx->ws[kWsTmp].ptr;
+    TRY(par_filter_impl(ctx, g, filter, dp, filt));
+    ParBpParams P = par_bp_params(ctx, g, true, flags);
+    TRY(check_launch(ctx, launch_par_fbp_bp(P, filt, di, ctx->stream), "parallel FBP backprojection"));
+    if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
+    return finish_out(ctx, dev, di, img, npix(g));
 }
 
 }  // extern "C"

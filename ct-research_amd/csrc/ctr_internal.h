@@ -27,6 +27,8 @@ struct RayParams {
     const double *xi;       // [nx] row positions at the isocentre
     const float *volT;      // optional [nz][nt][ns] copy of the volume, read by views whose
                             // central ray runs mostly along t (neighbouring rays then differ in s)
+    int parallel;           // parallel beam: ray (view, p) is the line through centre + p (cos, sin)
+                            // along (-sin, cos); view[4b+2..3] unused
 };
 
 // Per-view constants of the voxel-driven backprojectors, float:
@@ -68,7 +70,7 @@ struct SirtUpdate {
 // Row filter (pre-weight + convolution along p).
 struct FilterParams {
     int np, nx, rows;       // detector columns, rows per view, total rows
-    int mode;               // 0: w = D/sqrt(D^2+p^2+xi^2); 1: w = colw[p]
+    int mode;               // 0: w = D/sqrt(D^2+p^2+xi^2); 1: w = colw[p]; 2: w = 1
     float D;
     const float *pd;        // [np] column positions
     const float *xd;        // [nx] row positions
@@ -160,6 +162,24 @@ hipError_t launch_fan_bp_ref(const FanParams &P, const float *proj, float *img, 
 hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img, const SirtUpdate *upd,
                               int ones_input, hipStream_t st);
 hipError_t launch_fan_fbp_bp(const FanParams &P, const float *filt, float *img, hipStream_t st);
+
+// Parallel-beam pixel-driven kernels: the FBP backprojection (t = x cos + y sin
+// per pixel and view, linear interpolation on the uniform detector grid, times
+// dtheta) and the exact adjoint of the Siddon parallel projector.
+struct ParBpParams {
+    int nt, ns, np, nb;
+    float rt, rs, ct, cs;
+    double rt_d, rs_d, ct_d, cs_d;
+    float u_scale, u_off, umax;   // u = t*u_scale + u_off in sample units, valid in [0, umax)
+    float dtheta;
+    const float *view;            // FBP: [nb][2] sin, cos of the single-precision view angles
+    const double *view_d;         // adjoint: [nb][4] sin, cos (the projector's view table)
+    const double *t_d;            // adjoint: [np] ray offsets (the projector's)
+    int accumulate;
+};
+hipError_t launch_par_fbp_bp(const ParBpParams &P, const float *filt, float *img, hipStream_t st);
+hipError_t launch_par_adjoint(const ParBpParams &P, const float *proj, float *img, const SirtUpdate *U,
+                              int ones_input, hipStream_t st);
 
 // Utilities.
 hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st);

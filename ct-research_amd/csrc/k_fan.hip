@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_fan_adjoint(const float *__restrict__ p
             umax = fmaxf(umax, uu);
         }
         const float eu = 1e-3f + 4e-6f * fmaxf(fabsf(umin), fabsf(umax));
-        const int c0 = max((int)ceilf(fmaxf(umin - eu, -1.0f)), 0);
+        const int c0 = max((int)ceilf(fminf(fmaxf(umin - eu, -1.0f), (float)P.np)), 0);
         const int c1 = min((int)floorf(fminf(umax + eu, (float)P.np)), P.np - 1);
         const float *r = proj + (size_t)b * P.np;
         for (int c = c0; c <= c1; ++c) {
@@ -197,6 +197,106 @@ hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img,
         sirt = 1;
     }
     hipLaunchKernelGGL(k_fan_adjoint, pix_grid(P), dim3(256), 0, st, proj, img, P, U, sirt, ones_input);
+    return hipGetLastError();
+}
+
+// Parallel-beam FBP backprojection (FBPparallel.cu:41-89): one thread per
+// pixel, all views accumulated in a register and written once (the
+// reference launches one kernel per view and adds into the image).
+__global__ __launch_bounds__(256) void k_par_fbp_bp(const float *__restrict__ filt, float *__restrict__ img,
+                                                    ParBpParams P)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= P.nt || j >= P.ns) return;
+    const float x = ((float)i + 0.5f) * P.rt - P.ct, y = ((float)j + 0.5f) * P.rs - P.cs;
+    float acc = 0.0f;
+    for (int b = 0; b < P.nb; ++b) {
+        const float sb = P.view[2 * b], cb = P.view[2 * b + 1];
+        const float t = fmaf(x, cb, y * sb);
+        const float u = fmaf(t, P.u_scale, P.u_off);
+        if (u >= 0.0f && u < P.umax) {
+            const int iu = min((int)u, P.np - 2);
+            const float f = u - (float)iu;
+            const float *row = filt + (size_t)b * P.np;
+            acc = fmaf(fmaf(f, row[iu + 1] - row[iu], row[iu]), P.dtheta, acc);
+        }
+    }
+    const size_t o = (size_t)j * P.nt + i;
+    img[o] = P.accumulate ? img[o] + acc : acc;
+}
+
+// Exact adjoint of the parallel Siddon projector.  Per pixel and view the
+// candidate rays are the bins inside the pixel's shadow on the detector.  A
+// ray at signed distance dl (along n = (cos, sin)) from the pixel centre,
+// parameterised by mu along (-sin, cos) from the foot of the centre, has
+// offsets (dl cos - mu sin, dl sin + mu cos) from the centre; its length in
+// the pixel is the mu interval where both stay within the half pitches.  The
+// centre's detector coordinate is formed in FP64 so that dl keeps its
+// precision far from the rotation centre.
+__global__ __launch_bounds__(256) void k_par_adjoint(const float *__restrict__ proj, float *__restrict__ img,
+                                                     ParBpParams P, SirtUpdate U, int sirt, int ones)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= P.nt || j >= P.ns) return;
+    const double px = ((double)i + 0.5) * P.rt_d - P.ct_d, py = ((double)j + 0.5) * P.rs_d - P.cs_d;
+    const float hx = 0.5f * P.rt, hy = 0.5f * P.rs;
+    float acc = 0.0f;
+    for (int b = 0; b < P.nb; ++b) {
+        const double sbd = P.view_d[4 * b + 0], cbd = P.view_d[4 * b + 1];
+        const float sb = (float)sbd, cb = (float)cbd;
+        const double uc = fma(px, cbd, py * sbd);
+        const float reach = fmaf(fabsf(cb), hx, fabsf(sb) * hy);
+        float u0 = fmaf((float)uc - reach, P.u_scale, P.u_off), u1 = fmaf((float)uc + reach, P.u_scale, P.u_off);
+        const float ulo = fminf(u0, u1), uhi = fmaxf(u0, u1);
+        const float eu = 1e-3f + 4e-6f * fmaxf(fabsf(ulo), fabsf(uhi));
+        const int c0 = max((int)ceilf(fminf(fmaxf(ulo - eu, -1.0f), (float)P.np)), 0);
+        const int c1 = min((int)floorf(fminf(uhi + eu, (float)P.np)), P.np - 1);
+        const float *r = proj + (size_t)b * P.np;
+        for (int c = c0; c <= c1; ++c) {
+            const float dl = (float)(P.t_d[c] - uc);
+            const float ox = dl * cb, oy = dl * sb;
+            float lo = -kInf, hi = kInf;
+            if (sb != 0.0f) {
+                const float inv = 1.0f / sb;
+                const float l1 = (ox - hx) * inv, l2 = (ox + hx) * inv;
+                lo = fmaxf(lo, fminf(l1, l2));
+                hi = fminf(hi, fmaxf(l1, l2));
+            } else if (!(ox >= -hx && ox < hx)) {
+                continue;
+            }
+            if (cb != 0.0f) {
+                const float inv = 1.0f / cb;
+                const float l1 = (-hy - oy) * inv, l2 = (hy - oy) * inv;
+                lo = fmaxf(lo, fminf(l1, l2));
+                hi = fminf(hi, fmaxf(l1, l2));
+            } else if (!(oy >= -hy && oy < hy)) {
+                continue;
+            }
+            acc = fmaf(fmaxf(hi - lo, 0.0f), ones ? 1.0f : r[c], acc);
+        }
+    }
+    const size_t o = (size_t)j * P.nt + i;
+    if (sirt) {
+        const float xn = fmaf(U.lambda * U.vinv[o], acc, U.x[o]);
+        U.x[o] = fminf(fmaxf(xn, U.lo), U.hi);
+    } else {
+        img[o] = P.accumulate ? img[o] + acc : acc;
+    }
+}
+
+hipError_t launch_par_fbp_bp(const ParBpParams &P, const float *filt, float *img, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_par_fbp_bp, dim3((P.nt + 15) / 16, (P.ns + 15) / 16), dim3(256), 0, st, filt, img, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_par_adjoint(const ParBpParams &P, const float *proj, float *img, const SirtUpdate *U,
+                              int ones_input, hipStream_t st)
+{
+    SirtUpdate u{};
+    if (U) u = *U;
+    hipLaunchKernelGGL(k_par_adjoint, dim3((P.nt + 15) / 16, (P.ns + 15) / 16), dim3(256), 0, st, proj, img, P, u,
+                       U ? 1 : 0, ones_input);
     return hipGetLastError();
 }
 

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(64 *RW) void k_row_filter(const float *__restrict__
     const bool active = row < P.rows;
     if (active) {
         const int ix = row % P.nx;
-        const float xi = P.xd[ix];
+        const float xi = P.mode == 0 ? P.xd[ix] : 0.0f;
         const float *src = in + (size_t)row * np;
         for (int q = lane; q < RL; q += 64) {
             double v = 0.0;
@@ -52,8 +52,10 @@ __global__ __launch_bounds__(64 *RW) void k_row_filter(const float *__restrict__
                 if (P.mode == 0) {
                     const float pq = P.pd[q];
                     wgt = P.D / sqrtf(P.D * P.D + pq * pq + xi * xi);
-                } else {
+                } else if (P.mode == 1) {
                     wgt = P.colw[q];
+                } else {
+                    wgt = 1.0f;
                 }
                 v = (double)(src[q] * wgt);   // rounded to float like the in-place store of FDK.cu:36
             }
@@ -313,8 +315,10 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restr
                     const float pq = P.pd[q];
                     w0 = P.D / sqrtf(P.D * P.D + pq * pq + xi0 * xi0);
                     w1 = P.D / sqrtf(P.D * P.D + pq * pq + xi1 * xi1);
-                } else {
+                } else if (P.mode == 1) {
                     w0 = w1 = P.colw[q];
+                } else {
+                    w0 = w1 = 1.0f;
                 }
                 // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
                 if (has0) a = (double)(src0[q] * w0);

@@ -83,13 +83,26 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
                                            float &chord)
 {
     const double sb = g.view[4 * ib + 0], cb = g.view[4 * ib + 1];
-    const double st = g.view[4 * ib + 2], ss = g.view[4 * ib + 3];
     const double P = g.p[ip], X = g.xi[ix];
-    double dx = P * cb + g.D * sb, dy = P * sb - g.D * cb, dz = X;
-    const double inv_n = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    dx *= inv_n;
-    dy *= inv_n;
-    dz *= inv_n;
+    double st, ss, dx, dy, dz;
+    if (g.parallel) {
+        // foot point of the line and its unit direction (ProjectionParallel_2D.cu:22-28)
+        st = 0.5 * g.bt + P * cb;
+        ss = 0.5 * g.bs + P * sb;
+        dx = -sb;
+        dy = cb;
+        dz = 0.0;
+    } else {
+        st = g.view[4 * ib + 2];
+        ss = g.view[4 * ib + 3];
+        dx = P * cb + g.D * sb;
+        dy = P * sb - g.D * cb;
+        dz = X;
+        const double inv_n = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+        dx *= inv_n;
+        dy *= inv_n;
+        dz *= inv_n;
+    }
     double lo = -__builtin_huge_val(), hi = __builtin_huge_val();
     bool hit = clip_axis(st, dx, g.bt, lo, hi);
     hit = clip_axis(ss, dy, g.bs, lo, hi) && hit;

@@ -26,6 +26,7 @@ EXPORTS = [
     "ctr_copy_to_device", "ctr_copy_to_host", "ctr_cone_geom_check", "ctr_fan_geom_check",
     "ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_cone_filter", "ctr_fdk_bp", "ctr_fdk",
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
+    "ctr_par_geom_check", "ctr_par_fp", "ctr_par_bp_adjoint", "ctr_par_filter", "ctr_par_fbp",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
@@ -53,6 +54,12 @@ class FanGeomC(ctypes.Structure):
     _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("size", ctypes.c_double * 2),
                 ("distance", ctypes.c_double), ("n_beta", ctypes.c_int), ("n_p", ctypes.c_int),
                 ("beta", ctypes.c_void_p), ("p", ctypes.c_void_p)]
+
+
+class ParGeomC(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("size", ctypes.c_double * 2),
+                ("n_theta", ctypes.c_int), ("n_t", ctypes.c_int),
+                ("theta", ctypes.c_void_p), ("t", ctypes.c_void_p)]
 
 
 class SirtOptsC(ctypes.Structure):
@@ -101,11 +108,12 @@ def lib():
         L.ctr_ctx_synchronize.argtypes = [ctypes.c_void_p]
         L.ctr_cone_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_fan_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ctr_par_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         for name in ("ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_fdk_bp",
-                     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint"):
+                     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_par_fp", "ctr_par_bp_adjoint"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint]
-        for name in ("ctr_cone_filter", "ctr_fdk"):
+        for name in ("ctr_cone_filter", "ctr_fdk", "ctr_par_filter", "ctr_par_fbp"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_uint]
         for name in ("ctr_fan_filter", "ctr_fan_fbp"):
@@ -203,6 +211,30 @@ class FanGeom:
 
     def check(self, need_uniform=False):
         return lib().ctr_fan_geom_check(ctypes.byref(self.c), int(need_uniform))
+
+
+class ParGeom:
+    """Parallel-beam geometry: image nt x ns pixels (t = x fastest) over size
+    (mm), ray (theta, t) = the line (x - Cx) cos(theta) + (y - Cy) sin(theta) = t
+    (ProjectionParallel_2D.cpp:5-16, FBPparallel.cpp:7-14)."""
+
+    def __init__(self, nt, ns, size, theta, t):
+        self.nt, self.ns = int(nt), int(ns)
+        self.size = tuple(float(s) for s in size)
+        self.theta, self.t = _f64(theta), _f64(t)
+        self.c = ParGeomC(self.nt, self.ns, (ctypes.c_double * 2)(*self.size), len(self.theta), len(self.t),
+                          self.theta.ctypes.data, self.t.ctypes.data)
+
+    @property
+    def nvox(self):
+        return self.nt * self.ns
+
+    @property
+    def nray(self):
+        return len(self.theta) * len(self.t)
+
+    def check(self, need_uniform=False):
+        return lib().ctr_par_geom_check(ctypes.byref(self.c), int(need_uniform))
 
 
 def _is_device(x):
@@ -310,6 +342,19 @@ class Context:
 
     def fan_fbp(self, g, proj, detector=FAN_FLAT, filt=FILTER_SL_HALF, out=None, flags=0):
         return self._run(lib().ctr_fan_fbp, g, proj, g.nvox, out, flags, extra=(int(detector), int(filt)))
+
+    # -- parallel ---------------------------------------------------------------
+    def par_fp(self, g, img, out=None, flags=0):
+        return self._run(lib().ctr_par_fp, g, img, g.nray, out, flags)
+
+    def par_bp_adjoint(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_par_bp_adjoint, g, proj, g.nvox, out, flags)
+
+    def par_filter(self, g, proj, filt=FILTER_SL_HALF, out=None, flags=0):
+        return self._run(lib().ctr_par_filter, g, proj, g.nray, out, flags, extra=(int(filt),))
+
+    def par_fbp(self, g, proj, filt=FILTER_SL_HALF, out=None, flags=0):
+        return self._run(lib().ctr_par_fbp, g, proj, g.nvox, out, flags, extra=(int(filt),))
 
     # -- SIRT -------------------------------------------------------------------
     def _sirt(self, fn, g, b, n_iter, lam, lo, hi, x0, residuals, flags):

@@ -11,7 +11,7 @@ quirks listed in the flags of ctr.h are off unless ref_quirks=True.
 import numpy as np
 
 from . import _lib
-from ._lib import ConeGeom, FanGeom, default_context
+from ._lib import ConeGeom, FanGeom, ParGeom, default_context
 
 
 def _col(a):
@@ -79,6 +79,27 @@ def FBPfan(R, BetaScanRange, Pdomain, Size, Nt, Ns, Rscan, filt=_lib.FILTER_SL_H
     g = FanGeom(Nt, Ns, _col(Size)[:2], float(Rscan), _col(BetaScanRange).astype(np.float32),
                 _col(Pdomain).astype(np.float32))
     return ctx.fan_fbp(g, R, detector=_lib.FAN_FLAT, filt=filt).astype(np.float64)
+
+
+def ProjectionParallel_2D(picvector, height, width, Size, thetaRange, t_range, ctx=None):
+    """R = ProjectionParallel_2D(img, height, width, Size, theta, t)
+    (GPU_accelerated_Operator/ProjectionParallel_2D/ProjectionParallel_2D.cpp:3-50).
+    The image is [height][width] with width fastest; Size(1) is the vertical
+    extent (ProjectionParallel_2D.cpp:9-10), so the t axis is the width."""
+    ctx = ctx or default_context()
+    sz = _col(Size)
+    g = ParGeom(int(width), int(height), (float(sz[1]), float(sz[0])), thetaRange, t_range)
+    return ctx.par_fp(g, picvector).astype(np.float64)
+
+
+def FBPparallel(R, ThetaScanRange, Tdomain, Size, t_length, s_length, filt=_lib.FILTER_SL_HALF, ctx=None):
+    """img = FBPparallel(R, theta, Tdomain, Size, t_length, s_length)
+    (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FBP_parallel_GPU/FBPparallel.cpp:3-64).
+    theta and Tdomain are single in the reference; they are rounded to float32 here too."""
+    ctx = ctx or default_context()
+    g = ParGeom(int(t_length), int(s_length), _col(Size)[:2], _col(ThetaScanRange).astype(np.float32),
+                _col(Tdomain).astype(np.float32))
+    return ctx.par_fbp(g, R, filt=filt).astype(np.float64)
 
 
 def SIRT3D(R, Xigamadomain, Pdomain, BetaScanRange, Distance, Size, Nt, Ns, Nz, nIter, Lamda,

@@ -91,6 +91,17 @@ typedef struct ctr_fan_geom {
     const double *p;        /* [n_p] bin positions at the isocentre (mm), monotonic */
 } ctr_fan_geom;
 
+/* Parallel-beam geometry (the argument set of ProjectionParallel_2D / FBPparallel).
+ * Ray (theta, t) is the line (x - size[0]/2) cos(theta) + (y - size[1]/2) sin(theta) = t,
+ * x along t pixels, y along s pixels (ProjectionParallel_2D.cu:22-28). */
+typedef struct ctr_par_geom {
+    int nt, ns;             /* pixels along t (x) and s (y) */
+    double size[2];         /* physical extent of the image along t and s (mm) */
+    int n_theta, n_t;       /* views, detector bins */
+    const double *theta;    /* [n_theta] angles of the ray normals (rad) */
+    const double *t;        /* [n_t] signed ray offsets from the image centre (mm), monotonic */
+} ctr_par_geom;
+
 /* SIRT options (Iterative_algorithm/SIRT3D.m, SIRT.m). */
 typedef struct ctr_sirt_opts {
     int n_iter;             /* iterations (SIRT3D.m "Times") */
@@ -134,6 +145,7 @@ ctr_status ctr_copy_to_host(ctr_ctx *ctx, void *dst, const void *src, size_t byt
 /* Argument checks only (no device work). */
 ctr_status ctr_cone_geom_check(const ctr_cone_geom *g, int need_uniform_pitch);
 ctr_status ctr_fan_geom_check(const ctr_fan_geom *g, int need_uniform_pitch);
+ctr_status ctr_par_geom_check(const ctr_par_geom *g, int need_uniform_pitch);
 
 /* ---- cone beam --------------------------------------------------------- */
 
@@ -185,6 +197,29 @@ ctr_status ctr_fan_filter(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int
 /* Filtered backprojection.  Flat replaces FBPfan() (FBP_fan_GPU/FBPfan.h:31-32);
  * equiangular follows FanProjection.m:295-324 with weight 1/2 for a full scan. */
 ctr_status ctr_fan_fbp(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int filter, const float *proj, float *img,
+                       unsigned flags);
+
+/* ---- parallel beam ----------------------------------------------------- */
+
+/* Siddon line integrals of img [ns*nt] into proj [n_theta*n_t].
+ * Replaces ProjectionParallel_2D() (GPU_accelerated_Operator/ProjectionParallel_2D/ProjectionParallel_2D.h:31-33,
+ * kernel ProjectionParallel_2D.cu:9-220). */
+ctr_status ctr_par_fp(ctr_ctx *ctx, const ctr_par_geom *g, const float *img, float *proj, unsigned flags);
+
+/* Exact transpose of ctr_par_fp.  No reference counterpart: ART.m and
+ * SIRT.m use the transpose of the same Siddon system matrix. */
+ctr_status ctr_par_bp_adjoint(ctr_ctx *ctx, const ctr_par_geom *g, const float *proj, float *img, unsigned flags);
+
+/* Ramp filter along t (no pre-weight), scaled by the bin pitch of the
+ * single-precision t positions.  Replaces GFunction + Filter
+ * (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FBP_parallel_GPU/FBPparallel.cu:6-43). */
+ctr_status ctr_par_filter(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *filtered,
+                          unsigned flags);
+
+/* Filtered backprojection: linear detector interpolation x dtheta with theta
+ * and t rounded to single precision, as FBPparallel receives them.
+ * Replaces FBPpara() (FBP_parallel_GPU/FBPparallel.h:30-31, kernel FBPparallel.cu:46-96). */
+ctr_status ctr_par_fbp(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *img,
                        unsigned flags);
 
 /* ---- SIRT -------------------------------------------------------------- */

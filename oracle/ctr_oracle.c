@@ -732,8 +732,8 @@ static or_faxis make_faxis(const double *p, int n)
     a.pos = malloc(sizeof(float) * n);
     for (int i = 0; i < n; i++) a.pos[i] = (float)p[i];
     a.first = a.pos[0];
-    a.pitch = (double)fabsf(a.pos[1] - a.pos[0]);
-    a.step = ((double)a.pos[n - 1] - (double)a.pos[0]) / (n - 1);
+    a.pitch = n > 1 ? (double)fabsf(a.pos[1] - a.pos[0]) : 0.0;
+    a.step = n > 1 ? ((double)a.pos[n - 1] - (double)a.pos[0]) / (n - 1) : 0.0;
     a.lo = (double)(a.pos[0] < a.pos[n - 1] ? a.pos[0] : a.pos[n - 1]);
     a.hi = (double)(a.pos[0] > a.pos[n - 1] ? a.pos[0] : a.pos[n - 1]);
     a.n = n;
@@ -1053,6 +1053,155 @@ void or_sirt2d(const or_fan *g, const float *b, double *x, int n_iter, double la
         }
     }
     free(rows); free(cols); free(ax); free(bp); free(idx); free(len); free(work);
+}
+
+/* ------------------------------------------------------------------------
+ * Parallel beam (GPU_accelerated_Operator/ProjectionParallel_2D and
+ * FBP_parallel_GPU/FBPparallel.cu).  Image [s][t], t fastest.  Ray (theta, t)
+ * is the line (x - Ct) cos(theta) + (y - Cs) sin(theta) = t traversed along
+ * (-sin theta, cos theta) (ProjectionParallel_2D.cu:22-28: start and end
+ * points Center + t (cos, sin) -/+ Smax (-sin, cos)).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nt, ns;
+    double size[2];
+    int nb, np;
+    const double *theta, *t;
+} or_par;
+
+static void par_ray(const or_par *g, int b, int it, double S[3], double d[3])
+{
+    double th = g->theta[b], T = g->t[it];
+    double c = cos(th), s = sin(th);
+    S[0] = g->size[0] / 2 + T * c;
+    S[1] = g->size[1] / 2 + T * s;
+    S[2] = 0.5;
+    d[0] = -s;
+    d[1] = c;
+    d[2] = 0.0;
+}
+
+/* Exact Siddon line integrals (the operator ProjectionParallel_2D.cu:9-220
+ * computes with near-integer tolerance ERR = 1e-5 and a height+width-1 cap). */
+void or_par_fp_exact(const or_par *g, const float *img, double *proj)
+{
+    const int n[3] = {g->nt, g->ns, 1};
+    const double r[3] = {g->size[0] / g->nt, g->size[1] / g->ns, 1.0};
+    const long nray = (long)g->nb * g->np;
+#pragma omp parallel
+    {
+        int cap = max_segments(n);
+        long *idx = malloc(sizeof(long) * cap);
+        double *len = malloc(sizeof(double) * cap), *work = malloc(sizeof(double) * (cap + 4));
+#pragma omp for schedule(dynamic, 16)
+        for (long ray = 0; ray < nray; ray++) {
+            double S[3], d[3];
+            par_ray(g, (int)(ray / g->np), (int)(ray % g->np), S, d);
+            int m = siddon_exact(S, d, n, r, 2, idx, len, work);
+            double acc = 0.0;
+            for (int q = 0; q < m; q++) acc += len[q] * (double)img[idx[q]];
+            proj[ray] = acc;
+        }
+        free(idx); free(len); free(work);
+    }
+}
+
+/* Transpose of or_par_fp_exact: every ray scatters its Siddon lengths. */
+void or_par_adjoint(const or_par *g, const float *proj, double *img)
+{
+    const int n[3] = {g->nt, g->ns, 1};
+    const double r[3] = {g->size[0] / g->nt, g->size[1] / g->ns, 1.0};
+    const long nray = (long)g->nb * g->np;
+    memset(img, 0, sizeof(double) * (long)g->nt * g->ns);
+    long *idx = malloc(sizeof(long) * max_segments(n));
+    double *len = malloc(sizeof(double) * max_segments(n)), *work = malloc(sizeof(double) * (max_segments(n) + 4));
+    for (long ray = 0; ray < nray; ray++) {
+        double y = (double)proj[ray];
+        if (y == 0.0) continue;
+        double S[3], d[3];
+        par_ray(g, (int)(ray / g->np), (int)(ray % g->np), S, d);
+        int m = siddon_exact(S, d, n, r, 2, idx, len, work);
+        for (int q = 0; q < m; q++) img[idx[q]] += len[q] * y;
+    }
+    free(idx); free(len); free(work);
+}
+
+/* Long-double brute force of one parallel ray: clip against every pixel box. */
+double or_par_ray_brute(const or_par *g, const float *img, int b, int it)
+{
+    double Sd[3], dd[3];
+    par_ray(g, b, it, Sd, dd);
+    long double S[2] = {Sd[0], Sd[1]}, d[2] = {dd[0], dd[1]};
+    long double r[2] = {(long double)g->size[0] / g->nt, (long double)g->size[1] / g->ns};
+    long double acc = 0.0L;
+    for (int j = 0; j < g->ns; j++)
+        for (int i = 0; i < g->nt; i++) {
+            long double blo[2] = {i * r[0], j * r[1]}, bhi[2] = {(i + 1) * r[0], (j + 1) * r[1]};
+            long double lo = -INFINITY, hi = INFINITY;
+            int miss = 0;
+            for (int a = 0; a < 2; a++) {
+                if (d[a] == 0.0L) {
+                    if (!(S[a] >= blo[a] && S[a] < bhi[a])) miss = 1;
+                    continue;
+                }
+                long double t1 = (blo[a] - S[a]) / d[a], t2 = (bhi[a] - S[a]) / d[a];
+                long double a1 = t1 < t2 ? t1 : t2, a2 = t1 < t2 ? t2 : t1;
+                if (a1 > lo) lo = a1;
+                if (a2 < hi) hi = a2;
+            }
+            if (miss || !(hi > lo)) continue;
+            acc += (hi - lo) * (long double)img[(long)j * g->nt + i];
+        }
+    return (double)acc;
+}
+
+/* Parallel-beam filter (FBPparallel.cu:8-43): no pre-weight,
+ * Rcov[t] = dt * sum_g R[g] G[LT-1+t-g] with the float half-SL taps of
+ * GFunction (or half Ram-Lak); t positions arrive as single precision. */
+void or_par_filter(const or_par *g, int filter, const float *proj, double *out)
+{
+    or_faxis ap = make_faxis(g->t, g->np);
+    const double d = ap.pitch;
+    double *taps = malloc(sizeof(double) * g->np);
+    or_filter_taps(filter, 0, (double)(float)d, g->np, taps);
+    double *w = malloc(sizeof(double) * g->np), *o = malloc(sizeof(double) * g->np);
+    for (int b = 0; b < g->nb; b++) {
+        const float *R = proj + (long)b * g->np;
+        for (int q = 0; q < g->np; q++) w[q] = (double)R[q];
+        conv_row(w, g->np, taps, d, o);
+        memcpy(out + (long)b * g->np, o, sizeof(double) * g->np);
+    }
+    free(w); free(o); free(taps); free(ap.pos);
+}
+
+/* Parallel-beam FBP backprojection (FBPparallel.cu:46-96): for each pixel
+ * and view, t = x cos(theta) + y sin(theta) (pixel centre relative to the
+ * centre), linear interpolation on the detector / dt, times dtheta; theta
+ * and t arrive as single precision (FBPparallel.cpp:7-11). */
+void or_par_fbp_bp(const or_par *g, const float *filt, double *img)
+{
+    const int tl = g->nt, sl = g->ns;
+    const double rt = g->size[0] / tl, rs = g->size[1] / sl;
+    const double Ct = g->size[0] / 2, Cs = g->size[1] / 2;
+    or_faxis ap = make_faxis(g->t, g->np);
+    const double dtheta = fabs((double)(float)g->theta[1] - (double)(float)g->theta[0]);
+    memset(img, 0, sizeof(double) * (long)tl * sl);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int S = 0; S < sl; S++)
+        for (int T = 0; T < tl; T++) {
+            double acc = 0.0;
+            const double it = (T + 0.5) * rt - Ct, is = (S + 0.5) * rs - Cs;
+            for (int b = 0; b < g->nb; b++) {
+                const double th = (double)(float)g->theta[b];
+                const double t = it * cos(th) + is * sin(th);
+                const float *R = filt + (long)b * g->np;
+                int ip;
+                double wl, wu;
+                if (faxis_lerp(&ap, t, &ip, &wl, &wu)) acc += (wl * R[ip] + wu * R[ip + 1]) / ap.pitch * dtheta;
+            }
+            img[(long)S * tl + T] = acc;
+        }
+    free(ap.pos);
 }
 
 int or_num_threads(void)

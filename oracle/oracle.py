@@ -43,6 +43,11 @@ class _Fan(ctypes.Structure):
                 ("beta", ctypes.c_void_p), ("p", ctypes.c_void_p)]
 
 
+class _Par(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("size", ctypes.c_double * 2),
+                ("nb", ctypes.c_int), ("np", ctypes.c_int), ("theta", ctypes.c_void_p), ("t", ctypes.c_void_p)]
+
+
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
@@ -74,6 +79,17 @@ class Fan:
                       _ptr(self.beta), _ptr(self.p))
         self.nvox = nt * ns
         self.nray = len(self.beta) * len(self.p)
+
+
+class Par:
+    """Parallel-beam geometry: image [ns][nt] (t fastest), size (t extent, s extent), theta (rad), t (mm)."""
+
+    def __init__(self, nt, ns, size, theta, t):
+        self.theta, self.t = _f64(theta), _f64(t)
+        self.s = _Par(nt, ns, (ctypes.c_double * 2)(*size), len(self.theta), len(self.t),
+                      _ptr(self.theta), _ptr(self.t))
+        self.nvox = nt * ns
+        self.nray = len(self.theta) * len(self.t)
 
 
 def _call(name, g, *args):
@@ -190,6 +206,41 @@ def fan_fbp_bp(g, filtered, equiangular=False):
     out = np.zeros(g.nvox)
     y = _f32(filtered)
     _call("or_fan_fbp_bp", g, ctypes.c_int(int(equiangular)), _ptr(y), _ptr(out))
+    return out
+
+
+def par_fp_exact(g, img):
+    out = np.zeros(g.nray)
+    v = _f32(img)
+    _call("or_par_fp_exact", g, _ptr(v), _ptr(out))
+    return out
+
+
+def par_adjoint(g, proj):
+    out = np.zeros(g.nvox)
+    y = _f32(proj)
+    _call("or_par_adjoint", g, _ptr(y), _ptr(out))
+    return out
+
+
+def par_ray_brute(g, img, b, it):
+    fn = getattr(lib(), "or_par_ray_brute")
+    fn.restype = ctypes.c_double
+    v = _f32(img)
+    return fn(ctypes.byref(g.s), _ptr(v), ctypes.c_int(b), ctypes.c_int(it))
+
+
+def par_filter(g, proj, filt=0):
+    out = np.zeros(g.nray)
+    y = _f32(proj)
+    _call("or_par_filter", g, ctypes.c_int(filt), _ptr(y), _ptr(out))
+    return out
+
+
+def par_fbp_bp(g, filtered):
+    out = np.zeros(g.nvox)
+    y = _f32(filtered)
+    _call("or_par_fbp_bp", g, _ptr(y), _ptr(out))
     return out
 
 

@@ -113,6 +113,27 @@ def test_gateway_argument_errors():
     assert e.value.ident == "ctr:SIRT3D:nrhs"
 
 
+def test_parallel_gateway_argument_errors():
+    theta = np.deg2rad(np.arange(1, 31)).astype(np.float32)
+    t = (np.arange(40) - 19.5).astype(np.float32)
+    fbp = Mex("FBPparallel")
+    with pytest.raises(MexError) as e:
+        fbp(np.zeros(30 * 40, np.float32), theta, t, [32.0, 32.0], 32)
+    assert e.value.ident == "ctr:FBPparallel:nrhs"
+    with pytest.raises(MexError) as e:
+        fbp(np.zeros(30 * 40 - 1, np.float32), theta, t, [32.0, 32.0], 32, 32)
+    assert e.value.ident == "ctr:FBPparallel:size"
+    t_bad = t.copy()
+    t_bad[5] += 0.4
+    with pytest.raises(MexError) as e:
+        fbp(np.zeros(30 * 40, np.float32), theta, t_bad, [32.0, 32.0], 32, 32)
+    assert e.value.ident == "ctr:FBPparallel:EINVAL" and "non-uniform" in str(e.value)
+    fp = Mex("ProjectionParallel_2D")
+    with pytest.raises(MexError) as e:
+        fp(np.zeros(32 * 32, np.float32), 32, 32, [32.0, -1.0], theta, t)
+    assert e.value.ident == "ctr:ProjectionParallel_2D:EINVAL"
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_gateway_reports_missing_device():
     nt, ns, nz, size, beta, p, xi, D = cone_args()
@@ -163,3 +184,27 @@ def test_fan_gateways(ctx, oracle):
     assert errs(bp, ctx.fan_bp_ref(gs, R32))[1] == 0.0
     fbp = Mex("FBPfan")(R32, beta.astype(np.float32), p.astype(np.float32), [40.0, 40.0], n, n, 120.0)
     assert errs(fbp, ctx.fan_fbp(gs, R32))[1] == 0.0
+
+
+@pytest.mark.gpu
+def test_parallel_gateways(ctx, oracle):
+    """height != width and Size(1) != Size(2): the projector's t axis is the
+    width with extent Size(2) (ProjectionParallel_2D.cpp:6-10), the FBP's t
+    axis is t_length with extent Size(1) (FBPparallel.cu:47-48)."""
+    import ctr
+    height, width = 36, 44
+    theta = np.deg2rad(np.arange(1, 181, 2.5))
+    t = 0.6 * (np.arange(110) - 54.5)
+    img = np.random.default_rng(21).random(height * width).astype(np.float32)
+    R = Mex("ProjectionParallel_2D")(img, height, width, np.array([30.0, 40.0]), theta, t)
+    og = oracle.Par(width, height, (40.0, 30.0), theta, t)
+    assert errs(R, oracle.par_fp_exact(og, img))[0] < 2e-6
+    R32 = R.astype(np.float32)
+    th32, t32 = theta.astype(np.float32), t.astype(np.float32)
+    fbp = Mex("FBPparallel")(R32, th32, t32, [40.0, 30.0], width, height)
+    gs = ctr.ParGeom(width, height, (40.0, 30.0), th32, t32)
+    assert errs(fbp, ctx.par_fbp(gs, R32))[1] == 0.0
+    f_ref = oracle.par_filter(oracle.Par(width, height, (40.0, 30.0), th32, t32), R32)
+    with oracle.interp_mode(1):
+        ref = oracle.par_fbp_bp(oracle.Par(width, height, (40.0, 30.0), th32, t32), f_ref.astype(np.float32))
+    assert errs(fbp, ref)[0] < 2e-5

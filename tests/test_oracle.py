@@ -300,3 +300,89 @@ def test_grid_modes_agree_on_exact_grids(oracle):
     with oracle.interp_mode(1):
         b = oracle.fdk_bp(og, y)
     assert np.abs(a - b).max() < 1e-12 * np.abs(a).max()
+
+
+# ---- parallel beam ----------------------------------------------------------
+
+def small_par(O, nt=7, ns=6, size=(7.0, 6.6), nb=9, np_=11, pitch=0.93):
+    theta = np.concatenate([[0.0, np.pi / 2, np.pi / 4], np.random.default_rng(8).uniform(0, 2 * np.pi, nb - 3)])
+    t = (np.arange(np_) - (np_ - 1) / 2) * pitch + 0.013
+    return O.Par(nt, ns, size, theta, t)
+
+
+def box_chord_par(g, b, it):
+    """Chord of the line (x-Ct) cos + (y-Cs) sin = t through [0,W] x [0,H] (slab method)."""
+    th, t = g.theta[b], g.t[it]
+    W, H = g.s.size[0], g.s.size[1]
+    S = np.array([W / 2 + t * np.cos(th), H / 2 + t * np.sin(th)])
+    d = np.array([-np.sin(th), np.cos(th)])
+    lo, hi = -np.inf, np.inf
+    for a, bmax in enumerate((W, H)):
+        if d[a] == 0:
+            if not (0 <= S[a] < bmax):
+                return 0.0
+            continue
+        t1, t2 = (0 - S[a]) / d[a], (bmax - S[a]) / d[a]
+        lo, hi = max(lo, min(t1, t2)), min(hi, max(t1, t2))
+    return max(hi - lo, 0.0)
+
+
+def test_parallel_siddon_matches_brute_force(oracle):
+    g = small_par(oracle)
+    img = np.random.default_rng(9).uniform(0, 1, g.nvox).astype(np.float32)
+    fp = oracle.par_fp_exact(g, img).reshape(len(g.theta), len(g.t))
+    for b in range(len(g.theta)):
+        for it in range(len(g.t)):
+            assert abs(fp[b, it] - oracle.par_ray_brute(g, img, b, it)) < 1e-9
+
+
+def test_parallel_fp_of_ones_is_box_chord(oracle):
+    g = small_par(oracle)
+    fp = oracle.par_fp_exact(g, np.ones(g.nvox, np.float32)).reshape(len(g.theta), len(g.t))
+    for b in range(len(g.theta)):
+        for it in range(len(g.t)):
+            assert abs(fp[b, it] - box_chord_par(g, b, it)) < 1e-9
+
+
+def test_parallel_fp_of_disk_is_analytic_chord(oracle):
+    n, R = 256, 80.0
+    x = (np.arange(n) + 0.5) - n / 2
+    img = ((x[None, :] ** 2 + x[:, None] ** 2) <= R * R).astype(np.float32)
+    theta = np.array([0.1, 0.7, 1.9])
+    t = np.linspace(-75.3, 75.3, 41)
+    g = oracle.Par(n, n, (float(n), float(n)), theta, t)
+    fp = oracle.par_fp_exact(g, img).reshape(3, -1)
+    exact = 2 * np.sqrt(np.maximum(R * R - t * t, 0))
+    assert np.abs(fp - exact[None, :]).max() / exact.max() < 0.01
+
+
+@pytest.mark.parametrize("filt", [0, 1])
+def test_parallel_fbp_recovers_disk(oracle, filt):
+    """Analytic projections of a uniform disk (2 A sqrt(R^2 - t^2)) over 360
+    views of 1 degree, half-scaled filter: the FBP interior is A."""
+    n, R, A = 128, 40.0, 0.8
+    theta = np.deg2rad(np.arange(1, 361))
+    t = np.arange(-95.5, 96.0, 1.0)
+    g = oracle.Par(n, n, (float(n), float(n)), theta, t)
+    proj = np.tile(2 * A * np.sqrt(np.maximum(R * R - t * t, 0)), len(theta)).astype(np.float32)
+    filt_out = oracle.par_filter(g, proj, filt=filt)
+    img = oracle.par_fbp_bp(g, filt_out.astype(np.float32)).reshape(n, n)
+    c = (np.arange(n) + 0.5) - n / 2
+    inner = (c[None, :] ** 2 + c[:, None] ** 2) < (0.6 * R) ** 2
+    assert abs(img[inner].mean() / A - 1) < 0.02
+
+
+def test_parallel_adjoint_is_transpose(oracle):
+    g = small_par(oracle)
+    rng = np.random.default_rng(10)
+    x = rng.uniform(0, 1, g.nvox).astype(np.float32)
+    y = rng.uniform(0, 1, g.nray).astype(np.float32)
+    ax = oracle.par_fp_exact(g, x)
+    aty = oracle.par_adjoint(g, y)
+    lhs, rhs = float(np.dot(ax, y.astype(np.float64))), float(np.dot(x.astype(np.float64), aty))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # column sums of a one-hot pixel: the pixel's chords over all rays
+    e = np.zeros(g.nvox, np.float32)
+    e[17] = 1.0
+    assert np.isclose(oracle.par_adjoint(g, np.ones(g.nray, np.float32))[17], oracle.par_fp_exact(g, e).sum(),
+                      rtol=1e-12)
