@@ -202,8 +202,8 @@ ctr_status ctr_fan_fbp(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, int fi
 /* ---- parallel beam ----------------------------------------------------- */
 
 /* Siddon line integrals of img [ns*nt] into proj [n_theta*n_t].
- * Replaces ProjectionParallel_2D() (GPU_accelerated_Operator/ProjectionParallel_2D/ProjectionParallel_2D.h:31-33,
- * kernel ProjectionParallel_2D.cu:9-220). */
+ * Replaces ProjectionParallel_2D() (GPU_accelerated_Operator/ProjectionParallel_2D/ProjectionParallel_2D.h:33,
+ * kernel ProjectionParallel_2D.cu:9-253). */
 ctr_status ctr_par_fp(ctr_ctx *ctx, const ctr_par_geom *g, const float *img, float *proj, unsigned flags);
 
 /* Exact transpose of ctr_par_fp.  No reference counterpart: ART.m and
@@ -212,13 +212,13 @@ ctr_status ctr_par_bp_adjoint(ctr_ctx *ctx, const ctr_par_geom *g, const float *
 
 /* Ramp filter along t (no pre-weight), scaled by the bin pitch of the
  * single-precision t positions.  Replaces GFunction + Filter
- * (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FBP_parallel_GPU/FBPparallel.cu:6-43). */
+ * (Analytic_Algorithm/FBP_type_analytic_algorithm_GPU_cpp_sourcecode/FBP_parallel_GPU/FBPparallel.cu:6-39). */
 ctr_status ctr_par_filter(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *filtered,
                           unsigned flags);
 
 /* Filtered backprojection: linear detector interpolation x dtheta with theta
  * and t rounded to single precision, as FBPparallel receives them.
- * Replaces FBPpara() (FBP_parallel_GPU/FBPparallel.h:30-31, kernel FBPparallel.cu:46-96). */
+ * Replaces FBPpara() (FBP_parallel_GPU/FBPparallel.h:31, kernel FBPparallel.cu:41-89). */
 ctr_status ctr_par_fbp(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *img,
                        unsigned flags);
 

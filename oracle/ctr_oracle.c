@@ -1155,7 +1155,7 @@ double or_par_ray_brute(const or_par *g, const float *img, int b, int it)
     return (double)acc;
 }
 
-/* Parallel-beam filter (FBPparallel.cu:8-43): no pre-weight,
+/* Parallel-beam filter (FBPparallel.cu:6-39): no pre-weight,
  * Rcov[t] = dt * sum_g R[g] G[LT-1+t-g] with the float half-SL taps of
  * GFunction (or half Ram-Lak); t positions arrive as single precision. */
 void or_par_filter(const or_par *g, int filter, const float *proj, double *out)
@@ -1174,7 +1174,7 @@ void or_par_filter(const or_par *g, int filter, const float *proj, double *out)
     free(w); free(o); free(taps); free(ap.pos);
 }
 
-/* Parallel-beam FBP backprojection (FBPparallel.cu:46-96): for each pixel
+/* Parallel-beam FBP backprojection (FBPparallel.cu:41-89): for each pixel
  * and view, t = x cos(theta) + y sin(theta) (pixel centre relative to the
  * centre), linear interpolation on the detector / dt, times dtheta; theta
  * and t arrive as single precision (FBPparallel.cpp:7-11). */
