@@ -7,13 +7,28 @@ Runs on the 256^3 / 512^2 cone workload (1 mm voxels, SOD 600, SDD 1200,
 3-D Shepp-Logan phantom forward projected on the device, and prints one line
 per operator: device time per call (HIP events on the stream the library
 launches on).  Use it under rocprofv3 to attribute kernel time per operator.
+
+The par_* operators run the parallel-beam workload of FBP_parallel_GPU.m:
+512 x 512 pixels over 60 mm, 360 one-degree views, t from -47 mm in steps
+of 1.1 pixel (730 bins), on the 2-D Shepp-Logan phantom.
 """
 import argparse
 import os
 import sys
 import time
 
-OPS = ("cone_fp", "cone_filter", "fdk_bp", "cone_bp_ref", "cone_adjoint", "sirt3d")
+OPS = ("cone_fp", "cone_filter", "fdk_bp", "cone_bp_ref", "cone_adjoint", "sirt3d",
+       "par_fp", "par_filter", "par_fbp", "par_adjoint")
+
+
+def parallel_workload():
+    """FBP_parallel_GPU.m:16-35: 512^2 over Size 60, t = -47 : 1.1 * 60/512 : 47, theta = 1..360 degrees."""
+    import numpy as np
+    from . import _lib
+    res = 60.0 / 512
+    t = np.arange(-47.0, 47.0 + 1e-9, 1.1 * res)
+    theta = np.deg2rad(np.arange(1, 361))
+    return _lib.ParGeom(512, 512, (60.0, 60.0), theta, t)
 
 
 def main(argv=None):
@@ -58,12 +73,24 @@ def main(argv=None):
         return a.elapsed_time(b) / reps
 
     print(f"workload: {args.n}^3 volume, {len(g.beta)} views, {len(g.p)}x{len(g.xi)} detector", flush=True)
+    if any(o.startswith("par_") for o in ops):
+        gp = parallel_workload()
+        img = torch.from_numpy(phantoms.shepp_logan_2d(512).astype("float32").ravel()).cuda()
+        pproj = ctx.par_fp(gp, img)
+        pfilt = ctx.par_filter(gp, pproj)
+        pout = torch.empty_like(img)
+        torch.cuda.synchronize()
+        print(f"parallel workload: 512^2 image, {len(gp.theta)} views, {len(gp.t)} bins", flush=True)
     calls = {
         "cone_fp": lambda: ctx.cone_fp(g, vol, out=proj),
         "cone_filter": lambda: ctx.cone_filter(g, proj, out=filt),
         "fdk_bp": lambda: ctx.fdk_bp(g, filt, out=out),
         "cone_bp_ref": lambda: ctx.cone_bp_ref(g, proj, out=out),
         "cone_adjoint": lambda: ctx.cone_bp_adjoint(g, proj, out=out),
+        "par_fp": lambda: ctx.par_fp(gp, img, out=pproj),
+        "par_filter": lambda: ctx.par_filter(gp, pproj, out=pfilt),
+        "par_fbp": lambda: ctx.par_fbp(gp, pproj, out=pout),
+        "par_adjoint": lambda: ctx.par_bp_adjoint(gp, pproj, out=pout),
     }
     for o in ops:
         if o == "sirt3d":
