@@ -57,6 +57,16 @@ struct ParTables {
     DevBuf d_view, d_t, d_zero, d_vf;
 };
 
+struct CurvedTables {
+    uint64_t key = 0;
+    bool valid = false;
+    std::vector<float> h_view;             // [nb][2] sin, cos
+    std::vector<double> h_rowp, h_beta, h_gam;   // [nx] row scale; view and channel angles
+    int fft_len = 0;
+    std::vector<double> h_spec;            // [nx][fft_len] per-row spectrum / fft_len
+    DevBuf d_view, d_rowp, d_beta, d_gam, d_spec;
+};
+
 struct FootCache {
     DevBuf buf;
     uint64_t key = 0;
@@ -92,6 +102,7 @@ struct ctr_ctx {
     ConeTables cone;
     FanTables fan;
     ParTables par;
+    CurvedTables curved;
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
@@ -276,6 +287,38 @@ ctr_status check_par(const ctr_par_geom *g, int need_uniform)
     for (int i = 0; i < g->n_theta; ++i)
         if (!std::isfinite(g->theta[i])) return fail(nullptr, CTR_EINVAL, "theta[%d] is not finite", i);
     return check_axis(g->t, g->n_t, need_uniform != 0, "t");
+}
+
+ctr_status check_curved(const ctr_curved_geom *g)
+{
+    if (!g) return fail(nullptr, CTR_EINVAL, "geometry is NULL");
+    if (g->nt < 1 || g->ns < 1 || g->nz < 1) return fail(nullptr, CTR_EINVAL, "volume dimensions must be positive");
+    if ((double)g->nt * g->ns * g->nz > (double)std::numeric_limits<int>::max())
+        return fail(nullptr, CTR_ETOOBIG, "volume has more than 2^31-1 voxels");
+    for (int a = 0; a < 3; ++a)
+        if (!finite_pos(g->size[a])) return fail(nullptr, CTR_EINVAL, "size[%d] must be positive and finite", a);
+    if (!finite_pos(g->distance) || !finite_pos(g->distance_s2d))
+        return fail(nullptr, CTR_EINVAL, "distances must be positive and finite");
+    const double half = 0.5 * std::sqrt(g->size[0] * g->size[0] + g->size[1] * g->size[1]);
+    if (!(g->distance > half * (1.0 + 1e-6)))
+        return fail(nullptr, CTR_EINVAL, "source orbit (distance %g) intersects the volume (half diagonal %g)",
+                    g->distance, half);
+    if (g->n_beta < 2 || g->n_gamma < 2 || g->n_xi < 2)
+        return fail(nullptr, CTR_EINVAL, "need at least 2 views, 2 channels and 2 rows");
+    if (g->n_gamma > 1024) return fail(nullptr, CTR_ETOOBIG, "the curved-detector filter supports at most 1024 channels");
+    if ((double)g->n_beta * g->n_gamma * g->n_xi > (double)std::numeric_limits<int>::max())
+        return fail(nullptr, CTR_ETOOBIG, "more than 2^31-1 detector samples");
+    if (g->n_beta > 65535) return fail(nullptr, CTR_ETOOBIG, "at most 65535 views");
+    if (!std::isfinite(g->beta_start) || !finite_pos(g->fan_angle) || g->fan_angle >= 3.141592653589793)
+        return fail(nullptr, CTR_EINVAL, "beta_start must be finite and fan_angle in (0, pi)");
+    ctr_status s;
+    if ((s = check_axis(g->beta, g->n_beta, true, "beta")) != CTR_OK) return s;
+    if ((s = check_axis(g->gamma, g->n_gamma, true, "gamma")) != CTR_OK) return s;
+    if ((s = check_axis(g->xi, g->n_xi, true, "xi")) != CTR_OK) return s;
+    for (int i = 0; i < g->n_gamma; ++i)
+        if (!(std::fabs(g->gamma[i]) < 0.5 * 3.141592653589793))
+            return fail(nullptr, CTR_EINVAL, "gamma[%d] is not inside (-pi/2, pi/2)", i);
+    return CTR_OK;
 }
 
 ctr_status need_ctx(ctr_ctx *ctx)
@@ -689,6 +732,107 @@ ParBpParams par_bp_params(ctr_ctx *ctx, const ctr_par_geom *g, bool single_axis,
     return P;
 }
 
+uint64_t curved_key(const ctr_curved_geom *g)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(h, &g->nt, sizeof(int) * 3);
+    h = fnv(h, g->size, sizeof(double) * 3);
+    h = fnv(h, &g->distance, sizeof(double) * 2);
+    h = fnv(h, &g->n_beta, sizeof(int) * 3);
+    h = fnv(h, g->beta, sizeof(double) * g->n_beta);
+    h = fnv(h, g->gamma, sizeof(double) * g->n_gamma);
+    h = fnv(h, g->xi, sizeof(double) * g->n_xi);
+    h = fnv(h, &g->beta_start, sizeof(double) * 2);
+    return h;
+}
+
+// View table, row scales and the per-row spectra of the equiangular
+// Shepp-Logan filter (ImplementUI.cu:6-22): taps dgP G[n] with
+// G[0] = 2/(pi dgP)^2, G[n] = -2 n^2 / (pi^2 (4n^2-1) sin^2(n dgP)), dgP = dgamma P.
+ctr_status curved_tables(ctr_ctx *ctx, const ctr_curved_geom *g)
+{
+    CurvedTables &T = ctx->curved;
+    const uint64_t key = curved_key(g);
+    if (T.valid && T.key == key) return CTR_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    T.valid = false;
+    const double pi = 3.141592653589793;
+    const int nb = g->n_beta, ng = g->n_gamma, nx = g->n_xi;
+    T.h_view.resize(2 * (size_t)nb);
+    for (int b = 0; b < nb; ++b) {
+        T.h_view[2 * b] = (float)std::sin(g->beta[b]);
+        T.h_view[2 * b + 1] = (float)std::cos(g->beta[b]);
+    }
+    T.h_beta.assign(g->beta, g->beta + nb);
+    T.h_gam.assign(g->gamma, g->gamma + ng);
+    T.h_rowp.resize(nx);
+    const int L = fft_filter_length(ng);
+    if (!L) return fail(ctx, CTR_ETOOBIG, "no FFT length for %d channels", ng);
+    T.fft_len = L;
+    T.h_spec.assign((size_t)nx * L, 0.0);
+    std::vector<double> cosL(L), taps(ng);
+    for (int i = 0; i < L; ++i) cosL[i] = std::cos(2.0 * pi * i / L);
+    const double dg = std::fabs(g->gamma[1] - g->gamma[0]);
+    for (int x = 0; x < nx; ++x) {
+        const double P = g->distance_s2d / std::sqrt(g->distance_s2d * g->distance_s2d + g->xi[x] * g->xi[x]);
+        T.h_rowp[x] = P;
+        const double dgp = dg * P;
+        taps[0] = dgp * 2.0 / ((pi * dgp) * (pi * dgp));
+        for (int n = 1; n < ng; ++n) {
+            const double sn = std::sin(n * dgp);
+            taps[n] = dgp * (-2.0 * n * n / (pi * pi) / (4.0 * n * n - 1.0) / (sn * sn));
+        }
+        double *H = T.h_spec.data() + (size_t)x * L;
+        for (int k = 0; k < L; ++k) {
+            double acc = taps[0];
+            for (int m = 1; m < ng; ++m) acc += 2.0 * taps[m] * cosL[((long)k * m) % L];
+            H[k] = acc / L;
+        }
+    }
+    ctr_status s;
+    if ((s = upload(ctx, T.d_view, T.h_view.data(), sizeof(float) * T.h_view.size())) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_rowp, T.h_rowp.data(), sizeof(double) * nx)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_beta, T.h_beta.data(), sizeof(double) * nb)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_gam, T.h_gam.data(), sizeof(double) * ng)) != CTR_OK) return s;
+    if ((s = upload(ctx, T.d_spec, T.h_spec.data(), sizeof(double) * T.h_spec.size())) != CTR_OK) return s;
+    T.key = key;
+    T.valid = true;
+    return CTR_OK;
+}
+
+CurvedBpParams curved_bp_params(ctr_ctx *ctx, const ctr_curved_geom *g, unsigned flags)
+{
+    CurvedBpParams P{};
+    P.nt = g->nt;
+    P.ns = g->ns;
+    P.nz = g->nz;
+    P.ng = g->n_gamma;
+    P.nx = g->n_xi;
+    P.nb = g->n_beta;
+    P.rt = (float)(g->size[0] / g->nt);
+    P.rs = (float)(g->size[1] / g->ns);
+    P.rz = (float)(g->size[2] / g->nz);
+    P.ct = (float)(g->size[0] / 2);
+    P.cs = (float)(g->size[1] / 2);
+    P.cz = (float)(g->size[2] / 2);
+    P.off = (flags & CTR_REF_QUIRKS) ? -0.5f : 0.5f;
+    P.D = (float)g->distance;
+    P.Dsd = (float)g->distance_s2d;
+    // uniform grids through the end samples
+    const double dgs = (g->gamma[g->n_gamma - 1] - g->gamma[0]) / (g->n_gamma - 1);
+    P.g_scale = (float)(1.0 / dgs);
+    P.g_off = (float)(-g->gamma[0] / dgs);
+    P.gmax = (float)(g->n_gamma - 1);
+    const double dxs = (g->xi[g->n_xi - 1] - g->xi[0]) / (g->n_xi - 1);
+    P.x_scale = (float)(1.0 / dxs);
+    P.x_off = (float)(-g->xi[0] / dxs);
+    P.xmax = (float)(g->n_xi - 1);
+    P.dbeta = (float)std::fabs(g->beta[1] - g->beta[0]);
+    P.view = (const float *)ctx->curved.d_view.ptr;
+    P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
+    return P;
+}
+
 // Half-scaled ramp taps times the pitch, two-sided: h2[k] = d * h[|k-(n-1)|].
 ctr_status tap_table(ctr_ctx *ctx, int filter, int equiangular, double d, int n)
 {
@@ -800,6 +944,8 @@ size_t nvox2(const ctr_fan_geom *g) { return (size_t)g->nt * g->ns; }
 size_t nray2(const ctr_fan_geom *g) { return (size_t)g->n_beta * g->n_p; }
 size_t npix(const ctr_par_geom *g) { return (size_t)g->nt * g->ns; }
 size_t nrayp(const ctr_par_geom *g) { return (size_t)g->n_theta * g->n_t; }
+size_t nvoxc(const ctr_curved_geom *g) { return (size_t)g->nt * g->ns * g->nz; }
+size_t nrayc(const ctr_curved_geom *g) { return (size_t)g->n_beta * g->n_gamma * g->n_xi; }
 
 #define TRY(expr)                          \
     do {                                   \
@@ -882,6 +1028,27 @@ ctr_status par_filter_impl(ctr_ctx *ctx, const ctr_par_geom *g, int filter, cons
     F.spec = ctx->taps.fft_len ? (const double *)ctx->taps.d_spec.ptr : nullptr;
     F.fft_len = ctx->taps.fft_len;
     return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "parallel ramp filter");
+}
+
+// ImplementUI.cu:24-78 on the FFT path with one spectrum per detector row
+ctr_status curved_filter_impl(ctr_ctx *ctx, const ctr_curved_geom *g, const float *in, float *out)
+{
+    const CurvedTables &T = ctx->curved;
+    FilterParams F{};
+    F.np = g->n_gamma;
+    F.nx = g->n_xi;
+    F.rows = g->n_beta * g->n_xi;
+    F.mode = 3;
+    F.D = (float)g->distance;
+    F.spec = (const double *)T.d_spec.ptr;
+    F.fft_len = T.fft_len;
+    F.spec_stride = T.fft_len;
+    F.rowp = (const double *)T.d_rowp.ptr;
+    F.gamd = (const double *)T.d_gam.ptr;
+    F.betad = (const double *)T.d_beta.ptr;
+    F.start = g->beta_start;
+    F.fan = g->fan_angle;
+    return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "curved-detector filter");
 }
 
 ctr_status check_equiangular(ctr_ctx *ctx, const ctr_fan_geom *g)
@@ -1026,6 +1193,7 @@ ctr_status ctr_cone_geom_check(const ctr_cone_geom *g, int need_uniform_pitch) {
 
 ctr_status ctr_fan_geom_check(const ctr_fan_geom *g, int need_uniform_pitch) { return check_fan(g, need_uniform_pitch); }
 ctr_status ctr_par_geom_check(const ctr_par_geom *g, int need_uniform_pitch) { return check_par(g, need_uniform_pitch); }
+ctr_status ctr_curved_geom_check(const ctr_curved_geom *g) { return check_curved(g); }
 
 // ---- cone ------------------------------------------------------------------
 
@@ -1307,6 +1475,57 @@ ctr_status ctr_par_fbp(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const fl
     TRY(check_launch(ctx, launch_par_fbp_bp(P, filt, di, ctx->stream), "parallel FBP backprojection"));
     if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
     return finish_out(ctx, dev, di, img, npix(g));
+}
+
+// ---- curved detector, short scan ------------------------------------------------
+
+ctr_status ctr_curved_filter(ctr_ctx *ctx, const ctr_curved_geom *g, const float *proj, float *filtered,
+                             unsigned flags)
+{
+    TRY(check_curved(g));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(curved_tables(ctx, g));
+    const float *dp;
+    float *df;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nrayc(g), &dp));
+    TRY(stage_out(ctx, dev, false, kWsTmp, filtered, nrayc(g), &df));
+    TRY(curved_filter_impl(ctx, g, dp, df));
+    return finish_out(ctx, dev, df, filtered, nrayc(g));
+}
+
+ctr_status ctr_curved_bp(ctr_ctx *ctx, const ctr_curved_geom *g, const float *filtered, float *vol, unsigned flags)
+{
+    TRY(check_curved(g));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(curved_tables(ctx, g));
+    const float *df;
+    float *dv;
+    TRY(stage_in(ctx, dev, kWsIn, filtered, nrayc(g), &df));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvoxc(g), &dv));
+    const CurvedBpParams P = curved_bp_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_curved_bp(P, df, dv, ctx->stream), "curved-detector backprojection"));
+    return finish_out(ctx, dev, dv, vol, nvoxc(g));
+}
+
+ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *proj, float *vol, unsigned flags)
+{
+    TRY(check_curved(g));
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(curved_tables(ctx, g));
+    const float *dp;
+    float *dv;
+    TRY(stage_in(ctx, dev, kWsIn, proj, nrayc(g), &dp));
+    TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvoxc(g), &dv));
+    TRY(buf_reserve(ctx, ctx->ws[kWsTmp], nrayc(g) * sizeof(float)));
+    float *filt = (float *)ctx->ws[kWsTmp].ptr;
+    TRY(curved_filter_impl(ctx, g, dp, filt));
+    const CurvedBpParams P = curved_bp_params(ctx, g, flags);
+    TRY(check_launch(ctx, launch_curved_bp(P, filt, dv, ctx->stream), "curved-detector backprojection"));
+    if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
+    return finish_out(ctx, dev, dv, vol, nvoxc(g));
 }
 
 }  // extern "C"

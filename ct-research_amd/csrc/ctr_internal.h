@@ -70,7 +70,9 @@ struct SirtUpdate {
 // Row filter (pre-weight + convolution along p).
 struct FilterParams {
     int np, nx, rows;       // detector columns, rows per view, total rows
-    int mode;               // 0: w = D/sqrt(D^2+p^2+xi^2); 1: w = colw[p]; 2: w = 1
+    int mode;               // 0: w = D/sqrt(D^2+p^2+xi^2); 1: w = colw[p]; 2: w = 1;
+                            // 3: curved short scan, w = (D/P) cos(g P) Parker(b P, start P, g P, fan P)
+                            //    with P = rowp[row % nx], b = betad[row / nx], g = gamd[p]
     float D;
     const float *pd;        // [np] column positions
     const float *xd;        // [nx] row positions
@@ -78,7 +80,51 @@ struct FilterParams {
     const double *taps;     // [2np-1] two-sided taps times the pitch: h2[k] = d*h[|k-(np-1)|]
     const double *spec;     // [fft_len] real DFT of the circular taps divided by fft_len, or null (direct path)
     int fft_len;
+    int spec_stride;        // nonzero: spec holds one spectrum per detector row (row % nx) this many
+                            // doubles apart, and the FFT path pairs rows of equal row index
+    const double *rowp;     // mode 3: [nx] row scale Dsd / sqrt(Dsd^2 + xi^2) (FP64: the window's
+                            //         sine argument (pi + P (fan - b)) / (P (fan/2 + g)) is
+                            //         ill-conditioned in P near the end of the scan)
+    const double *gamd;     // mode 3: [np] channel angles
+    const double *betad;    // mode 3: [rows / nx] view angles
+    double start, fan;      // mode 3: Parker window start and full fan angle
 };
+
+// Parker short-scan window (ParkFunction.m:6-16): region tests in FP64,
+// the sine in FP32.  A ramp region of zero width (a channel at +-fan/2)
+// gives weight 0.
+__device__ __forceinline__ float parker_weight(double b, double st, double g, double fan)
+{
+    constexpr double pi = 3.141592653589793;
+    if (b >= st && b < st + fan - 2.0 * g) {
+        const double den = 0.5 * fan - g;
+        if (!(den > 0.0)) return 0.0f;
+        const float v = sinf((float)(0.25 * pi * b / den));
+        return v * v;
+    }
+    if (b >= st + fan - 2.0 * g && b <= st + pi - 2.0 * g) return 1.0f;
+    if (b > st + pi - 2.0 * g && b <= st + pi + fan) {
+        const double den = 0.5 * fan + g;
+        if (!(den > 0.0)) return 0.0f;
+        const float v = sinf((float)(0.25 * pi * (pi + fan - b) / den));
+        return v * v;
+    }
+    return 0.0f;
+}
+
+// Curved (cylindrical, equiangular) cone-beam FDK backprojection.
+struct CurvedBpParams {
+    int nt, ns, nz, ng, nx, nb;
+    float rt, rs, rz, ct, cs, cz;
+    float off;                    // voxel centre (i + off) r - c: 0.5, or -0.5 for the reference's offset
+    float D, Dsd;
+    float g_scale, g_off, gmax;   // channel coordinate gamma*g_scale + g_off, valid in [0, gmax)
+    float x_scale, x_off, xmax;   // row coordinate xi*x_scale + x_off, valid in [0, xmax)
+    float dbeta;
+    const float *view;            // [nb][2] sin, cos
+    int accumulate;
+};
+hipError_t launch_curved_bp(const CurvedBpParams &P, const float *filt, float *vol, hipStream_t st);
 
 // Fan-beam pixel-driven kernels.
 struct FanParams {

@@ -296,13 +296,21 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restr
 #pragma unroll
     for (int m = 0; m < NBF; ++m) tw.c[m] = twiddle(j + G * m, L);
 
-    const int npairs = (P.rows + 1) / 2;
+    // with one spectrum per detector row, a pair is one row index in two consecutive views
+    const int nviews = P.rows / P.nx;
+    const int npairs = P.spec_stride ? P.nx * ((nviews + 1) / 2) : (P.rows + 1) / 2;
     for (int base = blockIdx.x * GPB; base < npairs; base += gridDim.x * GPB) {
         const int pair = base + grp;   // every group takes part in the barriers, active or not
-        const int row0 = 2 * pair, row1 = row0 + 1;
+        const int row0 = P.spec_stride ? (2 * (pair / P.nx)) * P.nx + pair % P.nx : 2 * pair;
+        const int row1 = P.spec_stride ? row0 + P.nx : row0 + 1;
         const bool has0 = pair < npairs, has1 = has0 && row1 < P.rows;
         const float xi0 = P.mode == 0 && has0 ? P.xd[row0 % P.nx] : 0.0f;
         const float xi1 = P.mode == 0 && has1 ? P.xd[row1 % P.nx] : 0.0f;
+        const double *spec = P.spec + (P.spec_stride && has0 ? (size_t)(row0 % P.nx) * P.spec_stride : 0);
+        // mode 3: row scale and the scaled view angles of the two rows
+        const double prow = P.mode == 3 && has0 ? P.rowp[row0 % P.nx] : 1.0;
+        const double bp0 = P.mode == 3 && has0 ? P.betad[row0 / P.nx] * prow : 0.0;
+        const double bp1 = P.mode == 3 && has1 ? P.betad[row1 / P.nx] * prow : 0.0;
         const float *src0 = in + (size_t)row0 * np, *src1 = in + (size_t)row1 * np;
         double2 v[16];
 #pragma unroll
@@ -317,19 +325,32 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restr
                     w1 = P.D / sqrtf(P.D * P.D + pq * pq + xi1 * xi1);
                 } else if (P.mode == 1) {
                     w0 = w1 = P.colw[q];
-                } else {
+                } else if (P.mode == 2) {
                     w0 = w1 = 1.0f;
+                } else {
+                    // ImplementUI.cu:43-66: (D/P) cos(g P) Parker(b P, start P, g P, fan P)
+                    const double gp = P.gamd[q] * prow;
+                    const float base_w = (float)((double)P.D / prow) * cosf((float)gp);
+                    const double stp = P.start * prow, fanp = P.fan * prow;
+                    w0 = base_w * parker_weight(bp0, stp, gp, fanp);
+                    w1 = base_w * parker_weight(bp1, stp, gp, fanp);
                 }
-                // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
-                if (has0) a = (double)(src0[q] * w0);
-                if (has1) b = (double)(src1[q] * w1);
+                if (P.mode == 3) {
+                    // the reference keeps the weighted samples in FP64 (ImplementUI.cu:66)
+                    if (has0) a = (double)src0[q] * (double)w0;
+                    if (has1) b = (double)src1[q] * (double)w1;
+                } else {
+                    // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
+                    if (has0) a = (double)(src0[q] * w0);
+                    if (has1) b = (double)(src1[q] * w1);
+                }
             }
             v[t] = make_double2(a, b);
         }
         fft_group<L>(v, buf, j, tw);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
-            const double h = P.spec[j + G * t];   // H[k] / L, real
+            const double h = spec[j + G * t];   // H[k] / L, real
             v[t] = make_double2(v[t].x * h, -v[t].y * h);
         }
         fft_group<L>(v, buf, j, tw);
@@ -348,7 +369,7 @@ template <int L>
 hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStream_t st)
 {
     constexpr int GPB = kFftThreads / (L / 16);
-    const int npairs = (P.rows + 1) / 2;
+    const int npairs = P.spec_stride ? P.nx * ((P.rows / P.nx + 1) / 2) : (P.rows + 1) / 2;
     const int need = (npairs + GPB - 1) / GPB;
     const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
     hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);

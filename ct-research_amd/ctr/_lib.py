@@ -27,6 +27,7 @@ EXPORTS = [
     "ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_cone_filter", "ctr_fdk_bp", "ctr_fdk",
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
     "ctr_par_geom_check", "ctr_par_fp", "ctr_par_bp_adjoint", "ctr_par_filter", "ctr_par_fbp",
+    "ctr_curved_geom_check", "ctr_curved_filter", "ctr_curved_bp", "ctr_curved_fdk",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
@@ -60,6 +61,14 @@ class ParGeomC(ctypes.Structure):
     _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("size", ctypes.c_double * 2),
                 ("n_theta", ctypes.c_int), ("n_t", ctypes.c_int),
                 ("theta", ctypes.c_void_p), ("t", ctypes.c_void_p)]
+
+
+class CurvedGeomC(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("nz", ctypes.c_int),
+                ("size", ctypes.c_double * 3), ("distance", ctypes.c_double), ("distance_s2d", ctypes.c_double),
+                ("n_beta", ctypes.c_int), ("n_gamma", ctypes.c_int), ("n_xi", ctypes.c_int),
+                ("beta", ctypes.c_void_p), ("gamma", ctypes.c_void_p), ("xi", ctypes.c_void_p),
+                ("beta_start", ctypes.c_double), ("fan_angle", ctypes.c_double)]
 
 
 class SirtOptsC(ctypes.Structure):
@@ -109,8 +118,10 @@ def lib():
         L.ctr_cone_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_fan_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_par_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ctr_curved_geom_check.argtypes = [ctypes.c_void_p]
         for name in ("ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_fdk_bp",
-                     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_par_fp", "ctr_par_bp_adjoint"):
+                     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_par_fp", "ctr_par_bp_adjoint",
+                     "ctr_curved_filter", "ctr_curved_bp", "ctr_curved_fdk"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint]
         for name in ("ctr_cone_filter", "ctr_fdk", "ctr_par_filter", "ctr_par_fbp"):
@@ -237,6 +248,34 @@ class ParGeom:
         return lib().ctr_par_geom_check(ctypes.byref(self.c), int(need_uniform))
 
 
+class CurvedGeom:
+    """Curved (cylindrical, equiangular) cone-beam short scan: volume nt x ns x nz
+    over size (mm), projections [beta][xi][gamma] with gamma the channel angles
+    (rad) and xi the row positions (mm) on the cylinder of radius distance_s2d;
+    Parker window from beta_start over fan_angle (ImplementUI.cpp:7-24)."""
+
+    def __init__(self, nt, ns, nz, size, distance, distance_s2d, beta, gamma, xi, beta_start, fan_angle):
+        self.nt, self.ns, self.nz = int(nt), int(ns), int(nz)
+        self.size = tuple(float(v) for v in size)
+        self.distance, self.distance_s2d = float(distance), float(distance_s2d)
+        self.beta, self.gamma, self.xi = _f64(beta), _f64(gamma), _f64(xi)
+        self.beta_start, self.fan_angle = float(beta_start), float(fan_angle)
+        self.c = CurvedGeomC(self.nt, self.ns, self.nz, (ctypes.c_double * 3)(*self.size), self.distance,
+                             self.distance_s2d, len(self.beta), len(self.gamma), len(self.xi), self.beta.ctypes.data,
+                             self.gamma.ctypes.data, self.xi.ctypes.data, self.beta_start, self.fan_angle)
+
+    @property
+    def nvox(self):
+        return self.nt * self.ns * self.nz
+
+    @property
+    def nray(self):
+        return len(self.beta) * len(self.gamma) * len(self.xi)
+
+    def check(self):
+        return lib().ctr_curved_geom_check(ctypes.byref(self.c))
+
+
 def _is_device(x):
     return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
 
@@ -355,6 +394,16 @@ class Context:
 
     def par_fbp(self, g, proj, filt=FILTER_SL_HALF, out=None, flags=0):
         return self._run(lib().ctr_par_fbp, g, proj, g.nvox, out, flags, extra=(int(filt),))
+
+    # -- curved detector, short scan ----------------------------------------------
+    def curved_filter(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_curved_filter, g, proj, g.nray, out, flags)
+
+    def curved_bp(self, g, filtered, out=None, flags=0):
+        return self._run(lib().ctr_curved_bp, g, filtered, g.nvox, out, flags)
+
+    def curved_fdk(self, g, proj, out=None, flags=0):
+        return self._run(lib().ctr_curved_fdk, g, proj, g.nvox, out, flags)
 
     # -- SIRT -------------------------------------------------------------------
     def _sirt(self, fn, g, b, n_iter, lam, lo, hi, x0, residuals, flags):

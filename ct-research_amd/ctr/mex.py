@@ -11,7 +11,7 @@ quirks listed in the flags of ctr.h are off unless ref_quirks=True.
 import numpy as np
 
 from . import _lib
-from ._lib import ConeGeom, FanGeom, ParGeom, default_context
+from ._lib import ConeGeom, CurvedGeom, FanGeom, ParGeom, default_context
 
 
 def _col(a):
@@ -100,6 +100,21 @@ def FBPparallel(R, ThetaScanRange, Tdomain, Size, t_length, s_length, filt=_lib.
     g = ParGeom(int(t_length), int(s_length), _col(Size)[:2], _col(ThetaScanRange).astype(np.float32),
                 _col(Tdomain).astype(np.float32))
     return ctx.par_fbp(g, R, filt=filt).astype(np.float64)
+
+
+def ImplementUI(R, Xigamadomain, Gamadomain, BetaScanRange, betaStartAngle, MaxGama, Distance, Distance_s2d, Size,
+                z_length, ref_quirks=False, ctx=None):
+    """Display = ImplementUI(R, Xigamadomain, Gamadomain, BetaScanRange, StartAngle, GamaMax, Distance,
+    Distance_s2d, Size, z_length) (My_toolbox/Implement_UnitedImaging_Data/ImplementUI.cpp:3-57):
+    curved-detector short-scan FDK.  Size(1:2) are the in-plane pixel counts at
+    1 mm, Size(3) the z extent (mm); all three are truncated to integers as the
+    reference does (ImplementUI.cpp:20-22)."""
+    ctx = ctx or default_context()
+    sz = [int(v) for v in _col(Size)[:3]]
+    g = CurvedGeom(sz[0], sz[1], int(z_length), (float(sz[0]), float(sz[1]), float(sz[2])), float(Distance),
+                   float(Distance_s2d), BetaScanRange, Gamadomain, Xigamadomain, float(betaStartAngle),
+                   float(MaxGama))
+    return ctx.curved_fdk(g, R, flags=_flags(ref_quirks)).astype(np.float64)
 
 
 def SIRT3D(R, Xigamadomain, Pdomain, BetaScanRange, Distance, Size, Nt, Ns, Nz, nIter, Lamda,

@@ -10,7 +10,10 @@ launches on).  Use it under rocprofv3 to attribute kernel time per operator.
 
 The par_* operators run the parallel-beam workload of FBP_parallel_GPU.m:
 512 x 512 pixels over 60 mm, 360 one-degree views, t from -47 mm in steps
-of 1.1 pixel (730 bins), on the 2-D Shepp-Logan phantom.
+of 1.1 pixel (730 bins), on the 2-D Shepp-Logan phantom.  The curved_*
+operators run the short scan of ImplementUIData.m: 512 x 512 x 50 voxels,
+800 views over 240 degrees, 80 rows x 936 channels (uniform synthetic
+channel angles, uniform random projections).
 """
 import argparse
 import os
@@ -18,7 +21,7 @@ import sys
 import time
 
 OPS = ("cone_fp", "cone_filter", "fdk_bp", "cone_bp_ref", "cone_adjoint", "sirt3d",
-       "par_fp", "par_filter", "par_fbp", "par_adjoint")
+       "par_fp", "par_filter", "par_fbp", "par_adjoint", "curved_filter", "curved_bp")
 
 
 def parallel_workload():
@@ -81,6 +84,15 @@ def main(argv=None):
         pout = torch.empty_like(img)
         torch.cuda.synchronize()
         print(f"parallel workload: 512^2 image, {len(gp.theta)} views, {len(gp.t)} bins", flush=True)
+    if any(o.startswith("curved_") for o in ops):
+        from . import uidata
+        gc = uidata.scan_geometry(uidata.synthetic_detector_angles())
+        cproj = torch.rand(gc.nray, device="cuda")
+        cfilt = ctx.curved_filter(gc, cproj)
+        cvol = torch.empty(gc.nvox, device="cuda")
+        torch.cuda.synchronize()
+        print(f"curved workload: {gc.nt}x{gc.ns}x{gc.nz}, {len(gc.beta)} views, {len(gc.xi)}x{len(gc.gamma)} detector",
+              flush=True)
     calls = {
         "cone_fp": lambda: ctx.cone_fp(g, vol, out=proj),
         "cone_filter": lambda: ctx.cone_filter(g, proj, out=filt),
@@ -91,6 +103,8 @@ def main(argv=None):
         "par_filter": lambda: ctx.par_filter(gp, pproj, out=pfilt),
         "par_fbp": lambda: ctx.par_fbp(gp, pproj, out=pout),
         "par_adjoint": lambda: ctx.par_bp_adjoint(gp, pproj, out=pout),
+        "curved_filter": lambda: ctx.curved_filter(gc, cproj, out=cfilt),
+        "curved_bp": lambda: ctx.curved_bp(gc, cfilt, out=cvol),
     }
     for o in ops:
         if o == "sirt3d":

@@ -102,6 +102,22 @@ typedef struct ctr_par_geom {
     const double *t;        /* [n_t] signed ray offsets from the image centre (mm), monotonic */
 } ctr_par_geom;
 
+/* Curved (cylindrical, equiangular) cone-beam short-scan geometry (the argument
+ * set of ImplementUI: My_toolbox/Implement_UnitedImaging_Data/ImplementUI.cpp:7-24).
+ * Projections [n_beta][n_xi][n_gamma], gamma fastest. */
+typedef struct ctr_curved_geom {
+    int nt, ns, nz;         /* voxels along t, s, z */
+    double size[3];         /* physical extent of the volume (mm) */
+    double distance;        /* source-to-isocentre distance D (mm) */
+    double distance_s2d;    /* source-to-detector distance (mm): radius of the detector cylinder */
+    int n_beta, n_gamma, n_xi;
+    const double *beta;     /* [n_beta] view angles (rad), uniform */
+    const double *gamma;    /* [n_gamma] channel angles (rad), uniform, monotonic */
+    const double *xi;       /* [n_xi] row positions on the detector (mm), uniform, monotonic */
+    double beta_start;      /* short-scan start angle of the Parker window (rad) */
+    double fan_angle;       /* full fan angle of the Parker window (rad) */
+} ctr_curved_geom;
+
 /* SIRT options (Iterative_algorithm/SIRT3D.m, SIRT.m). */
 typedef struct ctr_sirt_opts {
     int n_iter;             /* iterations (SIRT3D.m "Times") */
@@ -146,6 +162,7 @@ ctr_status ctr_copy_to_host(ctr_ctx *ctx, void *dst, const void *src, size_t byt
 ctr_status ctr_cone_geom_check(const ctr_cone_geom *g, int need_uniform_pitch);
 ctr_status ctr_fan_geom_check(const ctr_fan_geom *g, int need_uniform_pitch);
 ctr_status ctr_par_geom_check(const ctr_par_geom *g, int need_uniform_pitch);
+ctr_status ctr_curved_geom_check(const ctr_curved_geom *g);
 
 /* ---- cone beam --------------------------------------------------------- */
 
@@ -221,6 +238,24 @@ ctr_status ctr_par_filter(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const
  * Replaces FBPpara() (FBP_parallel_GPU/FBPparallel.h:31, kernel FBPparallel.cu:41-89). */
 ctr_status ctr_par_fbp(ctr_ctx *ctx, const ctr_par_geom *g, int filter, const float *proj, float *img,
                        unsigned flags);
+
+/* ---- curved detector, short scan --------------------------------------- */
+
+/* Parker-weighted pre-weight (D/P) cos(gamma P) and per-row equiangular
+ * Shepp-Logan filter (full scale), P = Dsd / sqrt(Dsd^2 + xi^2).  Replaces
+ * GFunction + PreWeightFiltration (ImplementUI.cu:6-78).  At most 1024
+ * channels. */
+ctr_status ctr_curved_filter(ctr_ctx *ctx, const ctr_curved_geom *g, const float *proj, float *filtered,
+                             unsigned flags);
+
+/* Backprojection of filtered projections: bilinear sample at
+ * (atan(u/(D-s)), Dsd z/rho) times dbeta rho / L2^1.5.  Replaces the
+ * BackProjection kernel (ImplementUI.cu:80-153).  CTR_REF_QUIRKS places
+ * voxel centres at (i - 0.5) pitch instead of (i + 0.5) (ImplementUI.cu:110). */
+ctr_status ctr_curved_bp(ctr_ctx *ctx, const ctr_curved_geom *g, const float *filtered, float *vol, unsigned flags);
+
+/* Filter + backprojection.  Replaces FDKUI() (ImplementUI.h:31-33). */
+ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *proj, float *vol, unsigned flags);
 
 /* ---- SIRT -------------------------------------------------------------- */
 

@@ -1204,6 +1204,171 @@ void or_par_fbp_bp(const or_par *g, const float *filt, double *img)
     free(ap.pos);
 }
 
+/* ----------------------------------------------------------------------
+ * Curved (equiangular, cylindrical) cone-beam FDK with Parker short-scan
+ * weights: My_toolbox/Implement_UnitedImaging_Data/ImplementUI.cu:6-153
+ * (FDKUI), with ParkFunction.m:6-16.  Projections [beta][xi][gamma], gamma
+ * fastest; gamma are channel angles (rad), xi are row positions (mm) on the
+ * cylinder of radius Dsd; the source sits at (-D sin b, D cos b, 0).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nt, ns, nz;
+    double size[3];
+    double D, Dsd;
+    int nb, ng, nx;
+    const double *beta, *gamma, *xi;
+    double start, fan;
+} or_curved;
+
+/* ParkFunction.m:6-16 (fan = full fan angle).  The sine terms use beta
+ * itself, not beta - start, as the reference does.  A channel at exactly
+ * +-fan/2 can enter a ramp region of zero width through rounding, where the
+ * reference divides by zero (NaN); its weight is 0 here. */
+double or_parker(double beta, double start, double gama, double fan)
+{
+    if (beta >= start && beta < start + fan - 2 * gama) {
+        const double den = fan / 2 - gama;
+        if (!(den > 0.0)) return 0.0;
+        double v = sin(OR_PI / 4 * beta / den);
+        return v * v;
+    }
+    if (beta >= start + fan - 2 * gama && beta <= start + OR_PI - 2 * gama) return 1.0;
+    if (beta > start + OR_PI - 2 * gama && beta <= start + OR_PI + fan) {
+        const double den = fan / 2 + gama;
+        if (!(den > 0.0)) return 0.0;
+        double v = sin(OR_PI / 4 * (OR_PI + fan - beta) / den);
+        return v * v;
+    }
+    return 0.0;
+}
+
+/* Row scale of ImplementUI.cu:14,45: P = Dsd / sqrt(Dsd^2 + xi^2). */
+static double curved_row_scale(const or_curved *g, int x)
+{
+    return g->Dsd / sqrt(g->Dsd * g->Dsd + g->xi[x] * g->xi[x]);
+}
+
+/* Equiangular Shepp-Logan taps of row x (ImplementUI.cu:6-22), full scale:
+ * G[0] = 2/(pi dgP)^2, G[n] = -2 n^2 / (pi^2 (4n^2-1) sin^2(n dgP)), dgP = dgamma * P. */
+void or_curved_taps(const or_curved *g, int x, double *taps)
+{
+    const double P = curved_row_scale(g, x), dg = fabs(g->gamma[1] - g->gamma[0]) * P;
+    taps[0] = 2.0 / ((OR_PI * dg) * (OR_PI * dg));
+    for (int n = 1; n < g->ng; n++) {
+        double sn = sin(n * dg);
+        taps[n] = -2.0 * n * n / (OR_PI * OR_PI) / (4.0 * n * n - 1.0) / (sn * sn);
+    }
+}
+
+/* Pre-weight and filter (ImplementUI.cu:24-78): per row xi, with P the row
+ * scale, R * (D/P) cos(gamma P) * Parker(beta P, start P, gamma P, fan P),
+ * then Rcov[c] = dgamma P * sum_g R'[g] G[|c-g|]. */
+void or_curved_filter(const or_curved *g, const float *proj, double *out)
+{
+    const int ng = g->ng;
+    const double dg0 = fabs(g->gamma[1] - g->gamma[0]);
+#pragma omp parallel
+    {
+        double *taps = malloc(sizeof(double) * ng), *w = malloc(sizeof(double) * ng);
+#pragma omp for collapse(2) schedule(static)
+        for (int b = 0; b < g->nb; b++)
+            for (int x = 0; x < g->nx; x++) {
+                const double P = curved_row_scale(g, x);
+                or_curved_taps(g, x, taps);
+                const float *R = proj + ((long)b * g->nx + x) * ng;
+                for (int c = 0; c < ng; c++) {
+                    const double gp = g->gamma[c] * P;
+                    const double W = or_parker(g->beta[b] * P, g->start * P, gp, g->fan * P);
+                    w[c] = (double)R[c] * (g->D / P) * cos(gp) * W;
+                }
+                double *o = out + ((long)b * g->nx + x) * ng;
+                for (int c = 0; c < ng; c++) {
+                    double acc = 0.0;
+                    for (int q = 0; q < ng; q++) acc += w[q] * taps[abs(c - q)];
+                    o[c] = dg0 * P * acc;
+                }
+            }
+        free(taps); free(w);
+    }
+}
+
+/* Linear-interpolation coordinates on a double axis (interp modes as make_axis). */
+typedef struct { double first, pitch, step, lo, hi; int n; const double *pos; } or_daxis;
+
+static or_daxis make_daxis(const double *pos, int n)
+{
+    or_daxis a;
+    a.first = pos[0];
+    a.pitch = fabs(pos[1] - pos[0]);
+    a.step = (pos[n - 1] - pos[0]) / (n - 1);
+    if (g_interp_mode == 1) a.pitch = fabs(a.step);
+    a.lo = pos[0] < pos[n - 1] ? pos[0] : pos[n - 1];
+    a.hi = pos[0] > pos[n - 1] ? pos[0] : pos[n - 1];
+    a.n = n;
+    a.pos = pos;
+    return a;
+}
+
+/* weights returned as fractions (sum 1) */
+static int daxis_lerp(const or_daxis *a, double x, int *i1, double *w1, double *w2)
+{
+    if (!(x >= a->lo && x < a->hi)) return 0;
+    int i;
+    if (g_interp_mode == 1) {
+        double t = (x - a->first) / a->step;
+        i = (int)floor(t);
+        if (i > a->n - 2) i = a->n - 2;
+        if (i < 0) i = 0;
+        *w2 = t - i;
+        *w1 = 1.0 - *w2;
+    } else {
+        i = (int)floor(fabs(x - a->first) / a->pitch);
+        if (i > a->n - 2) i = a->n - 2;
+        *w2 = fabs(x - a->pos[i]) / a->pitch;
+        *w1 = fabs(a->pos[i + 1] - x) / a->pitch;
+    }
+    *i1 = i;
+    return 1;
+}
+
+/* Backprojection (ImplementUI.cu:80-153): voxel centre (i + o) r - C with
+ * o = +0.5, or the reference's -0.5 when quirks (ImplementUI.cu:110: the
+ * 1-based MATLAB offset of ImplementUIData.m:160 kept for 0-based indices);
+ * gamma = atan(u / (D - s)), xi = Dsd tan(asin(z / |S - v|)), value =
+ * bilinear sample * dbeta / L2 / (|S - v| / rho). */
+void or_curved_bp(const or_curved *g, const float *filt, double *vol, int quirks)
+{
+    const double rt = g->size[0] / g->nt, rs = g->size[1] / g->ns, rz = g->size[2] / g->nz;
+    const double Ct = g->size[0] / 2, Cs = g->size[1] / 2, Cz = g->size[2] / 2;
+    const double o = quirks ? -0.5 : 0.5;
+    const double dbeta = fabs(g->beta[1] - g->beta[0]);
+    or_daxis ag = make_daxis(g->gamma, g->ng), ax = make_daxis(g->xi, g->nx);
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int k = 0; k < g->nz; k++)
+        for (int j = 0; j < g->ns; j++)
+            for (int i = 0; i < g->nt; i++) {
+                const double x = (i + o) * rt - Ct, y = (j + o) * rs - Cs, z = (k + o) * rz - Cz;
+                double acc = 0.0;
+                for (int b = 0; b < g->nb; b++) {
+                    const double sb = sin(g->beta[b]), cb = cos(g->beta[b]);
+                    const double xs = -g->D * sb, ys = g->D * cb;
+                    const double L2 = (x - xs) * (x - xs) + (y - ys) * (y - ys) + z * z;
+                    const double u = x * cb + y * sb, s = -x * sb + y * cb;
+                    const double xi = g->Dsd * tan(asin(z / sqrt(L2)));
+                    const double ga = atan(u / (g->D - s));
+                    const double prop = sqrt(L2) / sqrt(L2 - z * z);
+                    int ig, ix;
+                    double g1, g2, x1, x2;
+                    if (!daxis_lerp(&ag, ga, &ig, &g1, &g2) || !daxis_lerp(&ax, xi, &ix, &x1, &x2)) continue;
+                    const float *F = filt + (long)b * g->nx * g->ng;
+                    const double v = x1 * g1 * F[(long)ix * g->ng + ig] + x2 * g1 * F[(long)(ix + 1) * g->ng + ig] +
+                                     x1 * g2 * F[(long)ix * g->ng + ig + 1] + x2 * g2 * F[(long)(ix + 1) * g->ng + ig + 1];
+                    acc += v / L2 * (dbeta / prop);
+                }
+                vol[((long)k * g->ns + j) * g->nt + i] = acc;
+            }
+}
+
 int or_num_threads(void)
 {
 #ifdef _OPENMP

@@ -48,6 +48,14 @@ class _Par(ctypes.Structure):
                 ("nb", ctypes.c_int), ("np", ctypes.c_int), ("theta", ctypes.c_void_p), ("t", ctypes.c_void_p)]
 
 
+class _Curved(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("nz", ctypes.c_int), ("size", ctypes.c_double * 3),
+                ("D", ctypes.c_double), ("Dsd", ctypes.c_double),
+                ("nb", ctypes.c_int), ("ng", ctypes.c_int), ("nx", ctypes.c_int),
+                ("beta", ctypes.c_void_p), ("gamma", ctypes.c_void_p), ("xi", ctypes.c_void_p),
+                ("start", ctypes.c_double), ("fan", ctypes.c_double)]
+
+
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
@@ -90,6 +98,19 @@ class Par:
                       _ptr(self.theta), _ptr(self.t))
         self.nvox = nt * ns
         self.nray = len(self.theta) * len(self.t)
+
+
+class Curved:
+    """Curved equiangular cone geometry: volume [nz][ns][nt], projections
+    [beta][xi][gamma]; gamma channel angles (rad), xi row positions (mm) on the
+    cylinder of radius Dsd; Parker window from start over the full fan angle."""
+
+    def __init__(self, nt, ns, nz, size, D, Dsd, beta, gamma, xi, start, fan):
+        self.beta, self.gamma, self.xi = _f64(beta), _f64(gamma), _f64(xi)
+        self.s = _Curved(nt, ns, nz, (ctypes.c_double * 3)(*size), D, Dsd, len(self.beta), len(self.gamma),
+                         len(self.xi), _ptr(self.beta), _ptr(self.gamma), _ptr(self.xi), start, fan)
+        self.nvox = nt * ns * nz
+        self.nray = len(self.beta) * len(self.gamma) * len(self.xi)
 
 
 def _call(name, g, *args):
@@ -241,6 +262,33 @@ def par_fbp_bp(g, filtered):
     out = np.zeros(g.nvox)
     y = _f32(filtered)
     _call("or_par_fbp_bp", g, _ptr(y), _ptr(out))
+    return out
+
+
+def parker(beta, start, gamma, fan):
+    fn = lib().or_parker
+    fn.restype = ctypes.c_double
+    fn.argtypes = [ctypes.c_double] * 4
+    return fn(beta, start, gamma, fan)
+
+
+def curved_taps(g, x):
+    out = np.zeros(len(g.gamma))
+    _call("or_curved_taps", g, ctypes.c_int(x), _ptr(out))
+    return out
+
+
+def curved_filter(g, proj):
+    out = np.zeros(g.nray)
+    y = _f32(proj)
+    _call("or_curved_filter", g, _ptr(y), _ptr(out))
+    return out
+
+
+def curved_bp(g, filtered, quirks=False):
+    out = np.zeros(g.nvox)
+    y = _f32(filtered)
+    _call("or_curved_bp", g, _ptr(y), _ptr(out), ctypes.c_int(int(quirks)))
     return out
 
 

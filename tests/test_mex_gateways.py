@@ -134,6 +134,28 @@ def test_parallel_gateway_argument_errors():
     assert e.value.ident == "ctr:ProjectionParallel_2D:EINVAL"
 
 
+def test_implementui_gateway_argument_errors():
+    gamma = 0.002 * (np.arange(64) - 31.5)
+    xi = np.arange(8) - 3.5
+    beta = 0.02 * np.arange(200)
+    ui = Mex("ImplementUI")
+    R = np.zeros(len(beta) * len(xi) * len(gamma), np.float32)
+    args = [xi, gamma, beta, 0.0, 0.126, 100.0, 200.0, [32.0, 32.0, 8.0], 8]
+    with pytest.raises(MexError) as e:
+        ui(R, *args[:-1])
+    assert e.value.ident == "ctr:ImplementUI:nrhs"
+    with pytest.raises(MexError) as e:
+        ui(R[:-1], *args)
+    assert e.value.ident == "ctr:ImplementUI:size"
+    with pytest.raises(MexError) as e:       # fan angle must be positive
+        ui(R, *args[:4], -0.1, *args[5:])
+    assert e.value.ident == "ctr:ImplementUI:EINVAL"
+    with pytest.raises(MexError) as e:       # more than 1024 channels
+        g2 = 0.0001 * (np.arange(1100) - 549.5)
+        ui(np.zeros(len(beta) * len(xi) * 1100, np.float32), xi, g2, *args[2:])
+    assert e.value.ident == "ctr:ImplementUI:ETOOBIG"
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_gateway_reports_missing_device():
     nt, ns, nz, size, beta, p, xi, D = cone_args()
@@ -208,3 +230,18 @@ def test_parallel_gateways(ctx, oracle):
     with oracle.interp_mode(1):
         ref = oracle.par_fbp_bp(oracle.Par(width, height, (40.0, 30.0), th32, t32), f_ref.astype(np.float32))
     assert errs(fbp, ref)[0] < 2e-5
+
+
+@pytest.mark.gpu
+def test_implementui_gateway(ctx):
+    """Size is truncated to integers; R may be double (the reference's class)."""
+    import ctr
+    gamma = 0.002 * (np.arange(64) - 31.5)
+    xi = np.arange(8) - 3.5
+    beta = 0.02 * np.arange(200)
+    R = np.random.default_rng(22).random(len(beta) * len(xi) * len(gamma))
+    out = Mex("ImplementUI")(R, xi, gamma, beta, 0.0, 0.126, 100.0, 200.0, [32.7, 32.2, 8.9], 8)
+    g = ctr.CurvedGeom(32, 32, 8, (32.0, 32.0, 8.0), 100.0, 200.0, beta, gamma, xi, 0.0, 0.126)
+    assert errs(out, ctx.curved_fdk(g, R.astype(np.float32)))[1] == 0.0
+    q = Mex("ImplementUI")(R, xi, gamma, beta, 0.0, 0.126, 100.0, 200.0, [32.0, 32.0, 8.0], 8, 1.0)
+    assert errs(q, ctx.curved_fdk(g, R.astype(np.float32), flags=ctr.REF_QUIRKS))[1] == 0.0

@@ -4,6 +4,8 @@ independent dense-matrix SIRT.  CPU only."""
 import numpy as np
 import pytest
 
+from analytic import curved_ball_projections
+
 
 def small_cone(O, nt=7, ns=6, nz=5, size=(7.0, 6.6, 4.5), D=25.0, nb=9, np_=11, nx=9, pp=0.9, px=0.7,
                beta0=None):
@@ -386,3 +388,63 @@ def test_parallel_adjoint_is_transpose(oracle):
     e[17] = 1.0
     assert np.isclose(oracle.par_adjoint(g, np.ones(g.nray, np.float32))[17], oracle.par_fp_exact(g, e).sum(),
                       rtol=1e-12)
+
+
+# ---- curved (equiangular) cone FDK with Parker weights ----------------------
+
+def small_curved(O, start=0.0):
+    D, Dsd = 150.0, 300.0
+    gamma = 0.002 * (np.arange(241) - 120)
+    xi = 1.0 * (np.arange(24) - 11.5)
+    fan = abs(gamma[-1] - gamma[0])
+    dbeta = 0.01
+    beta = start + dbeta * np.arange(int(np.ceil((np.pi + fan) / dbeta)) + 1)
+    return O.Curved(48, 48, 8, (48.0, 48.0, 8.0), D, Dsd, beta, gamma, xi, start, fan), (D, Dsd, beta, gamma, xi)
+
+
+def test_parker_weights_sum_to_one_over_conjugate_rays(oracle):
+    """Rays (b, g) and (b + pi + 2g, -g) are one line; over a short scan from 0
+    the window (ParkFunction.m) gives them weights summing to 1."""
+    fan = 0.5
+    rng = np.random.default_rng(17)
+    for _ in range(400):
+        g = rng.uniform(-fan / 2 + 1e-3, fan / 2 - 1e-3)
+        b = rng.uniform(0, np.pi + fan)
+        w = oracle.parker(b, 0.0, g, fan)
+        bc = b + np.pi + 2 * g if b + np.pi + 2 * g <= np.pi + fan else b - np.pi + 2 * g
+        wc = oracle.parker(bc, 0.0, -g, fan) if 0 <= bc <= np.pi + fan else 0.0
+        assert 0.0 <= w <= 1.0
+        assert abs(w + wc - 1.0) < 1e-12, (b, g, w, wc)
+    assert oracle.parker(-0.1, 0.0, 0.0, fan) == 0.0 and oracle.parker(np.pi + fan + 0.1, 0.0, 0.0, fan) == 0.0
+
+
+def test_curved_taps_match_equiangular_ramp(oracle):
+    """Row taps are the SL taps of the arc pitch times (n dg / sin(n dg))^2
+    (equiangular ramp, Kak & Slaney 3.4); their DFT is twice the SL-half
+    response |sin(w/2)|/(2 pi) of the flat filter at small n dg."""
+    g, _ = small_curved(oracle)
+    taps = oracle.curved_taps(g, 0)
+    P = 300.0 / np.sqrt(300.0 ** 2 + 11.5 ** 2)
+    dg = 0.002 * P
+    n = np.arange(1, 241)
+    flat = -2.0 / (np.pi ** 2 * dg ** 2 * (4 * n ** 2 - 1))
+    assert np.allclose(taps[1:], flat * (n * dg / np.sin(n * dg)) ** 2, rtol=1e-12)
+    assert np.isclose(taps[0], 2.0 / (np.pi * dg) ** 2, rtol=1e-14)
+
+
+@pytest.mark.parametrize("start", [0.0, 0.027])
+def test_curved_fdk_recovers_ball(oracle, start):
+    """Short scan (pi + fan) with Parker weights: the central slices of the
+    FDK of an analytic ball are its amplitude (also with the start-angle
+    offset that the window's sine terms ignore)."""
+    g, (D, Dsd, beta, gamma, xi) = small_curved(oracle, start)
+    proj = curved_ball_projections(beta, gamma, xi, D, Dsd, (3.0, -2.0, 0.0), 12.0, 1.0)
+    filt = oracle.curved_filter(g, proj)
+    with oracle.interp_mode(1):
+        vol = oracle.curved_bp(g, filt.astype(np.float32)).reshape(8, 48, 48)
+    c = np.arange(48) + 0.5 - 24
+    inside = ((c[None, :] - 3.0) ** 2 + (c[:, None] + 2.0) ** 2) < 8.0 ** 2
+    outside = ((c[None, :] - 3.0) ** 2 + (c[:, None] + 2.0) ** 2) > 16.0 ** 2
+    for k in (3, 4):
+        assert abs(vol[k][inside].mean() - 1.0) < 0.02
+        assert abs(vol[k][outside].mean()) < 0.02
