@@ -306,8 +306,9 @@ ctr_status check_curved(const ctr_curved_geom *g)
     if (g->n_beta < 2 || g->n_gamma < 2 || g->n_xi < 2)
         return fail(nullptr, CTR_EINVAL, "need at least 2 views, 2 channels and 2 rows");
     if (g->n_gamma > 1024) return fail(nullptr, CTR_ETOOBIG, "the curved-detector filter supports at most 1024 channels");
-    if ((double)g->n_beta * g->n_gamma * g->n_xi > (double)std::numeric_limits<int>::max())
-        return fail(nullptr, CTR_ETOOBIG, "more than 2^31-1 detector samples");
+    // the backprojection reads the filtered projections with 32-bit byte offsets
+    if ((double)g->n_beta * g->n_gamma * g->n_xi > (double)(1 << 30))
+        return fail(nullptr, CTR_ETOOBIG, "more than 2^30 detector samples");
     if (g->n_beta > 65535) return fail(nullptr, CTR_ETOOBIG, "at most 65535 views");
     if (!std::isfinite(g->beta_start) || !finite_pos(g->fan_angle) || g->fan_angle >= 3.141592653589793)
         return fail(nullptr, CTR_EINVAL, "beta_start must be finite and fan_angle in (0, pi)");

@@ -14,8 +14,7 @@
 namespace ctr {
 namespace {
 
-constexpr int kZc = 8;
-
+template <int kZc>
 __global__ __launch_bounds__(256) void k_curved_bp(const float *__restrict__ filt, float *__restrict__ vol,
                                                    const float *__restrict__ views, CurvedBpParams P)
 {
@@ -30,34 +29,46 @@ __global__ __launch_bounds__(256) void k_curved_bp(const float *__restrict__ fil
         acc[c] = 0.0f;
     }
     const int ng = P.ng;
-    const size_t view_stride = (size_t)P.nx * ng;
+    const size_t view_elems = (size_t)P.nx * ng;
     for (int b = 0; b < P.nb; ++b) {
         const float sb = views[2 * b], cb = views[2 * b + 1];
         const float u = fmaf(x, cb, y * sb), s = fmaf(-x, sb, y * cb);
         // source (-D sin b, D cos b, 0): in-plane distance^2 = u^2 + (D - s)^2
         const float ds = P.D - s;
         const float rho2 = fmaf(u, u, ds * ds);
-        const float gi = fmaf(atanf(u / ds), P.g_scale, P.g_off);
+        const float gi = fmaf(atanf(u * __builtin_amdgcn_rcpf(ds)), P.g_scale, P.g_off);
         if (!(gi >= 0.0f && gi < P.gmax)) continue;
         const int i0 = min((int)gi, ng - 2);
         const float fg = gi - (float)i0;
-        const float inv_rho = rsqrtf(rho2), rho = rho2 * inv_rho;
+        const float inv_rho = __builtin_amdgcn_rsqf(rho2), rho = rho2 * inv_rho;
         const float xs = P.Dsd * inv_rho * P.x_scale;   // row coordinate per unit z
         const float wb = P.dbeta * rho;
-        const float *F = filt + (size_t)b * view_stride + i0;
+        const float *Fv = filt + (size_t)b * view_elems;
+        // branch-free over the chunk, so that all its loads are in flight together:
+        // rows are clamped into the detector and out-of-range voxels get weight 0
+        float v0[kZc], v1[kZc], v2[kZc], v3[kZc], fxs[kZc], msk[kZc];
 #pragma unroll
         for (int c = 0; c < kZc; ++c) {
             const float xi = fmaf(zc[c], xs, P.x_off);
-            if (xi >= 0.0f && xi < P.xmax) {
-                const int j0 = min((int)xi, P.nx - 2);
-                const float fx = xi - (float)j0;
-                const float *r0 = F + (size_t)j0 * ng, *r1 = r0 + ng;
-                const float a = fmaf(fg, r0[1] - r0[0], r0[0]);
-                const float bq = fmaf(fg, r1[1] - r1[0], r1[0]);
-                const float L2 = fmaf(zc[c], zc[c], rho2);
-                const float w = wb / (L2 * sqrtf(L2));
-                acc[c] = fmaf(fmaf(fx, bq - a, a), w, acc[c]);
-            }
+            msk[c] = (xi >= 0.0f && xi < P.xmax) ? 1.0f : 0.0f;
+            const float xcl = __builtin_amdgcn_fmed3f(xi, 0.0f, P.xmax);
+            const int j0 = min((int)xcl, P.nx - 2);
+            fxs[c] = xcl - (float)j0;
+            // the view's base pointer is uniform; 32-bit offsets within the view
+            const float *q = Fv + (unsigned)(j0 * ng + i0);
+            v0[c] = q[0];
+            v1[c] = q[1];
+            v2[c] = q[ng];
+            v3[c] = q[ng + 1];
+        }
+#pragma unroll
+        for (int c = 0; c < kZc; ++c) {
+            const float a = fmaf(fg, v1[c] - v0[c], v0[c]);
+            const float bq = fmaf(fg, v3[c] - v2[c], v2[c]);
+            // weight dbeta rho / L2^1.5 (ImplementUI.cu:145: / L2 * dbeta / (sqrt(L2) / rho))
+            const float L2 = fmaf(zc[c], zc[c], rho2);
+            const float r = __builtin_amdgcn_rsqf(L2);
+            acc[c] = fmaf(fmaf(fxs[c], bq - a, a), msk[c] * wb * (r * r * r), acc[c]);
         }
     }
 #pragma unroll
@@ -70,13 +81,42 @@ __global__ __launch_bounds__(256) void k_curved_bp(const float *__restrict__ fil
     }
 }
 
+template <int ZC>
+hipError_t launch_zc(const CurvedBpParams &P, const float *filt, float *vol, hipStream_t st)
+{
+    const dim3 grid((P.nt + 15) / 16, (P.ns + 15) / 16, (P.nz + ZC - 1) / ZC);
+    hipLaunchKernelGGL((k_curved_bp<ZC>), grid, dim3(256), 0, st, filt, vol, P.view, P);
+    return hipGetLastError();
+}
+
+// z-chunk per thread: the per-view set-up (about 30 instructions) is shared
+// by the chunk's voxels (about 20 each); chunks past nz are wasted work
+int choose_zc(int nz)
+{
+    static const int cand[] = {4, 8, 10, 12, 16};
+    int best = 8;
+    long best_cost = -1;
+    for (int zc : cand) {
+        const long cost = (long)((nz + zc - 1) / zc) * (30 + 20 * zc);
+        if (best_cost < 0 || cost < best_cost) {
+            best = zc;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 }  // namespace
 
 hipError_t launch_curved_bp(const CurvedBpParams &P, const float *filt, float *vol, hipStream_t st)
 {
-    const dim3 grid((P.nt + 15) / 16, (P.ns + 15) / 16, (P.nz + kZc - 1) / kZc);
-    hipLaunchKernelGGL(k_curved_bp, grid, dim3(256), 0, st, filt, vol, P.view, P);
-    return hipGetLastError();
+    switch (choose_zc(P.nz)) {
+    case 4: return launch_zc<4>(P, filt, vol, st);
+    case 10: return launch_zc<10>(P, filt, vol, st);
+    case 12: return launch_zc<12>(P, filt, vol, st);
+    case 16: return launch_zc<16>(P, filt, vol, st);
+    default: return launch_zc<8>(P, filt, vol, st);
+    }
 }
 
 }  // namespace ctr

@@ -121,7 +121,9 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     const double lam_end = hi - lo;
     chord = (float)lam_end;
 
-    const unsigned nbytes = (unsigned)((size_t)g.nt * g.ns * g.nz * sizeof(float));
+    // num_records is 32 bits: 2^30 voxels (4 GiB) saturate to 2^32 - 1, above every valid offset
+    const size_t vbytes = (size_t)g.nt * g.ns * g.nz * sizeof(float);
+    const unsigned nbytes = vbytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (unsigned)vbytes;
     __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)(tr ? g.volT : vol), 0, (int)nbytes, kBufDword3);
     int idx = az.i0 * (g.nt * g.ns) + as.i0 * st_s + at.i0 * st_t;
