@@ -1052,6 +1052,56 @@ ctr_status curved_filter_impl(ctr_ctx *ctx, const ctr_curved_geom *g, const floa
     return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "curved-detector filter");
 }
 
+// Shared body of the system-matrix entry points: per-ray counts, their
+// prefix sum into row_ptr, then (with col/val) the rows.
+ctr_status sysmat_impl(ctr_ctx *ctx, const RayParams &R, size_t nray, int64_t *row_ptr, int32_t *col, float *val,
+                       int64_t *nnz, unsigned flags)
+{
+    if (!nnz || !row_ptr) return fail(ctx, CTR_EINVAL, "row_ptr and nnz must not be NULL");
+    if ((col == nullptr) != (val == nullptr)) return fail(ctx, CTR_EINVAL, "col and val must both be given or both NULL");
+    if (nray > (size_t)std::numeric_limits<int>::max()) return fail(ctx, CTR_ETOOBIG, "more than 2^31-1 rays");
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const int n = (int)nray;
+    TRY(buf_reserve(ctx, ctx->ws[kWsG], nray * sizeof(long long)));
+    long long *counts = (long long *)ctx->ws[kWsG].ptr;
+    long long *rp = (long long *)row_ptr;
+    if (!dev) {
+        TRY(buf_reserve(ctx, ctx->ws[kWsS], (nray + 1) * sizeof(long long)));
+        rp = (long long *)ctx->ws[kWsS].ptr;
+    }
+    TRY(check_launch(ctx, launch_sysmat_count(R, counts, ctx->stream), "system-matrix row counts"));
+    HIP_TRY(ctx, hipMemsetAsync(rp, 0, sizeof(long long), ctx->stream));
+    size_t tmp_bytes = 0;
+    HIP_TRY(ctx, scan_counts(counts, rp + 1, n, nullptr, tmp_bytes, ctx->stream));
+    TRY(buf_reserve(ctx, ctx->ws[kWsD], tmp_bytes > 0 ? tmp_bytes : 1));
+    HIP_TRY(ctx, scan_counts(counts, rp + 1, n, ctx->ws[kWsD].ptr, tmp_bytes, ctx->stream));
+    long long total = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&total, rp + nray, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (col) {
+        if (*nnz < total)
+            return fail(ctx, CTR_EINVAL, "col/val capacity %lld is below the %lld entries", (long long)*nnz, total);
+        int *dc = (int *)col;
+        float *dv = val;
+        if (!dev) {
+            TRY(buf_reserve(ctx, ctx->ws[kWsIn], (size_t)(total > 0 ? total : 1) * sizeof(int)));
+            TRY(buf_reserve(ctx, ctx->ws[kWsOut], (size_t)(total > 0 ? total : 1) * sizeof(float)));
+            dc = (int *)ctx->ws[kWsIn].ptr;
+            dv = (float *)ctx->ws[kWsOut].ptr;
+        }
+        TRY(check_launch(ctx, launch_sysmat_fill(R, rp, dc, dv, ctx->stream), "system-matrix rows"));
+        if (!dev && total > 0) {
+            HIP_TRY(ctx, hipMemcpyAsync(col, dc, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(val, dv, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        }
+    }
+    if (!dev)
+        HIP_TRY(ctx, hipMemcpyAsync(row_ptr, rp, (nray + 1) * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *nnz = total;
+    return CTR_OK;
+}
+
 ctr_status check_equiangular(ctr_ctx *ctx, const ctr_fan_geom *g)
 {
     const double g0 = std::atan(g->p[0] / g->distance), d0 = std::atan(g->p[1] / g->distance) - g0;
@@ -1527,6 +1577,35 @@ ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *p
     TRY(check_launch(ctx, launch_curved_bp(P, filt, dv, ctx->stream), "curved-detector backprojection"));
     if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
     return finish_out(ctx, dev, dv, vol, nvoxc(g));
+}
+
+// ---- Siddon system matrix -----------------------------------------------------
+
+ctr_status ctr_cone_sysmat(ctr_ctx *ctx, const ctr_cone_geom *g, int64_t *row_ptr, int32_t *col, float *val,
+                           int64_t *nnz, unsigned flags)
+{
+    TRY(check_cone(g, 0));
+    TRY(need_ctx(ctx));
+    TRY(cone_tables(ctx, g));
+    return sysmat_impl(ctx, cone_ray_params(ctx, g), nray3(g), row_ptr, col, val, nnz, flags);
+}
+
+ctr_status ctr_fan_sysmat(ctr_ctx *ctx, const ctr_fan_geom *g, int64_t *row_ptr, int32_t *col, float *val,
+                          int64_t *nnz, unsigned flags)
+{
+    TRY(check_fan(g, 0));
+    TRY(need_ctx(ctx));
+    TRY(fan_tables(ctx, g));
+    return sysmat_impl(ctx, fan_ray_params(ctx, g), nray2(g), row_ptr, col, val, nnz, flags);
+}
+
+ctr_status ctr_par_sysmat(ctr_ctx *ctx, const ctr_par_geom *g, int64_t *row_ptr, int32_t *col, float *val,
+                          int64_t *nnz, unsigned flags)
+{
+    TRY(check_par(g, 0));
+    TRY(need_ctx(ctx));
+    TRY(par_tables(ctx, g));
+    return sysmat_impl(ctx, par_ray_params(ctx, g), nrayp(g), row_ptr, col, val, nnz, flags);
 }
 
 }  // extern "C"

@@ -146,6 +146,12 @@ struct FanParams {
 
 // ---- launchers (return hipError_t of the launch) --------------------------
 hipError_t launch_ray_fp(const RayParams &g, const float *vol, float *proj, hipStream_t st);
+// Siddon system matrix: counts[ray] = entries of the ray's row; then the rows
+// at row_ptr[ray] (column = voxel index, value = segment length).
+hipError_t launch_sysmat_count(const RayParams &g, long long *counts, hipStream_t st);
+hipError_t launch_sysmat_fill(const RayParams &g, const long long *row_ptr, int *col, float *val, hipStream_t st);
+// Inclusive prefix sum of n counts (hipCUB); tmp == nullptr queries tmp_bytes.
+hipError_t scan_counts(const long long *counts, long long *out, int n, void *tmp, size_t &tmp_bytes, hipStream_t st);
 // SIRT residual: r = (b - A x) / (rowlen + 1e-9); partials[block] = (sum e^2, sum |e|)
 hipError_t launch_ray_fp_residual(const RayParams &g, const float *vol, const float *b, float *r,
                                   double *partials, int *nblocks, hipStream_t st);

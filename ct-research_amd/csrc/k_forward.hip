@@ -77,10 +77,17 @@ __device__ __forceinline__ bool clip_axis(double s, double d, double b, double &
     return true;
 }
 
-// Line integral of one ray; also returns the chord length of the ray
-// through the volume box (the row sum A*1 of the Siddon system matrix).
-__device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const RayParams &g, int ib, int ip, int ix,
-                                           float &chord)
+// Walk of one ray through the volume: per-axis crossings after the entry
+// point, the exit parameter and the first voxel.
+struct RayWalk {
+    AxisSetup at, as, az;
+    double lam_end;   // chord length through the volume box
+    int idx0;         // first voxel (in the layout the strides describe)
+};
+
+// Set-up of ray (view ib, column ip, row ix); false on a miss.  With tr the
+// t and s strides are those of a [z][t][s] copy of the volume.
+__device__ __forceinline__ bool ray_walk(const RayParams &g, int ib, int ip, int ix, bool tr, RayWalk &w)
 {
     const double sb = g.view[4 * ib + 0], cb = g.view[4 * ib + 1];
     const double P = g.p[ip], X = g.xi[ix];
@@ -107,47 +114,115 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     bool hit = clip_axis(st, dx, g.bt, lo, hi);
     hit = clip_axis(ss, dy, g.bs, lo, hi) && hit;
     hit = clip_axis(g.sz, dz, g.bz, lo, hi) && hit;
-    chord = 0.0f;
-    if (!hit || !(hi > lo)) return 0.0f;
+    if (!hit || !(hi > lo)) return false;
+    const int st_t = tr ? g.ns : 1, st_s = tr ? 1 : g.nt;
+    w.at = setup_axis(st, dx, lo, hi, g.rt, g.nt, st_t);
+    w.as = setup_axis(ss, dy, lo, hi, g.rs, g.ns, st_s);
+    w.az = setup_axis(g.sz, dz, lo, hi, g.rz, g.nz, g.nt * g.ns);
+    w.lam_end = hi - lo;
+    w.idx0 = w.az.i0 * (g.nt * g.ns) + w.as.i0 * st_s + w.at.i0 * st_t;
+    return true;
+}
 
+// Line integral of one ray; also returns the chord length of the ray
+// through the volume box (the row sum A*1 of the Siddon system matrix).
+__device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const RayParams &g, int ib, int ip, int ix,
+                                           float &chord)
+{
     // views whose central ray runs mostly along t read the [z][t][s] copy, so
     // that the neighbouring rays of a wavefront (neighbouring p, offset in s)
     // gather from neighbouring addresses
-    const bool tr = g.volT != nullptr && fabs(sb) > fabs(cb);
-    const int st_t = tr ? g.ns : 1, st_s = tr ? 1 : g.nt;
-    const AxisSetup at = setup_axis(st, dx, lo, hi, g.rt, g.nt, st_t);
-    const AxisSetup as = setup_axis(ss, dy, lo, hi, g.rs, g.ns, st_s);
-    const AxisSetup az = setup_axis(g.sz, dz, lo, hi, g.rz, g.nz, g.nt * g.ns);
-    const double lam_end = hi - lo;
-    chord = (float)lam_end;
+    const bool tr = g.volT != nullptr && fabs(g.view[4 * ib + 0]) > fabs(g.view[4 * ib + 1]);
+    RayWalk w;
+    chord = 0.0f;
+    if (!ray_walk(g, ib, ip, ix, tr, w)) return 0.0f;
+    chord = (float)w.lam_end;
 
     // num_records is 32 bits: 2^30 voxels (4 GiB) saturate to 2^32 - 1, above every valid offset
     const size_t vbytes = (size_t)g.nt * g.ns * g.nz * sizeof(float);
     const unsigned nbytes = vbytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (unsigned)vbytes;
     __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)(tr ? g.volT : vol), 0, (int)nbytes, kBufDword3);
-    int idx = az.i0 * (g.nt * g.ns) + as.i0 * st_s + at.i0 * st_t;
+    int idx = w.idx0;
     const double inf = __builtin_huge_val();
-    double nt = at.n > 0 ? at.next0 : inf;
-    double ns = as.n > 0 ? as.next0 : inf;
-    double nz = az.n > 0 ? az.next0 : inf;
+    double nt = w.at.n > 0 ? w.at.next0 : inf;
+    double ns = w.as.n > 0 ? w.as.next0 : inf;
+    double nz = w.az.n > 0 ? w.az.next0 : inf;
     double lam = 0.0;
     float acc = 0.0f;
-    const int total = at.n + as.n + az.n + 1;
+    const int total = w.at.n + w.as.n + w.az.n + 1;
     for (int k = 0; k < total; ++k) {
-        const double m = fmin(fmin(nt, ns), fmin(nz, lam_end));
+        const double m = fmin(fmin(nt, ns), fmin(nz, w.lam_end));
         const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (unsigned)idx * 4u, 0, 0));
         acc = fmaf((float)(m - lam), v, acc);
         lam = m;
         // step the axis whose plane comes first (t before s before z on ties)
         const bool bt = nt <= ns && nt <= nz;
         const bool bs = !bt && ns <= nz;
-        idx += bt ? at.stride : (bs ? as.stride : az.stride);
-        nt = bt ? nt + at.step : nt;
-        ns = bs ? ns + as.step : ns;
-        nz = (!bt && !bs) ? nz + az.step : nz;
+        idx += bt ? w.at.stride : (bs ? w.as.stride : w.az.stride);
+        nt = bt ? nt + w.at.step : nt;
+        ns = bs ? ns + w.as.step : ns;
+        nz = (!bt && !bs) ? nz + w.az.step : nz;
     }
     return acc;
+}
+
+// Siddon system-matrix row of one ray: the voxels the walk above visits with
+// a segment of positive length, in visiting order (the rows GenSysMatFan.m /
+// GenSysMatCircular3D.m assemble; A x is exactly trace_ray).  kFill false
+// only counts them; kFill true writes at most `limit` entries (the count).
+template <bool kFill>
+__device__ __forceinline__ int sysmat_row(const RayParams &g, int ib, int ip, int ix, int *__restrict__ col,
+                                          float *__restrict__ val, long long limit)
+{
+    RayWalk w;
+    if (!ray_walk(g, ib, ip, ix, false, w)) return 0;
+    int idx = w.idx0, cnt = 0;
+    const double inf = __builtin_huge_val();
+    double nt = w.at.n > 0 ? w.at.next0 : inf;
+    double ns = w.as.n > 0 ? w.as.next0 : inf;
+    double nz = w.az.n > 0 ? w.az.next0 : inf;
+    double lam = 0.0;
+    const int total = w.at.n + w.as.n + w.az.n + 1;
+    for (int k = 0; k < total; ++k) {
+        const double m = fmin(fmin(nt, ns), fmin(nz, w.lam_end));
+        if (m > lam) {
+            if (kFill && cnt < limit) {
+                col[cnt] = idx;
+                val[cnt] = (float)(m - lam);
+            }
+            ++cnt;
+        }
+        lam = m;
+        const bool bt = nt <= ns && nt <= nz;
+        const bool bs = !bt && ns <= nz;
+        idx += bt ? w.at.stride : (bs ? w.as.stride : w.az.stride);
+        nt = bt ? nt + w.at.step : nt;
+        ns = bs ? ns + w.as.step : ns;
+        nz = (!bt && !bs) ? nz + w.az.step : nz;
+    }
+    return cnt;
+}
+
+__global__ __launch_bounds__(kBX *kBY) void k_sysmat_count(long long *__restrict__ counts, RayParams g)
+{
+    const int ip = blockIdx.x * kBX + threadIdx.x;
+    const int ix = blockIdx.y * kBY + threadIdx.y;
+    const int ib = blockIdx.z;
+    if (ip >= g.np || ix >= g.nx) return;
+    counts[((size_t)ib * g.nx + ix) * g.np + ip] = sysmat_row<false>(g, ib, ip, ix, nullptr, nullptr, 0);
+}
+
+__global__ __launch_bounds__(kBX *kBY) void k_sysmat_fill(const long long *__restrict__ row_ptr, int *__restrict__ col,
+                                                         float *__restrict__ val, RayParams g)
+{
+    const int ip = blockIdx.x * kBX + threadIdx.x;
+    const int ix = blockIdx.y * kBY + threadIdx.y;
+    const int ib = blockIdx.z;
+    if (ip >= g.np || ix >= g.nx) return;
+    const size_t ray = ((size_t)ib * g.nx + ix) * g.np + ip;
+    const long long o = row_ptr[ray];
+    sysmat_row<true>(g, ib, ip, ix, col + o, val + o, row_ptr[ray + 1] - o);
 }
 
 __global__ __launch_bounds__(kBX *kBY) void k_ray_fp(const float *__restrict__ vol, float *__restrict__ proj, RayParams g)
@@ -232,6 +307,20 @@ int ray_fp_blocks(const RayParams &g)
 {
     const dim3 grid = fp_grid(g);
     return (int)(grid.x * grid.y * grid.z);
+}
+
+hipError_t launch_sysmat_count(const RayParams &g, long long *counts, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sysmat_count, dim3((g.np + kBX - 1) / kBX, (g.nx + kBY - 1) / kBY, g.nb), dim3(kBX, kBY), 0,
+                       st, counts, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_sysmat_fill(const RayParams &g, const long long *row_ptr, int *col, float *val, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sysmat_fill, dim3((g.np + kBX - 1) / kBX, (g.nx + kBY - 1) / kBY, g.nb), dim3(kBX, kBY), 0,
+                       st, row_ptr, col, val, g);
+    return hipGetLastError();
 }
 
 hipError_t launch_ray_fp(const RayParams &g, const float *vol, float *proj, hipStream_t st)

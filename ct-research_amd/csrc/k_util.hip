@@ -2,6 +2,8 @@
 // deterministic two-level reductions of per-block partial sums).
 #include "ctr_internal.h"
 
+#include <hipcub/hipcub.hpp>
+
 namespace ctr {
 namespace {
 
@@ -171,6 +173,11 @@ hipError_t launch_sum_sq_sum(const float *x, size_t n, double *partials, int *nb
     *nblocks = g;
     hipLaunchKernelGGL(k_sum_sq_sum, dim3(g), dim3(256), 0, st, x, n, partials);
     return hipGetLastError();
+}
+
+hipError_t scan_counts(const long long *counts, long long *out, int n, void *tmp, size_t &tmp_bytes, hipStream_t st)
+{
+    return hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, out, n, st);
 }
 
 }  // namespace ctr

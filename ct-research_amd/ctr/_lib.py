@@ -28,6 +28,7 @@ EXPORTS = [
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
     "ctr_par_geom_check", "ctr_par_fp", "ctr_par_bp_adjoint", "ctr_par_filter", "ctr_par_fbp",
     "ctr_curved_geom_check", "ctr_curved_filter", "ctr_curved_bp", "ctr_curved_fdk",
+    "ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
@@ -119,6 +120,9 @@ def lib():
         L.ctr_fan_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_par_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_curved_geom_check.argtypes = [ctypes.c_void_p]
+        for name in ("ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_uint]
         for name in ("ctr_cone_fp", "ctr_cone_bp_ref", "ctr_cone_bp_adjoint", "ctr_fdk_bp",
                      "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_par_fp", "ctr_par_bp_adjoint",
                      "ctr_curved_filter", "ctr_curved_bp", "ctr_curved_fdk"):
@@ -404,6 +408,38 @@ class Context:
 
     def curved_fdk(self, g, proj, out=None, flags=0):
         return self._run(lib().ctr_curved_fdk, g, proj, g.nvox, out, flags)
+
+    # -- Siddon system matrix ----------------------------------------------------
+    def _sysmat(self, fn, g, device):
+        nnz = ctypes.c_int64(0)
+        if device:
+            import torch
+            self._bind_torch_stream()
+            row_ptr = torch.empty(g.nray + 1, dtype=torch.int64, device="cuda")
+            self._check(fn(self.h, ctypes.byref(g.c), row_ptr.data_ptr(), None, None, ctypes.byref(nnz),
+                           DEVICE_PTRS))
+            col = torch.empty(max(nnz.value, 1), dtype=torch.int32, device="cuda")
+            val = torch.empty(max(nnz.value, 1), dtype=torch.float32, device="cuda")
+            self._check(fn(self.h, ctypes.byref(g.c), row_ptr.data_ptr(), col.data_ptr(), val.data_ptr(),
+                           ctypes.byref(nnz), DEVICE_PTRS))
+            return row_ptr, col[:nnz.value], val[:nnz.value]
+        row_ptr = np.zeros(g.nray + 1, np.int64)
+        self._check(fn(self.h, ctypes.byref(g.c), row_ptr.ctypes.data, None, None, ctypes.byref(nnz), 0))
+        col = np.empty(max(nnz.value, 1), np.int32)
+        val = np.empty(max(nnz.value, 1), np.float32)
+        self._check(fn(self.h, ctypes.byref(g.c), row_ptr.ctypes.data, col.ctypes.data, val.ctypes.data,
+                       ctypes.byref(nnz), 0))
+        return row_ptr, col[:nnz.value], val[:nnz.value]
+
+    def cone_sysmat(self, g, device=False):
+        """CSR (row_ptr int64, col int32, val float32) of the Siddon system matrix."""
+        return self._sysmat(lib().ctr_cone_sysmat, g, device)
+
+    def fan_sysmat(self, g, device=False):
+        return self._sysmat(lib().ctr_fan_sysmat, g, device)
+
+    def par_sysmat(self, g, device=False):
+        return self._sysmat(lib().ctr_par_sysmat, g, device)
 
     # -- SIRT -------------------------------------------------------------------
     def _sirt(self, fn, g, b, n_iter, lam, lo, hi, x0, residuals, flags):

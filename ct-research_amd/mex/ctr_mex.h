@@ -169,4 +169,34 @@ inline mxArray *double_column(const std::vector<double> &v)
     return out;
 }
 
+// The system matrix of one geometry as a MATLAB sparse double matrix
+// (rays x voxels, as sparse(L, J, W, ...) of GenSysMatFan.m:85 builds it).
+// sysmat(row_ptr, col, val, &nnz, flags) is one of ctr_*_sysmat bound to its
+// context and geometry.  MATLAB stores compressed columns with ascending row
+// indices, so the CSR rows are transposed by a counting sort over columns.
+template <class Sysmat>
+inline mxArray *sparse_system_matrix(Sysmat sysmat, size_t n_rays, size_t n_vox, const char *op)
+{
+    std::vector<int64_t> row_ptr(n_rays + 1);
+    int64_t nnz = 0;
+    check(sysmat(row_ptr.data(), nullptr, nullptr, &nnz, 0u), op);
+    std::vector<int32_t> col((size_t)(nnz > 0 ? nnz : 1));
+    std::vector<float> val(col.size());
+    check(sysmat(row_ptr.data(), col.data(), val.data(), &nnz, 0u), op);
+    mxArray *out = mxCreateSparse(n_rays, n_vox, (size_t)nnz, mxREAL);
+    mwIndex *jc = mxGetJc(out), *ir = mxGetIr(out);
+    double *pr = mxGetPr(out);
+    for (size_t j = 0; j <= n_vox; ++j) jc[j] = 0;
+    for (int64_t k = 0; k < nnz; ++k) ++jc[(size_t)col[k] + 1];
+    for (size_t j = 0; j < n_vox; ++j) jc[j + 1] += jc[j];
+    std::vector<mwIndex> next(jc, jc + n_vox);
+    for (size_t r = 0; r < n_rays; ++r)
+        for (int64_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+            const mwIndex dst = next[(size_t)col[k]]++;
+            ir[dst] = (mwIndex)r;
+            pr[dst] = (double)val[k];
+        }
+    return out;
+}
+
 }  // namespace ctr_mex

@@ -38,6 +38,7 @@
 #define CTR_CTR_H_
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -256,6 +257,27 @@ ctr_status ctr_curved_bp(ctr_ctx *ctx, const ctr_curved_geom *g, const float *fi
 
 /* Filter + backprojection.  Replaces FDKUI() (ImplementUI.h:31-33). */
 ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *proj, float *vol, unsigned flags);
+
+/* ---- Siddon system matrix ------------------------------------------------
+ * The matrix A of the Siddon projector in CSR form: row r is ray r in the
+ * projection layout, column j is voxel j in the volume layout, the value is
+ * the ray's segment length in the voxel (mm); A x equals the forward
+ * projection.  Each row lists its voxels in the order the ray visits them,
+ * without repeats.  Replaces the MATLAB builders GenSysMatCircular3D.m:8-100,
+ * GenSysMatFan.m:6-85 and GenSysMatParal.m (the SysMatrix of SIRT.m:79,
+ * MLOSEM.m:30).
+ *
+ * row_ptr: [n_rays + 1] int64; col: int32 voxel indices; val: float.
+ * Call with col = val = NULL to fill row_ptr and *nnz.  Then call with col
+ * and val of capacity *nnz (in: the capacity, out: the entries written) to
+ * fill all three.  With CTR_DEVICE_PTRS the arrays are device memory; nnz is
+ * always a host pointer. */
+ctr_status ctr_cone_sysmat(ctr_ctx *ctx, const ctr_cone_geom *g, int64_t *row_ptr, int32_t *col, float *val,
+                           int64_t *nnz, unsigned flags);
+ctr_status ctr_fan_sysmat(ctr_ctx *ctx, const ctr_fan_geom *g, int64_t *row_ptr, int32_t *col, float *val,
+                          int64_t *nnz, unsigned flags);
+ctr_status ctr_par_sysmat(ctr_ctx *ctx, const ctr_par_geom *g, int64_t *row_ptr, int32_t *col, float *val,
+                          int64_t *nnz, unsigned flags);
 
 /* ---- SIRT -------------------------------------------------------------- */
 

@@ -1126,6 +1126,41 @@ void or_par_adjoint(const or_par *g, const float *proj, double *img)
     free(idx); free(len); free(work);
 }
 
+/* The exact Siddon segments of one ray (its system-matrix row), in visiting
+ * order; returns their number (at most nt + ns + nz + 8). */
+int or_cone_segments(const or_cone *g, long ray, long *idx, double *len)
+{
+    const int n[3] = {g->nt, g->ns, g->nz};
+    const double r[3] = {g->size[0] / g->nt, g->size[1] / g->ns, g->size[2] / g->nz};
+    double S[3], d[3], *work = malloc(sizeof(double) * (max_segments(n) + 4));
+    cone_ray(g, (int)(ray / ((long)g->nx * g->np)), (int)(ray % g->np), (int)((ray / g->np) % g->nx), S, d);
+    int m = siddon_exact(S, d, n, r, 3, idx, len, work);
+    free(work);
+    return m;
+}
+
+int or_fan_segments(const or_fan *g, long ray, long *idx, double *len)
+{
+    const int n[3] = {g->nt, g->ns, 1};
+    const double r[3] = {g->size[0] / g->nt, g->size[1] / g->ns, 1.0};
+    double S[3], d[3], *work = malloc(sizeof(double) * (max_segments(n) + 4));
+    fan_ray(g, (int)(ray / g->np), (int)(ray % g->np), S, d);
+    int m = siddon_exact(S, d, n, r, 2, idx, len, work);
+    free(work);
+    return m;
+}
+
+int or_par_segments(const or_par *g, long ray, long *idx, double *len)
+{
+    const int n[3] = {g->nt, g->ns, 1};
+    const double r[3] = {g->size[0] / g->nt, g->size[1] / g->ns, 1.0};
+    double S[3], d[3], *work = malloc(sizeof(double) * (max_segments(n) + 4));
+    par_ray(g, (int)(ray / g->np), (int)(ray % g->np), S, d);
+    int m = siddon_exact(S, d, n, r, 2, idx, len, work);
+    free(work);
+    return m;
+}
+
 /* Long-double brute force of one parallel ray: clip against every pixel box. */
 double or_par_ray_brute(const or_par *g, const float *img, int b, int it)
 {

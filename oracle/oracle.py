@@ -265,6 +265,31 @@ def par_fbp_bp(g, filtered):
     return out
 
 
+def _segments(name, g, ray, cap):
+    fn = getattr(lib(), name)
+    fn.restype = ctypes.c_int
+    idx = np.zeros(cap, np.int64)
+    ln = np.zeros(cap)
+    m = fn(ctypes.byref(g.s), ctypes.c_long(ray), _ptr(idx), _ptr(ln))
+    return idx[:m], ln[:m]
+
+
+def cone_segments(g, ray):
+    """Exact Siddon segments (voxel index, length) of one ray, in visiting order."""
+    s = g.s
+    return _segments("or_cone_segments", g, ray, s.nt + s.ns + s.nz + 8)
+
+
+def fan_segments(g, ray):
+    s = g.s
+    return _segments("or_fan_segments", g, ray, s.nt + s.ns + 9)
+
+
+def par_segments(g, ray):
+    s = g.s
+    return _segments("or_par_segments", g, ray, s.nt + s.ns + 9)
+
+
 def parker(beta, start, gamma, fan):
     fn = lib().or_parker
     fn.restype = ctypes.c_double

@@ -11,7 +11,9 @@
 struct mxArray_tag {
     mxClassID cls;
     size_t m, n;
-    std::vector<unsigned char> data;
+    std::vector<unsigned char> data;   // dense: m*n elements; sparse: nzmax doubles
+    bool sparse = false;
+    std::vector<mwIndex> ir, jc;       // sparse: row indices [nzmax], column starts [n+1]
 };
 
 namespace {
@@ -29,6 +31,9 @@ void *mxGetData(const mxArray *a) { return (void *)a->data.data(); }
 size_t mxGetM(const mxArray *a) { return a->m; }
 size_t mxGetN(const mxArray *a) { return a->n; }
 size_t mxGetNumberOfElements(const mxArray *a) { return a->m * a->n; }
+mwIndex *mxGetIr(const mxArray *a) { return const_cast<mwIndex *>(a->ir.data()); }
+mwIndex *mxGetJc(const mxArray *a) { return const_cast<mwIndex *>(a->jc.data()); }
+int mxIsSparse(const mxArray *a) { return a->sparse ? 1 : 0; }
 int mxIsSingle(const mxArray *a) { return a->cls == mxSINGLE_CLASS; }
 int mxIsDouble(const mxArray *a) { return a->cls == mxDOUBLE_CLASS; }
 int mxIsComplex(const mxArray *) { return 0; }
@@ -49,6 +54,17 @@ mxArray *mxCreateDoubleMatrix(size_t m, size_t n, mxComplexity)
 {
     mxArray *a = new mxArray_tag{mxDOUBLE_CLASS, m, n, {}};
     a->data.assign(m * n * 8, 0);
+    return a;
+}
+
+mxArray *mxCreateSparse(size_t m, size_t n, size_t nzmax, mxComplexity)
+{
+    if (nzmax == 0) nzmax = 1;   // as MATLAB does
+    mxArray *a = new mxArray_tag{mxDOUBLE_CLASS, m, n, {}};
+    a->sparse = true;
+    a->data.assign(nzmax * 8, 0);
+    a->ir.assign(nzmax, 0);
+    a->jc.assign(n + 1, 0);
     return a;
 }
 
@@ -91,6 +107,9 @@ const void *fm_data(const mxArray *a) { return a->data.data(); }
 size_t fm_m(const mxArray *a) { return a->m; }
 size_t fm_n(const mxArray *a) { return a->n; }
 int fm_class(const mxArray *a) { return (int)a->cls; }
+int fm_is_sparse(const mxArray *a) { return a->sparse ? 1 : 0; }
+const mwIndex *fm_ir(const mxArray *a) { return a->ir.data(); }
+const mwIndex *fm_jc(const mxArray *a) { return a->jc.data(); }
 
 typedef void (*mexfn_t)(int, mxArray **, int, const mxArray **);
 

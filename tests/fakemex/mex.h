@@ -9,6 +9,7 @@
 typedef enum { mxUNKNOWN_CLASS = 0, mxDOUBLE_CLASS = 6, mxSINGLE_CLASS = 7, mxINT32_CLASS = 12 } mxClassID;
 typedef enum { mxREAL = 0, mxCOMPLEX = 1 } mxComplexity;
 typedef struct mxArray_tag mxArray;
+typedef size_t mwIndex;   /* 64-bit MATLAB: sparse row indices and column starts */
 
 #ifdef __cplusplus
 extern "C" {
@@ -24,6 +25,11 @@ int mxIsDouble(const mxArray *a);
 int mxIsComplex(const mxArray *a);
 mxClassID mxGetClassID(const mxArray *a);
 mxArray *mxCreateDoubleMatrix(size_t m, size_t n, mxComplexity c);
+/* sparse (compressed column): pr/ir hold nzmax entries, jc holds n + 1 column starts */
+mxArray *mxCreateSparse(size_t m, size_t n, size_t nzmax, mxComplexity c);
+mwIndex *mxGetIr(const mxArray *a);
+mwIndex *mxGetJc(const mxArray *a);
+int mxIsSparse(const mxArray *a);
 void mexErrMsgIdAndTxt(const char *id, const char *fmt, ...);
 int mexPrintf(const char *fmt, ...);
 int mexAtExit(void (*fn)(void));

@@ -28,6 +28,9 @@ class Mex:
             f.fm_data.argtypes = [ctypes.c_void_p]
             f.fm_m.restype = f.fm_n.restype = ctypes.c_size_t
             f.fm_m.argtypes = f.fm_n.argtypes = [ctypes.c_void_p]
+            f.fm_is_sparse.argtypes = [ctypes.c_void_p]
+            f.fm_ir.restype = f.fm_jc.restype = ctypes.c_void_p
+            f.fm_ir.argtypes = f.fm_jc.argtypes = [ctypes.c_void_p]
             f.fm_call.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                   ctypes.c_char_p, ctypes.c_char_p]
         self.lib = ctypes.CDLL(os.path.join(FM, f"mex_{name}.so"))
@@ -56,8 +59,16 @@ class Mex:
         outs = []
         for k in range(nlhs):
             a = arr_out[k]
-            n = Mex._fm.fm_m(a) * Mex._fm.fm_n(a)
-            outs.append(np.ctypeslib.as_array((ctypes.c_double * n).from_address(Mex._fm.fm_data(a))).copy())
+            m, n = Mex._fm.fm_m(a), Mex._fm.fm_n(a)
+            if Mex._fm.fm_is_sparse(a):
+                import scipy.sparse as sp
+                jc = np.ctypeslib.as_array((ctypes.c_size_t * (n + 1)).from_address(Mex._fm.fm_jc(a))).copy()
+                nz = int(jc[-1])
+                ir = np.ctypeslib.as_array((ctypes.c_size_t * max(nz, 1)).from_address(Mex._fm.fm_ir(a)))[:nz].copy()
+                pr = np.ctypeslib.as_array((ctypes.c_double * max(nz, 1)).from_address(Mex._fm.fm_data(a)))[:nz].copy()
+                outs.append(sp.csc_matrix((pr, ir, jc), shape=(m, n)))
+            else:
+                outs.append(np.ctypeslib.as_array((ctypes.c_double * (m * n)).from_address(Mex._fm.fm_data(a))).copy())
             Mex._fm.fm_destroy(a)
         return outs[0] if nlhs == 1 else outs
 
@@ -156,6 +167,19 @@ def test_implementui_gateway_argument_errors():
     assert e.value.ident == "ctr:ImplementUI:ETOOBIG"
 
 
+def test_sysmat_gateway_argument_errors():
+    with pytest.raises(MexError) as e:       # rotation centre off the image centre
+        Mex("GenSysMatFan")(32, 32, [32.0, 32.0], np.arange(10.0), np.arange(8.0) - 3.5, 100.0, 16.5, 16.0)
+    assert e.value.ident == "ctr:GenSysMatFan:centre"
+    with pytest.raises(MexError) as e:
+        Mex("GenSysMatParal")(32, 32, [32.0, 32.0], 16.0, 16.0, np.arange(10.0))
+    assert e.value.ident == "ctr:GenSysMatParal:nrhs"
+    with pytest.raises(MexError) as e:
+        Mex("GenSysMatCircular3D")([8, 8, 4.5], 4.0, 4.0, 2.25, np.arange(8.0), np.arange(4.0), np.arange(10.0),
+                                   60.0, [1.0, 1.0, 1.0])
+    assert e.value.ident == "ctr:GenSysMatCircular3D:value"
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_gateway_reports_missing_device():
     nt, ns, nz, size, beta, p, xi, D = cone_args()
@@ -245,3 +269,37 @@ def test_implementui_gateway(ctx):
     assert errs(out, ctx.curved_fdk(g, R.astype(np.float32)))[1] == 0.0
     q = Mex("ImplementUI")(R, xi, gamma, beta, 0.0, 0.126, 100.0, 200.0, [32.0, 32.0, 8.0], 8, 1.0)
     assert errs(q, ctx.curved_fdk(g, R.astype(np.float32), flags=ctr.REF_QUIRKS))[1] == 0.0
+
+
+@pytest.mark.gpu
+def test_sysmat_gateways(ctx):
+    """The builders return MATLAB sparse matrices equal to ctr_*_sysmat, whose
+    products are the projectors' outputs."""
+    import ctr
+    import scipy.sparse as sp
+    height, width, size = 36, 40, [40.0, 40.0]
+    res = 40.0 / height
+    beta = (2 * np.pi / 24) * np.arange(1, 25)
+    p = 0.9 * (np.arange(60) - 29.5)
+    A = Mex("GenSysMatFan")(height, width, size, beta, p, 120.0, width * res / 2, height * res / 2)
+    g = ctr.FanGeom(width, height, (width * res, height * res), 120.0, beta, p)
+    rp, col, val = ctx.fan_sysmat(g)
+    B = sp.csr_matrix((val.astype(np.float64), col, rp), shape=A.shape)
+    assert A.shape == (24 * 60, height * width) and abs(A - B).max() == 0.0
+    x = np.random.default_rng(27).random(height * width).astype(np.float32)
+    assert errs(A @ x.astype(np.float64), ctx.fan_fp(g, x))[0] < 1e-6
+
+    theta = np.deg2rad(np.arange(1, 181, 6.0))
+    t = 0.8 * (np.arange(70) - 34.5)
+    A = Mex("GenSysMatParal")(height, width, size, width * res / 2, height * res / 2, theta, t)
+    gp = ctr.ParGeom(width, height, (width * res, height * res), theta, t)
+    assert errs(A @ x.astype(np.float64), ctx.par_fp(gp, x))[0] < 1e-6
+
+    b_deg = np.arange(0.0, 360.0, 20.0)
+    pc = (0.8 * (np.arange(24) - 11.5)).astype(np.float32)
+    xc = (0.8 * (np.arange(12) - 5.5)).astype(np.float32)
+    A = Mex("GenSysMatCircular3D")([16, 14, 6], 8.0, 7.7, 2.7, pc, xc, b_deg, 60.0, [1.0, 1.1, 0.9])
+    gc = ctr.ConeGeom(16, 14, 6, (16.0, 15.4, 5.4), 60.0, np.deg2rad(b_deg).astype(np.float32), pc, xc)
+    v = np.random.default_rng(28).random(16 * 14 * 6).astype(np.float32)
+    assert A.shape == (len(b_deg) * 12 * 24, 16 * 14 * 6)
+    assert errs(A @ v.astype(np.float64), ctx.cone_fp(gc, v))[0] < 1e-6

@@ -448,3 +448,14 @@ def test_curved_fdk_recovers_ball(oracle, start):
     for k in (3, 4):
         assert abs(vol[k][inside].mean() - 1.0) < 0.02
         assert abs(vol[k][outside].mean()) < 0.02
+
+
+def test_oracle_row_segments_sum_to_projection(oracle):
+    """The per-ray segment lists are the rows behind the exact projectors."""
+    g = small_par(oracle)
+    img = np.random.default_rng(23).uniform(0, 1, g.nvox).astype(np.float32)
+    fp = oracle.par_fp_exact(g, img)
+    for ray in range(g.nray):
+        idx, ln = oracle.par_segments(g, ray)
+        assert len(np.unique(idx)) == len(idx) and np.all(ln > 0)
+        assert abs(float(np.dot(ln, img[idx])) - fp[ray]) < 1e-12
