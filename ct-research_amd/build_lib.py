@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libctr.so")
 
 SOURCES = ["ctr_api.cpp", "k_forward.hip", "k_backproject.hip", "k_filter.hip", "k_fan.hip", "k_util.hip", "k_cnn.hip",
-           "k_curved.hip"]
+           "k_curved.hip", "k_fdkerr.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

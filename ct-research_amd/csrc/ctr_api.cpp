@@ -67,6 +67,10 @@ struct CurvedTables {
     DevBuf d_view, d_rowp, d_beta, d_gam, d_spec;
 };
 
+struct FdkErrTables {
+    DevBuf d_cos, d_sin, d_u, d_t, d_p, d_g, d_cp;
+};
+
 struct FootCache {
     DevBuf buf;
     uint64_t key = 0;
@@ -103,6 +107,7 @@ struct ctr_ctx {
     FanTables fan;
     ParTables par;
     CurvedTables curved;
+    FdkErrTables fdkerr;
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
@@ -1606,6 +1611,98 @@ ctr_status ctr_par_sysmat(ctr_ctx *ctx, const ctr_par_geom *g, int64_t *row_ptr,
     TRY(need_ctx(ctx));
     TRY(par_tables(ctx, g));
     return sysmat_impl(ctx, par_ray_params(ctx, g), nrayp(g), row_ptr, col, val, nnz, flags);
+}
+
+// ---- iterative FDK ------------------------------------------------------------
+
+ctr_status ctr_fdk_error_slice(ctr_ctx *ctx, const ctr_fdkerr_geom *a, const float *vol, float *err, unsigned flags)
+{
+    if (!a) return fail(ctx, CTR_EINVAL, "arguments are NULL");
+    if (a->nt < 1 || a->ns < 1 || a->nz < 1) return fail(ctx, CTR_EINVAL, "volume dimensions must be positive");
+    if ((double)a->nt * a->ns * a->nz > (double)(1 << 30)) return fail(ctx, CTR_ETOOBIG, "more than 2^30 voxels");
+    for (int k = 0; k < 3; ++k)
+        if (!finite_pos(a->size[k])) return fail(ctx, CTR_EINVAL, "size[%d] must be positive and finite", k);
+    if (!finite_pos(a->distance)) return fail(ctx, CTR_EINVAL, "distance must be positive and finite");
+    if (a->n_beta < 2 || a->n_u < 2 || a->n_p < 2 || a->n_g < 1)
+        return fail(ctx, CTR_EINVAL, "need at least 2 views, 2 rows U, 2 samples P and 1 tap");
+    if (a->t < 0 || a->t >= a->nt) return fail(ctx, CTR_EINVAL, "slice t = %d is outside [0, %d)", a->t, a->nt);
+    if ((double)a->n_beta * a->n_u * a->n_p > (double)(1 << 30) || a->n_beta > 65535 || a->nz > 65535)
+        return fail(ctx, CTR_ETOOBIG, "n_beta * n_u * n_p above 2^30, or more than 65535 views or slices");
+    const double half = 0.5 * std::sqrt(a->size[0] * a->size[0] + a->size[1] * a->size[1]);
+    if (!(a->distance > half * (1.0 + 1e-6)))
+        return fail(ctx, CTR_EINVAL, "source orbit (distance %g) intersects the volume (half diagonal %g)", a->distance,
+                    half);
+    if (!a->beta || !a->u || !a->p || !a->g) return fail(ctx, CTR_EINVAL, "an array argument is NULL");
+    for (int i = 0; i < a->n_beta; ++i)
+        if (!std::isfinite(a->beta[i])) return fail(ctx, CTR_EINVAL, "beta[%d] is not finite", i);
+    for (int i = 0; i < a->n_u; ++i)
+        if (!std::isfinite(a->u[i])) return fail(ctx, CTR_EINVAL, "u[%d] is not finite", i);
+    for (int i = 0; i < a->n_p; ++i)
+        if (!std::isfinite(a->p[i])) return fail(ctx, CTR_EINVAL, "p[%d] is not finite", i);
+    for (int i = 0; i < a->n_g; ++i)
+        if (!std::isfinite(a->g[i])) return fail(ctx, CTR_EINVAL, "g[%d] is not finite", i);
+    const float dbeta = std::fabs(a->beta[1] - a->beta[0]), du = std::fabs(a->u[1] - a->u[0]);
+    const double pint = std::fabs(a->p[1] - a->p[0]);
+    if (!(pint > 0.0)) return fail(ctx, CTR_EINVAL, "p[1] - p[0] must not be 0");
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const int nb = a->n_beta, nu = a->n_u, np = a->n_p;
+    const double D = a->distance;
+    // host tables, rounded as the reference's float intermediates are
+    std::vector<float> cs(nb), sn(nb), tt((size_t)nu * np);
+    for (int b = 0; b < nb; ++b) {
+        cs[b] = (float)std::cos((double)a->beta[b]);
+        sn[b] = (float)std::sin((double)a->beta[b]);
+    }
+    for (int iu = 0; iu < nu; ++iu)
+        for (int ip = 0; ip < np; ++ip) tt[(size_t)iu * np + ip] = (float)(a->p[ip] * (D - (double)a->u[iu]) / D);
+    std::vector<double> cp(np);
+    for (int ip = 0; ip < np; ++ip) {
+        const double P = a->p[ip];
+        cp[ip] = std::sqrt(1 + (P / D) * (P / D)) * D / std::sqrt(D * D + P * P) * (double)du * pint * (double)dbeta;
+    }
+    FdkErrTables &T = ctx->fdkerr;
+    TRY(upload(ctx, T.d_cos, cs.data(), sizeof(float) * nb));
+    TRY(upload(ctx, T.d_sin, sn.data(), sizeof(float) * nb));
+    TRY(upload(ctx, T.d_u, a->u, sizeof(float) * nu));
+    TRY(upload(ctx, T.d_t, tt.data(), sizeof(float) * tt.size()));
+    TRY(upload(ctx, T.d_p, a->p, sizeof(double) * np));
+    TRY(upload(ctx, T.d_g, a->g, sizeof(double) * a->n_g));
+    TRY(upload(ctx, T.d_cp, cp.data(), sizeof(double) * np));
+    const size_t nvox = (size_t)a->nt * a->ns * a->nz, nout = (size_t)nb * a->nz * a->ns;
+    const float *dv;
+    float *de;
+    TRY(stage_in(ctx, dev, kWsIn, vol, nvox, &dv));
+    TRY(stage_out(ctx, dev, false, kWsOut, err, nout, &de));
+    TRY(buf_reserve(ctx, ctx->ws[kWsG], (size_t)nb * nu * np * sizeof(int)));
+    FdkErrParams P{};
+    P.nt = a->nt;
+    P.ns = a->ns;
+    P.nz = a->nz;
+    P.nb = nb;
+    P.nu = nu;
+    P.np = np;
+    P.ng = a->n_g;
+    P.t = a->t;
+    P.r0 = a->size[0] / a->nt;
+    P.r1 = a->size[1] / a->ns;
+    P.r2 = a->size[2] / a->nz;
+    P.ct = a->size[0] / 2;
+    P.cs = a->size[1] / 2;
+    P.cz = a->size[2] / 2;
+    P.D = D;
+    P.pint = pint;
+    P.cosb = (const float *)T.d_cos.ptr;
+    P.sinb = (const float *)T.d_sin.ptr;
+    P.u = (const float *)T.d_u.ptr;
+    P.tab_t = (const float *)T.d_t.ptr;
+    P.p = (const double *)T.d_p.ptr;
+    P.g = (const double *)T.d_g.ptr;
+    P.cp = (const double *)T.d_cp.ptr;
+    TRY(check_launch(ctx, launch_fdk_error_slice(P, dv, (int *)ctx->ws[kWsG].ptr, de, ctx->stream),
+                     "iterative-FDK error slice"));
+    if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the tables are rewritten by the next call
+    return finish_out(ctx, dev, de, err, nout);
 }
 
 }  // extern "C"

@@ -233,6 +233,20 @@ hipError_t launch_par_fbp_bp(const ParBpParams &P, const float *filt, float *img
 hipError_t launch_par_adjoint(const ParBpParams &P, const float *proj, float *img, const SirtUpdate *U,
                               int ones_input, hipStream_t st);
 
+// Iterative-FDK error slice (IterativeFDKerrCompute.cu).
+struct FdkErrParams {
+    int nt, ns, nz, nb, nu, np, ng, t;
+    double r0, r1, r2, ct, cs, cz, D, pint;
+    const float *cosb, *sinb;   // [nb] float(cos(double beta)), float(sin(...))
+    const float *u;             // [nu] integration rows U
+    const float *tab_t;         // [nu][np] float(P (D - U) / D)
+    const double *p;            // [np] detector samples P
+    const double *g;            // [ng] ramp taps
+    const double *cp;           // [np] sqrt(1 + (P/D)^2) D / sqrt(D^2 + P^2) dU PInt dbeta
+};
+// xy: [nb][nu][np] int scratch for the voxel map
+hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *xy, float *out, hipStream_t st);
+
 // Utilities.
 hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st);
 hipError_t launch_transpose_st(const float *in, float *out, int nt, int ns, int nz, hipStream_t st);   // [z][t][s] <- [z][s][t]

@@ -1,0 +1,104 @@
+// k_fdkerr.hip - error slice of the iterative FDK correction for gfx950
+// (My_novel_algorithm/IterativeFDK/IterativeFDKerrCompute.cu:12-75).
+//
+// Output [beta][z][s] for one voxel slice t: for every view, integration row
+// U and detector sample P, the difference between the voxel reached at
+// (P (D-U)/D, U) in the rotated frame (its z moved along the ray) and the
+// voxel of the same (t, s) in slice z, times the ramp tap at the voxel's
+// projected offset.  The reference launches one kernel per (view, U) with
+// float accumulation; here one launch covers everything and each output
+// accumulates in FP64 registers.
+//
+// Every voxel index is a floor of float/double intermediates, so those keep
+// the reference's types and this file is compiled without FMA contraction:
+// the indices then depend only on IEEE-rounded operations.  The (view, U, P)
+// -> (x, y) voxel map does not depend on the output element and comes from a
+// table built by k_fdkerr_xy; the tap indices of each output are formed into
+// LDS once per chunk of 256 detector samples.
+#include "ctr_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ctr {
+namespace {
+
+constexpr int kFeThreads = 64;
+constexpr int kFeChunk = 256;   // detector samples per LDS tap-index table (64 KiB)
+
+__global__ __launch_bounds__(256) void k_fdkerr_xy(FdkErrParams P, int *__restrict__ xy)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t n = (size_t)P.nb * P.nu * P.np;
+    if (i >= n) return;
+    const int ip = (int)(i % P.np), iu = (int)((i / P.np) % P.nu), b = (int)(i / ((size_t)P.np * P.nu));
+    const float cb = P.cosb[b], sb = P.sinb[b];
+    const float U = P.u[iu], T = P.tab_t[(size_t)iu * P.np + ip];
+    const float tcb = T * cb, usb = U * sb, tsb = T * sb, ucb = U * cb;
+    const float X = tcb - usb, Y = tsb + ucb;
+    const double fx = floor(((double)X + P.ct) / P.r0), fy = floor(((double)Y + P.cs) / P.r1);
+    xy[i] = (fx >= 0.0 && fx < (double)P.nt && fy >= 0.0 && fy < (double)P.ns) ? (int)fy * P.nt + (int)fx : -1;
+}
+
+__global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const float *__restrict__ pic,
+                                                       const int *__restrict__ xy, float *__restrict__ out)
+{
+    __shared__ int fil[kFeChunk * kFeThreads];   // tap index of (P, thread) for a chunk of P, -1 outside the taps
+    const int s = blockIdx.x * kFeThreads + threadIdx.x;
+    const int z = blockIdx.y, b = blockIdx.z;
+    const float cb = P.cosb[b], sb = P.sinb[b];
+    const float image_t = (float)((P.t + 0.5) * P.r0 - P.ct);
+    const float image_s = (float)((s + 0.5) * P.r1 - P.cs);
+    const float image_z = (float)((z + 0.5) * P.r2 - P.cz);
+    const float ts = image_t * sb, tc = image_t * cb, ss = image_s * sb, sc = image_s * cb;
+    const float dect_t = tc + ss, dect_s = sc - ts, dect_z = image_z;
+    const double D = P.D;
+    const float LR = (float)(D / (D - (double)dect_s));
+    const float dtl = dect_t * LR;
+    const size_t plane = (size_t)P.nt * P.ns;
+    const float *slice = pic + (size_t)z * plane;
+    const double half_lr2 = 0.5 * ((double)LR * LR);
+    double acc = 0.0;
+    // each thread reads only its own column of the table: no barrier needed
+    for (int p0 = 0; p0 < P.np; p0 += kFeChunk) {
+        const int pn = min(kFeChunk, P.np - p0);
+        for (int k = 0; k < pn; ++k) {
+            const double ff = floor(((double)dtl - P.p[p0 + k]) / P.pint) + P.np;
+            fil[k * kFeThreads + threadIdx.x] = (ff >= 0.0 && ff < (double)P.ng) ? (int)ff : -1;
+        }
+        if (s >= P.ns) continue;
+        for (int iu = 0; iu < P.nu; ++iu) {
+            const float U = P.u[iu];
+            const float num = (dect_s - U) * dect_z;
+            const float Zd = (float)((double)dect_z + (double)num / (D - (double)dect_s));
+            const double fz = floor(((double)Zd + P.cz) / P.r2);
+            if (!(fz >= 0.0 && fz < (double)P.nz)) continue;
+            const float *moved = pic + (size_t)fz * plane;
+            const int *row = xy + ((size_t)b * P.nu + iu) * P.np + p0;
+            const double *cp = P.cp + p0;
+            for (int k = 0; k < pn; ++k) {
+                const int f = fil[k * kFeThreads + threadIdx.x];
+                const int o = row[k];
+                if (f >= 0 && o >= 0) {
+                    const float diff = moved[o] - slice[o];
+                    acc += (double)diff * half_lr2 * (P.g[f] * cp[k]);
+                }
+            }
+        }
+    }
+    if (s < P.ns) out[((size_t)b * P.nz + z) * P.ns + s] = (float)acc;
+}
+
+}  // namespace
+
+hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *xy, float *out, hipStream_t st)
+{
+    const size_t n = (size_t)P.nb * P.nu * P.np;
+    hipLaunchKernelGGL(k_fdkerr_xy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, xy);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid((P.ns + kFeThreads - 1) / kFeThreads, P.nz, P.nb);
+    hipLaunchKernelGGL(k_fdkerr, grid, dim3(kFeThreads), 0, st, P, pic, xy, out);
+    return hipGetLastError();
+}
+
+}  // namespace ctr

@@ -28,7 +28,7 @@ EXPORTS = [
     "ctr_fan_fp", "ctr_fan_bp_ref", "ctr_fan_bp_adjoint", "ctr_fan_filter", "ctr_fan_fbp",
     "ctr_par_geom_check", "ctr_par_fp", "ctr_par_bp_adjoint", "ctr_par_filter", "ctr_par_fbp",
     "ctr_curved_geom_check", "ctr_curved_filter", "ctr_curved_bp", "ctr_curved_fdk",
-    "ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat",
+    "ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat", "ctr_fdk_error_slice",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
@@ -70,6 +70,13 @@ class CurvedGeomC(ctypes.Structure):
                 ("n_beta", ctypes.c_int), ("n_gamma", ctypes.c_int), ("n_xi", ctypes.c_int),
                 ("beta", ctypes.c_void_p), ("gamma", ctypes.c_void_p), ("xi", ctypes.c_void_p),
                 ("beta_start", ctypes.c_double), ("fan_angle", ctypes.c_double)]
+
+
+class FdkErrGeomC(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("nz", ctypes.c_int), ("size", ctypes.c_double * 3),
+                ("distance", ctypes.c_double), ("n_beta", ctypes.c_int), ("n_u", ctypes.c_int),
+                ("n_p", ctypes.c_int), ("n_g", ctypes.c_int), ("beta", ctypes.c_void_p), ("u", ctypes.c_void_p),
+                ("p", ctypes.c_void_p), ("g", ctypes.c_void_p), ("t", ctypes.c_int)]
 
 
 class SirtOptsC(ctypes.Structure):
@@ -120,6 +127,8 @@ def lib():
         L.ctr_fan_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_par_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_curved_geom_check.argtypes = [ctypes.c_void_p]
+        L.ctr_fdk_error_slice.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_uint]
         for name in ("ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_uint]
@@ -280,6 +289,31 @@ class CurvedGeom:
         return lib().ctr_curved_geom_check(ctypes.byref(self.c))
 
 
+class FdkErrGeom:
+    """Arguments of the iterative-FDK error slice (IterativeFDKerrCompute.cpp:5-28):
+    volume nt x ns x nz over size, views beta and rows u (float32), detector
+    samples p and ramp taps g (float64), slice t (0-based)."""
+
+    def __init__(self, nt, ns, nz, size, distance, beta, u, p, g, t):
+        self.nt, self.ns, self.nz = int(nt), int(ns), int(nz)
+        self.size = tuple(float(v) for v in size)
+        self.distance = float(distance)
+        self.beta, self.u = _f32(beta).copy(), _f32(u).copy()
+        self.p, self.g = _f64(p), _f64(g)
+        self.t = int(t)
+        self.c = FdkErrGeomC(self.nt, self.ns, self.nz, (ctypes.c_double * 3)(*self.size), self.distance,
+                             len(self.beta), len(self.u), len(self.p), len(self.g), self.beta.ctypes.data,
+                             self.u.ctypes.data, self.p.ctypes.data, self.g.ctypes.data, self.t)
+
+    @property
+    def nvox(self):
+        return self.nt * self.ns * self.nz
+
+    @property
+    def nout(self):
+        return len(self.beta) * self.nz * self.ns
+
+
 def _is_device(x):
     return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
 
@@ -408,6 +442,10 @@ class Context:
 
     def curved_fdk(self, g, proj, out=None, flags=0):
         return self._run(lib().ctr_curved_fdk, g, proj, g.nvox, out, flags)
+
+    # -- iterative FDK -------------------------------------------------------------
+    def fdk_error_slice(self, a, vol, out=None, flags=0):
+        return self._run(lib().ctr_fdk_error_slice, a, vol, a.nout, out, flags)
 
     # -- Siddon system matrix ----------------------------------------------------
     def _sysmat(self, fn, g, device):

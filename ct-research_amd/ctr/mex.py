@@ -11,7 +11,7 @@ quirks listed in the flags of ctr.h are off unless ref_quirks=True.
 import numpy as np
 
 from . import _lib
-from ._lib import ConeGeom, CurvedGeom, FanGeom, ParGeom, default_context
+from ._lib import ConeGeom, CurvedGeom, FanGeom, FdkErrGeom, ParGeom, default_context
 
 
 def _col(a):
@@ -115,6 +115,17 @@ def ImplementUI(R, Xigamadomain, Gamadomain, BetaScanRange, betaStartAngle, MaxG
                    float(Distance_s2d), BetaScanRange, Gamadomain, Xigamadomain, float(betaStartAngle),
                    float(MaxGama))
     return ctx.curved_fdk(g, R, flags=_flags(ref_quirks)).astype(np.float64)
+
+
+def IterativeFDKerrCompute(PriorImage, t_length, s_length, z_length, Size, BetaScanRange, Udomain, Pdomain, Distance,
+                           G, t, ctx=None):
+    """ErrorSlice = IterativeFDKerrCompute(PriorImage, t_length, s_length, z_length, Size, BetaScanRange,
+    Udomain, Pdomain, Distance, G, t) (My_novel_algorithm/IterativeFDK/IterativeFDKerrCompute.cpp:3-56);
+    t is 1-based as in MATLAB."""
+    ctx = ctx or default_context()
+    a = FdkErrGeom(t_length, s_length, z_length, _col(Size)[:3], float(Distance), _col(BetaScanRange),
+                   _col(Udomain), _col(Pdomain), _col(G), int(t) - 1)
+    return ctx.fdk_error_slice(a, PriorImage).astype(np.float64)
 
 
 def SIRT3D(R, Xigamadomain, Pdomain, BetaScanRange, Distance, Size, Nt, Ns, Nz, nIter, Lamda,

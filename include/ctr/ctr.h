@@ -119,6 +119,20 @@ typedef struct ctr_curved_geom {
     double fan_angle;       /* full fan angle of the Parker window (rad) */
 } ctr_curved_geom;
 
+/* Arguments of the iterative-FDK error slice (the argument set of
+ * IterativeFDKerrCompute: My_novel_algorithm/IterativeFDK/IterativeFDKerrCompute.cpp:5-28). */
+typedef struct ctr_fdkerr_geom {
+    int nt, ns, nz;         /* voxels along t, s, z */
+    double size[3];         /* physical extent of the volume (mm) */
+    double distance;        /* source-to-isocentre distance (mm) */
+    int n_beta, n_u, n_p, n_g;
+    const float *beta;      /* [n_beta] view angles (rad), uniform */
+    const float *u;         /* [n_u] integration rows U (mm) along the source axis, uniform */
+    const double *p;        /* [n_p] detector samples (mm), uniform */
+    const double *g;        /* [n_g] ramp taps, centre at index n_p - 1 (IterativeFDK.m:303-312) */
+    int t;                  /* voxel slice along t, 0-based */
+} ctr_fdkerr_geom;
+
 /* SIRT options (Iterative_algorithm/SIRT3D.m, SIRT.m). */
 typedef struct ctr_sirt_opts {
     int n_iter;             /* iterations (SIRT3D.m "Times") */
@@ -257,6 +271,15 @@ ctr_status ctr_curved_bp(ctr_ctx *ctx, const ctr_curved_geom *g, const float *fi
 
 /* Filter + backprojection.  Replaces FDKUI() (ImplementUI.h:31-33). */
 ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *proj, float *vol, unsigned flags);
+
+/* ---- iterative FDK ------------------------------------------------------- */
+
+/* Error slice of the iterative FDK correction for voxel slice t of vol
+ * [nz*ns*nt]: err [n_beta*nz*ns] (s fastest), summed over U and P as
+ * IterativeFDK.m:330-333 then sums it over the views.  Replaces
+ * IterativeFDKerrCompute() (IterativeFDKerrCompute.cu:77-332, kernel :12-75).
+ * Accumulates in FP64 (the reference in FP32). */
+ctr_status ctr_fdk_error_slice(ctr_ctx *ctx, const ctr_fdkerr_geom *a, const float *vol, float *err, unsigned flags);
 
 /* ---- Siddon system matrix ------------------------------------------------
  * The matrix A of the Siddon projector in CSR form: row r is ray r in the

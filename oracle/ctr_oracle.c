@@ -1404,6 +1404,71 @@ void or_curved_bp(const or_curved *g, const float *filt, double *vol, int quirks
             }
 }
 
+/* ----------------------------------------------------------------------
+ * Error slice of the iterative FDK correction
+ * (My_novel_algorithm/IterativeFDK/IterativeFDKerrCompute.cu:12-75): for voxel
+ * slice t of a volume [z][s][t] and every view, row U of the integration
+ * domain and detector sample P, the difference between the voxel reached at
+ * (P (D-U)/D, U) in the rotated frame (its z moved along the ray) and the
+ * voxel of the same (t, s) in slice z, weighted by the ramp tap at the
+ * voxel's projected offset.  Output [beta][z][s].  The float/double types of
+ * the reference's intermediates are kept, because each voxel index is a
+ * floor of them; cos/sin of the view angles are float(cos(double beta)).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nt, ns, nz;
+    double size[3];
+    double D;
+    int nb, nu, np, ng, t;
+    const float *beta, *u;
+    const double *p, *g;
+} or_fdkerr;
+
+void or_fdk_error_slice(const or_fdkerr *a, const float *pic, double *out)
+{
+    const double r0 = a->size[0] / a->nt, r1 = a->size[1] / a->ns, r2 = a->size[2] / a->nz;
+    const double Ct = a->size[0] / 2, Cs = a->size[1] / 2, Cz = a->size[2] / 2;
+    const double D = a->D;
+    const float dbeta = fabsf(a->beta[1] - a->beta[0]), dU = fabsf(a->u[1] - a->u[0]);
+    const double PInt = fabs(a->p[1] - a->p[0]);
+    const long plane = (long)a->nt * a->ns;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int b = 0; b < a->nb; b++)
+        for (int z = 0; z < a->nz; z++)
+            for (int s = 0; s < a->ns; s++) {
+                const float cb = (float)cos((double)a->beta[b]), sb = (float)sin((double)a->beta[b]);
+                const float image_t = (float)((a->t + 0.5) * r0 - Ct);
+                const float image_s = (float)((s + 0.5) * r1 - Cs);
+                const float image_z = (float)((z + 0.5) * r2 - Cz);
+                const float ts = image_t * sb, tc = image_t * cb, ss = image_s * sb, sc = image_s * cb;
+                const float dect_t = tc + ss, dect_s = sc - ts, dect_z = image_z;
+                const float LR = (float)(D / (D - (double)dect_s));
+                const float dtl = dect_t * LR;
+                double acc = 0.0;
+                for (int iu = 0; iu < a->nu; iu++) {
+                    const float U = a->u[iu];
+                    const float num = (dect_s - U) * dect_z;
+                    const float Zd = (float)((double)dect_z + (double)num / (D - (double)dect_s));
+                    for (int ip = 0; ip < a->np; ip++) {
+                        const double P = a->p[ip];
+                        const double ff = floor(((double)dtl - P) / PInt) + a->np;
+                        if (!(ff >= 0 && ff < a->ng)) continue;
+                        const float T = (float)(P * (D - (double)U) / D);
+                        const float tcb = T * cb, usb = U * sb, tsb = T * sb, ucb = U * cb;
+                        const float X = tcb - usb, Y = tsb + ucb;
+                        const double fx = floor(((double)X + Ct) / r0), fy = floor(((double)Y + Cs) / r1);
+                        const double fz = floor(((double)Zd + Cz) / r2);
+                        if (!(fx >= 0 && fx < a->nt && fy >= 0 && fy < a->ns && fz >= 0 && fz < a->nz)) continue;
+                        const long xy = (long)fy * a->nt + (long)fx;
+                        const float diff = pic[(long)fz * plane + xy] - pic[(long)z * plane + xy];
+                        acc += 0.5 * ((double)LR * LR) * diff * sqrt(1 + (P / D) * (P / D)) * a->g[(long)ff] * D /
+                               sqrt(D * D + P * P) * dU * PInt * dbeta;
+                    }
+                }
+                out[((long)b * a->nz + z) * a->ns + s] = acc;
+            }
+}
+
 int or_num_threads(void)
 {
 #ifdef _OPENMP

@@ -56,6 +56,13 @@ class _Curved(ctypes.Structure):
                 ("start", ctypes.c_double), ("fan", ctypes.c_double)]
 
 
+class _FdkErr(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("ns", ctypes.c_int), ("nz", ctypes.c_int), ("size", ctypes.c_double * 3),
+                ("D", ctypes.c_double), ("nb", ctypes.c_int), ("nu", ctypes.c_int), ("np", ctypes.c_int),
+                ("ng", ctypes.c_int), ("t", ctypes.c_int), ("beta", ctypes.c_void_p), ("u", ctypes.c_void_p),
+                ("p", ctypes.c_void_p), ("g", ctypes.c_void_p)]
+
+
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
@@ -111,6 +118,19 @@ class Curved:
                          len(self.xi), _ptr(self.beta), _ptr(self.gamma), _ptr(self.xi), start, fan)
         self.nvox = nt * ns * nz
         self.nray = len(self.beta) * len(self.gamma) * len(self.xi)
+
+
+class FdkErr:
+    """Arguments of the iterative-FDK error slice: volume [nz][ns][nt] over size,
+    slice t (0-based), beta and u float32, p and the taps g float64."""
+
+    def __init__(self, nt, ns, nz, size, D, beta, u, p, g, t):
+        self.beta, self.u = _f32(beta), _f32(u)
+        self.p, self.g = _f64(p), _f64(g)
+        self.s = _FdkErr(nt, ns, nz, (ctypes.c_double * 3)(*size), D, len(self.beta), len(self.u), len(self.p),
+                         len(self.g), t, _ptr(self.beta), _ptr(self.u), _ptr(self.p), _ptr(self.g))
+        self.nvox = nt * ns * nz
+        self.nout = len(self.beta) * nz * ns
 
 
 def _call(name, g, *args):
@@ -288,6 +308,13 @@ def fan_segments(g, ray):
 def par_segments(g, ray):
     s = g.s
     return _segments("or_par_segments", g, ray, s.nt + s.ns + 9)
+
+
+def fdk_error_slice(a, vol):
+    out = np.zeros(a.nout)
+    v = _f32(vol)
+    _call("or_fdk_error_slice", a, _ptr(v), _ptr(out))
+    return out
 
 
 def parker(beta, start, gamma, fan):

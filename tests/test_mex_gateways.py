@@ -303,3 +303,21 @@ def test_sysmat_gateways(ctx):
     v = np.random.default_rng(28).random(16 * 14 * 6).astype(np.float32)
     assert A.shape == (len(b_deg) * 12 * 24, 16 * 14 * 6)
     assert errs(A @ v.astype(np.float64), ctx.cone_fp(gc, v))[0] < 1e-6
+
+
+@pytest.mark.gpu
+def test_iterative_fdk_gateway(ctx):
+    """t is 1-based at the gateway; double inputs are accepted where the
+    reference reads single."""
+    import ctr
+    from analytic import iterative_fdk_args
+    a = iterative_fdk_args(12, n_views=10, pint=2.0)
+    vol = np.random.default_rng(33).random(12 ** 3).astype(np.float32)
+    out = Mex("IterativeFDKerrCompute")(vol, 12, 12, 12, np.array(a["size"]), a["beta"], a["u"], a["p"], a["D"],
+                                        a["g"], 4)
+    g = ctr.FdkErrGeom(12, 12, 12, a["size"], a["D"], a["beta"], a["u"], a["p"], a["g"], 3)
+    assert errs(out, ctx.fdk_error_slice(g, vol))[1] == 0.0
+    out2 = Mex("IterativeFDKerrCompute")(vol.astype(np.float64), 12, 12, 12, np.array(a["size"]),
+                                         a["beta"].astype(np.float64), a["u"].astype(np.float64), a["p"], a["D"],
+                                         a["g"], 4)
+    assert errs(out2, out)[1] == 0.0

@@ -459,3 +459,35 @@ def test_oracle_row_segments_sum_to_projection(oracle):
         idx, ln = oracle.par_segments(g, ray)
         assert len(np.unique(idx)) == len(idx) and np.all(ln > 0)
         assert abs(float(np.dot(ln, img[idx])) - fp[ray]) < 1e-12
+
+
+# ---- iterative FDK error slice ------------------------------------------------
+
+def small_fdkerr(O, n=16, t=5):
+    from analytic import iterative_fdk_args
+    a = iterative_fdk_args(n, n_views=24, pint=1.6)
+    return O.FdkErr(n, n, n, a["size"], a["D"], a["beta"], a["u"][::3], a["p"], a["g"], t)
+
+
+def test_fdk_error_slice_vanishes_for_z_invariant_volumes(oracle):
+    """FDK is exact for objects constant along z: every term is a difference
+    of two voxels of one (x, y) column."""
+    a = small_fdkerr(oracle)
+    col = np.random.default_rng(29).random((16, 16)).astype(np.float32)
+    vol = np.broadcast_to(col, (16, 16, 16)).copy()
+    assert np.abs(oracle.fdk_error_slice(a, vol.ravel())).max() == 0.0
+
+
+def test_fdk_error_slice_linear_and_z_mirror_symmetric(oracle):
+    """Linear in the volume; a volume mirrored about the central z plane gives
+    the error mirrored in z (the moved height is odd in z)."""
+    a = small_fdkerr(oracle)
+    rng = np.random.default_rng(30)
+    f = rng.random((16, 16, 16)).astype(np.float32)
+    h = rng.random((16, 16, 16)).astype(np.float32)
+    ef, eh = oracle.fdk_error_slice(a, f.ravel()), oracle.fdk_error_slice(a, h.ravel())
+    e2 = oracle.fdk_error_slice(a, (2 * f + h).ravel())
+    assert np.abs(e2 - (2 * ef + eh)).max() < 1e-5 * np.abs(e2).max() and np.abs(ef).max() > 0
+    sym = (f + f[::-1]).astype(np.float32)
+    e = oracle.fdk_error_slice(a, sym.ravel()).reshape(24, 16, 16)
+    assert np.abs(e - e[:, ::-1, :]).max() < 1e-5 * np.abs(e).max()
