@@ -71,6 +71,11 @@ struct FdkErrTables {
     DevBuf d_cos, d_sin, d_u, d_t, d_p, d_g, d_cp;
 };
 
+struct PhantomTable {
+    std::vector<PhantomEll> h;
+    DevBuf d;
+};
+
 struct FootCache {
     DevBuf buf;
     uint64_t key = 0;
@@ -108,6 +113,7 @@ struct ctr_ctx {
     ParTables par;
     CurvedTables curved;
     FdkErrTables fdkerr;
+    PhantomTable phantom;
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
@@ -1611,6 +1617,85 @@ ctr_status ctr_par_sysmat(ctr_ctx *ctx, const ctr_par_geom *g, int64_t *row_ptr,
     TRY(need_ctx(ctx));
     TRY(par_tables(ctx, g));
     return sysmat_impl(ctx, par_ray_params(ctx, g), nrayp(g), row_ptr, col, val, nnz, flags);
+}
+
+// ---- phantoms and metrics -----------------------------------------------------
+
+ctr_status ctr_phantom_ellipsoids(ctr_ctx *ctx, const double *ell, int n_ell, int nt, int ns, int nz, float *vol,
+                                  unsigned flags)
+{
+    if (nt < 1 || ns < 1 || nz < 1) return fail(ctx, CTR_EINVAL, "volume dimensions must be positive");
+    if ((double)nt * ns * nz > (double)std::numeric_limits<int>::max())
+        return fail(ctx, CTR_ETOOBIG, "more than 2^31-1 voxels");
+    if (n_ell < 0 || n_ell > 65536) return fail(ctx, CTR_EINVAL, "n_ell must be in [0, 65536]");
+    if (n_ell > 0 && !ell) return fail(ctx, CTR_EINVAL, "ell is NULL");
+    TRY(need_ctx(ctx));
+    PhantomTable &T = ctx->phantom;
+    T.h.resize(n_ell > 0 ? n_ell : 1);
+    const double deg = 3.141592653589793 / 180.0;
+    for (int k = 0; k < n_ell; ++k) {
+        const double *e = ell + 8 * (size_t)k;
+        for (int j = 0; j < 8; ++j)
+            if (!std::isfinite(e[j])) return fail(ctx, CTR_EINVAL, "ellipsoid %d has a non-finite entry", k);
+        if (!(e[1] > 0 && e[2] > 0 && e[3] > 0)) return fail(ctx, CTR_EINVAL, "ellipsoid %d has a non-positive semi-axis", k);
+        PhantomEll &p = T.h[k];
+        const double ph = e[7] * deg;
+        p.A = e[0];
+        p.a = e[1];
+        p.b = e[2];
+        p.c = e[3];
+        p.cp = std::cos(ph);
+        p.sp = std::sin(ph);
+        p.sx = p.cp * e[4] + p.sp * e[5];    // phantom3d.m:97-104: alpha * [x0; y0; z0]
+        p.sy = -p.sp * e[4] + p.cp * e[5];
+        p.z0 = e[6];
+    }
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    TRY(upload(ctx, T.d, T.h.data(), sizeof(PhantomEll) * T.h.size()));
+    const size_t n = (size_t)nt * ns * nz;
+    float *dv;
+    TRY(stage_out(ctx, dev, false, kWsOut, vol, n, &dv));
+    TRY(check_launch(ctx, launch_phantom((const PhantomEll *)T.d.ptr, n_ell, nt, ns, nz, dv, ctx->stream),
+                     "phantom rasterizer"));
+    if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the table is rewritten by the next call
+    return finish_out(ctx, dev, dv, vol, n);
+}
+
+// (sum (x - ref)^2, sum ref^2) of two device or host vectors
+static ctr_status diff_sums(ctr_ctx *ctx, const float *x, const float *ref, size_t n, unsigned flags, double out[2])
+{
+    if (!out) return fail(ctx, CTR_EINVAL, "out is NULL");
+    if (n == 0) return fail(ctx, CTR_EINVAL, "n must be positive");
+    TRY(need_ctx(ctx));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const float *dx, *dr;
+    TRY(stage_in(ctx, dev, kWsIn, x, n, &dx));
+    TRY(stage_in(ctx, dev, kWsTmp, ref, n, &dr));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * 4096));
+    TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2));
+    int nb = 0;
+    double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
+    TRY(check_launch(ctx, launch_diff_sums(dx, dr, n, part, &nb, ctx->stream), "difference sums"));
+    TRY(check_launch(ctx, launch_reduce_partials(part, nb, sums, ctx->stream), "reduce"));
+    HIP_TRY(ctx, hipMemcpyAsync(out, sums, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return CTR_OK;
+}
+
+ctr_status ctr_rrmse(ctr_ctx *ctx, const float *x, const float *ref, size_t n, double *out, unsigned flags)
+{
+    double s[2];
+    TRY(diff_sums(ctx, x, ref, n, flags, s));
+    *out = std::sqrt(s[0] / (double)n) / (std::sqrt(s[1] / (double)n) + std::numeric_limits<double>::epsilon());
+    return CTR_OK;
+}
+
+ctr_status ctr_local_error(ctr_ctx *ctx, const float *cur, const float *prev, size_t n, double *out, unsigned flags)
+{
+    double s[2];
+    TRY(diff_sums(ctx, cur, prev, n, flags, s));
+    *out = std::sqrt(s[0]) / std::sqrt(s[1]);
+    return CTR_OK;
 }
 
 // ---- iterative FDK ------------------------------------------------------------

@@ -247,6 +247,15 @@ struct FdkErrParams {
 // xy: [nb][nu][np] int scratch for the voxel map
 hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *xy, float *out, hipStream_t st);
 
+// Ellipsoid of the phantom rasterizer: amplitude, semi-axes, rotated centre
+// (sx, sy) = R(phi) (x0, y0), z0, cos/sin of the rotation about z.
+struct PhantomEll {
+    double A, a, b, c, sx, sy, z0, cp, sp;
+};
+hipError_t launch_phantom(const PhantomEll *ell, int n_ell, int nt, int ns, int nz, float *vol, hipStream_t st);
+// per-block (sum (x - ref)^2, sum ref^2)
+hipError_t launch_diff_sums(const float *x, const float *ref, size_t n, double *partials, int *nblocks, hipStream_t st);
+
 // Utilities.
 hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st);
 hipError_t launch_transpose_st(const float *in, float *out, int nt, int ns, int nz, hipStream_t st);   // [z][t][s] <- [z][s][t]

@@ -121,6 +121,55 @@ __global__ __launch_bounds__(256) void k_sum_sq_sum(const float *x, size_t n, do
     }
 }
 
+// Per-block (sum (x - ref)^2, sum ref^2) for rRMSE and LocalError.
+__global__ __launch_bounds__(256) void k_diff_sums(const float *x, const float *ref, size_t n, double *partials)
+{
+    __shared__ double red[2][4];
+    double a = 0.0, b = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double r = ref[i], d = (double)x[i] - r;
+        a += d * d;
+        b += r * r;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        partials[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// Sum of ellipsoid indicators at the voxel centres (phantom3d.m:71-110), FP64
+// in the order of the reference's expression, contraction off so that the
+// boundary test is the plain IEEE one.
+__global__ __launch_bounds__(256) void k_phantom(const PhantomEll *__restrict__ ell, int n_ell, int nt, int ns, int nz,
+                                                 float *__restrict__ vol)
+{
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t n = (size_t)nt * ns * nz;
+    if (i >= n) return;
+    const int it = (int)(i % nt), is = (int)((i / nt) % ns), iz = (int)(i / ((size_t)nt * ns));
+    // grid on [-1, 1] with the end points: (k - (n-1)/2) / ((n-1)/2); t is MATLAB's y, s its x
+    const double y = nt > 1 ? (it - (nt - 1) / 2.0) / ((nt - 1) / 2.0) : 0.0;
+    const double x = ns > 1 ? (is - (ns - 1) / 2.0) / ((ns - 1) / 2.0) : 0.0;
+    const double z = nz > 1 ? (iz - (nz - 1) / 2.0) / ((nz - 1) / 2.0) : 0.0;
+    double acc = 0.0;
+    for (int k = 0; k < n_ell; ++k) {
+        const PhantomEll e = ell[k];
+        const double p = e.cp * x + e.sp * y - e.sx, q = -e.sp * x + e.cp * y - e.sy, r = z - e.z0;
+        const double u = p * p / (e.a * e.a) + q * q / (e.b * e.b);
+        const double w = r * r / (e.c * e.c);
+        if (u + w <= 1.0) acc += e.A;
+    }
+    vol[i] = (float)acc;
+}
+
 int grid_for(size_t n)
 {
     size_t g = (n + 255) / 256;
@@ -172,6 +221,21 @@ hipError_t launch_sum_sq_sum(const float *x, size_t n, double *partials, int *nb
     const int g = grid_for(n);
     *nblocks = g;
     hipLaunchKernelGGL(k_sum_sq_sum, dim3(g), dim3(256), 0, st, x, n, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_sums(const float *x, const float *ref, size_t n, double *partials, int *nblocks, hipStream_t st)
+{
+    const int g = grid_for(n);
+    *nblocks = g;
+    hipLaunchKernelGGL(k_diff_sums, dim3(g), dim3(256), 0, st, x, ref, n, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_phantom(const PhantomEll *ell, int n_ell, int nt, int ns, int nz, float *vol, hipStream_t st)
+{
+    const size_t n = (size_t)nt * ns * nz;
+    hipLaunchKernelGGL(k_phantom, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ell, n_ell, nt, ns, nz, vol);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ EXPORTS = [
     "ctr_par_geom_check", "ctr_par_fp", "ctr_par_bp_adjoint", "ctr_par_filter", "ctr_par_fbp",
     "ctr_curved_geom_check", "ctr_curved_filter", "ctr_curved_bp", "ctr_curved_fdk",
     "ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat", "ctr_fdk_error_slice",
+    "ctr_phantom_ellipsoids", "ctr_rrmse", "ctr_local_error",
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
@@ -127,6 +128,11 @@ def lib():
         L.ctr_fan_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_par_geom_check.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ctr_curved_geom_check.argtypes = [ctypes.c_void_p]
+        L.ctr_phantom_ellipsoids.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint]
+        for name in ("ctr_rrmse", "ctr_local_error"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_uint]
         L.ctr_fdk_error_slice.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_uint]
         for name in ("ctr_cone_sysmat", "ctr_fan_sysmat", "ctr_par_sysmat"):
@@ -442,6 +448,42 @@ class Context:
 
     def curved_fdk(self, g, proj, out=None, flags=0):
         return self._run(lib().ctr_curved_fdk, g, proj, g.nvox, out, flags)
+
+    # -- phantoms and metrics ------------------------------------------------------
+    def phantom_ellipsoids(self, ell, nt, ns=None, nz=None, device=False):
+        """Rasterize an [n][8] ellipsoid table (phantom3d.m) on the device; [nz][ns][nt]."""
+        ns = nt if ns is None else ns
+        nz = nt if nz is None else nz
+        e = _f64(np.asarray(ell, np.float64).reshape(-1, 8))   # flat, 8 values per ellipsoid
+        n_ell = e.size // 8
+        if device:
+            import torch
+            self._bind_torch_stream()
+            out = torch.empty(nt * ns * nz, dtype=torch.float32, device="cuda")
+            self._check(lib().ctr_phantom_ellipsoids(self.h, e.ctypes.data, n_ell, nt, ns, nz, out.data_ptr(),
+                                                     DEVICE_PTRS))
+            return out.reshape(nz, ns, nt)
+        out = np.empty(nt * ns * nz, np.float32)
+        self._check(lib().ctr_phantom_ellipsoids(self.h, e.ctypes.data, n_ell, nt, ns, nz, out.ctypes.data, 0))
+        return out.reshape(nz, ns, nt)
+
+    def _metric(self, fn, x, ref):
+        v = ctypes.c_double(0.0)
+        if _is_device(x):
+            self._bind_torch_stream()
+            assert x.is_contiguous() and ref.is_contiguous() and x.numel() == ref.numel()
+            self._check(fn(self.h, x.data_ptr(), ref.data_ptr(), x.numel(), ctypes.byref(v), DEVICE_PTRS))
+        else:
+            a, b = _f32(x).ravel(), _f32(ref).ravel()
+            assert a.size == b.size
+            self._check(fn(self.h, a.ctypes.data, b.ctypes.data, a.size, ctypes.byref(v), 0))
+        return v.value
+
+    def rrmse(self, x, ref):
+        return self._metric(lib().ctr_rrmse, x, ref)
+
+    def local_error(self, cur, prev):
+        return self._metric(lib().ctr_local_error, cur, prev)
 
     # -- iterative FDK -------------------------------------------------------------
     def fdk_error_slice(self, a, vol, out=None, flags=0):

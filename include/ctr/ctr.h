@@ -281,6 +281,23 @@ ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *p
  * Accumulates in FP64 (the reference in FP32). */
 ctr_status ctr_fdk_error_slice(ctr_ctx *ctx, const ctr_fdkerr_geom *a, const float *vol, float *err, unsigned flags);
 
+/* ---- phantoms and metrics ------------------------------------------------ */
+
+/* Sum of ellipsoid indicators (My_toolbox/My_phantom/phantom3d.m:71-110) at the
+ * voxel centres of a grid spanning [-1, 1] on each axis (end points
+ * included), written as vol [nz*ns*nt]: t is phantom3d's y and s its x (the
+ * layout a plain reshape of its array gives, FDK_GPU.m:74).  ell [n_ell][8]:
+ * A, a, b, c, x0, y0, z0, phi (degrees), as phantom3d.m's tables. */
+ctr_status ctr_phantom_ellipsoids(ctr_ctx *ctx, const double *ell, int n_ell, int nt, int ns, int nz, float *vol,
+                                  unsigned flags);
+
+/* rRMSE (My_toolbox/Evalution_parameter/rRMSE.m:2):
+ * sqrt(mean((x-ref)^2)) / (sqrt(mean(ref^2)) + eps), FP64 on the device. */
+ctr_status ctr_rrmse(ctr_ctx *ctx, const float *x, const float *ref, size_t n, double *out, unsigned flags);
+
+/* LocalError (My_toolbox/Evalution_parameter/LocalError.m:2): ||cur - prev|| / ||prev||. */
+ctr_status ctr_local_error(ctr_ctx *ctx, const float *cur, const float *prev, size_t n, double *out, unsigned flags);
+
 /* ---- Siddon system matrix ------------------------------------------------
  * The matrix A of the Siddon projector in CSR form: row r is ray r in the
  * projection layout, column j is voxel j in the volume layout, the value is
