@@ -58,3 +58,23 @@ def config_cone_256(n_views=360):
 def config_cone_512():
     """512^3 at 0.5 mm, 720 views, 1024x768 detector at 0.8 mm, SOD 700, SDD 1100."""
     return cone_circular(512, 0.5, 720, 1024, 768, 0.8, 700.0, 1100.0)
+
+
+def iterative_fdk_args(n, n_views=360, size=60.0, D=200.0, pint=0.4):
+    """The configuration of IterativeFDK.m:5-55,293-312 (My_novel_algorithm/IterativeFDK) for an n^3 volume over
+    size mm: P = -1.1 Rplane : pint : 1.1 Rplane, U = Rpic - dU (1..LU) with
+    dU the voxel pitch, views 1..360 degrees, half-less SL taps G [2 LP - 1]."""
+    res = size / n
+    rpic, rplane = size * np.sqrt(3) / 2, size * np.sqrt(2) / 2
+    maxp = rplane * 1.1
+    p = np.arange(-maxp, maxp + 1e-9, pint)
+    lp = len(p)
+    du = res
+    lu = int(np.floor(2 * rpic / du))
+    u = (rpic - du * np.arange(1, lu + 1)).astype(np.float32)
+    beta = np.deg2rad(np.arange(1, n_views + 1) * (360.0 / n_views)).astype(np.float32)
+    k = np.arange(lp)
+    half = -1.0 / (np.pi * pint) ** 2 / (4.0 * k ** 2 - 1.0)
+    half[0] = 1.0 / (np.pi * pint) ** 2
+    g = np.concatenate([half[:0:-1], half])
+    return dict(size=(size, size, size), D=D, beta=beta, u=u, p=p, g=g)

@@ -13,7 +13,10 @@ The par_* operators run the parallel-beam workload of FBP_parallel_GPU.m:
 of 1.1 pixel (730 bins), on the 2-D Shepp-Logan phantom.  The curved_*
 operators run the short scan of ImplementUIData.m: 512 x 512 x 50 voxels,
 800 views over 240 degrees, 80 rows x 936 channels (uniform synthetic
-channel angles, uniform random projections).
+channel angles, uniform random projections).  fan_sysmat builds the
+system matrix of the fan configuration C1 (256^2 over 256 mm, 360 views, 512
+equiangular bins at 1e-3 rad, SOD 500); fdk_error_slice runs
+IterativeFDK.m's 128^3 / 360-view slice.
 """
 import argparse
 import os
@@ -21,7 +24,8 @@ import sys
 import time
 
 OPS = ("cone_fp", "cone_filter", "fdk_bp", "cone_bp_ref", "cone_adjoint", "sirt3d",
-       "par_fp", "par_filter", "par_fbp", "par_adjoint", "curved_filter", "curved_bp")
+       "par_fp", "par_filter", "par_fbp", "par_adjoint", "curved_filter", "curved_bp", "fan_sysmat",
+       "fdk_error_slice")
 
 
 def parallel_workload():
@@ -93,6 +97,17 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(f"curved workload: {gc.nt}x{gc.ns}x{gc.nz}, {len(gc.beta)} views, {len(gc.xi)}x{len(gc.gamma)} detector",
               flush=True)
+    if "fan_sysmat" in ops:
+        import numpy as np
+        gam = 1e-3 * (np.arange(512) - 255.5)
+        gf = ctr.FanGeom(256, 256, (256.0, 256.0), 500.0, np.deg2rad(np.arange(1, 361)), 500.0 * np.tan(gam))
+    if "fdk_error_slice" in ops:
+        import numpy as np
+        from .geometry import iterative_fdk_args
+        ea = iterative_fdk_args(128)
+        ge = ctr.FdkErrGeom(128, 128, 128, ea["size"], ea["D"], ea["beta"], ea["u"], ea["p"], ea["g"], 63)
+        evol = torch.rand(ge.nvox, device="cuda")
+        eout = torch.empty(ge.nout, device="cuda")
     calls = {
         "cone_fp": lambda: ctx.cone_fp(g, vol, out=proj),
         "cone_filter": lambda: ctx.cone_filter(g, proj, out=filt),
@@ -105,6 +120,8 @@ def main(argv=None):
         "par_adjoint": lambda: ctx.par_bp_adjoint(gp, pproj, out=pout),
         "curved_filter": lambda: ctx.curved_filter(gc, cproj, out=cfilt),
         "curved_bp": lambda: ctx.curved_bp(gc, cfilt, out=cvol),
+        "fan_sysmat": lambda: ctx.fan_sysmat(gf, device=True),
+        "fdk_error_slice": lambda: ctx.fdk_error_slice(ge, evol, out=eout),
     }
     for o in ops:
         if o == "sirt3d":
@@ -116,7 +133,7 @@ def main(argv=None):
             ms = (time.perf_counter() - t0) * 1e3 / args.sirt_iters
             print(f"{o:14s} {ms:9.3f} ms per iteration (wall, {args.sirt_iters} iterations)", flush=True)
         else:
-            reps = 1 if o == "cone_adjoint" and args.n == 512 else args.reps
+            reps = 1 if (o == "cone_adjoint" and args.n == 512) or o == "fdk_error_slice" else args.reps
             print(f"{o:14s} {timed(calls[o], reps):9.3f} ms", flush=True)
     ctx.close()
 

@@ -7,7 +7,7 @@ import time
 import numpy as np
 import pytest
 
-from analytic import iterative_fdk_args
+from ctr.geometry import iterative_fdk_args
 from conftest import errs, report
 
 pytestmark = pytest.mark.gpu

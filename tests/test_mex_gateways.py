@@ -310,7 +310,7 @@ def test_iterative_fdk_gateway(ctx):
     """t is 1-based at the gateway; double inputs are accepted where the
     reference reads single."""
     import ctr
-    from analytic import iterative_fdk_args
+    from ctr.geometry import iterative_fdk_args
     a = iterative_fdk_args(12, n_views=10, pint=2.0)
     vol = np.random.default_rng(33).random(12 ** 3).astype(np.float32)
     out = Mex("IterativeFDKerrCompute")(vol, 12, 12, 12, np.array(a["size"]), a["beta"], a["u"], a["p"], a["D"],

@@ -464,7 +464,7 @@ def test_oracle_row_segments_sum_to_projection(oracle):
 # ---- iterative FDK error slice ------------------------------------------------
 
 def small_fdkerr(O, n=16, t=5):
-    from analytic import iterative_fdk_args
+    from ctr.geometry import iterative_fdk_args
     a = iterative_fdk_args(n, n_views=24, pint=1.6)
     return O.FdkErr(n, n, n, a["size"], a["D"], a["beta"], a["u"][::3], a["p"], a["g"], t)
 
