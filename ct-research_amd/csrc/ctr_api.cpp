@@ -1759,7 +1759,6 @@ ctr_status ctr_fdk_error_slice(ctr_ctx *ctx, const ctr_fdkerr_geom *a, const flo
     float *de;
     TRY(stage_in(ctx, dev, kWsIn, vol, nvox, &dv));
     TRY(stage_out(ctx, dev, false, kWsOut, err, nout, &de));
-    TRY(buf_reserve(ctx, ctx->ws[kWsG], (size_t)nb * nu * np * sizeof(int)));
     FdkErrParams P{};
     P.nt = a->nt;
     P.ns = a->ns;
@@ -1784,6 +1783,7 @@ ctr_status ctr_fdk_error_slice(ctr_ctx *ctx, const ctr_fdkerr_geom *a, const flo
     P.p = (const double *)T.d_p.ptr;
     P.g = (const double *)T.d_g.ptr;
     P.cp = (const double *)T.d_cp.ptr;
+    TRY(buf_reserve(ctx, ctx->ws[kWsG], launch_fdk_error_slice_scratch(P)));
     TRY(check_launch(ctx, launch_fdk_error_slice(P, dv, (int *)ctx->ws[kWsG].ptr, de, ctx->stream),
                      "iterative-FDK error slice"));
     if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the tables are rewritten by the next call

@@ -244,8 +244,14 @@ struct FdkErrParams {
     const double *g;            // [ng] ramp taps
     const double *cp;           // [np] sqrt(1 + (P/D)^2) D / sqrt(D^2 + P^2) dU PInt dbeta
 };
-// xy: [nb][nu][np] int scratch for the voxel map
+// xy: int scratch of launch_fdk_error_slice_scratch(P) bytes (the voxel map
+// [nb][nu][np], then the per-(view, U) sample intervals)
 hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *xy, float *out, hipStream_t st);
+inline size_t launch_fdk_error_slice_scratch(const FdkErrParams &P)
+{
+    const size_t n = (size_t)P.nb * P.nu * P.np;
+    return (((n + 1) & ~(size_t)1) + 2 * (size_t)P.nb * P.nu) * sizeof(int);
+}
 
 // Ellipsoid of the phantom rasterizer: amplitude, semi-axes, rotated centre
 // (sx, sy) = R(phi) (x0, y0), z0, cos/sin of the rotation about z.

@@ -30,6 +30,8 @@ __global__ __launch_bounds__(256) void k_curved_bp(const float *__restrict__ fil
     }
     const int ng = P.ng;
     const size_t view_elems = (size_t)P.nx * ng;
+    const unsigned row_bytes = (unsigned)ng * 4u;
+    const float xlast = __builtin_bit_cast(float, __builtin_bit_cast(int, P.xmax) - 1);   // largest float below xmax
     for (int b = 0; b < P.nb; ++b) {
         const float sb = views[2 * b], cb = views[2 * b + 1];
         const float u = fmaf(x, cb, y * sb), s = fmaf(-x, sb, y * cb);
@@ -46,20 +48,27 @@ __global__ __launch_bounds__(256) void k_curved_bp(const float *__restrict__ fil
         const float *Fv = filt + (size_t)b * view_elems;
         // branch-free over the chunk, so that all its loads are in flight together:
         // rows are clamped into the detector and out-of-range voxels get weight 0
-        float v0[kZc], v1[kZc], v2[kZc], v3[kZc], fxs[kZc], msk[kZc];
+        float v0[kZc], v1[kZc], v2[kZc], v3[kZc], fxs[kZc];
+        bool in[kZc];
 #pragma unroll
         for (int c = 0; c < kZc; ++c) {
+            // 0 <= xi < xmax exactly when clamping into [0, xlast] leaves xi unchanged;
+            // the clamped row index is then at most nx - 2
             const float xi = fmaf(zc[c], xs, P.x_off);
-            msk[c] = (xi >= 0.0f && xi < P.xmax) ? 1.0f : 0.0f;
-            const float xcl = __builtin_amdgcn_fmed3f(xi, 0.0f, P.xmax);
-            const int j0 = min((int)xcl, P.nx - 2);
-            fxs[c] = xcl - (float)j0;
-            // the view's base pointer is uniform; 32-bit offsets within the view
-            const float *q = Fv + (unsigned)(j0 * ng + i0);
-            v0[c] = q[0];
-            v1[c] = q[1];
-            v2[c] = q[ng];
-            v3[c] = q[ng + 1];
+            const float xcl = __builtin_amdgcn_fmed3f(xi, 0.0f, xlast);
+            in[c] = xi == xcl;
+            const int j0 = (int)xcl;
+            fxs[c] = __builtin_amdgcn_fractf(xcl);
+            // the view's base pointer is uniform; 32-bit byte offsets within the view
+            // (scalar base + vector offset addressing)
+            const unsigned ob = (unsigned)(j0 * ng + i0) * 4u;
+            float2 p0, p1;   // 4-byte aligned pairs
+            __builtin_memcpy(&p0, reinterpret_cast<const char *>(Fv) + ob, sizeof p0);
+            __builtin_memcpy(&p1, reinterpret_cast<const char *>(Fv) + (ob + row_bytes), sizeof p1);
+            v0[c] = p0.x;
+            v1[c] = p0.y;
+            v2[c] = p1.x;
+            v3[c] = p1.y;
         }
 #pragma unroll
         for (int c = 0; c < kZc; ++c) {
@@ -68,7 +77,7 @@ __global__ __launch_bounds__(256) void k_curved_bp(const float *__restrict__ fil
             // weight dbeta rho / L2^1.5 (ImplementUI.cu:145: / L2 * dbeta / (sqrt(L2) / rho))
             const float L2 = fmaf(zc[c], zc[c], rho2);
             const float r = __builtin_amdgcn_rsqf(L2);
-            acc[c] = fmaf(fmaf(fxs[c], bq - a, a), msk[c] * wb * (r * r * r), acc[c]);
+            acc[c] = fmaf(fmaf(fxs[c], bq - a, a), in[c] ? wb * (r * r * r) : 0.0f, acc[c]);
         }
     }
 #pragma unroll

@@ -39,8 +39,26 @@ __global__ __launch_bounds__(256) void k_fdkerr_xy(FdkErrParams P, int *__restri
     xy[i] = (fx >= 0.0 && fx < (double)P.nt && fy >= 0.0 && fy < (double)P.ns) ? (int)fy * P.nt + (int)fx : -1;
 }
 
+// Per (view, U) the samples P whose voxel lies in the image form one
+// interval (the points (P (D-U)/D, U) of a line in the rotated frame); its
+// first and last index, or (1, 0) when empty.
+__global__ __launch_bounds__(256) void k_fdkerr_range(FdkErrParams P, const int *__restrict__ xy, int2 *__restrict__ rng)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.nb * P.nu) return;
+    const int *row = xy + (size_t)i * P.np;
+    int lo = P.np, hi = -1;
+    for (int ip = 0; ip < P.np; ++ip)
+        if (row[ip] >= 0) {
+            lo = min(lo, ip);
+            hi = ip;
+        }
+    rng[i] = lo <= hi ? make_int2(lo, hi) : make_int2(1, 0);
+}
+
 __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const float *__restrict__ pic,
-                                                       const int *__restrict__ xy, float *__restrict__ out)
+                                                       const int *__restrict__ xy, const int2 *__restrict__ rng,
+                                                       float *__restrict__ out)
 {
     __shared__ int fil[kFeChunk * kFeThreads];   // tap index of (P, thread) for a chunk of P, -1 outside the taps
     const int s = blockIdx.x * kFeThreads + threadIdx.x;
@@ -67,6 +85,10 @@ __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const flo
         }
         if (s >= P.ns) continue;
         for (int iu = 0; iu < P.nu; ++iu) {
+            // samples of this chunk whose voxel lies in the image
+            const int2 r = rng[(size_t)b * P.nu + iu];
+            const int k0 = max(r.x - p0, 0), k1 = min(r.y - p0 + 1, pn);
+            if (k0 >= k1) continue;
             const float U = P.u[iu];
             const float num = (dect_s - U) * dect_z;
             const float Zd = (float)((double)dect_z + (double)num / (D - (double)dect_s));
@@ -75,7 +97,7 @@ __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const flo
             const float *moved = pic + (size_t)fz * plane;
             const int *row = xy + ((size_t)b * P.nu + iu) * P.np + p0;
             const double *cp = P.cp + p0;
-            for (int k = 0; k < pn; ++k) {
+            for (int k = k0; k < k1; ++k) {
                 const int f = fil[k * kFeThreads + threadIdx.x];
                 const int o = row[k];
                 if (f >= 0 && o >= 0) {
@@ -96,8 +118,13 @@ hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *
     hipLaunchKernelGGL(k_fdkerr_xy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, xy);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    // the (view, U) sample intervals follow the voxel map in the same scratch
+    int2 *rng = reinterpret_cast<int2 *>(xy + ((n + 1) & ~(size_t)1));
+    const int nbu = P.nb * P.nu;
+    hipLaunchKernelGGL(k_fdkerr_range, dim3((nbu + 255) / 256), dim3(256), 0, st, P, xy, rng);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     const dim3 grid((P.ns + kFeThreads - 1) / kFeThreads, P.nz, P.nb);
-    hipLaunchKernelGGL(k_fdkerr, grid, dim3(kFeThreads), 0, st, P, pic, xy, out);
+    hipLaunchKernelGGL(k_fdkerr, grid, dim3(kFeThreads), 0, st, P, pic, xy, rng, out);
     return hipGetLastError();
 }
 

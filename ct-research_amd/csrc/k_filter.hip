@@ -12,6 +12,8 @@
 // FP32 rate on gfx950, and the reference accumulates in double (:38-45).
 #include "ctr_internal.h"
 
+#include <cstdlib>
+
 namespace ctr {
 namespace {
 
@@ -281,8 +283,8 @@ __device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const
 }
 
 template <int L>
-__global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restrict__ in, float *__restrict__ out,
-                                                          FilterParams P)
+__device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, float *__restrict__ out,
+                                                const FilterParams &P)
 {
     constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1, GPB = kFftThreads / G;
     constexpr int SLOT = L + L / 16;
@@ -366,13 +368,37 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restr
 }
 
 template <int L>
+__global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restrict__ in, float *__restrict__ out,
+                                                          FilterParams P)
+{
+    fft_filter_body<L>(in, out, P);
+}
+
+// the same with the registers capped for two waves per SIMD (some spilled)
+template <int L>
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fft_filter_occ2(
+    const float *__restrict__ in, float *__restrict__ out, FilterParams P)
+{
+    fft_filter_body<L>(in, out, P);
+}
+
+template <int L>
 hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStream_t st)
 {
     constexpr int GPB = kFftThreads / (L / 16);
     const int npairs = P.spec_stride ? P.nx * ((P.rows / P.nx + 1) / 2) : (P.rows + 1) / 2;
     const int need = (npairs + GPB - 1) / GPB;
     const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
-    hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+    // two waves per SIMD (a few spilled registers) beat one for L <= 1024: 1.40 -> 1.11 ms
+    // for 360 x 512 rows of 512; at L = 2048 the cap spills too much.  CTR_FFT_OCC=1 forces one.
+    static const int occ = [] {
+        const char *e = std::getenv("CTR_FFT_OCC");
+        return e ? std::atoi(e) : 2;
+    }();
+    if (occ == 2 && L <= 1024)
+        hipLaunchKernelGGL((k_fft_filter_occ2<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+    else
+        hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
     return hipGetLastError();
 }
 
