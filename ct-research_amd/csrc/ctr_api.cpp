@@ -1629,16 +1629,14 @@ ctr_status ctr_phantom_ellipsoids(ctr_ctx *ctx, const double *ell, int n_ell, in
         return fail(ctx, CTR_ETOOBIG, "more than 2^31-1 voxels");
     if (n_ell < 0 || n_ell > 65536) return fail(ctx, CTR_EINVAL, "n_ell must be in [0, 65536]");
     if (n_ell > 0 && !ell) return fail(ctx, CTR_EINVAL, "ell is NULL");
-    TRY(need_ctx(ctx));
-    PhantomTable &T = ctx->phantom;
-    T.h.resize(n_ell > 0 ? n_ell : 1);
+    std::vector<PhantomEll> h(n_ell > 0 ? n_ell : 1);
     const double deg = 3.141592653589793 / 180.0;
     for (int k = 0; k < n_ell; ++k) {
         const double *e = ell + 8 * (size_t)k;
         for (int j = 0; j < 8; ++j)
             if (!std::isfinite(e[j])) return fail(ctx, CTR_EINVAL, "ellipsoid %d has a non-finite entry", k);
         if (!(e[1] > 0 && e[2] > 0 && e[3] > 0)) return fail(ctx, CTR_EINVAL, "ellipsoid %d has a non-positive semi-axis", k);
-        PhantomEll &p = T.h[k];
+        PhantomEll &p = h[k];
         const double ph = e[7] * deg;
         p.A = e[0];
         p.a = e[1];
@@ -1650,6 +1648,9 @@ ctr_status ctr_phantom_ellipsoids(ctr_ctx *ctx, const double *ell, int n_ell, in
         p.sy = -p.sp * e[4] + p.cp * e[5];
         p.z0 = e[6];
     }
+    TRY(need_ctx(ctx));
+    PhantomTable &T = ctx->phantom;
+    T.h = std::move(h);
     const bool dev = flags & CTR_DEVICE_PTRS;
     TRY(upload(ctx, T.d, T.h.data(), sizeof(PhantomEll) * T.h.size()));
     const size_t n = (size_t)nt * ns * nz;

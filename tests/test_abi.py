@@ -116,3 +116,66 @@ def test_physical_mapping():
     assert np.allclose(np.diff(gam), 1e-3) and len(f.p) == 512
     c5 = ctr.geometry.config_cone_512()
     assert np.isclose(c5.p[1] - c5.p[0], 0.8 * 700 / 1100, rtol=1e-4) and len(c5.p) == 1024 and len(c5.xi) == 768
+
+
+def test_parallel_validation():
+    import ctr
+    EINVAL, ETOOBIG = ctr._lib.EINVAL, ctr._lib.ETOOBIG
+    th, t = np.deg2rad(np.arange(1, 181)), 0.5 * (np.arange(90) - 44.5)
+    assert ctr.ParGeom(32, 32, (32.0, 32.0), th, t).check(True) == 0
+    assert ctr.ParGeom(0, 32, (32.0, 32.0), th, t).check() == EINVAL
+    assert ctr.ParGeom(32, 32, (32.0, 32.0), [0.1, np.nan], t).check() == EINVAL
+    assert ctr.ParGeom(32, 32, (32.0, 32.0), th, [0.0, 1.0, 1.0]).check() == EINVAL        # not monotonic
+    t_nu = t.copy()
+    t_nu[10] += 0.2
+    assert ctr.ParGeom(32, 32, (32.0, 32.0), th, t_nu).check(False) == 0
+    assert ctr.ParGeom(32, 32, (32.0, 32.0), th, t_nu).check(True) == EINVAL             # FBP needs a uniform pitch
+    assert ctr.ParGeom(32, 32, (32.0, 32.0), np.zeros(70000), t).check() == ETOOBIG
+    assert ctr.ParGeom(40000, 40000, (1.0, 1.0), th, t).check() == ETOOBIG
+
+
+def test_curved_validation():
+    import ctr
+    EINVAL, ETOOBIG = ctr._lib.EINVAL, ctr._lib.ETOOBIG
+    gam = 0.002 * (np.arange(101) - 50)
+    xi = np.arange(8) - 3.5
+    beta = 0.02 * np.arange(200)
+
+    def geom(**kw):
+        d = dict(nt=32, ns=32, nz=8, size=(32.0, 32.0, 8.0), D=100.0, Dsd=200.0, beta=beta, gamma=gam, xi=xi,
+                 start=0.0, fan=0.2)
+        d.update(kw)
+        return ctr.CurvedGeom(d["nt"], d["ns"], d["nz"], d["size"], d["D"], d["Dsd"], d["beta"], d["gamma"],
+                              d["xi"], d["start"], d["fan"])
+    assert geom().check() == 0
+    assert geom(D=10.0).check() == EINVAL                                  # source orbit inside the volume
+    assert geom(fan=0.0).check() == EINVAL and geom(fan=4.0).check() == EINVAL
+    assert geom(gamma=0.001 * (np.arange(1100) - 549.5)).check() == ETOOBIG   # FFT filter limit
+    g2 = gam.copy()
+    g2[3] += 1e-4
+    assert geom(gamma=g2).check() == EINVAL                                 # non-uniform channels
+    assert geom(gamma=np.array([0.0, 2.0])).check() == EINVAL              # outside (-pi/2, pi/2)
+    assert geom(beta=np.array([0.0])).check() == EINVAL
+
+
+def test_error_slice_and_helpers_reject_bad_arguments_without_a_context():
+    import ctr
+    from ctr.geometry import iterative_fdk_args
+    L = ctr.lib()
+    EINVAL = ctr._lib.EINVAL
+    a = iterative_fdk_args(8, n_views=6, pint=4.0)
+    vol = np.zeros(8 ** 3, np.float32)
+    out = np.zeros(6 * 8 * 8, np.float32)
+    bad_t = ctr.FdkErrGeom(8, 8, 8, a["size"], a["D"], a["beta"], a["u"], a["p"], a["g"], 8)
+    assert L.ctr_fdk_error_slice(None, ctypes.byref(bad_t.c), vol.ctypes.data, out.ctypes.data, 0) == EINVAL
+    assert b"slice t" in L.ctr_last_error(None)
+    near = ctr.FdkErrGeom(8, 8, 8, a["size"], 20.0, a["beta"], a["u"], a["p"], a["g"], 3)
+    assert L.ctr_fdk_error_slice(None, ctypes.byref(near.c), vol.ctypes.data, out.ctypes.data, 0) == EINVAL
+    ok = ctr.FdkErrGeom(8, 8, 8, a["size"], a["D"], a["beta"], a["u"], a["p"], a["g"], 3)
+    assert L.ctr_fdk_error_slice(None, ctypes.byref(ok.c), vol.ctypes.data, out.ctypes.data, 0) == EINVAL
+    assert b"context is NULL" in L.ctr_last_error(None)
+    ell = np.array([[1.0, 0.5, 0.5, -0.1, 0, 0, 0, 0]])
+    assert L.ctr_phantom_ellipsoids(None, ell.ctypes.data, 1, 8, 8, 8, vol.ctypes.data, 0) == EINVAL
+    assert b"semi-axis" in L.ctr_last_error(None)
+    v = ctypes.c_double()
+    assert L.ctr_rrmse(None, vol.ctypes.data, vol.ctypes.data, 0, ctypes.byref(v), 0) == EINVAL
