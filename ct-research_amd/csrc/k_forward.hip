@@ -4,15 +4,15 @@
 // is 64 columns x 4 rows of one view, so a wavefront holds 64 neighbouring,
 // nearly parallel rays that walk neighbouring voxels (shared cache lines).
 //
-// Per ray the entry/exit of the volume box and the first crossing of each
-// axis are computed in FP64 (near-parallel rays need it).  The traversal
-// keeps the three next-crossing parameters in FP64 too (FP64 adds issue at
-// the FP32 rate on gfx950), advancing the axis whose plane comes first with
-// selects rather than branches, so a wavefront never splits three ways.  The
-// loop runs the FP64 setup's crossing count; segments are clipped at the exit
-// parameter, and the volume is read through a range-checked buffer
-// descriptor, so a rounding-level extra step past the exit reads zero with
-// zero weight.
+// Per ray the entry/exit of the volume box, the first crossing of each axis
+// and the crossing counts are computed in FP64 (near-parallel rays need it).
+// The traversal runs in FP32: the next crossing of each axis is formed as
+// first + count * step with one fma (no accumulated drift), and the axis
+// whose plane comes first advances with selects rather than branches, so a
+// wavefront never splits three ways.  The loop runs the FP64 setup's crossing
+// count and the last segment ends at the exit parameter.  The volume is read
+// through a range-checked buffer descriptor.  (The system-matrix walk keeps
+// FP64 parameters: it is an export, not a hot loop.)
 //
 // Geometry: ProjectionCone_3D.cu:26-42 (source, end point at 2D along the
 // ray through (p, xi) of the virtual isocentre detector); segment weight is
@@ -141,30 +141,52 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     // num_records is 32 bits: 2^30 voxels (4 GiB) saturate to 2^32 - 1, above every valid offset
     const size_t vbytes = (size_t)g.nt * g.ns * g.nz * sizeof(float);
     const unsigned nbytes = vbytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (unsigned)vbytes;
-    __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)(tr ? g.volT : vol), 0, (int)nbytes, kBufDword3);
-    int idx = w.idx0;
-    const double inf = __builtin_huge_val();
-    double nt = w.at.n > 0 ? w.at.next0 : inf;
-    double ns = w.as.n > 0 ? w.as.next0 : inf;
-    double nz = w.az.n > 0 ? w.az.next0 : inf;
-    double lam = 0.0;
-    float acc = 0.0f;
-    const int total = w.at.n + w.as.n + w.az.n + 1;
-    for (int k = 0; k < total; ++k) {
-        const double m = fmin(fmin(nt, ns), fmin(nz, w.lam_end));
-        const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (unsigned)idx * 4u, 0, 0));
-        acc = fmaf((float)(m - lam), v, acc);
-        lam = m;
-        // step the axis whose plane comes first (t before s before z on ties)
+    // base and size are the same for the whole block (one view of one
+    // problem); stating it keeps the descriptor in scalar registers where the
+    // compiler cannot prove it (the batched kernels read them from a job table)
+    // (readfirstlane returns int: each half goes through unsigned, so the low
+    // half is zero-extended, not sign-extended into the high one)
+    const unsigned long long base = (unsigned long long)(tr ? g.volT : vol);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+    const unsigned long long ubase = ((unsigned long long)hi << 32) | (unsigned long long)lo;
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)ubase, 0, (int)__builtin_amdgcn_readfirstlane(nbytes), kBufDword3);
+
+    // FP32 traversal of the FP64 set-up: the c-th crossing of an axis is
+    // first + c * step, one rounding each (an accumulated next += step drifts
+    // by the square root of the crossing count in ulps).  Parameters are
+    // relative to the entry point, so they stay below the chord length.
+    const float inf = __builtin_huge_valf();
+    const float t0 = w.at.n > 0 ? (float)w.at.next0 : inf, dt = (float)w.at.step;
+    const float s0 = w.as.n > 0 ? (float)w.as.next0 : inf, ds = (float)w.as.step;
+    const float z0 = w.az.n > 0 ? (float)w.az.next0 : inf, dz = (float)w.az.step;
+    // byte strides; negative ones wrap, as unsigned offsets do
+    const unsigned bt_st = (unsigned)w.at.stride * 4u, bs_st = (unsigned)w.as.stride * 4u,
+                   bz_st = (unsigned)w.az.stride * 4u;
+    float nt = t0, ns = s0, nz = z0, ct = 0.0f, cs = 0.0f, cz = 0.0f;   // crossings passed per axis
+    unsigned off = (unsigned)w.idx0 * 4u;
+    float lam = 0.0f, acc = 0.0f;
+    // every counted crossing ends a segment; the last segment ends at the exit
+    const int crossings = w.at.n + w.as.n + w.az.n;
+    for (int k = 0; k < crossings; ++k) {
+        // the axis whose plane comes first steps (t before s before z on ties)
         const bool bt = nt <= ns && nt <= nz;
         const bool bs = !bt && ns <= nz;
-        idx += bt ? w.at.stride : (bs ? w.as.stride : w.az.stride);
-        nt = bt ? nt + w.at.step : nt;
-        ns = bs ? ns + w.as.step : ns;
-        nz = (!bt && !bs) ? nz + w.az.step : nz;
+        const float m = bt ? nt : (bs ? ns : nz);
+        const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+        acc = fmaf(m - lam, v, acc);
+        lam = m;
+        off += bt ? bt_st : (bs ? bs_st : bz_st);
+        ct = bt ? ct + 1.0f : ct;
+        cs = bs ? cs + 1.0f : cs;
+        cz = (!bt && !bs) ? cz + 1.0f : cz;
+        nt = fmaf(ct, dt, t0);
+        ns = fmaf(cs, ds, s0);
+        nz = fmaf(cz, dz, z0);
     }
-    return acc;
+    const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+    return fmaf((float)w.lam_end - lam, v, acc);
 }
 
 // Siddon system-matrix row of one ray: the voxels the walk above visits with
