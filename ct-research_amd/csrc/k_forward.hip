@@ -124,39 +124,36 @@ __device__ __forceinline__ bool ray_walk(const RayParams &g, int ib, int ip, int
     return true;
 }
 
-// Line integral of one ray; also returns the chord length of the ray
-// through the volume box (the row sum A*1 of the Siddon system matrix).
-__device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const RayParams &g, int ib, int ip, int ix,
-                                           float &chord)
+// Minimum / maximum of an int over the wavefront; every lane must be active.
+__device__ __forceinline__ int wave_min_int(int v)
 {
-    // views whose central ray runs mostly along t read the [z][t][s] copy, so
-    // that the neighbouring rays of a wavefront (neighbouring p, offset in s)
-    // gather from neighbouring addresses
-    const bool tr = g.volT != nullptr && fabs(g.view[4 * ib + 0]) > fabs(g.view[4 * ib + 1]);
-    RayWalk w;
-    chord = 0.0f;
-    if (!ray_walk(g, ib, ip, ix, tr, w)) return 0.0f;
-    chord = (float)w.lam_end;
+    for (int m = 32; m > 0; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_int(int v)
+{
+    for (int m = 32; m > 0; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+    return v;
+}
 
-    // num_records is 32 bits: 2^30 voxels (4 GiB) saturate to 2^32 - 1, above every valid offset
-    const size_t vbytes = (size_t)g.nt * g.ns * g.nz * sizeof(float);
-    const unsigned nbytes = vbytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (unsigned)vbytes;
-    // base and size are the same for the whole block (one view of one
-    // problem); stating it keeps the descriptor in scalar registers where the
-    // compiler cannot prove it (the batched kernels read them from a job table)
-    // (readfirstlane returns int: each half goes through unsigned, so the low
-    // half is zero-extended, not sign-extended into the high one)
-    const unsigned long long base = (unsigned long long)(tr ? g.volT : vol);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
-    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
-    const unsigned long long ubase = ((unsigned long long)hi << 32) | (unsigned long long)lo;
-    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)ubase, 0, (int)__builtin_amdgcn_readfirstlane(nbytes), kBufDword3);
+// min(a, b) for finite or infinite operands as one v_med3 (no NaN quieting)
+__device__ __forceinline__ float min_nc(float a, float b)
+{
+    return __builtin_amdgcn_fmed3f(a, b, -__builtin_huge_valf());
+}
 
-    // FP32 traversal of the FP64 set-up: the c-th crossing of an axis is
-    // first + c * step, one rounding each (an accumulated next += step drifts
-    // by the square root of the crossing count in ulps).  Parameters are
-    // relative to the entry point, so they stay below the chord length.
+__device__ __forceinline__ float load_vox(__amdgpu_buffer_rsrc_t rs, unsigned off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+
+// One crossing at a time: the axis whose plane comes first advances (t
+// before s before z on ties).  The c-th crossing of an axis is first +
+// c * step, one rounding each (an accumulated next += step drifts by the
+// square root of the crossing count in ulps).  Parameters are relative to
+// the entry point, so they stay below the chord length.
+__device__ __forceinline__ float walk_crossings(__amdgpu_buffer_rsrc_t rs, const RayWalk &w)
+{
     const float inf = __builtin_huge_valf();
     const float t0 = w.at.n > 0 ? (float)w.at.next0 : inf, dt = (float)w.at.step;
     const float s0 = w.as.n > 0 ? (float)w.as.next0 : inf, ds = (float)w.as.step;
@@ -170,12 +167,10 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     // every counted crossing ends a segment; the last segment ends at the exit
     const int crossings = w.at.n + w.as.n + w.az.n;
     for (int k = 0; k < crossings; ++k) {
-        // the axis whose plane comes first steps (t before s before z on ties)
         const bool bt = nt <= ns && nt <= nz;
         const bool bs = !bt && ns <= nz;
         const float m = bt ? nt : (bs ? ns : nz);
-        const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-        acc = fmaf(m - lam, v, acc);
+        acc = fmaf(m - lam, load_vox(rs, off), acc);
         lam = m;
         off += bt ? bt_st : (bs ? bs_st : bz_st);
         ct = bt ? ct + 1.0f : ct;
@@ -185,8 +180,112 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
         ns = fmaf(cs, ds, s0);
         nz = fmaf(cz, dz, z0);
     }
-    const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-    return fmaf((float)w.lam_end - lam, v, acc);
+    return fmaf((float)w.lam_end - lam, load_vox(rs, off), acc);
+}
+
+// Line integral of one ray; also returns the chord length of the ray
+// through the volume box (the row sum A*1 of the Siddon system matrix).
+// Every lane of the wavefront must call it (valid false for lanes without a
+// ray): the wavefront decides together how it walks.
+//
+// A wavefront holds 64 neighbouring rays of one detector row.  When, for all
+// of them, the in-plane axis D that the view runs along has the shortest
+// crossing step (by 0.1%, so float rounding cannot put two crossings of
+// another axis between two D planes), the rays walk D's slabs in lockstep:
+// at iteration u every ray is in the same absolute slab, so the wavefront's
+// loads fall on a few neighbouring cache lines (one row of the layout whose
+// fastest axis is the other in-plane axis).  In a slab a ray crosses each
+// other axis at most once, so the slab is at most three segments: before
+// the first crossing, between the two, after the second.  The segments and
+// voxels are those of the one-crossing-at-a-time walk, which the other
+// wavefronts run.  No crossing within rounding of the exit is taken that the
+// FP64 set-up did not count, so no weighted load falls outside the volume
+// (where the two layouts differ).
+__device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const RayParams &g, int ib, int ip, int ix,
+                                           bool valid, float &chord)
+{
+    // views whose central ray runs mostly along t read the [z][t][s] copy, so
+    // that the neighbouring rays of a wavefront (neighbouring p, offset in s)
+    // gather from neighbouring addresses
+    const bool along_t = fabs(g.view[4 * ib + 0]) > fabs(g.view[4 * ib + 1]);
+    const bool tr = g.volT != nullptr && along_t;
+    RayWalk w;
+    const bool hit = valid && ray_walk(g, ib, ip, ix, tr, w);
+    chord = hit ? (float)w.lam_end : 0.0f;
+
+    // num_records is 32 bits: 2^30 voxels (4 GiB) saturate to 2^32 - 1, above every valid offset
+    const size_t vbytes = (size_t)g.nt * g.ns * g.nz * sizeof(float);
+    const unsigned nbytes = vbytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (unsigned)vbytes;
+    // base and size are the same for the whole block (one view of one
+    // problem); stating it keeps the descriptor in scalar registers where the
+    // compiler cannot prove it (the batched kernels read them from a job
+    // table).  readfirstlane returns int: each half goes through unsigned, so
+    // the low half is zero-extended, not sign-extended into the high one.
+    const unsigned long long base = (unsigned long long)(tr ? g.volT : vol);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+    const unsigned long long ubase = ((unsigned long long)hi << 32) | (unsigned long long)lo;
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)ubase, 0, (int)__builtin_amdgcn_readfirstlane(nbytes), kBufDword3);
+
+    // lockstep slabs: D is t or s by the view, A the other in-plane axis, B = z
+    const AxisSetup aD = along_t ? w.at : w.as;
+    const AxisSetup aA = along_t ? w.as : w.at;
+    const AxisSetup &aB = w.az;
+    bool ok = true;
+    if (hit) {
+        const double lim = 1.001 * aD.step;
+        ok = aD.stride != 0 && (aA.n == 0 || aA.step >= lim) && (aB.n == 0 || aB.step >= lim);
+    }
+    const unsigned long long bad = __ballot(hit && !ok);
+    const unsigned long long up = __ballot(hit && aD.stride > 0), down = __ballot(hit && aD.stride < 0);
+    if (bad != 0 || (up != 0 && down != 0)) return hit ? walk_crossings(rs, w) : 0.0f;
+
+    // wavefront-relative slab index u: absolute D index istart + dir * u
+    const int dir = up != 0 ? 1 : -1;
+    const int first = hit ? aD.i0 : (dir > 0 ? 0x7fffffff : -0x7fffffff);
+    const int istart = dir > 0 ? wave_min_int(first) : wave_max_int(first);
+    const int o = hit ? dir * (aD.i0 - istart) : 0;   // this ray's first slab, >= 0
+    const int uend = __builtin_amdgcn_readfirstlane(wave_max_int(hit ? o + aD.n : -1));   // uniform
+    const int nD = hit ? aD.n : -1;
+
+    const float inf = __builtin_huge_valf();
+    const float D0 = hit ? (float)aD.next0 : inf, dD = hit ? (float)aD.step : 0.0f;
+    const float A0 = hit && aA.n > 0 ? (float)aA.next0 : inf, dA = hit ? (float)aA.step : 0.0f;
+    const float B0 = hit && aB.n > 0 ? (float)aB.next0 : inf, dB = hit ? (float)aB.step : 0.0f;
+    const float lend = hit ? (float)w.lam_end : 0.0f;
+    // 8 ulps below the exit: the float crossings of the FP64-counted planes
+    // are within about 2 ulps of their exact values
+    const float lcnt = __builtin_bit_cast(float, max(__builtin_bit_cast(int, lend) - 8, 0));
+    const unsigned sD = (unsigned)aD.stride * 4u, sA = (unsigned)aA.stride * 4u, sB = (unsigned)aB.stride * 4u;
+    unsigned off = hit ? (unsigned)w.idx0 * 4u : 0u;
+    float pA = A0, pB = B0, cA = 0.0f, cB = 0.0f, kf = 0.0f, s = 0.0f, acc = 0.0f;
+    for (int u = 0; u <= uend; ++u) {
+        const int k = u - o;
+        if (k < 0 || k > nD) continue;
+        // slab k ends at the k-th D crossing, the last one at the exit
+        const float e = min_nc(fmaf(kf, dD, D0), lend);
+        // crossings count up to ec: one past the last counted crossing of an
+        // axis is at or beyond the exit, within rounding of it
+        const float ec = min_nc(e, lcnt);
+        const bool hasA = pA < ec, hasB = pB < ec;
+        // p1 <= p2: the crossings inside the slab in order, ec where there is none
+        const float p1 = min_nc(min_nc(pA, pB), ec);
+        const float p2 = min_nc(__builtin_amdgcn_fmed3f(pA, pB, ec), ec);
+        const unsigned offm = off + (pA <= pB ? sA : sB);
+        const unsigned off3 = off + (hasA ? sA : 0u) + (hasB ? sB : 0u);
+        acc = fmaf(p1 - s, load_vox(rs, off), acc);
+        acc = fmaf(p2 - p1, load_vox(rs, offm), acc);
+        acc = fmaf(e - p2, load_vox(rs, off3), acc);
+        cA = hasA ? cA + 1.0f : cA;
+        cB = hasB ? cB + 1.0f : cB;
+        pA = fmaf(cA, dA, A0);
+        pB = fmaf(cB, dB, B0);
+        off = off3 + sD;
+        s = e;
+        kf += 1.0f;
+    }
+    return hit ? acc : 0.0f;
 }
 
 // Siddon system-matrix row of one ray: the voxels the walk above visits with
@@ -252,10 +351,10 @@ __global__ __launch_bounds__(kBX *kBY) void k_ray_fp(const float *__restrict__ v
     const int ip = blockIdx.x * kBX + threadIdx.x;
     const int ix = blockIdx.y * kBY + threadIdx.y;
     const int ib = blockIdx.z;
-    if (ip >= g.np || ix >= g.nx) return;
+    const bool valid = ip < g.np && ix < g.nx;
     float chord;
-    const float acc = trace_ray(vol, g, ib, ip, ix, chord);
-    proj[((size_t)ib * g.nx + ix) * g.np + ip] = acc;
+    const float acc = trace_ray(vol, g, ib, ip, ix, valid, chord);
+    if (valid) proj[((size_t)ib * g.nx + ix) * g.np + ip] = acc;
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -275,9 +374,10 @@ __device__ __forceinline__ void fp_residual_block(const float *__restrict__ vol,
     const int ip = bx * kBX + threadIdx.x;
     const int ix = by * kBY + threadIdx.y;
     double e2 = 0.0, e1 = 0.0;
-    if (ip < g.np && ix < g.nx) {
-        float chord;
-        const float acc = trace_ray(vol, g, ib, ip, ix, chord);
+    const bool valid = ip < g.np && ix < g.nx;
+    float chord;
+    const float acc = trace_ray(vol, g, ib, ip, ix, valid, chord);
+    if (valid) {
         const size_t ray = ((size_t)ib * g.nx + ix) * g.np + ip;
         const float e = b[ray] - acc;
         r[ray] = e / (chord + 1e-9f);
