@@ -334,7 +334,8 @@ constexpr int kCMax = 6;   // LDS slack after each staged row and column table: 
 // Slab interval of one axis for a ray parameterised S + lam*dir: the voxel
 // [lo_rel, lo_rel + pitch] (relative to S) holds lam in [l1, l2] sorted;
 // inv = 1/dir, with inv == 0 marking a ray parallel to the slab (inside iff
-// the source coordinate lies in [lo, lo + pitch)).
+// the source coordinate lies in [lo, lo + pitch)): the whole line, or the
+// empty interval [inf, inf] (lo <= hi always).
 __device__ __forceinline__ void slab_interval(float lo_rel, float pitch, float inv, float &lo, float &hi)
 {
     if (inv != 0.0f) {
@@ -344,8 +345,18 @@ __device__ __forceinline__ void slab_interval(float lo_rel, float pitch, float i
     } else {
         const bool in = lo_rel <= 0.0f && 0.0f < lo_rel + pitch;
         lo = in ? -kInf : kInf;
-        hi = in ? kInf : -kInf;
+        hi = kInf;
     }
+}
+
+// |[zl, zh] n [lo, hi]| for lo <= hi finite and zl <= zh (either may be
+// infinite): the interval [zl, zh] clamped into [lo, hi].  Equal to
+// max(min(hi, zh) - max(lo, zl), 0) bit for bit, in three instructions: the
+// clamps are v_med3, which the compiler does not guard with NaN quieting of
+// operands from other blocks as it does fminf/fmaxf.
+__device__ __forceinline__ float overlap(float zl, float zh, float lo, float hi)
+{
+    return __builtin_amdgcn_fmed3f(zh, lo, hi) - __builtin_amdgcn_fmed3f(zl, lo, hi);
 }
 
 __device__ __forceinline__ float safe_rcp(float x) { return x != 0.0f ? 1.0f / x : 0.0f; }
@@ -478,9 +489,12 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                 float l0, h0, l1, h1;
                 slab_interval(tlo, P.rt, ia, l0, h0);
                 slab_interval(slo, P.rs, ib, l1, h1);
-                const bool valid = cb + q <= cu1;
-                clo[q] = valid ? fmaxf(l0, l1) : kInf;
-                chi[q] = valid ? fminf(h0, h1) : -kInf;
+                // the ray's interval in the voxel column; invalid or empty
+                // candidates become [0, 0], which overlaps nothing
+                const float lo_ = fmaxf(l0, l1), hi_ = fminf(h0, h1);
+                const bool live = cb + q <= cu1 && lo_ < hi_;
+                clo[q] = live ? lo_ : 0.0f;
+                chi[q] = live ? hi_ : 0.0f;
             }
 #pragma unroll
             for (int k = 0; k < TZ; ++k) {
@@ -498,14 +512,14 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                         const float *sy = yd + (r - f.y) * WP + (cb - f.x);
 #pragma unroll
                         for (int q = 0; q < CM; ++q) {
-                            const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
+                            const float len = overlap(zl, zh, clo[q], chi[q]);
                             a_k = fmaf(len, sy[q], a_k);
                         }
                     } else {
                         const float xi = P.xd[r], xi2 = fmaf(xi, xi, P.D * P.D);
 #pragma unroll
                         for (int q = 0; q < CM; ++q) {
-                            const float len = fmaxf(fminf(chi[q], zh) - fmaxf(clo[q], zl), 0.0f);
+                            const float len = overlap(zl, zh, clo[q], chi[q]);
                             if (len > 0.0f) {
                                 const int c = min(cb + q, P.np - 1);
                                 const float p = P.pd[c];
