@@ -23,7 +23,7 @@ namespace {
 
 thread_local std::string g_last_error;
 
-enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsT, kWsCount };
+enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsT, kWsYn, kWsCount };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -39,6 +39,7 @@ struct ConeTables {
     std::vector<float> h_pf, h_xf;
     std::vector<float> h_zq;        // [nz + 32][2]: zs^2, hz/|zs| per slice (zs = slice centre - source z)
     DevBuf d_view, d_p, d_xi, d_bp, d_pf, d_xf, d_zq;
+    DevBuf d_ainv, d_binv, d_xinv, d_norm;   // exact-adjoint tables, built on the device
 };
 
 struct FanTables {
@@ -355,6 +356,8 @@ uint64_t cone_key(const ctr_cone_geom *g)
     return h;
 }
 
+BpParams cone_bp_params_from(const ConeTables &T, const ctr_cone_geom *g, unsigned flags);
+
 ctr_status cone_tables_into(ctr_ctx *ctx, ConeTables &T, const ctr_cone_geom *g)
 {
     const uint64_t key = cone_key(g);
@@ -408,6 +411,15 @@ ctr_status cone_tables_into(ctr_ctx *ctx, ConeTables &T, const ctr_cone_geom *g)
     if ((s = upload(ctx, T.d_pf, T.h_pf.data(), sizeof(float) * T.h_pf.size())) != CTR_OK) return s;
     if ((s = upload(ctx, T.d_xf, T.h_xf.data(), sizeof(float) * T.h_xf.size())) != CTR_OK) return s;
     if ((s = upload(ctx, T.d_zq, T.h_zq.data(), sizeof(float) * T.h_zq.size())) != CTR_OK) return s;
+    const size_t nvc = (size_t)nb * g->n_p, nrc = (size_t)g->n_xi * g->n_p;
+    if ((s = buf_reserve(ctx, T.d_ainv, nvc * sizeof(float))) != CTR_OK) return s;
+    if ((s = buf_reserve(ctx, T.d_binv, nvc * sizeof(float))) != CTR_OK) return s;
+    if ((s = buf_reserve(ctx, T.d_xinv, (size_t)g->n_xi * sizeof(float))) != CTR_OK) return s;
+    if ((s = buf_reserve(ctx, T.d_norm, nrc * sizeof(float))) != CTR_OK) return s;
+    const BpParams P = cone_bp_params_from(T, g, 0);
+    if ((s = check_launch(ctx, launch_adjoint_tables(P, ctx->stream), "adjoint tables")) != CTR_OK) return s;
+    // later users may run on other streams (batched SIRT)
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     T.key = key;
     T.valid = true;
     return CTR_OK;
@@ -433,6 +445,7 @@ RayParams cone_ray_params_from(const ConeTables &T, const ctr_cone_geom *g)
     r.view = (const double *)T.d_view.ptr;
     r.p = (const double *)T.d_p.ptr;
     r.xi = (const double *)T.d_xi.ptr;
+    r.residual_norm = 1;
     return r;
 }
 
@@ -470,6 +483,10 @@ BpParams cone_bp_params_from(const ConeTables &T, const ctr_cone_geom *g, unsign
     P.zq = (const float2 *)T.d_zq.ptr;
     P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
     P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
+    P.adj_inva = (const float *)T.d_ainv.ptr;
+    P.adj_invb = (const float *)T.d_binv.ptr;
+    P.adj_invxi = (const float *)T.d_xinv.ptr;
+    P.adj_norm = (const float *)T.d_norm.ptr;
     return P;
 }
 
@@ -1307,8 +1324,12 @@ ctr_status ctr_cone_bp_adjoint(ctr_ctx *ctx, const ctr_cone_geom *g, const float
     TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvox3(g), &dv));
     BpParams P = cone_bp_params(ctx, g, flags);
     TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+    // the kernel reads its input times the ray norm
+    TRY(buf_reserve(ctx, ctx->ws[kWsYn], nray3(g) * sizeof(float)));
+    float *yn = (float *)ctx->ws[kWsYn].ptr;
+    TRY(check_launch(ctx, launch_scale_by_norm(P, dp, yn, ctx->stream), "adjoint input scaling"));
     TRY(check_launch(ctx,
-                     launch_adjoint(P, ctx->foot_adj.plan, (const int4 *)ctx->foot_adj.buf.ptr, dp, dv, nullptr, 0,
+                     launch_adjoint(P, ctx->foot_adj.plan, (const int4 *)ctx->foot_adj.buf.ptr, yn, dv, nullptr, 0,
                                     ctx->stream),
                      "cone adjoint"));
     return finish_out(ctx, dev, dv, vol, nvox3(g));

@@ -29,6 +29,8 @@ struct RayParams {
                             // central ray runs mostly along t (neighbouring rays then differ in s)
     int parallel;           // parallel beam: ray (view, p) is the line through centre + p (cos, sin)
                             // along (-sin, cos); view[4b+2..3] unused
+    int residual_norm;      // the SIRT residual pass stores r times the ray norm (the cone
+                            // adjoint's input form); the fan adjoint reads r itself
 };
 
 // Per-view constants of the voxel-driven backprojectors, float:
@@ -56,7 +58,20 @@ struct BpParams {
     const float2 *zq;       // [nz + 32]: (zs^2, hz/|zs|) per slice, zs = slice centre relative to the source
     int accumulate;
     int quirks;
+    // exact-adjoint tables (geometry only, built with the cone tables):
+    const float *adj_inva;  // [nb][np] 1/(p cos b + D sin b), 0 where that is 0
+    const float *adj_invb;  // [nb][np] 1/(p sin b - D cos b), 0 where that is 0
+    const float *adj_invxi; // [nx] 1/xi, 0 at xi = 0
+    const float *adj_norm;  // [nx][np] ray norm sqrt(p^2 + xi^2 + D^2); the adjoint reads its
+                            // input multiplied by it (this table itself for A'1)
 };
+
+// The ray norm of detector sample (p, xi), in the float operations that the
+// adjoint's input scaling uses everywhere (tables, residual, pre-pass).
+__host__ __device__ inline float ray_norm(float p, float xi, float D)
+{
+    return sqrtf(fmaf(p, p, fmaf(xi, xi, D * D)));
+}
 
 enum BpKind { kBpFdk = 0, kBpRef = 1 };
 
@@ -171,15 +186,23 @@ BpPlan adjoint_plan(const BpParams &P, int cm);      // cm: 3, 4 or 6 candidate 
 hipError_t launch_bp_footprints(const BpParams &P, const BpPlan &L, int4 *foot, hipStream_t st);
 hipError_t launch_bp_tiled(int kind, const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj,
                            float *vol, hipStream_t st);
+// proj is the adjoint's input already multiplied by the ray norm (P.adj_norm):
+// the SIRT residual kernels write it so, other callers scale with
+// launch_scale_by_norm first.  ones_input computes A'1 from P.adj_norm.
 hipError_t launch_adjoint(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
                           const SirtUpdate *upd, int ones_input, hipStream_t st);
+// Fills P.adj_inva, adj_invb, adj_invxi and adj_norm (the pointers name the
+// destinations; P's views, pd and xd must be on the device).
+hipError_t launch_adjoint_tables(const BpParams &P, hipStream_t st);
+// out[b][r][c] = in[b][r][c] * P.adj_norm[r][c] over all views.
+hipError_t launch_scale_by_norm(const BpParams &P, const float *in, float *out, hipStream_t st);
 
 // Batched exact adjoint (independent problems in one launch).  P.tiles_* are
 // the problem's plan; jobs live in device memory, sorted by block0.
 struct AdjointJob {
     BpParams P;
     const int4 *foot;
-    const float *proj;      // weighted residual (ignored when ones_input)
+    const float *proj;      // weighted residual times the ray norm (ignored when ones_input)
     float *vol;             // output (when not a SIRT update)
     SirtUpdate U;
     int block0, nblocks;    // this job's blocks in the launch

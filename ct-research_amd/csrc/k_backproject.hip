@@ -16,6 +16,7 @@
 // different XCDs round-robin, so tile = (bid % 8) * (nwg / 8) + bid / 8
 // gives each XCD a contiguous z-slab of tiles whose footprints share rows
 // of the detector in that XCD's L2.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 
@@ -441,26 +442,22 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
         const int W = f.z & 0xffff, H = f.z >> 16;
         const bool fits = (f.w & kFootFits) != 0;
         const BpView vw = views[b];
-        const float *src = proj + (size_t)b * P.nx * P.np;
+        // the input times the ray norm; A'1 reads the norm table for every view
+        const float *src = ones ? P.adj_norm : proj + (size_t)b * P.nx * P.np;
         if (fits) {
+            // plain copies of the footprint and of the view's geometry tables
             __syncthreads();
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
             for (int r = wave; r < H; r += kThreads / 64) {
-                const float xi = P.xd[f.y + r];
-                const float xi2 = fmaf(xi, xi, P.D * P.D);
                 const float *row = src + (size_t)(f.y + r) * P.np + f.x;
-                for (int c = lane; c < W; c += 64) {
-                    const float p = P.pd[f.x + c];
-                    const float nrm = sqrtf(fmaf(p, p, xi2));
-                    yd[r * WP + c] = ones ? nrm : row[c] * nrm;
-                }
+                for (int c = lane; c < W; c += 64) yd[r * WP + c] = row[c];
             }
+            const float *ta = P.adj_inva + (size_t)b * P.np + f.x, *tb = P.adj_invb + (size_t)b * P.np + f.x;
             for (int c = threadIdx.x; c < W; c += kThreads) {
-                const float p = P.pd[f.x + c];
-                inva[c] = safe_rcp(fmaf(p, vw.cb, P.D * vw.sb));
-                invb[c] = safe_rcp(fmaf(p, vw.sb, -P.D * vw.cb));
+                inva[c] = ta[c];
+                invb[c] = tb[c];
             }
-            for (int r = threadIdx.x; r < H; r += kThreads) invxi[r] = safe_rcp(P.xd[f.y + r]);
+            for (int r = threadIdx.x; r < H; r += kThreads) invxi[r] = P.adj_invxi[f.y + r];
             __syncthreads();
         }
         if (!col_in) continue;
@@ -482,9 +479,9 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                     ia = inva[cb - f.x + q];
                     ib = invb[cb - f.x + q];
                 } else {
-                    const float p = P.pd[min(cb + q, P.np - 1)];
-                    ia = safe_rcp(fmaf(p, vw.cb, P.D * vw.sb));
-                    ib = safe_rcp(fmaf(p, vw.sb, -P.D * vw.cb));
+                    const size_t c = (size_t)b * P.np + min(cb + q, P.np - 1);
+                    ia = P.adj_inva[c];
+                    ib = P.adj_invb[c];
                 }
                 float l0, h0, l1, h1;
                 slab_interval(tlo, P.rt, ia, l0, h0);
@@ -505,7 +502,7 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                 rv1 = min(rv1, rhi_lim);
                 float a_k = 0.0f;
                 for (int r = rv0; r <= rv1; ++r) {
-                    const float ix = fits ? invxi[r - f.y] : safe_rcp(P.xd[r]);
+                    const float ix = fits ? invxi[r - f.y] : P.adj_invxi[r];
                     float zl, zh;
                     slab_interval(zb, P.rz, ix, zl, zh);
                     if (fits) {
@@ -516,16 +513,10 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                             a_k = fmaf(len, sy[q], a_k);
                         }
                     } else {
-                        const float xi = P.xd[r], xi2 = fmaf(xi, xi, P.D * P.D);
 #pragma unroll
                         for (int q = 0; q < CM; ++q) {
                             const float len = overlap(zl, zh, clo[q], chi[q]);
-                            if (len > 0.0f) {
-                                const int c = min(cb + q, P.np - 1);
-                                const float p = P.pd[c];
-                                const float y = ones ? 1.0f : src[(size_t)r * P.np + c];
-                                a_k = fmaf(len, y * sqrtf(fmaf(p, p, xi2)), a_k);
-                            }
+                            if (len > 0.0f) a_k = fmaf(len, src[(size_t)r * P.np + min(cb + q, P.np - 1)], a_k);
                         }
                     }
                 }
@@ -578,6 +569,36 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
     const int local = (int)blockIdx.x - jobs[p].block0;
     adjoint_tile<TZ, WP, CAPF, CM>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
                                    xcd_tile(local, jobs[p].nblocks), lds);
+}
+
+// Geometry tables of the exact adjoint: per (view, column) the reciprocals
+// of the ray direction's t and s components, per row 1/xi, per (row, column)
+// the ray norm.  One thread per (view, column) and per (row, column).
+__global__ __launch_bounds__(256) void k_adjoint_tables(BpParams P, float *__restrict__ inva,
+                                                        float *__restrict__ invb, float *__restrict__ invxi,
+                                                        float *__restrict__ norm)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nvc = (long)P.nb * P.np, nrc = (long)P.nx * P.np;
+    if (e < nvc) {
+        const int b = (int)(e / P.np), c = (int)(e % P.np);
+        const BpView vw = P.views[b];
+        const float p = P.pd[c];
+        inva[e] = safe_rcp(fmaf(p, vw.cb, P.D * vw.sb));
+        invb[e] = safe_rcp(fmaf(p, vw.sb, -P.D * vw.cb));
+    }
+    if (e < nrc) {
+        const int r = (int)(e / P.np), c = (int)(e % P.np);
+        norm[e] = ray_norm(P.pd[c], P.xd[r], P.D);
+        if (c == 0) invxi[r] = safe_rcp(P.xd[r]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scale_by_norm(const float *__restrict__ in, const float *__restrict__ norm,
+                                                       float *__restrict__ out, long per_view, long n)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256)
+        out[e] = in[e] * norm[e % per_view];
 }
 
 constexpr int kBpTZ = 16;
@@ -753,6 +774,23 @@ hipError_t launch_adjoint_batched(int wp, int cm, const AdjointJob *jobs, int nj
     if (cm == 3) return launch_adjoint_batched_cm<3>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
     if (cm == 4) return launch_adjoint_batched_cm<4>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
     return launch_adjoint_batched_cm<6>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
+}
+
+hipError_t launch_adjoint_tables(const BpParams &P, hipStream_t st)
+{
+    const long n = std::max((long)P.nb * P.np, (long)P.nx * P.np);
+    hipLaunchKernelGGL(k_adjoint_tables, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P,
+                       const_cast<float *>(P.adj_inva), const_cast<float *>(P.adj_invb),
+                       const_cast<float *>(P.adj_invxi), const_cast<float *>(P.adj_norm));
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_by_norm(const BpParams &P, const float *in, float *out, hipStream_t st)
+{
+    const long per_view = (long)P.nx * P.np, n = per_view * P.nb;
+    const long blocks = std::min((n + 255) / 256, 65536L);
+    hipLaunchKernelGGL(k_scale_by_norm, dim3((unsigned)blocks), dim3(256), 0, st, in, P.adj_norm, out, per_view, n);
+    return hipGetLastError();
 }
 
 }  // namespace ctr

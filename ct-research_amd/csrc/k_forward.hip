@@ -365,7 +365,8 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // SIRT residual pass: r = (b - A x) / (A 1 + 1e-9) and per-block partial sums
 // of (b - A x)^2 and |b - A x| (SIRT3D.m:80-81, :87) for block (bx, by) of
-// view ib; the block's pair goes to partials[2 blk].
+// view ib; the block's pair goes to partials[2 blk].  For the cone beam r is
+// stored times the ray norm, the form in which the exact adjoint reads it.
 __device__ __forceinline__ void fp_residual_block(const float *__restrict__ vol, const float *__restrict__ b,
                                                   float *__restrict__ r, double *__restrict__ partials,
                                                   const RayParams &g, int bx, int by, int ib, size_t blk)
@@ -380,7 +381,8 @@ __device__ __forceinline__ void fp_residual_block(const float *__restrict__ vol,
     if (valid) {
         const size_t ray = ((size_t)ib * g.nx + ix) * g.np + ip;
         const float e = b[ray] - acc;
-        r[ray] = e / (chord + 1e-9f);
+        const float rr = e / (chord + 1e-9f);
+        r[ray] = g.residual_norm ? rr * ray_norm((float)g.p[ip], (float)g.xi[ix], (float)g.D) : rr;
         e2 = (double)e * (double)e;
         e1 = fabs((double)e);
     }
