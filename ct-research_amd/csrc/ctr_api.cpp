@@ -97,7 +97,7 @@ struct TapTable {
 // until ctr_autotune_cone measures them).
 struct Tuning {
     int bp_pitch = 80;      // row pitch of the FDK / BP_REF footprint buffers
-    int adj_cm = 4;         // candidate-column chunk of the exact adjoint
+    int adj_cm = 3;         // candidate-column chunk of the exact adjoint
     int fp_transpose = 1;   // forward projector reads the [z][t][s] copy for views along t
 };
 

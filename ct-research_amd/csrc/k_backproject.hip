@@ -542,12 +542,14 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
 
 // The per-pair work is short dependent chains (slab clip, LDS read, fma), so
 // the adjoint runs latency-bound at low occupancy.  Four z-slices per thread
-// at four waves per SIMD (128 VGPRs) measured fastest at 256^3 / 360 views:
-// 43.4 ms, against 56.6 ms for eight slices at the two waves that their 197
-// registers allow, 48 ms for eight slices forced to three waves (spills) and
-// 50 ms for four slices at three.
+// at four waves per SIMD measured 43.4 ms at 256^3 / 360 views, against
+// 56.6 ms for eight slices at the two waves that their 197 registers allow,
+// 48 ms for eight slices forced to three waves (spills) and 50 ms for four
+// slices at three.  With the staging reduced to copies, five waves (96
+// VGPRs, a few spills) and 3-column chunks are faster again: 30.2 ms against
+// 31.5 ms at four waves.
 template <int TZ, int WP, int CAPF, int CM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
                                                       const int4 *__restrict__ foot, const BpView *__restrict__ views,
                                                       SirtUpdate U, int sirt, int ones)
 {
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
 // Many independent problems in one launch: block b belongs to the job whose
 // block range holds it (a scalar scan of the small job table).
 template <int TZ, int WP, int CAPF, int CM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
                                                               int ones)
 {
     __shared__ float lds[CAPF];
