@@ -1732,6 +1732,7 @@ ctr_status ctr_fdk_error_slice(ctr_ctx *ctx, const ctr_fdkerr_geom *a, const flo
     if (!finite_pos(a->distance)) return fail(ctx, CTR_EINVAL, "distance must be positive and finite");
     if (a->n_beta < 2 || a->n_u < 2 || a->n_p < 2 || a->n_g < 1)
         return fail(ctx, CTR_EINVAL, "need at least 2 views, 2 rows U, 2 samples P and 1 tap");
+    if (a->n_g > 32767) return fail(ctx, CTR_ETOOBIG, "more than 32767 taps (tap indices are 16-bit)");
     if (a->t < 0 || a->t >= a->nt) return fail(ctx, CTR_EINVAL, "slice t = %d is outside [0, %d)", a->t, a->nt);
     if ((double)a->n_beta * a->n_u * a->n_p > (double)(1 << 30) || a->n_beta > 65535 || a->nz > 65535)
         return fail(ctx, CTR_ETOOBIG, "n_beta * n_u * n_p above 2^30, or more than 65535 views or slices");

@@ -14,7 +14,7 @@
 // the indices then depend only on IEEE-rounded operations.  The (view, U, P)
 // -> (x, y) voxel map does not depend on the output element and comes from a
 // table built by k_fdkerr_xy; the tap indices of each output are formed into
-// LDS once per chunk of 256 detector samples.
+// LDS once per chunk of 32 detector samples.
 #include "ctr_internal.h"
 
 #pragma clang fp contract(off)
@@ -23,7 +23,12 @@ namespace ctr {
 namespace {
 
 constexpr int kFeThreads = 64;
-constexpr int kFeChunk = 256;   // detector samples per LDS tap-index table (64 KiB)
+// detector samples per LDS tap-index table: 32 samples x 64 threads of
+// 16-bit indices is 4 KiB per one-wave block, so the 64 VGPRs, not LDS,
+// bound the occupancy (8 waves per SIMD).  A 256-sample table of ints
+// (64 KiB) left two waves per CU: 628 ms for the 128^3 / 360-view slice,
+// against 149 ms with 64-sample chunks and 112 ms with 32.
+constexpr int kFeChunk = 32;
 
 __global__ __launch_bounds__(256) void k_fdkerr_xy(FdkErrParams P, int *__restrict__ xy)
 {
@@ -60,7 +65,7 @@ __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const flo
                                                        const int *__restrict__ xy, const int2 *__restrict__ rng,
                                                        float *__restrict__ out)
 {
-    __shared__ int fil[kFeChunk * kFeThreads];   // tap index of (P, thread) for a chunk of P, -1 outside the taps
+    __shared__ short fil[kFeChunk * kFeThreads];   // tap index of (P, thread) for a chunk of P, -1 outside the taps
     const int s = blockIdx.x * kFeThreads + threadIdx.x;
     const int z = blockIdx.y, b = blockIdx.z;
     const float cb = P.cosb[b], sb = P.sinb[b];
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const flo
         const int pn = min(kFeChunk, P.np - p0);
         for (int k = 0; k < pn; ++k) {
             const double ff = floor(((double)dtl - P.p[p0 + k]) / P.pint) + P.np;
-            fil[k * kFeThreads + threadIdx.x] = (ff >= 0.0 && ff < (double)P.ng) ? (int)ff : -1;
+            fil[k * kFeThreads + threadIdx.x] = (short)((ff >= 0.0 && ff < (double)P.ng) ? (int)ff : -1);
         }
         if (s >= P.ns) continue;
         for (int iu = 0; iu < P.nu; ++iu) {

@@ -129,7 +129,7 @@ typedef struct ctr_fdkerr_geom {
     const float *beta;      /* [n_beta] view angles (rad), uniform */
     const float *u;         /* [n_u] integration rows U (mm) along the source axis, uniform */
     const double *p;        /* [n_p] detector samples (mm), uniform */
-    const double *g;        /* [n_g] ramp taps, centre at index n_p - 1 (IterativeFDK.m:303-312) */
+    const double *g;        /* [n_g] ramp taps, centre at index n_p - 1 (IterativeFDK.m:303-312); n_g <= 32767 */
     int t;                  /* voxel slice along t, 0-based */
 } ctr_fdkerr_geom;
 
