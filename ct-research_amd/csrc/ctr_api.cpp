@@ -66,6 +66,8 @@ struct CurvedTables {
     int fft_len = 0;
     std::vector<double> h_spec;            // [nx][fft_len] per-row spectrum / fft_len
     DevBuf d_view, d_rowp, d_beta, d_gam, d_spec;
+    uint64_t foot_key = 0;                 // backprojection tile footprints (per geometry and voxel offset)
+    DevBuf d_foot;
 };
 
 struct FdkErrTables {
@@ -862,6 +864,25 @@ CurvedBpParams curved_bp_params(ctr_ctx *ctx, const ctr_curved_geom *g, unsigned
     return P;
 }
 
+// Tile plan and per-(tile, view) footprints of the curved backprojection,
+// computed on the device once per geometry and voxel-centre convention.
+ctr_status curved_footprints(ctr_ctx *ctx, CurvedBpParams &P)
+{
+    CurvedTables &T = ctx->curved;
+    const size_t n = curved_plan(P);
+    uint64_t key = fnv(T.key, &P.off, sizeof P.off);
+    key = fnv(key, &P.tz, sizeof P.tz);
+    if (!(T.d_foot.ptr && T.foot_key == key)) {
+        ctr_status s = buf_reserve(ctx, T.d_foot, n * sizeof(int4));
+        if (s != CTR_OK) return s;
+        s = check_launch(ctx, launch_curved_footprints(P, (int4 *)T.d_foot.ptr, ctx->stream), "curved footprints");
+        if (s != CTR_OK) return s;
+        T.foot_key = key;
+    }
+    P.foot = (const int4 *)T.d_foot.ptr;
+    return CTR_OK;
+}
+
 // Half-scaled ramp taps times the pitch, two-sided: h2[k] = d * h[|k-(n-1)|].
 ctr_status tap_table(ctr_ctx *ctx, int filter, int equiangular, double d, int n)
 {
@@ -1587,7 +1608,8 @@ ctr_status ctr_curved_bp(ctr_ctx *ctx, const ctr_curved_geom *g, const float *fi
     float *dv;
     TRY(stage_in(ctx, dev, kWsIn, filtered, nrayc(g), &df));
     TRY(stage_out(ctx, dev, (flags & CTR_ACCUMULATE) != 0, kWsOut, vol, nvoxc(g), &dv));
-    const CurvedBpParams P = curved_bp_params(ctx, g, flags);
+    CurvedBpParams P = curved_bp_params(ctx, g, flags);
+    TRY(curved_footprints(ctx, P));
     TRY(check_launch(ctx, launch_curved_bp(P, df, dv, ctx->stream), "curved-detector backprojection"));
     return finish_out(ctx, dev, dv, vol, nvoxc(g));
 }
@@ -1605,7 +1627,8 @@ ctr_status ctr_curved_fdk(ctr_ctx *ctx, const ctr_curved_geom *g, const float *p
     TRY(buf_reserve(ctx, ctx->ws[kWsTmp], nrayc(g) * sizeof(float)));
     float *filt = (float *)ctx->ws[kWsTmp].ptr;
     TRY(curved_filter_impl(ctx, g, dp, filt));
-    const CurvedBpParams P = curved_bp_params(ctx, g, flags);
+    CurvedBpParams P = curved_bp_params(ctx, g, flags);
+    TRY(curved_footprints(ctx, P));
     TRY(check_launch(ctx, launch_curved_bp(P, filt, dv, ctx->stream), "curved-detector backprojection"));
     if (dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));   // the filtered workspace is reused by the next call
     return finish_out(ctx, dev, dv, vol, nvoxc(g));

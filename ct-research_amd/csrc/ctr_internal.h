@@ -138,7 +138,15 @@ struct CurvedBpParams {
     float dbeta;
     const float *view;            // [nb][2] sin, cos
     int accumulate;
+    // tiles of 16 x 16 columns x tz slices and their per-view detector
+    // footprints (curved_plan, launch_curved_footprints)
+    int tz, tiles_t, tiles_s, tiles_z;
+    const int4 *foot;             // [tiles][nb]: (channel0, row0, W | H << 16, flags)
 };
+// Tile plan of the curved backprojection: fills P.tz and P.tiles_*; returns
+// the number of footprint entries (tiles x views).
+size_t curved_plan(CurvedBpParams &P);
+hipError_t launch_curved_footprints(const CurvedBpParams &P, int4 *foot, hipStream_t st);
 hipError_t launch_curved_bp(const CurvedBpParams &P, const float *filt, float *vol, hipStream_t st);
 
 // Fan-beam pixel-driven kernels.
