@@ -171,6 +171,10 @@ def test_error_slice_and_helpers_reject_bad_arguments_without_a_context():
     assert b"slice t" in L.ctr_last_error(None)
     near = ctr.FdkErrGeom(8, 8, 8, a["size"], 20.0, a["beta"], a["u"], a["p"], a["g"], 3)
     assert L.ctr_fdk_error_slice(None, ctypes.byref(near.c), vol.ctypes.data, out.ctypes.data, 0) == EINVAL
+    # tap indices are 16-bit on the device
+    many = ctr.FdkErrGeom(8, 8, 8, a["size"], a["D"], a["beta"], a["u"], a["p"], np.zeros(32768), 3)
+    assert L.ctr_fdk_error_slice(None, ctypes.byref(many.c), vol.ctypes.data, out.ctypes.data, 0) == ctr._lib.ETOOBIG
+    assert b"32767 taps" in L.ctr_last_error(None)
     ok = ctr.FdkErrGeom(8, 8, 8, a["size"], a["D"], a["beta"], a["u"], a["p"], a["g"], 3)
     assert L.ctr_fdk_error_slice(None, ctypes.byref(ok.c), vol.ctypes.data, out.ctypes.data, 0) == EINVAL
     assert b"context is NULL" in L.ctr_last_error(None)
