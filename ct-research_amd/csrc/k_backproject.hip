@@ -545,11 +545,12 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
 // at four waves per SIMD measured 43.4 ms at 256^3 / 360 views, against
 // 56.6 ms for eight slices at the two waves that their 197 registers allow,
 // 48 ms for eight slices forced to three waves (spills) and 50 ms for four
-// slices at three.  With the staging reduced to copies, five waves (96
-// VGPRs, a few spills) and 3-column chunks are faster again: 30.2 ms against
-// 31.5 ms at four waves.
+// slices at three.  With the staging reduced to copies and 3-column chunks:
+// four slices at five waves (96 VGPRs, a few spills) 31.2 ms, six slices at
+// four waves (128 VGPRs) 30.0 ms, eight at three 33.0 ms and eight at four
+// (spills) 31.9 ms, all with the input scaling pass.
 template <int TZ, int WP, int CAPF, int CM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
                                                       const int4 *__restrict__ foot, const BpView *__restrict__ views,
                                                       SirtUpdate U, int sirt, int ones)
 {
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))
 // Many independent problems in one launch: block b belongs to the job whose
 // block range holds it (a scalar scan of the small job table).
 template <int TZ, int WP, int CAPF, int CM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
                                                               int ones)
 {
     __shared__ float lds[CAPF];
@@ -606,7 +607,7 @@ __global__ __launch_bounds__(256) void k_scale_by_norm(const float *__restrict__
 constexpr int kBpTZ = 16;
 // (row pitch, rows) of the two FDK/BP_REF footprint buffers, 2 x 32 KB at most
 constexpr int kBpShapes[3][2] = {{64, 64}, {72, 56}, {80, 50}};
-constexpr int kAdjTZ = 4, kAdjCap = 6144;
+constexpr int kAdjTZ = 6, kAdjCap = 6144;
 
 }  // namespace
 
