@@ -50,6 +50,27 @@ def test_curved_bp(ctx, oracle, quirks):
     assert e < 2e-5 and m < 2e-4
 
 
+def test_curved_bp_footprints_beyond_lds(ctx, oracle):
+    """Channels 0.0005 rad and rows 0.3 mm apart: a tile's footprint spans
+    several hundred channels and more rows than an LDS buffer holds, so the
+    backprojection reads global memory for those tiles."""
+    import ctr
+    D, Dsd = 120.0, 240.0
+    gamma = 0.0005 * (np.arange(701) - 350.2)
+    xi = 0.3 * (np.arange(60) - 29.5)
+    fan = abs(gamma[-1] - gamma[0])
+    beta = 0.05 * np.arange(int(np.ceil((np.pi + fan) / 0.05)) + 1)
+    args = (24, 20, 8, (24.0, 20.0, 8.0), D, Dsd, beta, gamma, xi, 0.0, fan)
+    g, og = ctr.CurvedGeom(*args), oracle.Curved(*args)
+    y = np.random.default_rng(23).random(g.nray).astype(np.float32)
+    gpu = ctx.curved_bp(g, y)
+    with oracle.interp_mode(1):
+        ref = oracle.curved_bp(og, y)
+    e, m = errs(gpu, ref)
+    report("curved_bp[beyond LDS]", rel_l2=e, max_rel=m)
+    assert e < 2e-5 and m < 2e-4
+
+
 def test_curved_fdk_matches_oracle_and_accumulates(ctx, oracle):
     import ctr
     g, og, (D, Dsd, beta, gamma, xi) = curved_case(oracle, 0.027)
