@@ -119,6 +119,10 @@ struct ctr_ctx {
     PhantomTable phantom;
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
+    // SIRT's V = 1/(A'1 + 1e-9) depends on the geometry only: ws[kWsVinv] keeps
+    // it for the operator with this key, while the buffer is unchanged
+    uint64_t vinv_key = 0;
+    const void *vinv_ptr = nullptr;
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
     std::vector<ctr_cnn *> nets;   // networks created on this context (released with it)
     std::map<uint64_t, Tuning> tune;   // autotuned choices per cone geometry
@@ -1854,10 +1858,25 @@ ctr_status sirt_opts_check(ctr_ctx *ctx, const ctr_sirt_opts *o)
 
 // Shared SIRT driver: fp_resid(x, r, partials, &nblk) and adj_update(r, upd)
 // close over the geometry.
+// V = 1/(A'1 + 1e-9) into ws[kWsVinv] (reserved by the caller), unless it
+// already holds V of the operator `key` (0: always recompute)
+template <typename AdjOnes>
+ctr_status sirt_norm(ctr_ctx *ctx, uint64_t key, size_t nvox, hipStream_t st, AdjOnes adj_ones)
+{
+    float *vinv = (float *)ctx->ws[kWsVinv].ptr;
+    if (key != 0 && ctx->vinv_key == key && ctx->vinv_ptr == (const void *)vinv) return CTR_OK;
+    ctx->vinv_key = 0;
+    TRY(adj_ones(vinv));
+    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
+    ctx->vinv_key = key;
+    ctx->vinv_ptr = vinv;
+    return CTR_OK;
+}
+
 template <typename FpResid, typename AdjOnes, typename AdjUpdate>
 ctr_status sirt_loop(ctr_ctx *ctx, size_t nvox, size_t nray, const float *b_in, float *x_in,
                             const ctr_sirt_opts *o, unsigned flags, int nblk_fp, FpResid fp_resid, AdjOnes adj_ones,
-                            AdjUpdate adj_update)
+                            AdjUpdate adj_update, uint64_t vkey)
 {
     const bool dev = flags & CTR_DEVICE_PTRS;
     const float *b;
@@ -1873,8 +1892,7 @@ ctr_status sirt_loop(ctr_ctx *ctx, size_t nvox, size_t nray, const float *b_in, 
     double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
     if (!o->use_x0) TRY(check_launch(ctx, launch_fill(x, 0.0f, nvox, ctx->stream), "fill"));
     // V = 1/(A'1 + 1e-9): column sums from the adjoint of ones
-    TRY(adj_ones(vinv));
-    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, ctx->stream), "reciprocal"));
+    TRY(sirt_norm(ctx, vkey, nvox, ctx->stream, adj_ones));
     // ||b||^2 and sum(b) for the residual normalisations (slot n_iter+1)
     int nb2 = 0;
     TRY(check_launch(ctx, launch_sum_sq_sum(b, nray, part, &nb2, ctx->stream), "norm"));
@@ -1932,7 +1950,8 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
     auto upd = [&](const float *rr, SirtUpdate *U) {
         return check_launch(ctx, launch_adjoint(P, AL, AF, rr, nullptr, U, 0, ctx->stream), "SIRT update");
     };
-    return sirt_loop(ctx, nvox3(g), nray3(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd);
+    return sirt_loop(ctx, nvox3(g), nray3(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd,
+                     fnv(ctx->foot_adj.key, "cone", 4));
 }
 
 ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
@@ -1955,7 +1974,8 @@ ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float
     auto upd = [&](const float *rr, SirtUpdate *U) {
         return check_launch(ctx, launch_fan_adjoint(P, rr, nullptr, U, 0, ctx->stream), "SIRT update");
     };
-    return sirt_loop(ctx, nvox2(g), nray2(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd);
+    return sirt_loop(ctx, nvox2(g), nray2(g), b, x, opts, flags, ray_fp_blocks(R), fp, ones, upd,
+                     fnv(ctx->fan.key, "fan", 3));
 }
 
 }  // extern "C"
@@ -2299,8 +2319,9 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_
     TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
     float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr, *r = (float *)ctx->ws[kWsR].ptr;
     double *part = (double *)ctx->ws[kWsPart].ptr;
-    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, st), "SIRT column sums"));
-    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
+    TRY(sirt_norm(ctx, fnv(ctx->foot_adj.key, "cone", 4), nvox, st, [&](float *v) {
+        return check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, v, nullptr, 1, st), "SIRT column sums");
+    }));
     int nblk = 0;
     RayParams Rx = R;
     TRY(fp_transposed(ctx, Rx, x, ctx->ws[kWsT], st, tuning_for(ctx, ctx->cone.key).fp_transpose));
@@ -2347,8 +2368,9 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     float *r_spare = o->judge_vs_plain ? (float *)ctx->ws[kWsR2].ptr : nullptr;
     double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
     if (!o->use_x0) TRY(check_launch(ctx, launch_fill(x, 0.0f, nvox, st), "fill"));
-    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, vinv, nullptr, 1, st), "SIRT column sums"));
-    TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
+    TRY(sirt_norm(ctx, fnv(ctx->foot_adj.key, "cone", 4), nvox, st, [&](float *v) {
+        return check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, v, nullptr, 1, st), "SIRT column sums");
+    }));
     // reduction of a vector to (sum of squares, sum) read back to the host
     auto host_sums = [&](double *out2) -> ctr_status {
         HIP_TRY(ctx, hipMemcpyAsync(out2, sums, 2 * sizeof(double), hipMemcpyDeviceToHost, st));

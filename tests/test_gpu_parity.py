@@ -289,6 +289,37 @@ def test_sirt3d(ctx, oracle):
     assert x.min() >= 0.0
 
 
+def test_sirt3d_normalisation_reuse():
+    """V = 1/(A'1 + 1e-9) is kept per geometry between calls: a repeated call,
+    a call after another geometry and the SIRT step all match a fresh context."""
+    import ctr
+    d, beta, p, xi = cone_case("base")
+    g1 = ctr.ConeGeom(24, 24, 12, (24.0, 24.0, 12.0), 120.0, beta[::2], p, xi)
+    g2 = ctr.ConeGeom(20, 28, 10, (20.0, 28.0, 10.0), 110.0, beta[1::2], p, xi)
+    rng = np.random.default_rng(31)
+    b1 = rng.random(g1.nray).astype(np.float32)
+    b2 = rng.random(g2.nray).astype(np.float32)
+    x0 = rng.random(g1.nvox).astype(np.float32)
+    fresh = ctr.Context(0)
+    ref1, _, _ = fresh.sirt3d(g1, b1, 4)
+    ref2, _, _ = fresh.sirt3d(g2, b2, 4)
+    fresh.close()
+    fresh = ctr.Context(0)
+    ref_step = fresh.sirt3d_step(g1, b1, x0)
+    fresh.close()
+    c = ctr.Context(0)
+    try:
+        a, _, _ = c.sirt3d(g1, b1, 4)
+        again, _, _ = c.sirt3d(g1, b1, 4)
+        other, _, _ = c.sirt3d(g2, b2, 4)
+        back, _, _ = c.sirt3d(g1, b1, 4)
+        step = c.sirt3d_step(g1, b1, x0)
+    finally:
+        c.close()
+    for got, ref in ((a, ref1), (again, ref1), (other, ref2), (back, ref1), (step, ref_step)):
+        assert np.array_equal(got, ref)
+
+
 def test_sirt2d(ctx, oracle):
     g, og = make_fan(oracle)
     img = np.zeros((g.ns, g.nt), np.float32)
