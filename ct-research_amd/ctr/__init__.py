@@ -8,7 +8,7 @@ from ._lib import (CNN_LAYERS, CNN_NUM_PARAMS, Cnn, cnn_init_params, cnn_split_p
 from ._lib import (ACCUMULATE, AUTOTUNE, DEVICE_PTRS, FAN_EQUIANGULAR, FAN_FLAT, FILTER_RL_HALF, FILTER_SL_HALF,
                    REF_QUIRKS, ConeGeom, Context, CtrError, CurvedGeom, FanGeom, FdkErrGeom, ParGeom, default_context, lib, version, LIB_PATH,
                    EXPORTS)
-from . import geometry, metrics, phantoms, uidata
+from . import checkpoint, geometry, metrics, phantoms, uidata
 from .mex import (FDK, Backprojection, Backprojection3D, BackprojectionCone_3D, BackprojectionFan2D,
                   BackprojectionFan_2D, FBPfan, FBPparallel, ProjectionCone_3D, ProjectionFan_2D,
                   ProjectionParallel_2D, SIRT3D, ImplementUI,
