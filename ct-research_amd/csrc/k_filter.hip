@@ -106,12 +106,15 @@ __global__ __launch_bounds__(64 *RW) void k_row_filter(const float *__restrict__
 // one complex transform pair, z' = IDFT(DFT(z) H) = y0 + i y1, computed as
 // conj(DFT(conj(DFT(z)) H / L)).  FP64 throughout; H / L comes from the host.
 //
-// Transforms are Stockham radix-16, radix-16, radix-(L/256) passes over
-// G = L/16 threads that each hold 16 points; after the last pass thread j
-// holds X[j + G t] (t = 0..15), which is exactly the input distribution of
-// the first pass, so the spectrum multiply and the second transform follow
-// in registers.  LDS holds one padded transform (index i at i + i/16) per
-// group of G threads.
+// Transforms are Stockham passes of radix 8, then one of radix 2 or 4 when
+// L is not a power of 8, over G = L/8 threads that each hold 8 points.  After
+// the last pass thread j holds X[j + G t] (t = 0..7), which is exactly the
+// input distribution of the first pass, so the spectrum multiply and the
+// second transform follow in registers.  LDS holds one padded transform
+// (index i at i + i/8) per group of G threads.  With 8 points per thread the
+// kernel needs about 100 registers, so four workgroups of 256 threads share
+// a CU (a radix-16 design held 16 points per thread and ran at one or two
+// waves per SIMD).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -138,24 +141,7 @@ __device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, doub
     a3 = csub(t1, t3);
 }
 
-constexpr double kC8 = 0.92387953251128675613;   // cos(pi/8)
-constexpr double kS8 = 0.38268343236508977173;   // sin(pi/8)
 constexpr double kR2 = 0.70710678118654752440;   // sqrt(1/2)
-
-// exp(-2 pi i m / 16) for the products m = n1 * k1 of the 4 x 4 split
-__device__ __forceinline__ double2 w16(int m)
-{
-    switch (m & 15) {
-    case 0: return make_double2(1.0, 0.0);
-    case 1: return make_double2(kC8, -kS8);
-    case 2: return make_double2(kR2, -kR2);
-    case 3: return make_double2(kS8, -kC8);
-    case 4: return make_double2(0.0, -1.0);
-    case 6: return make_double2(-kR2, -kR2);
-    case 9: return make_double2(-kC8, kS8);
-    default: return make_double2(0.0, 0.0);   // not used by dft16
-    }
-}
 
 __device__ __forceinline__ void dft8(double2 *v)
 {
@@ -176,41 +162,7 @@ __device__ __forceinline__ void dft8(double2 *v)
     v[7] = csub(e3, o3);
 }
 
-__device__ __forceinline__ void dft16(double2 *v)
-{
-    // x[n1 + 4 n2]: 4-point DFTs over n2, twiddle w16^(n1 k1), 4-point DFTs over n1 -> X[k1 + 4 k2]
-    double2 a[4][4];
-#pragma unroll
-    for (int n1 = 0; n1 < 4; ++n1) {
-        a[n1][0] = v[n1];
-        a[n1][1] = v[n1 + 4];
-        a[n1][2] = v[n1 + 8];
-        a[n1][3] = v[n1 + 12];
-        dft4(a[n1][0], a[n1][1], a[n1][2], a[n1][3]);
-    }
-#pragma unroll
-    for (int n1 = 1; n1 < 4; ++n1)
-#pragma unroll
-        for (int k1 = 1; k1 < 4; ++k1) a[n1][k1] = n1 * k1 == 4 ? mul_mi(a[n1][k1]) : cmul(a[n1][k1], w16(n1 * k1));
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        dft4(a[0][k1], a[1][k1], a[2][k1], a[3][k1]);
-        v[k1] = a[0][k1];
-        v[k1 + 4] = a[1][k1];
-        v[k1 + 8] = a[2][k1];
-        v[k1 + 12] = a[3][k1];
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void dft_small(double2 *u)
-{
-    if constexpr (R == 2) dft2(u[0], u[1]);
-    else if constexpr (R == 4) dft4(u[0], u[1], u[2], u[3]);
-    else dft8(u);
-}
-
-__device__ __forceinline__ int fft_pad(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int fft_pad(int i) { return i + (i >> 3); }
 
 // w = exp(-2 pi i num / den)
 __device__ __forceinline__ double2 twiddle(int num, int den)
@@ -220,83 +172,117 @@ __device__ __forceinline__ double2 twiddle(int num, int den)
     return make_double2(c, s);
 }
 
-constexpr int kFftThreads = 128;
+constexpr int kFftThreads = 256;
 
-// Per-thread twiddle bases; the powers w^r are formed by recurrence in each
-// pass (relative error below 16 ulp), which keeps the registers for the data.
+// Pass structure of a length-L transform: radix-8 passes with strides
+// Ns = 1, 8 (, 64), then a radix-RL pass (RL = 1: none) with Ns = 8^NP8.
 template <int L>
-struct FftTwiddles {
-    static constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1;
-    double2 b;          // pass 2: w256^(j % 16)
-    double2 c[NBF];     // pass 3: wL^jj, jj = j + G m
+struct FftPlan {
+    static constexpr int G = L / 8;                           // threads per transform
+    static constexpr int NP8 = L == 256 ? 2 : 3;              // radix-8 passes
+    static constexpr int RL = L / (NP8 == 2 ? 64 : 512);      // 4, 1, 2, 4 for L = 256 .. 2048
+    static constexpr int NBF = RL > 1 ? 8 / RL : 1;           // last-pass butterflies per thread
 };
 
-// One forward transform of the 16 points v[t] = x[j + G t] of this thread's
+// Per-thread twiddle bases, computed once per thread; the powers w^r are
+// formed by recurrence in each pass (relative error below 8 ulp).
+template <int L>
+struct FftTwiddles {
+    double2 p2, p3;                     // radix-8 passes 2 and 3: w^(j mod Ns) of Ns R = 64, 512
+    double2 c[FftPlan<L>::NBF];         // last pass: w_L^(j + G m)
+};
+
+template <int L>
+__device__ __forceinline__ FftTwiddles<L> fft_twiddles(int j)
+{
+    using F = FftPlan<L>;
+    FftTwiddles<L> tw;
+    tw.p2 = twiddle(j & 7, 64);
+    tw.p3 = twiddle(j & 63, 512);
+#pragma unroll
+    for (int m = 0; m < F::NBF; ++m) tw.c[m] = twiddle(j + F::G * m, L);
+    return tw;
+}
+
+// v[r] *= w^r for r = 1..n-1
+__device__ __forceinline__ void twiddle_powers(double2 *v, int n, double2 w0)
+{
+    double2 w = w0;
+    for (int r = 1; r < n; ++r) {
+        v[r] = cmul(v[r], w);
+        if (r + 1 < n) w = cmul(w, w0);
+    }
+}
+
+// Radix-8 pass with stride Ns on v[r] = in[j + G r]; the result goes to the
+// group's LDS slot at (j / Ns) 8 Ns + j mod Ns + r Ns, and the next pass's
+// input v[r] = out[j + G r] is read back.
+template <int L>
+__device__ __forceinline__ void fft8_pass(double2 *v, double2 *buf, int j, int Ns)
+{
+    constexpr int G = FftPlan<L>::G;
+    dft8(v);
+    const int k = j & (Ns - 1);
+    const int d = (j - k) * 8 + k;
+    __syncthreads();   // the group's previous readers of buf are done
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[fft_pad(d + r * Ns)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = buf[fft_pad(j + G * r)];
+}
+
+// One forward transform of the 8 points v[t] = x[j + G t] of this thread's
 // group; on return v[t] = X[j + G t].  buf is the group's padded LDS slot.
 template <int L>
 __device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const FftTwiddles<L> &tw)
 {
-    constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1;
-    // pass 1: radix 16, Ns = 1 -> X1[16 j + r]
-    dft16(v);
-    __syncthreads();   // the group's previous readers of buf are done
-#pragma unroll
-    for (int r = 0; r < 16; ++r) buf[fft_pad(16 * j + r)] = v[r];
-    __syncthreads();
-    // pass 2: radix 16, Ns = 16
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = buf[fft_pad(j + G * r)];
-    {
-        double2 w = tw.b;
-#pragma unroll
-        for (int r = 1; r < 16; ++r) {
-            v[r] = cmul(v[r], w);
-            if (r < 15) w = cmul(w, tw.b);
+    using F = FftPlan<L>;
+    fft8_pass<L>(v, buf, j, 1);
+    if constexpr (F::NP8 == 2) {
+        // L = 256: radix 8 (Ns = 8), then radix 4 (Ns = 64)
+        twiddle_powers(v, 8, tw.p2);
+        fft8_pass<L>(v, buf, j, 8);
+    } else {
+        twiddle_powers(v, 8, tw.p2);
+        fft8_pass<L>(v, buf, j, 8);
+        twiddle_powers(v, 8, tw.p3);
+        if constexpr (F::RL == 1) {
+            // L = 512: the third radix-8 pass is the last; thread j's outputs are
+            // X[j + 64 r] = X[j + G r], already in place
+            dft8(v);
+        } else {
+            fft8_pass<L>(v, buf, j, 64);
         }
     }
-    dft16(v);
-    if constexpr (RL > 1) {
-        __syncthreads();   // every thread has read its pass-2 inputs
-        const int d0 = (j >> 4) * 256 + (j & 15);
+    if constexpr (F::RL > 1) {
+        // butterflies b = j + G m (m < NBF) read X'[b + r L/RL] = v[m + NBF r] and
+        // write X[b + r L/RL] to the same registers
 #pragma unroll
-        for (int r = 0; r < 16; ++r) buf[fft_pad(d0 + 16 * r)] = v[r];
-        __syncthreads();
-        // pass 3: radix RL, Ns = 256, butterflies jj = j + G m -> X[jj + 256 r] = X[j + G (m + NBF r)]
+        for (int m = 0; m < F::NBF; ++m) {
+            double2 u[F::RL];
 #pragma unroll
-        for (int m = 0; m < NBF; ++m) {
-            const int jj = j + G * m;
-            double2 u[RL];
+            for (int r = 0; r < F::RL; ++r) u[r] = v[m + F::NBF * r];
+            twiddle_powers(u, F::RL, tw.c[m]);
+            if constexpr (F::RL == 2) dft2(u[0], u[1]);
+            else dft4(u[0], u[1], u[2], u[3]);
 #pragma unroll
-            for (int r = 0; r < RL; ++r) u[r] = buf[fft_pad(jj + 256 * r)];
-            double2 w = tw.c[m];
-#pragma unroll
-            for (int r = 1; r < RL; ++r) {
-                u[r] = cmul(u[r], w);
-                if (r + 1 < RL) w = cmul(w, tw.c[m]);
-            }
-            dft_small<RL>(u);
-#pragma unroll
-            for (int r = 0; r < RL; ++r) v[m + NBF * r] = u[r];
+            for (int r = 0; r < F::RL; ++r) v[m + F::NBF * r] = u[r];
         }
     }
-    // L == 256: thread j < 16 holds X[j + 16 r] = v[r] after pass 2 already
 }
 
 template <int L>
 __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, float *__restrict__ out,
                                                 const FilterParams &P)
 {
-    constexpr int G = L / 16, RL = L / 256, NBF = RL > 1 ? 16 / RL : 1, GPB = kFftThreads / G;
-    constexpr int SLOT = L + L / 16;
+    constexpr int G = FftPlan<L>::G, GPB = kFftThreads / G;
+    constexpr int SLOT = L + L / 8;
     __shared__ double2 lds[GPB * SLOT];
     const int grp = threadIdx.x / G, j = threadIdx.x % G;
     double2 *buf = lds + grp * SLOT;
     const int np = P.np;
-
-    FftTwiddles<L> tw;
-    tw.b = twiddle(j & 15, 256);
-#pragma unroll
-    for (int m = 0; m < NBF; ++m) tw.c[m] = twiddle(j + G * m, L);
+    const FftTwiddles<L> tw = fft_twiddles<L>(j);
 
     // with one spectrum per detector row, a pair is one row index in two consecutive views
     const int nviews = P.rows / P.nx;
@@ -314,9 +300,11 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
         const double bp0 = P.mode == 3 && has0 ? P.betad[row0 / P.nx] * prow : 0.0;
         const double bp1 = P.mode == 3 && has1 ? P.betad[row1 / P.nx] * prow : 0.0;
         const float *src0 = in + (size_t)row0 * np, *src1 = in + (size_t)row1 * np;
-        double2 v[16];
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
+        // the weighted samples go through LDS: a rolled loop keeps the
+        // weight arithmetic of one sample live at a time
+        __syncthreads();   // the group's previous readers of buf are done
+#pragma unroll 1
+        for (int t = 0; t < 8; ++t) {
             const int q = j + G * t;
             double a = 0.0, b = 0.0;
             if (q < np) {
@@ -347,17 +335,21 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
                     if (has1) b = (double)(src1[q] * w1);
                 }
             }
-            v[t] = make_double2(a, b);
+            buf[fft_pad(q)] = make_double2(a, b);
         }
+        __syncthreads();
+        double2 v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[t] = buf[fft_pad(j + G * t)];
         fft_group<L>(v, buf, j, tw);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
+        for (int t = 0; t < 8; ++t) {
             const double h = spec[j + G * t];   // H[k] / L, real
             v[t] = make_double2(v[t].x * h, -v[t].y * h);
         }
         fft_group<L>(v, buf, j, tw);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
+        for (int t = 0; t < 8; ++t) {
             const int q = j + G * t;
             if (q < np) {
                 if (has0) out[(size_t)row0 * np + q] = (float)v[t].x;
@@ -374,31 +366,14 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restr
     fft_filter_body<L>(in, out, P);
 }
 
-// the same with the registers capped for two waves per SIMD (some spilled)
-template <int L>
-__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fft_filter_occ2(
-    const float *__restrict__ in, float *__restrict__ out, FilterParams P)
-{
-    fft_filter_body<L>(in, out, P);
-}
-
 template <int L>
 hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStream_t st)
 {
-    constexpr int GPB = kFftThreads / (L / 16);
+    constexpr int GPB = kFftThreads / FftPlan<L>::G;
     const int npairs = P.spec_stride ? P.nx * ((P.rows / P.nx + 1) / 2) : (P.rows + 1) / 2;
     const int need = (npairs + GPB - 1) / GPB;
     const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
-    // two waves per SIMD (a few spilled registers) beat one for L <= 1024: 1.40 -> 1.11 ms
-    // for 360 x 512 rows of 512; at L = 2048 the cap spills too much.  CTR_FFT_OCC=1 forces one.
-    static const int occ = [] {
-        const char *e = std::getenv("CTR_FFT_OCC");
-        return e ? std::atoi(e) : 2;
-    }();
-    if (occ == 2 && L <= 1024)
-        hipLaunchKernelGGL((k_fft_filter_occ2<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
-    else
-        hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+    hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
     return hipGetLastError();
 }
 
