@@ -147,6 +147,19 @@ __device__ __forceinline__ void issue_view(const float *__restrict__ proj, const
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, buf + r * WP, 4, voff, (unsigned)((f.y + r) * P.np) * 4u, 0, 0);
 }
 
+// Direct global -> LDS copy of n consecutive floats, elements first ..
+// first + n - 1 of a buffer (its row selected by the scalar byte offset
+// soff), into dst[0 .. n): 64 per instruction, the last chunk under an EXEC
+// mask, so nothing past dst[n - 1] is written.  The caller keeps
+// first + n - 1 inside the buffer.
+__device__ __forceinline__ void dma_span(__amdgpu_buffer_rsrc_t rs, float *dst, unsigned soff, int first, int n,
+                                         int lane)
+{
+    for (int c0 = 0; c0 < n; c0 += 64)
+        if (lane < n - c0)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + c0, 4, (unsigned)(first + c0 + lane) * 4u, soff, 0, 0);
+}
+
 // Voxel-driven backprojection of all views in one pass.  LDS holds two
 // footprint buffers (HMAX rows of pitch WP floats, 64 staged columns per
 // row): while the tile interpolates view b from one buffer, the DMA engine
@@ -466,19 +479,29 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
         // the input times the ray norm; A'1 reads the norm table for every view
         const float *src = ones ? P.adj_norm : proj + (size_t)b * P.nx * P.np;
         if (fits) {
-            // plain copies of the footprint and of the view's geometry tables
+            // direct global -> LDS copies of the footprint rows and of the
+            // view's geometry tables; every wave's copies are complete after
+            // its vmcnt wait, and the barrier publishes them to the block
             __syncthreads();
-            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            for (int r = wave; r < H; r += kThreads / 64) {
-                const float *row = src + (size_t)(f.y + r) * P.np + f.x;
-                for (int c = lane; c < W; c += 64) yd[r * WP + c] = row[c];
+            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+            const unsigned vbytes = (unsigned)((size_t)P.nx * P.np * sizeof(float));
+            const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc((void *)src, 0, (int)vbytes, kBufDword3);
+            for (int r = wave; r < H; r += kThreads / 64)
+                dma_span(ry, yd + r * WP, (unsigned)((f.y + r) * P.np) * 4u, f.x, W, lane);
+            if (wave == 0) {
+                const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(P.adj_inva + (size_t)b * P.np), 0, P.np * (int)sizeof(float), kBufDword3);
+                dma_span(ra, inva, 0u, f.x, W, lane);
+            } else if (wave == 1) {
+                const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(P.adj_invb + (size_t)b * P.np), 0, P.np * (int)sizeof(float), kBufDword3);
+                dma_span(rb, invb, 0u, f.x, W, lane);
+            } else if (wave == 2) {
+                const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)P.adj_invxi, 0, P.nx * (int)sizeof(float), kBufDword3);
+                dma_span(rx, invxi, 0u, f.y, H, lane);
             }
-            const float *ta = P.adj_inva + (size_t)b * P.np + f.x, *tb = P.adj_invb + (size_t)b * P.np + f.x;
-            for (int c = threadIdx.x; c < W; c += kThreads) {
-                inva[c] = ta[c];
-                invb[c] = tb[c];
-            }
-            for (int r = threadIdx.x; r < H; r += kThreads) invxi[r] = P.adj_invxi[f.y + r];
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
         if (!col_in) continue;
