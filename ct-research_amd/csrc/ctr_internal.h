@@ -275,13 +275,19 @@ struct FdkErrParams {
     const double *g;            // [ng] ramp taps
     const double *cp;           // [np] sqrt(1 + (P/D)^2) D / sqrt(D^2 + P^2) dU PInt dbeta
 };
-// xy: int scratch of launch_fdk_error_slice_scratch(P) bytes (the voxel map
-// [nb][nu][np], then the per-(view, U) sample intervals)
+// xy: scratch of launch_fdk_error_slice_scratch(P) bytes: the voxel map
+// [nb][nu][np] (int), the per-(view, U) sample intervals (int2), then a copy
+// of the volume with z fastest (float [ns][nt][nz])
 hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *xy, float *out, hipStream_t st);
-inline size_t launch_fdk_error_slice_scratch(const FdkErrParams &P)
+constexpr int kFdkErrRowPad = 8;   // ints readable past the intervals (the kernel's unrolled map reads)
+inline size_t fdk_error_slice_zcopy_offset(const FdkErrParams &P)   // in ints, a multiple of 64 (256 bytes)
 {
     const size_t n = (size_t)P.nb * P.nu * P.np;
-    return (((n + 1) & ~(size_t)1) + 2 * (size_t)P.nb * P.nu) * sizeof(int);
+    return (((n + 1) & ~(size_t)1) + 2 * (size_t)P.nb * P.nu + kFdkErrRowPad + 63) & ~(size_t)63;
+}
+inline size_t launch_fdk_error_slice_scratch(const FdkErrParams &P)
+{
+    return (fdk_error_slice_zcopy_offset(P) + (size_t)P.nt * P.ns * P.nz) * sizeof(int);
 }
 
 // Ellipsoid of the phantom rasterizer: amplitude, semi-axes, rotated centre

@@ -14,7 +14,10 @@
 // the indices then depend only on IEEE-rounded operations.  The (view, U, P)
 // -> (x, y) voxel map does not depend on the output element and comes from a
 // table built by k_fdkerr_xy; the tap indices of each output are formed into
-// LDS once per chunk of 32 detector samples.
+// LDS once per chunk of 32 detector samples.  The volume is read from a copy
+// with z fastest: the lanes of a wave share the voxel (x, y) of a sample and
+// differ in their moved slice, so their gather falls into a few cache lines
+// of one z column instead of one line per slice.
 #include "ctr_internal.h"
 
 #pragma clang fp contract(off)
@@ -29,6 +32,8 @@ constexpr int kFeThreads = 64;
 // (64 KiB) left two waves per CU: 628 ms for the 128^3 / 360-view slice,
 // against 149 ms with 64-sample chunks and 112 ms with 32.
 constexpr int kFeChunk = 32;
+constexpr int kFeUnroll = 4;   // detector samples per step of the inner loop
+static_assert(kFeUnroll - 1 <= kFdkErrRowPad, "the unrolled map reads stay inside the scratch buffer");
 
 __global__ __launch_bounds__(256) void k_fdkerr_xy(FdkErrParams P, int *__restrict__ xy)
 {
@@ -61,7 +66,18 @@ __global__ __launch_bounds__(256) void k_fdkerr_range(FdkErrParams P, const int 
     rng[i] = lo <= hi ? make_int2(lo, hi) : make_int2(1, 0);
 }
 
-__global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const float *__restrict__ pic,
+// pz[(y nt + x) nz + z] = pic[(z ns + y) nt + x]
+__global__ __launch_bounds__(256) void k_fdkerr_zcopy(FdkErrParams P, const float *__restrict__ pic,
+                                                      float *__restrict__ pz)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t plane = (size_t)P.nt * P.ns;
+    if (i >= plane * P.nz) return;
+    const size_t z = i % P.nz, o = i / P.nz;
+    pz[i] = pic[z * plane + o];
+}
+
+__global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const float *__restrict__ pz,
                                                        const int *__restrict__ xy, const int2 *__restrict__ rng,
                                                        float *__restrict__ out)
 {
@@ -77,8 +93,7 @@ __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const flo
     const double D = P.D;
     const float LR = (float)(D / (D - (double)dect_s));
     const float dtl = dect_t * LR;
-    const size_t plane = (size_t)P.nt * P.ns;
-    const float *slice = pic + (size_t)z * plane;
+    const float *slice = pz + z;   // voxel o of slice z at slice[o * nz]
     const double half_lr2 = 0.5 * ((double)LR * LR);
     double acc = 0.0;
     // each thread reads only its own column of the table: no barrier needed
@@ -99,15 +114,41 @@ __global__ __launch_bounds__(kFeThreads) void k_fdkerr(FdkErrParams P, const flo
             const float Zd = (float)((double)dect_z + (double)num / (D - (double)dect_s));
             const double fz = floor(((double)Zd + P.cz) / P.r2);
             if (!(fz >= 0.0 && fz < (double)P.nz)) continue;
-            const float *moved = pic + (size_t)fz * plane;
+            const float *moved = pz + (size_t)fz;
             const int *row = xy + ((size_t)b * P.nu + iu) * P.np + p0;
             const double *cp = P.cp + p0;
-            for (int k = k0; k < k1; ++k) {
-                const int f = fil[k * kFeThreads + threadIdx.x];
-                const int o = row[k];
-                if (f >= 0 && o >= 0) {
-                    const float diff = moved[o] - slice[o];
-                    acc += (double)diff * half_lr2 * (P.g[f] * cp[k]);
+            // four samples per step, their gathers all in flight before the
+            // arithmetic: indices are clamped into range (a sample outside the
+            // image or the taps, or past k1, reads element 0) and such terms
+            // are not added.  row[k1 .. k1 + kFeUnroll - 2] may lie past the
+            // map's last row; the sample intervals and kFdkErrRowPad ints of
+            // padding follow it in the same buffer, and those values are never
+            // used as indices.
+            for (int k = k0; k < k1; k += kFeUnroll) {
+                int o[kFeUnroll], f[kFeUnroll];
+                bool ok[kFeUnroll];
+#pragma unroll
+                for (int q = 0; q < kFeUnroll; ++q) {
+                    const bool in = k + q < k1;
+                    o[q] = row[k + q];
+                    f[q] = fil[min(k + q, k1 - 1) * kFeThreads + threadIdx.x];
+                    o[q] = in ? o[q] : -1;
+                    ok[q] = in && f[q] >= 0 && o[q] >= 0;
+                }
+                float mv[kFeUnroll], sv[kFeUnroll];
+                double gv[kFeUnroll];
+#pragma unroll
+                for (int q = 0; q < kFeUnroll; ++q) {
+                    const size_t oz = (size_t)max(o[q], 0) * P.nz;
+                    mv[q] = moved[oz];
+                    sv[q] = slice[oz];
+                    gv[q] = P.g[max(f[q], 0)];
+                }
+#pragma unroll
+                for (int q = 0; q < kFeUnroll; ++q) {
+                    const float diff = mv[q] - sv[q];
+                    const double term = (double)diff * half_lr2 * (gv[q] * cp[min(k + q, k1 - 1)]);
+                    acc = ok[q] ? acc + term : acc;
                 }
             }
         }
@@ -128,8 +169,12 @@ hipError_t launch_fdk_error_slice(const FdkErrParams &P, const float *pic, int *
     const int nbu = P.nb * P.nu;
     hipLaunchKernelGGL(k_fdkerr_range, dim3((nbu + 255) / 256), dim3(256), 0, st, P, xy, rng);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    float *pz = reinterpret_cast<float *>(xy + fdk_error_slice_zcopy_offset(P));
+    const size_t nvox = (size_t)P.nt * P.ns * P.nz;
+    hipLaunchKernelGGL(k_fdkerr_zcopy, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st, P, pic, pz);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     const dim3 grid((P.ns + kFeThreads - 1) / kFeThreads, P.nz, P.nb);
-    hipLaunchKernelGGL(k_fdkerr, grid, dim3(kFeThreads), 0, st, P, pic, xy, rng, out);
+    hipLaunchKernelGGL(k_fdkerr, grid, dim3(kFeThreads), 0, st, P, pz, xy, rng, out);
     return hipGetLastError();
 }
 
