@@ -452,6 +452,11 @@ RayParams cone_ray_params_from(const ConeTables &T, const ctr_cone_geom *g)
     r.p = (const double *)T.d_p.ptr;
     r.xi = (const double *)T.d_xi.ptr;
     r.residual_norm = 1;
+    // masked second/third voxel reads once the volume exceeds the 256 MiB
+    // Infinity Cache (measured: 512^3 301 -> 270 ms, 256^3 19.3 -> 20.4 ms);
+    // CTR_FP_SKIP=0/1 overrides
+    r.skip_empty = (size_t)g->nt * g->ns * g->nz * sizeof(float) > ((size_t)256 << 20);
+    if (const char *e = std::getenv("CTR_FP_SKIP")) r.skip_empty = std::atoi(e) != 0;
     return r;
 }
 

@@ -31,6 +31,9 @@ struct RayParams {
                             // along (-sin, cos); view[4b+2..3] unused
     int residual_norm;      // the SIRT residual pass stores r times the ray norm (the cone
                             // adjoint's input form); the fan adjoint reads r itself
+    int skip_empty;         // a slab's second and third voxels are read only by lanes whose
+                            // segment in them is not empty (pays once the volume outgrows the
+                            // Infinity Cache; costs instructions below that)
 };
 
 // Per-view constants of the voxel-driven backprojectors, float:

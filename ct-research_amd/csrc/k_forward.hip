@@ -201,6 +201,7 @@ __device__ __forceinline__ float walk_crossings(__amdgpu_buffer_rsrc_t rs, const
 // wavefronts run.  No crossing within rounding of the exit is taken that the
 // FP64 set-up did not count, so no weighted load falls outside the volume
 // (where the two layouts differ).
+template <bool kSkip>
 __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const RayParams &g, int ib, int ip, int ix,
                                            bool valid, float &chord)
 {
@@ -274,9 +275,21 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
         const float p2 = min_nc(__builtin_amdgcn_fmed3f(pA, pB, ec), ec);
         const unsigned offm = off + (pA <= pB ? sA : sB);
         const unsigned off3 = off + (hasA ? sA : 0u) + (hasB ? sB : 0u);
+        const float w2 = p2 - p1, w3 = e - p2;
         acc = fmaf(p1 - s, load_vox(rs, off), acc);
-        acc = fmaf(p2 - p1, load_vox(rs, offm), acc);
-        acc = fmaf(e - p2, load_vox(rs, off3), acc);
+        if (kSkip) {
+            // the second and third voxels are read only by lanes whose
+            // segment in them is not empty (a crossing inside the slab); a
+            // skipped term is 0 * v, so the sum is unchanged bit for bit
+            float v2 = 0.0f, v3 = 0.0f;
+            if (w2 != 0.0f) v2 = load_vox(rs, offm);
+            if (w3 != 0.0f) v3 = load_vox(rs, off3);
+            acc = fmaf(w2, v2, acc);
+            acc = fmaf(w3, v3, acc);
+        } else {
+            acc = fmaf(w2, load_vox(rs, offm), acc);
+            acc = fmaf(w3, load_vox(rs, off3), acc);
+        }
         cA = hasA ? cA + 1.0f : cA;
         cB = hasB ? cB + 1.0f : cB;
         pA = fmaf(cA, dA, A0);
@@ -353,7 +366,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_ray_fp(const float *__restrict__ v
     const int ib = blockIdx.z;
     const bool valid = ip < g.np && ix < g.nx;
     float chord;
-    const float acc = trace_ray(vol, g, ib, ip, ix, valid, chord);
+    const float acc = g.skip_empty ? trace_ray<true>(vol, g, ib, ip, ix, valid, chord)
+                                  : trace_ray<false>(vol, g, ib, ip, ix, valid, chord);
     if (valid) proj[((size_t)ib * g.nx + ix) * g.np + ip] = acc;
 }
 
@@ -377,7 +391,8 @@ __device__ __forceinline__ void fp_residual_block(const float *__restrict__ vol,
     double e2 = 0.0, e1 = 0.0;
     const bool valid = ip < g.np && ix < g.nx;
     float chord;
-    const float acc = trace_ray(vol, g, ib, ip, ix, valid, chord);
+    const float acc = g.skip_empty ? trace_ray<true>(vol, g, ib, ip, ix, valid, chord)
+                                  : trace_ray<false>(vol, g, ib, ip, ix, valid, chord);
     if (valid) {
         const size_t ray = ((size_t)ib * g.nx + ix) * g.np + ip;
         const float e = b[ray] - acc;

@@ -24,6 +24,11 @@ def gpu_available():
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch carries its own HIP runtime: initialise it on the device before the
+    # library's runtime creates its streams, so that the tests which hand torch
+    # tensors to the library find torch's side ready whatever ran before them
+    import torch
+    torch.cuda.init()
     import ctr
     c = ctr.Context(0)
     yield c
