@@ -136,10 +136,19 @@ __device__ __forceinline__ int wave_max_int(int v)
     return v;
 }
 
-// min(a, b) for finite or infinite operands as one v_med3 (no NaN quieting)
-__device__ __forceinline__ float min_nc(float a, float b)
+// min(a, b) for finite or infinite operands as one v_med3 (no NaN quieting).
+// ninf is -inf held in a register the compiler cannot see through
+// (opaque_ninf): with a literal -inf it rewrites the med3 as an IEEE min and
+// re-canonicalises every operand defined outside the loop on every slab.
+__device__ __forceinline__ float min_nc(float a, float b, float ninf)
 {
-    return __builtin_amdgcn_fmed3f(a, b, -__builtin_huge_valf());
+    return __builtin_amdgcn_fmed3f(a, b, ninf);
+}
+__device__ __forceinline__ float opaque_ninf()
+{
+    float v;
+    asm volatile("v_mov_b32 %0, 0xff800000" : "=v"(v));
+    return v;
 }
 
 __device__ __forceinline__ float load_vox(__amdgpu_buffer_rsrc_t rs, unsigned off)
@@ -246,9 +255,11 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     const int dir = up != 0 ? 1 : -1;
     const int first = hit ? aD.i0 : (dir > 0 ? 0x7fffffff : -0x7fffffff);
     const int istart = dir > 0 ? wave_min_int(first) : wave_max_int(first);
-    const int o = hit ? dir * (aD.i0 - istart) : 0;   // this ray's first slab, >= 0
+    // this ray's first slab, >= 0; a lane without a hit starts 2^30 slabs
+    // late, so its k = u - o never enters [0, nD]
+    const int o = hit ? dir * (aD.i0 - istart) : (1 << 30);
     const int uend = __builtin_amdgcn_readfirstlane(wave_max_int(hit ? o + aD.n : -1));   // uniform
-    const int nD = hit ? aD.n : -1;
+    const int nD = hit ? aD.n : 0;
 
     const float inf = __builtin_huge_valf();
     const float D0 = hit ? (float)aD.next0 : inf, dD = hit ? (float)aD.step : 0.0f;
@@ -261,18 +272,19 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
     const unsigned sD = (unsigned)aD.stride * 4u, sA = (unsigned)aA.stride * 4u, sB = (unsigned)aB.stride * 4u;
     unsigned off = hit ? (unsigned)w.idx0 * 4u : 0u;
     float pA = A0, pB = B0, cA = 0.0f, cB = 0.0f, kf = 0.0f, s = 0.0f, acc = 0.0f;
+    const float ninf = opaque_ninf();
     for (int u = 0; u <= uend; ++u) {
         const int k = u - o;
-        if (k < 0 || k > nD) continue;
+        if ((unsigned)k > (unsigned)nD) continue;   // k outside [0, nD]
         // slab k ends at the k-th D crossing, the last one at the exit
-        const float e = min_nc(fmaf(kf, dD, D0), lend);
+        const float e = min_nc(fmaf(kf, dD, D0), lend, ninf);
         // crossings count up to ec: one past the last counted crossing of an
         // axis is at or beyond the exit, within rounding of it
-        const float ec = min_nc(e, lcnt);
+        const float ec = min_nc(e, lcnt, ninf);
         const bool hasA = pA < ec, hasB = pB < ec;
         // p1 <= p2: the crossings inside the slab in order, ec where there is none
-        const float p1 = min_nc(min_nc(pA, pB), ec);
-        const float p2 = min_nc(__builtin_amdgcn_fmed3f(pA, pB, ec), ec);
+        const float p1 = min_nc(min_nc(pA, pB, ninf), ec, ninf);
+        const float p2 = min_nc(__builtin_amdgcn_fmed3f(pA, pB, ec), ec, ninf);
         const unsigned offm = off + (pA <= pB ? sA : sB);
         const unsigned off3 = off + (hasA ? sA : 0u) + (hasB ? sB : 0u);
         const float w2 = p2 - p1, w3 = e - p2;
