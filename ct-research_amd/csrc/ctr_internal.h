@@ -222,6 +222,21 @@ hipError_t launch_adjoint_batched(int wp, int cm, const AdjointJob *jobs, int nj
                                   int ones_input, hipStream_t st);
 
 // Batched SIRT residual pass: r = (b - A x)/(A1 + 1e-9) and per-block (sum e^2, sum |e|) per job.
+// The job of flat block `bid` in a batched launch: the last job whose first
+// block is <= bid (block0 ascending, jobs[0].block0 == 0), by binary search
+// over uniform values (scalar loads).
+template <typename Job>
+__device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int njobs, int bid)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bid >= jobs[mid].block0) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct FpJob {
     RayParams g;
     const float *vol;

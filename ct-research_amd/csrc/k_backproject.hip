@@ -609,8 +609,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
                                                               int ones)
 {
     __shared__ float lds[CAPF];
-    int p = 0;
-    while (p + 1 < njobs && (int)blockIdx.x >= jobs[p + 1].block0) ++p;
+    const int p = find_job(jobs, njobs, (int)blockIdx.x);
     const BpParams P = jobs[p].P;
     const SirtUpdate U = jobs[p].U;
     const int local = (int)blockIdx.x - jobs[p].block0;

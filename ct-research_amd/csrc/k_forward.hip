@@ -442,8 +442,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual(const float *__res
 // Independent problems in one launch: a flat grid over the jobs' block ranges.
 __global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual_batched(const FpJob *__restrict__ jobs, int njobs)
 {
-    int p = 0;
-    while (p + 1 < njobs && (int)blockIdx.x >= jobs[p + 1].block0) ++p;
+    const int p = find_job(jobs, njobs, (int)blockIdx.x);
     const RayParams g = jobs[p].g;
     const int local = (int)blockIdx.x - jobs[p].block0, gx = jobs[p].gx, gy = jobs[p].gy;
     const int bx = local % gx, by = (local / gx) % gy, ib = local / (gx * gy);
