@@ -22,6 +22,10 @@ using namespace ctr;
 namespace {
 
 thread_local std::string g_last_error;
+// Failures on this thread, counted: a context's message is the thread's
+// latest only while no later failure (e.g. a geometry check that runs
+// before the context is looked at) happened on the thread.
+thread_local unsigned long long g_error_seq = 0;
 
 enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsT, kWsYn, kWsCount };
 
@@ -110,6 +114,7 @@ struct ctr_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::string last_error;
+    unsigned long long error_seq = 0;   // g_error_seq of the failure that set last_error
     DevBuf ws[kWsCount];
     ConeTables cone;
     FanTables fan;
@@ -120,9 +125,9 @@ struct ctr_ctx {
     TapTable taps;
     FootCache foot_bp, foot_adj;   // footprint tables of the interpolating BP and of the exact adjoint
     // SIRT's V = 1/(A'1 + 1e-9) depends on the geometry only: ws[kWsVinv] keeps
-    // it for the operator with this key, while the buffer is unchanged
+    // it for the operator with this key; reserve_vinv clears the key whenever
+    // the buffer may be reallocated, and sirt_norm sets it once V is written
     uint64_t vinv_key = 0;
-    const void *vinv_ptr = nullptr;
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
     std::vector<ctr_cnn *> nets;   // networks created on this context (released with it)
     std::map<uint64_t, Tuning> tune;   // autotuned choices per cone geometry
@@ -141,7 +146,11 @@ ctr_status fail(ctr_ctx *ctx, ctr_status st, const char *fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (ctx) ctx->last_error = buf;
+    ++g_error_seq;
+    if (ctx) {
+        ctx->last_error = buf;
+        ctx->error_seq = g_error_seq;
+    }
     g_last_error = buf;
     return st;
 }
@@ -182,6 +191,15 @@ ctr_status buf_reserve(ctr_ctx *ctx, DevBuf &b, size_t bytes)
     b.ptr = p;
     b.bytes = bytes;
     return CTR_OK;
+}
+
+// The workspace of SIRT's V: a reallocation (even one that fails, or that
+// returns the old address) invalidates the V kept in it.
+ctr_status reserve_vinv(ctr_ctx *ctx, size_t bytes)
+{
+    DevBuf &b = ctx->ws[kWsVinv];
+    if (!(b.ptr && bytes <= b.bytes)) ctx->vinv_key = 0;
+    return buf_reserve(ctx, b, bytes);
 }
 
 void buf_free(DevBuf &b)
@@ -489,9 +507,15 @@ BpParams cone_bp_params_from(const ConeTables &T, const ctr_cone_geom *g, unsign
     P.hs = 0.5f * P.rs;
     P.hz = 0.5f * P.rz;
     P.views = (const BpView *)T.d_bp.ptr;
+    P.view_d = (const double *)T.d_view.ptr;
+    P.pdd = (const double *)T.d_p.ptr;
     P.pd = (const float *)T.d_pf.ptr;
     P.xd = (const float *)T.d_xf.ptr;
     P.zq = (const float2 *)T.d_zq.ptr;
+    {
+        const char *e = getenv("CTR_BP_ROWSIN");
+        P.rows_in = (e && !strcmp(e, "0")) ? 0 : 1;
+    }
     P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
     P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
     P.adj_inva = (const float *)T.d_ainv.ptr;
@@ -664,6 +688,7 @@ FanParams fan_params(ctr_ctx *ctx, const ctr_fan_geom *g, unsigned flags)
     P.dbeta = g->n_beta >= 2 ? (float)std::fabs(g->beta[1] - g->beta[0]) : 0.0f;
     P.view = (const double *)T.d_view.ptr;
     P.pf = (const float *)T.d_pf.ptr;
+    P.pdbl = (const double *)T.d_p.ptr;
     P.gam = (const float *)T.d_gam.ptr;
     P.accumulate = (flags & CTR_ACCUMULATE) ? 1 : 0;
     P.quirks = (flags & CTR_REF_QUIRKS) ? 1 : 0;
@@ -1194,7 +1219,13 @@ const char *ctr_status_string(int status)
     }
 }
 
-const char *ctr_last_error(const ctr_ctx *ctx) { return ctx ? ctx->last_error.c_str() : g_last_error.c_str(); }
+const char *ctr_last_error(const ctr_ctx *ctx)
+{
+    // a failure raised before the call reached its context (argument and
+    // geometry checks) is newer than the context's own last message
+    if (!ctx || g_error_seq > ctx->error_seq) return g_last_error.c_str();
+    return ctx->last_error.c_str();
+}
 
 ctr_status ctr_ctx_create(int device, ctr_ctx **out)
 {
@@ -1459,7 +1490,7 @@ ctr_status ctr_fan_bp_ref(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj
 
 ctr_status ctr_fan_bp_adjoint(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags)
 {
-    TRY(check_fan(g, 1));
+    TRY(check_fan(g, 0));   // any monotone detector
     TRY(need_ctx(ctx));
     const bool dev = flags & CTR_DEVICE_PTRS;
     TRY(fan_tables(ctx, g));
@@ -1869,12 +1900,11 @@ template <typename AdjOnes>
 ctr_status sirt_norm(ctr_ctx *ctx, uint64_t key, size_t nvox, hipStream_t st, AdjOnes adj_ones)
 {
     float *vinv = (float *)ctx->ws[kWsVinv].ptr;
-    if (key != 0 && ctx->vinv_key == key && ctx->vinv_ptr == (const void *)vinv) return CTR_OK;
+    if (key != 0 && ctx->vinv_key == key) return CTR_OK;
     ctx->vinv_key = 0;
     TRY(adj_ones(vinv));
     TRY(check_launch(ctx, launch_reciprocal_eps(vinv, nvox, 1e-9f, st), "reciprocal"));
     ctx->vinv_key = key;
-    ctx->vinv_ptr = vinv;
     return CTR_OK;
 }
 
@@ -1889,7 +1919,7 @@ ctr_status sirt_loop(ctr_ctx *ctx, size_t nvox, size_t nray, const float *b_in, 
     TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
     TRY(stage_out(ctx, dev, o->use_x0 != 0, kWsOut, x_in, nvox, &x));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
-    TRY(buf_reserve(ctx, ctx->ws[kWsVinv], nvox * sizeof(float)));
+    TRY(reserve_vinv(ctx, nvox * sizeof(float)));
     const size_t nsum = (size_t)o->n_iter + 2;
     TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(nblk_fp, 4096)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * nsum));
@@ -1962,7 +1992,7 @@ ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, floa
 ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
                       unsigned flags)
 {
-    TRY(check_fan(g, 1));
+    TRY(check_fan(g, 0));   // any monotone detector
     TRY(need_ctx(ctx));
     TRY(sirt_opts_check(ctx, opts));
     TRY(fan_tables(ctx, g));
@@ -2017,6 +2047,7 @@ struct ctr_cnn {
     DevBuf wf[4], wb[4];                    // packed BF16 forward / input-gradient weights
     DevBuf a0, a1, a2, a3, out, d4, da, db; // activations and deltas (one patch or volume)
     DevBuf partial, lsum, xin, gin, tin;
+    DevBuf tx, tg, full;                    // tiled pass: a tile's input boxes, the whole output
     int step = 0;
 };
 
@@ -2078,6 +2109,45 @@ ctr_status cnn_forward_dev(ctr_cnn *net, const CnnShape &S, const float *x, cons
     return CTR_OK;
 }
 
+// The network over a volume in tiles of T^3 output voxels: each tile's input
+// box reaches `halo` voxels past it, cropped at the volume's faces (where the
+// whole-volume pass pads with zeros as well).  Inside the volume the zero
+// padding at a box face reaches 4 voxels inwards (four 3^3 layers), so with
+// halo >= 4 every output voxel comes from the same inputs, in the same
+// order, as in the whole-volume pass: the result is identical.  Workspaces
+// scale with the tile, not the volume.  Result into out (full volume).
+ctr_status cnn_forward_tiled(ctr_cnn *net, const CnnShape &S, const float *x, const float *g, float g_scale, float *out,
+                             int T, int halo)
+{
+    ctr_ctx *ctx = net->ctx;
+    hipStream_t st = ctx->stream;
+    const int bmax_x = std::min(S.nx, T + 2 * halo), bmax_y = std::min(S.ny, T + 2 * halo),
+              bmax_z = std::min(S.nz, T + 2 * halo);
+    const size_t nbox = (size_t)bmax_x * bmax_y * bmax_z;
+    TRY(buf_reserve(ctx, net->tx, nbox * sizeof(float)));
+    TRY(buf_reserve(ctx, net->tg, nbox * sizeof(float)));
+    TRY(cnn_reserve(net, nbox, false));
+    float *tx = (float *)net->tx.ptr, *tg = (float *)net->tg.ptr;
+    for (int oz = 0; oz < S.nz; oz += T)
+        for (int oy = 0; oy < S.ny; oy += T)
+            for (int ox = 0; ox < S.nx; ox += T) {
+                const int wx = std::min(T, S.nx - ox), wy = std::min(T, S.ny - oy), wz = std::min(T, S.nz - oz);
+                const int x0 = std::max(0, ox - halo), y0 = std::max(0, oy - halo), z0 = std::max(0, oz - halo);
+                const int x1 = std::min(S.nx, ox + wx + halo), y1 = std::min(S.ny, oy + wy + halo),
+                          z1 = std::min(S.nz, oz + wz + halo);
+                const CnnShape B{x1 - x0, y1 - y0, z1 - z0};
+                TRY(check_launch(ctx, launch_box_copy(x, S.nx, S.ny, x0, y0, z0, tx, B.nx, B.ny, 0, 0, 0, B.nx, B.ny,
+                                                      B.nz, st), "tile input"));
+                TRY(check_launch(ctx, launch_box_copy(g, S.nx, S.ny, x0, y0, z0, tg, B.nx, B.ny, 0, 0, 0, B.nx, B.ny,
+                                                      B.nz, st), "tile input"));
+                TRY(cnn_forward_dev(net, B, tx, tg, g_scale, false));
+                TRY(check_launch(ctx, launch_box_copy((const float *)net->out.ptr, B.nx, B.ny, ox - x0, oy - y0,
+                                                      oz - z0, out, S.nx, S.ny, ox, oy, oz, wx, wy, wz, st),
+                                 "tile output"));
+            }
+    return CTR_OK;
+}
+
 // Backward pass after cnn_forward_dev(keep): accumulates dloss/dparams into
 // net->grad and the patch's sum of squared errors into lsum[slot].
 ctr_status cnn_backward_dev(ctr_cnn *net, const CnnShape &S, const float *target, int border, float inv_count,
@@ -2127,7 +2197,8 @@ void cnn_release(ctr_cnn *net)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf *b : {&net->params, &net->m, &net->v, &net->grad, &net->a0, &net->a1, &net->a2, &net->a3, &net->out,
-                      &net->d4, &net->da, &net->db, &net->partial, &net->lsum, &net->xin, &net->gin, &net->tin})
+                      &net->d4, &net->da, &net->db, &net->partial, &net->lsum, &net->xin, &net->gin, &net->tin,
+                      &net->tx, &net->tg, &net->full})
         buf_free(*b);
     for (int l = 0; l < 4; ++l) {
         buf_free(net->wf[l]);
@@ -2235,6 +2306,27 @@ ctr_status ctr_cnn_forward(ctr_cnn *net, int nx, int ny, int nz, const float *x,
     return finish_out(ctx, false, (const float *)net->out.ptr, out, n);
 }
 
+ctr_status ctr_cnn_forward_tiled(ctr_cnn *net, int nx, int ny, int nz, const float *x, const float *g, float g_scale,
+                                 float *out, int tile, int halo, unsigned flags)
+{
+    TRY(need_net(net));
+    ctr_ctx *ctx = net->ctx;
+    TRY(check_cnn_shape(ctx, nx, ny, nz));
+    if (!std::isfinite(g_scale)) return fail(ctx, CTR_EINVAL, "g_scale must be finite");
+    if (tile < 8) return fail(ctx, CTR_EINVAL, "tile must be at least 8 voxels");
+    if (halo < 4) return fail(ctx, CTR_EINVAL, "halo must be at least 4 voxels (the network's receptive radius)");
+    TRY(check_cnn_shape(ctx, tile + 2 * halo, tile + 2 * halo, tile + 2 * halo));
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const size_t n = (size_t)nx * ny * nz;
+    const float *dx, *dg;
+    float *dout;
+    TRY(stage_in(ctx, dev, kWsIn, x, n, &dx));
+    TRY(stage_in(ctx, dev, kWsTmp, g, n, &dg));
+    TRY(stage_out(ctx, dev, false, kWsOut, out, n, &dout));
+    TRY(cnn_forward_tiled(net, CnnShape{nx, ny, nz}, dx, dg, g_scale, dout, tile, halo));
+    return finish_out(ctx, dev, dout, out, n);
+}
+
 ctr_status ctr_cnn_train_step(ctr_cnn *net, int n_patch, int nx, int ny, int nz, const float *x, const float *g,
                               const float *g_scale, const float *target, int border, float step_lr, double *loss,
                               float *grad, unsigned flags)
@@ -2318,7 +2410,7 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_
     TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
     TRY(stage_in(ctx, dev, kWsTmp, x_in, nvox, &x));
     TRY(stage_out(ctx, dev, false, kWsOut, s_out, nvox, &s));
-    TRY(buf_reserve(ctx, ctx->ws[kWsVinv], nvox * sizeof(float)));
+    TRY(reserve_vinv(ctx, nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsG], nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
@@ -2362,12 +2454,21 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     float *x;
     TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
     TRY(stage_out(ctx, dev, o->use_x0 != 0, kWsOut, x_in, nvox, &x));
-    for (int s : {kWsVinv, kWsG, kWsS, kWsD, kWsX2}) TRY(buf_reserve(ctx, ctx->ws[s], nvox * sizeof(float)));
+    TRY(reserve_vinv(ctx, nvox * sizeof(float)));
+    for (int s : {kWsG, kWsS, kWsD, kWsX2}) TRY(buf_reserve(ctx, ctx->ws[s], nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
     if (o->judge_vs_plain) TRY(buf_reserve(ctx, ctx->ws[kWsR2], nray * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * 4));
-    TRY(cnn_reserve(net, nvox, false));
+    if (o->cnn_tile < 0) return fail(ctx, CTR_EINVAL, "cnn_tile must be >= 0");
+    // patch-wise network: tiles of cnn_tile^3 with 8-voxel halos, workspaces of one tile
+    const bool tiled = o->cnn_tile > 0 && (g->nt > o->cnn_tile || g->ns > o->cnn_tile || g->nz > o->cnn_tile);
+    if (tiled) {
+        if (o->cnn_tile < 8) return fail(ctx, CTR_EINVAL, "cnn_tile must be 0 or at least 8");
+        TRY(buf_reserve(ctx, net->full, nvox * sizeof(float)));
+    } else {
+        TRY(cnn_reserve(net, nvox, false));
+    }
     float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr, *sb = (float *)ctx->ws[kWsS].ptr;
     float *r = (float *)ctx->ws[kWsR].ptr, *alt = (float *)ctx->ws[kWsX2].ptr;
     float *r_spare = o->judge_vs_plain ? (float *)ctx->ws[kWsR2].ptr : nullptr;
@@ -2416,12 +2517,14 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
         const double rms = std::sqrt(h[0] / (double)nvox);
         float *cand = (cur == x) ? alt : x;
         float scale = 0.0f;
+        const float *corr = tiled ? (const float *)net->full.ptr : (const float *)net->out.ptr;
         if (rms > 0.0 && std::isfinite(rms)) {
-            TRY(cnn_forward_dev(net, S, cur, sb, (float)(1.0 / rms), false));
+            if (tiled) TRY(cnn_forward_tiled(net, S, cur, sb, (float)(1.0 / rms), (float *)net->full.ptr, o->cnn_tile, 8));
+            else TRY(cnn_forward_dev(net, S, cur, sb, (float)(1.0 / rms), false));
             scale = (float)rms;
         }
-        TRY(check_launch(ctx, launch_learned_candidate(cur, sb, (const float *)net->out.ptr, scale, o->min_val,
-                                                       o->max_val, nvox, cand, st), "learned step"));
+        TRY(check_launch(ctx, launch_learned_candidate(cur, sb, corr, scale, o->min_val, o->max_val, nvox, cand, st),
+                         "learned step"));
         double res_c = 0.0;
         TRY(residual(cand, &res_c));
         int acc = 1;
@@ -2671,6 +2774,30 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
 // Autotuner
 // ===========================================================================
 extern "C" {
+
+ctr_status ctr_cone_bp_plan_stats(ctr_ctx *ctx, const ctr_cone_geom *g, int adjoint, long long counts[6])
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    if (!counts) return fail(ctx, CTR_EINVAL, "counts is NULL");
+    TRY(cone_tables(ctx, g));
+    const BpParams P = cone_bp_params(ctx, g, 0);
+    FootCache &fc = adjoint ? ctx->foot_adj : ctx->foot_bp;
+    TRY(ensure_footprints(ctx, P, fc, adjoint != 0));
+    std::vector<int4> h(fc.plan.nfoot);
+    HIP_TRY(ctx, hipMemcpyAsync(h.data(), fc.buf.ptr, h.size() * sizeof(int4), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 6; ++k) counts[k] = 0;
+    for (const int4 &f : h) {
+        ++counts[0];
+        if (f.w & kFootEmpty) ++counts[1];
+        else if (!(f.w & kFootFits)) ++counts[5];
+        else if (f.w & kFootInterior) ++counts[2];
+        else if (!adjoint && (f.w & kFootRowsIn)) ++counts[3];
+        else ++counts[4];
+    }
+    return CTR_OK;
+}
 
 ctr_status ctr_autotune_cone(ctr_ctx *ctx, const ctr_cone_geom *g, ctr_tune_result *out)
 {

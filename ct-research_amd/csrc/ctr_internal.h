@@ -61,12 +61,16 @@ struct BpParams {
     const float2 *zq;       // [nz + 32]: (zs^2, hz/|zs|) per slice, zs = slice centre relative to the source
     int accumulate;
     int quirks;
+    int rows_in;            // FDK: tiles that leave the detector only in u take the clamped interior loop
+                            // (0: the masked loop, for validation; CTR_BP_ROWSIN=0)
     // exact-adjoint tables (geometry only, built with the cone tables):
     const float *adj_inva;  // [nb][np] 1/(p cos b + D sin b), 0 where that is 0
     const float *adj_invb;  // [nb][np] 1/(p sin b - D cos b), 0 where that is 0
     const float *adj_invxi; // [nx] 1/xi, 0 at xi = 0
     const float *adj_norm;  // [nx][np] ray norm sqrt(p^2 + xi^2 + D^2); the adjoint reads its
                             // input multiplied by it (this table itself for A'1)
+    const double *view_d;   // [nb][4] sin, cos, source t, source s in FP64 (the projector's rays)
+    const double *pdd;      // [np] detector columns in FP64
 };
 
 // The ray norm of detector sample (p, xi), in the float operations that the
@@ -164,6 +168,7 @@ struct FanParams {
     double rt_d, rs_d, ct_d, cs_d, u_scale_d, u_off_d;   // FP64 copies for the reference-quirk path
     const double *view;     // [nb][4]: sin, cos, source t, source s
     const float *pf;        // [np] bin positions (float)
+    const double *pdbl;     // [np] bin positions (FP64, as the forward projector's rays)
     const float *gam;       // [np] bin angles (float, equiangular)
     int accumulate;
     int quirks;
@@ -185,6 +190,9 @@ int ray_fp_blocks(const RayParams &g);
 
 // Tiling of the voxel-driven backprojection and its per-(tile, view) detector
 // footprint table (int4 per entry, nfoot entries), computed once per geometry.
+// Flags of a (tile, view) footprint (the .w of its int4).
+enum { kFootEmpty = 1, kFootInterior = 2, kFootFits = 4, kFootRowsIn = 8 };
+
 struct BpPlan {
     int faces;              // footprints of voxel boxes (exact adjoint) instead of voxel centres
     int tz, wp, capf, hmax;
@@ -258,6 +266,11 @@ hipError_t launch_iter_inc(int *iter, hipStream_t st);
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
 // transform length of the FFT filter path for np columns (0: direct convolution)
 int fft_filter_length(int np);
+// Copy a wx x wy x wz box from src (row sx, plane sx*sy) at (sx0, sy0, sz0) to
+// dst (row dx, plane dx*dy) at (dx0, dy0, dz0).
+hipError_t launch_box_copy(const float *src, int sx, int sy, int sx0, int sy0, int sz0, float *dst, int dx, int dy,
+                           int dx0, int dy0, int dz0, int wx, int wy, int wz, hipStream_t st);
+
 
 hipError_t launch_fan_bp_ref(const FanParams &P, const float *proj, float *img, hipStream_t st);
 hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img, const SirtUpdate *upd,

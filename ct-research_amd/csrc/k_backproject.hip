@@ -36,7 +36,6 @@ __device__ __forceinline__ int xcd_tile(int bid, int nwg)
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-enum { kFootEmpty = 1, kFootInterior = 2, kFootFits = 4, kFootRowsIn = 8 };
 
 // ---------------------------------------------------------------------------
 // Per-(tile, view) detector footprints.  The rectangle of detector samples a
@@ -253,7 +252,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_bp_tiled(const float *__restric
                     acc[k] = fmaf(chord_len(rho2, mts, zq[z0 + k]), fmaf(fv, bot - top, top), acc[k]);
                 }
             }
-        } else if (KIND == kBpFdk && fits && (f.w & kFootRowsIn)) {
+        } else if (KIND == kBpFdk && fits && (f.w & kFootRowsIn) && P.rows_in) {
             // every sample is on the detector rows and only u leaves the
             // detector: the interior loop, with the column clamped into the
             // staged rectangle and the weights of off-detector columns zeroed
@@ -649,11 +648,14 @@ __global__ __launch_bounds__(256) void k_adjoint_tables(BpParams P, float *__res
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     const long nvc = (long)P.nb * P.np, nrc = (long)P.nx * P.np;
     if (e < nvc) {
+        // the direction components p cos + D sin and p sin - D cos cancel for
+        // rays nearly parallel to a slab: formed in FP64 from the projector's
+        // own view table, so that the reciprocal keeps float precision
         const int b = (int)(e / P.np), c = (int)(e % P.np);
-        const BpView vw = P.views[b];
-        const float p = P.pd[c];
-        inva[e] = safe_rcp(fmaf(p, vw.cb, P.D * vw.sb));
-        invb[e] = safe_rcp(fmaf(p, vw.sb, -P.D * vw.cb));
+        const double sb = P.view_d[4 * b + 0], cb = P.view_d[4 * b + 1], p = P.pdd[c], D = (double)P.D;
+        const double a = p * cb + D * sb, bq = p * sb - D * cb;
+        inva[e] = a != 0.0 ? (float)(1.0 / a) : 0.0f;
+        invb[e] = bq != 0.0 ? (float)(1.0 / bq) : 0.0f;
     }
     if (e < nrc) {
         const int r = (int)(e / P.np), c = (int)(e % P.np);

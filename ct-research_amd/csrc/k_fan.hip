@@ -76,55 +76,83 @@ __global__ __launch_bounds__(256) void k_fan_bp_ref(const float *__restrict__ pr
     img[o] = P.accumulate ? img[o] + acc : acc;
 }
 
-// Exact adjoint of the fan Siddon projector: for each view the candidate
-// rays are the bins inside the projection of the pixel's square; the ray
-// S + lam (a, b), a = p cos + D sin, b = p sin - D cos, meets the pixel for
-// lam in the intersection of its t and s slabs; length = dlam sqrt(p^2+D^2).
+// Exact adjoint of the fan Siddon projector.  Per pixel and view the
+// candidate rays are the bins whose positions fall inside the projection of
+// the pixel's square, found by binary search over the monotone bin
+// positions (so equiangular and other non-uniform detectors work).  The ray
+// S + lam (a, b) through bin p, a = p cos + D sin, b = p sin - D cos, meets
+// the pixel for lam in the intersection of its t and s slabs; its length is
+// dlam sqrt(p^2 + D^2).  The slab parameters are measured from the source,
+// D away, so they are formed in FP64 with the FP64 bin positions of the
+// forward projector: a 2-D image is small enough for that.
+
+// First index c of the monotone array pos[0..n) with pos[c] >= x (ascending)
+// or pos[c] <= x (descending); n when there is none.
+__device__ __forceinline__ int bin_lower(const float *__restrict__ pos, int n, bool asc, float x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const float v = pos[mid];
+        if (asc ? v < x : v > x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_fan_adjoint(const float *__restrict__ proj, float *__restrict__ img,
                                                      FanParams P, SirtUpdate U, int sirt, int ones)
 {
     const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (i >= P.nt || j >= P.ns) return;
-    const float tlo = (float)i * P.rt, slo = (float)j * P.rs;
+    const double tlo = (double)i * P.rt_d, thi = tlo + P.rt_d, slo = (double)j * P.rs_d, shi = slo + P.rs_d;
+    const double D = (double)P.D;
+    const bool asc = P.pf[P.np - 1] >= P.pf[0];
     float acc = 0.0f;
     for (int b = 0; b < P.nb; ++b) {
-        const float sb = (float)P.view[4 * b + 0], cb = (float)P.view[4 * b + 1];
-        const float st = (float)P.view[4 * b + 2], ss = (float)P.view[4 * b + 3];
+        const double sb = P.view[4 * b + 0], cb = P.view[4 * b + 1];
+        const double st = P.view[4 * b + 2], ss = P.view[4 * b + 3];
         float umin = kInf, umax = -kInf;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float x = tlo + ((c & 1) ? P.rt : 0.0f) - P.ct, y = slo + ((c & 2) ? P.rs : 0.0f) - P.cs;
-            const float u = x * cb + y * sb, s = -x * sb + y * cb;
+            const float x = (float)(((c & 1) ? thi : tlo) - P.ct_d), y = (float)(((c & 2) ? shi : slo) - P.cs_d);
+            const float u = x * (float)cb + y * (float)sb, s = -x * (float)sb + y * (float)cb;
             const float pd = u * P.D / (P.D - s);
-            const float uu = fmaf(pd, P.u_scale, P.u_off);
-            umin = fminf(umin, uu);
-            umax = fmaxf(umax, uu);
+            umin = fminf(umin, pd);
+            umax = fmaxf(umax, pd);
         }
-        const float eu = 1e-3f + 4e-6f * fmaxf(fabsf(umin), fabsf(umax));
-        const int c0 = max((int)ceilf(fminf(fmaxf(umin - eu, -1.0f), (float)P.np)), 0);
-        const int c1 = min((int)floorf(fminf(umax + eu, (float)P.np)), P.np - 1);
+        // a conservative bin range: a bin outside the pixel's shadow gets length 0 below
+        const float eu = 1e-3f + 1e-5f * fmaxf(fabsf(umin), fabsf(umax));
+        int c0, c1;
+        if (asc) {
+            c0 = bin_lower(P.pf, P.np, true, umin - eu);
+            c1 = bin_lower(P.pf, P.np, true, umax + eu) - 1;   // last bin <= umax + eu (ties included by the margin)
+        } else {
+            c0 = bin_lower(P.pf, P.np, false, umax + eu);
+            c1 = bin_lower(P.pf, P.np, false, umin - eu) - 1;
+        }
+        c0 = max(c0, 0);
+        c1 = min(c1, P.np - 1);
         const float *r = proj + (size_t)b * P.np;
         for (int c = c0; c <= c1; ++c) {
-            const float p = P.pf[c];
-            const float a = fmaf(p, cb, P.D * sb), bq = fmaf(p, sb, -P.D * cb);
-            float lo = -kInf, hi = kInf;
-            if (a != 0.0f) {
-                const float inv = 1.0f / a;
-                const float l1 = (tlo - st) * inv, l2 = (tlo + P.rt - st) * inv;
-                lo = fmaxf(lo, fminf(l1, l2));
-                hi = fminf(hi, fmaxf(l1, l2));
-            } else if (!(st >= tlo && st < tlo + P.rt)) {
+            const double p = P.pdbl[c];
+            const double a = p * cb + D * sb, bq = p * sb - D * cb;
+            double lo = -INFINITY, hi = INFINITY;
+            if (a != 0.0) {
+                const double l1 = (tlo - st) / a, l2 = (thi - st) / a;
+                lo = fmax(lo, fmin(l1, l2));
+                hi = fmin(hi, fmax(l1, l2));
+            } else if (!(st >= tlo && st < thi)) {
                 continue;
             }
-            if (bq != 0.0f) {
-                const float inv = 1.0f / bq;
-                const float l1 = (slo - ss) * inv, l2 = (slo + P.rs - ss) * inv;
-                lo = fmaxf(lo, fminf(l1, l2));
-                hi = fminf(hi, fmaxf(l1, l2));
-            } else if (!(ss >= slo && ss < slo + P.rs)) {
+            if (bq != 0.0) {
+                const double l1 = (slo - ss) / bq, l2 = (shi - ss) / bq;
+                lo = fmax(lo, fmin(l1, l2));
+                hi = fmin(hi, fmax(l1, l2));
+            } else if (!(ss >= slo && ss < shi)) {
                 continue;
             }
-            const float len = fmaxf(hi - lo, 0.0f) * sqrtf(fmaf(p, p, P.D * P.D));
+            if (!(hi > lo)) continue;
+            const float len = (float)((hi - lo) * sqrt(p * p + D * D));
             acc = fmaf(len, ones ? 1.0f : r[c], acc);
         }
     }

@@ -12,6 +12,7 @@
 // FP32 rate on gfx950, and the reference accumulates in double (:38-45).
 #include "ctr_internal.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace ctr {
@@ -272,7 +273,7 @@ __device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const
     }
 }
 
-template <int L>
+template <int L, int MODE>
 __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, float *__restrict__ out,
                                                 const FilterParams &P)
 {
@@ -292,13 +293,13 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
         const int row0 = P.spec_stride ? (2 * (pair / P.nx)) * P.nx + pair % P.nx : 2 * pair;
         const int row1 = P.spec_stride ? row0 + P.nx : row0 + 1;
         const bool has0 = pair < npairs, has1 = has0 && row1 < P.rows;
-        const float xi0 = P.mode == 0 && has0 ? P.xd[row0 % P.nx] : 0.0f;
-        const float xi1 = P.mode == 0 && has1 ? P.xd[row1 % P.nx] : 0.0f;
+        const float xi0 = MODE == 0 && has0 ? P.xd[row0 % P.nx] : 0.0f;
+        const float xi1 = MODE == 0 && has1 ? P.xd[row1 % P.nx] : 0.0f;
         const double *spec = P.spec + (P.spec_stride && has0 ? (size_t)(row0 % P.nx) * P.spec_stride : 0);
         // mode 3: row scale and the scaled view angles of the two rows
-        const double prow = P.mode == 3 && has0 ? P.rowp[row0 % P.nx] : 1.0;
-        const double bp0 = P.mode == 3 && has0 ? P.betad[row0 / P.nx] * prow : 0.0;
-        const double bp1 = P.mode == 3 && has1 ? P.betad[row1 / P.nx] * prow : 0.0;
+        const double prow = MODE == 3 && has0 ? P.rowp[row0 % P.nx] : 1.0;
+        const double bp0 = MODE == 3 && has0 ? P.betad[row0 / P.nx] * prow : 0.0;
+        const double bp1 = MODE == 3 && has1 ? P.betad[row1 / P.nx] * prow : 0.0;
         const float *src0 = in + (size_t)row0 * np, *src1 = in + (size_t)row1 * np;
         // the weighted samples go through LDS: a rolled loop keeps the
         // weight arithmetic of one sample live at a time
@@ -309,13 +310,13 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
             double a = 0.0, b = 0.0;
             if (q < np) {
                 float w0, w1;
-                if (P.mode == 0) {
+                if constexpr (MODE == 0) {
                     const float pq = P.pd[q];
                     w0 = P.D / sqrtf(P.D * P.D + pq * pq + xi0 * xi0);
                     w1 = P.D / sqrtf(P.D * P.D + pq * pq + xi1 * xi1);
-                } else if (P.mode == 1) {
+                } else if constexpr (MODE == 1) {
                     w0 = w1 = P.colw[q];
-                } else if (P.mode == 2) {
+                } else if constexpr (MODE == 2) {
                     w0 = w1 = 1.0f;
                 } else {
                     // ImplementUI.cu:43-66: (D/P) cos(g P) Parker(b P, start P, g P, fan P)
@@ -325,7 +326,7 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
                     w0 = base_w * parker_weight(bp0, stp, gp, fanp);
                     w1 = base_w * parker_weight(bp1, stp, gp, fanp);
                 }
-                if (P.mode == 3) {
+                if constexpr (MODE == 3) {
                     // the reference keeps the weighted samples in FP64 (ImplementUI.cu:66)
                     if (has0) a = (double)src0[q] * (double)w0;
                     if (has1) b = (double)src1[q] * (double)w1;
@@ -359,11 +360,13 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
     }
 }
 
-template <int L>
+// One kernel per pre-weight mode: the curved-detector weights (FP64 Parker
+// window) need registers that the other modes should not hold.
+template <int L, int MODE>
 __global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restrict__ in, float *__restrict__ out,
                                                           FilterParams P)
 {
-    fft_filter_body<L>(in, out, P);
+    fft_filter_body<L, MODE>(in, out, P);
 }
 
 template <int L>
@@ -373,7 +376,13 @@ hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStr
     const int npairs = P.spec_stride ? P.nx * ((P.rows / P.nx + 1) / 2) : (P.rows + 1) / 2;
     const int need = (npairs + GPB - 1) / GPB;
     const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
-    hipLaunchKernelGGL((k_fft_filter<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+    switch (P.mode) {
+    case 0: hipLaunchKernelGGL((k_fft_filter<L, 0>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
+    case 1: hipLaunchKernelGGL((k_fft_filter<L, 1>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
+    case 2: hipLaunchKernelGGL((k_fft_filter<L, 2>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
+    case 3: hipLaunchKernelGGL((k_fft_filter<L, 3>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

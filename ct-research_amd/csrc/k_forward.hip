@@ -292,7 +292,8 @@ __device__ __forceinline__ float trace_ray(const float *__restrict__ vol, const 
         if (kSkip) {
             // the second and third voxels are read only by lanes whose
             // segment in them is not empty (a crossing inside the slab); a
-            // skipped term is 0 * v, so the sum is unchanged bit for bit
+            // skipped term is 0 * v, so the sum is unchanged bit for bit for
+            // finite volumes (an infinite v would make that term NaN)
             float v2 = 0.0f, v3 = 0.0f;
             if (w2 != 0.0f) v2 = load_vox(rs, offm);
             if (w3 != 0.0f) v3 = load_vox(rs, off3);

@@ -2,6 +2,8 @@
 // deterministic two-level reductions of per-block partial sums).
 #include "ctr_internal.h"
 
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 namespace ctr {
@@ -179,6 +181,33 @@ int grid_for(size_t n)
 }
 
 }  // namespace
+
+// Copy a wx x wy x wz box between two volumes ([z][y][x], x fastest): from
+// src (row sx, plane sx*sy) at (sx0, sy0, sz0) to dst (row dx, plane dx*dy)
+// at (dx0, dy0, dz0).  Both boxes lie inside their volumes.
+__global__ __launch_bounds__(256) void k_box_copy(const float *__restrict__ src, int sx, int sy, int sx0, int sy0,
+                                                  int sz0, float *__restrict__ dst, int dx, int dy, int dx0, int dy0,
+                                                  int dz0, int wx, int wy, int wz)
+{
+    const size_t n = (size_t)wx * wy * wz;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int x = (int)(e % wx);
+        const size_t r = e / wx;
+        const int y = (int)(r % wy), z = (int)(r / wy);
+        dst[((size_t)(dz0 + z) * dy + (dy0 + y)) * dx + (dx0 + x)] = src[((size_t)(sz0 + z) * sy + (sy0 + y)) * sx + (sx0 + x)];
+    }
+}
+
+hipError_t launch_box_copy(const float *src, int sx, int sy, int sx0, int sy0, int sz0, float *dst, int dx, int dy,
+                           int dx0, int dy0, int dz0, int wx, int wy, int wz, hipStream_t st)
+{
+    const size_t n = (size_t)wx * wy * wz;
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_box_copy, dim3(blocks), dim3(256), 0, st, src, sx, sy, sx0, sy0, sz0, dst, dx, dy, dx0, dy0,
+                       dz0, wx, wy, wz);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st)
 {

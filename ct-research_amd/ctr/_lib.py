@@ -33,6 +33,7 @@ EXPORTS = [
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
+    "ctr_cone_bp_plan_stats", "ctr_cnn_forward_tiled",
 ]
 
 CNN_NUM_PARAMS = 57985
@@ -99,7 +100,8 @@ class SirtProblemC(ctypes.Structure):
 class LearnedOptsC(ctypes.Structure):
     _fields_ = [("n_steps", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
                 ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
-                ("residual_l2", ctypes.c_void_p), ("accepted", ctypes.c_void_p), ("judge_vs_plain", ctypes.c_int)]
+                ("residual_l2", ctypes.c_void_p), ("accepted", ctypes.c_void_p), ("judge_vs_plain", ctypes.c_int),
+                ("cnn_tile", ctypes.c_int)]
 
 
 _lib = None
@@ -159,6 +161,10 @@ def lib():
         L.ctr_cnn_get_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.ctr_cnn_forward.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_uint]
+        L.ctr_cnn_forward_tiled.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_uint]
+        L.ctr_cone_bp_plan_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.ctr_cnn_train_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
@@ -552,6 +558,12 @@ class Context:
         return {"bp_pitch": r.bp_pitch, "adj_cm": r.adj_cm, "fp_transpose": r.fp_transpose,
                 "bp_ms": list(r.bp_ms), "adj_ms": list(r.adj_ms), "fp_ms": list(r.fp_ms)}
 
+    def bp_plan_stats(self, g, adjoint=False):
+        """Classes of the backprojection plan's (tile, view) pairs for geometry g."""
+        c = (ctypes.c_longlong * 6)()
+        self._check(lib().ctr_cone_bp_plan_stats(self.h, ctypes.byref(g.c), int(bool(adjoint)), c))
+        return dict(zip(("pairs", "empty", "interior", "rows_in", "edge", "global"), (int(v) for v in c)))
+
     def sirt3d_batched(self, problems, n_iter, lam=1.0, lo=0.0, hi=np.inf, x0s=None, flags=0):
         """problems: [(ConeGeom, b)], b numpy or CUDA tensors (all alike).
         Returns ([x], [residual_l2]) with one graph-captured launch sequence per iteration."""
@@ -598,12 +610,14 @@ class Context:
                                           float(lam), 0))
         return s
 
-    def learned_sirt3d(self, g, b, net, n_steps, lam=1.0, lo=0.0, hi=np.inf, x0=None, judge_vs_plain=False, flags=0):
-        """Residual-judged learned SIRT; returns (x, residual_l2[n_steps+1], accepted[n_steps])."""
+    def learned_sirt3d(self, g, b, net, n_steps, lam=1.0, lo=0.0, hi=np.inf, x0=None, judge_vs_plain=False, flags=0,
+                       cnn_tile=0):
+        """Residual-judged learned SIRT; returns (x, residual_l2[n_steps+1], accepted[n_steps]).
+        cnn_tile > 0 runs the network in tiles of cnn_tile^3 voxels (8-voxel halos)."""
         r2 = np.zeros(n_steps + 1)
         acc = np.zeros(max(n_steps, 1), dtype=np.int32)
         opts = LearnedOptsC(int(n_steps), float(lam), float(lo), float(hi), int(x0 is not None),
-                            r2.ctypes.data, acc.ctypes.data, int(bool(judge_vs_plain)))
+                            r2.ctypes.data, acc.ctypes.data, int(bool(judge_vs_plain)), int(cnn_tile))
         fn = lib().ctr_learned_sirt3d
         if _is_device(b):
             import torch
@@ -691,6 +705,24 @@ class Cnn:
         out = np.zeros(n, dtype=np.float32)
         self.ctx._check(lib().ctr_cnn_forward(self.h, nx, ny, nz, xx.ctypes.data, gg.ctypes.data, float(g_scale),
                                               out.ctypes.data, 0))
+        return out
+
+    def forward_tiled(self, shape, x, g, g_scale=1.0, tile=128, halo=8):
+        """forward() computed in tiles of tile^3 voxels with halo-voxel halos (identical result)."""
+        nx, ny, nz = shape
+        n = nx * ny * nz
+        fn = lib().ctr_cnn_forward_tiled
+        if _is_device(x):
+            import torch
+            self.ctx._bind_torch_stream()
+            out = torch.empty(n, dtype=torch.float32, device=x.device)
+            self.ctx._check(fn(self.h, nx, ny, nz, x.data_ptr(), g.data_ptr(), ctypes.c_float(g_scale), out.data_ptr(),
+                               int(tile), int(halo), DEVICE_PTRS))
+            return out
+        xx, gg = _f32(x), _f32(g)
+        out = np.zeros(n, dtype=np.float32)
+        self.ctx._check(fn(self.h, nx, ny, nz, xx.ctypes.data, gg.ctypes.data, ctypes.c_float(g_scale), out.ctypes.data,
+                           int(tile), int(halo), 0))
         return out
 
     def train_step(self, shape, x, g, g_scale, target, border=0, lr=0.0, want_grad=False):

@@ -27,6 +27,7 @@ import numpy as np
 
 from . import _lib, geometry, phantoms
 
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "learned_sirt_256_90.npz")
 STARTS = (0, 1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48)
 AHEAD = 16
 SNAPSHOTS = tuple(sorted(set(STARTS) | {k + AHEAD for k in STARTS}))
@@ -171,6 +172,27 @@ def evaluate(ctx, net, seed=2000, n=256, views=90, n_plain=200, n_learned=10, lo
     }
     log(json.dumps(out), flush=True)
     return out
+
+
+def m2_measure(ctx, seed=2000, n=256, views=90, n_plain=200):
+    """The time-to-residual comparison on the held-out problem, as one dict:
+    plain SIRT's 200-iteration residual is the target; the plain time is the
+    time plain SIRT takes to first reach it; the learned time is the wall
+    time of the learned loop with the packaged weights."""
+    import torch
+    net = _lib.Cnn(ctx, _lib.cnn_init_params(0, 0.0))
+    try:
+        with np.load(WEIGHTS, allow_pickle=False) as f:
+            net.set_params(f["params"])
+        r = evaluate(ctx, net, seed=seed, n=n, views=views, n_plain=n_plain, log=lambda *a, **k: None)
+    finally:
+        net.close()
+    torch.cuda.empty_cache()
+    return {"method": "cnn-step", "problem": r["problem"], "target_residual": r["target_residual"],
+            "plain_s_to_target": r["plain_s"], "plain_ms_per_iter": r["plain_ms_per_iter"],
+            "learned_s": r["learned_s"], "learned_residual": r["learned_residual"],
+            "reached_target": r["reached_target"], "image_rel_l2_vs_plain200": r["image_rel_l2_vs_plain200"],
+            "speedup": r["speedup_at_target"], "speedup_at_equal_residual": r["speedup_at_equal_residual"]}
 
 
 def main(argv=None):

@@ -182,6 +182,10 @@ ctr_status ctr_curved_geom_check(const ctr_curved_geom *g);
 /* ---- cone beam --------------------------------------------------------- */
 
 /* Siddon line integrals of vol [nz*ns*nt] into proj [n_beta*n_xi*n_p].
+ * The volume must be finite: a non-finite voxel makes the rays through it
+ * non-finite, and a ray that only grazes it (zero segment length) may or
+ * may not be, depending on the voxel-read path (masked above 256 MiB of
+ * volume); rays away from it are unaffected.
  * Replaces ProjectionCone_3D() (GPU_accelerated_Operator/ProjectionCone_3D/ProjectionCone_3D.h:32-35,
  * kernel ProjectionCone_3D.cu:8-379). */
 ctr_status ctr_cone_fp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *vol, float *proj, unsigned flags);
@@ -218,7 +222,8 @@ ctr_status ctr_fan_fp(ctr_ctx *ctx, const ctr_fan_geom *g, const float *img, flo
  * GPU_accelerated_Operator/BackprojectionFan_2D/BackprojectionFan2D.cu:119 (kernel :7-116). */
 ctr_status ctr_fan_bp_ref(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags);
 
-/* Exact transpose of ctr_fan_fp. */
+/* Exact transpose of ctr_fan_fp, for any monotone detector (uniform or not,
+ * e.g. equiangular p = D tan(gamma)). */
 ctr_status ctr_fan_bp_adjoint(ctr_ctx *ctx, const ctr_fan_geom *g, const float *proj, float *img, unsigned flags);
 
 /* Fan pre-weight and ramp filter (flat: FBP_fan_GPU/FBPfan.cu:21-43; equiangular:
@@ -327,7 +332,7 @@ ctr_status ctr_par_sysmat(ctr_ctx *ctx, const ctr_par_geom *g, int64_t *row_ptr,
 ctr_status ctr_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
                       unsigned flags);
 
-/* 2-D fan-beam SIRT (Iterative_algorithm/SIRT.m:126-154). */
+/* 2-D fan-beam SIRT (Iterative_algorithm/SIRT.m:126-154), any monotone detector. */
 ctr_status ctr_sirt2d(ctr_ctx *ctx, const ctr_fan_geom *g, const float *b, float *x, const ctr_sirt_opts *opts,
                       unsigned flags);
 
@@ -346,6 +351,16 @@ typedef struct ctr_tune_result {
 } ctr_tune_result;
 
 ctr_status ctr_autotune_cone(ctr_ctx *ctx, const ctr_cone_geom *g, ctr_tune_result *out);
+
+/* How the cone-beam backprojection plan of this geometry classifies its
+ * (tile, view) pairs: counts[0] all pairs, [1] empty (the tile projects off
+ * the detector), [2] interior (every sample on the detector), [3] rows-in
+ * (only u leaves the detector; FDK takes its clamped interior loop unless
+ * the environment sets CTR_BP_ROWSIN=0), [4] other edge pairs staged in LDS,
+ * [5] footprints too large for LDS (read from global memory).  adjoint != 0
+ * reports the exact adjoint's plan ([3] is then 0).  A diagnostic with no
+ * counterpart in the reference. */
+ctr_status ctr_cone_bp_plan_stats(ctr_ctx *ctx, const ctr_cone_geom *g, int adjoint, long long counts[6]);
 
 /* Many independent SIRT3D problems (uneven volumes, views and detectors)
  * advanced together: each iteration is one batched residual launch, one
@@ -383,6 +398,15 @@ ctr_status ctr_cnn_get_params(ctr_cnn *net, float *params, size_t n);
 ctr_status ctr_cnn_forward(ctr_cnn *net, int nx, int ny, int nz, const float *x, const float *g, float g_scale,
                            float *out, unsigned flags);
 
+/* The same output as ctr_cnn_forward, computed tile by tile: tiles of tile^3
+ * output voxels whose input boxes reach halo voxels past them (cropped at the
+ * volume's faces).  halo >= 4 (the network's receptive radius) makes the
+ * result identical to the whole-volume pass; the workspaces hold one tile, so
+ * volumes whose activations would not fit whole (512^3 and up) still run.
+ * No counterpart in the reference. */
+ctr_status ctr_cnn_forward_tiled(ctr_cnn *net, int nx, int ny, int nz, const float *x, const float *g, float g_scale,
+                                 float *out, int tile, int halo, unsigned flags);
+
 /* Mean squared error of CNN(x_i, g_i * g_scale[i]) against target_i over the
  * voxels at least `border` from every face, averaged over n_patch patches of
  * nx x ny x nz (x, g, target: [n_patch][nz][ny][nx]; g_scale: host [n_patch]).
@@ -402,6 +426,8 @@ typedef struct ctr_learned_opts {
     int *accepted;          /* optional host [n_steps]: 1 learned step kept, 0 fell back to the plain step */
     int judge_vs_plain;     /* nonzero: keep the learned step only if its residual is below the plain
                                step's (one more forward projection per step) */
+    int cnn_tile;           /* 0: the network runs over the whole volume; > 0: in tiles of cnn_tile^3
+                               voxels with 8-voxel halos (same result, workspaces of one tile) */
 } ctr_learned_opts;
 
 /* The plain SIRT step at x: s = lambda * V .* A'(W .* (b - A x)) (the

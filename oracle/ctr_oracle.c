@@ -1477,3 +1477,13 @@ int or_num_threads(void)
     return 1;
 #endif
 }
+
+/* Thread count of the OpenMP loops above (the CPU baseline's 1-thread figure). */
+void or_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -389,3 +389,10 @@ def num_threads():
     fn = lib().or_num_threads
     fn.restype = ctypes.c_int
     return fn()
+
+
+def set_num_threads(n):
+    """Threads of the oracle's OpenMP loops (n > 0)."""
+    fn = lib().or_set_num_threads
+    fn.restype = None
+    fn(ctypes.c_int(int(n)))

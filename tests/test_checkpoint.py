@@ -31,6 +31,35 @@ def test_state_rejects_bad_input(tmp_path):
         load_sirt_state(stem)
 
 
+def test_crash_between_volume_and_header_keeps_old_state(tmp_path):
+    """A save interrupted after the new volume is written and before the
+    header is replaced resumes from the old state, never the new volume
+    under the old count."""
+    from ctr import checkpoint
+    rng = np.random.default_rng(8)
+    x3, x6 = rng.random(60).astype(np.float32), rng.random(60).astype(np.float32)
+    stem = str(tmp_path / "run")
+    save_sirt_state(stem, x3, 3, (3, 4, 5))
+    checkpoint.write_volume(stem, x6, 6)            # the crash: no header replace after it
+    y, k, _ = load_sirt_state(stem)
+    assert k == 3 and np.array_equal(y, x3)
+    save_sirt_state(stem, x6, 6, (3, 4, 5))         # a completed save commits the new state
+    y, k, h = load_sirt_state(stem)
+    assert k == 6 and np.array_equal(y, x6)
+    assert not (tmp_path / "run.k3.raw").exists()   # the superseded volume is removed
+
+
+def test_corrupt_volume_is_rejected(tmp_path):
+    stem = str(tmp_path / "c")
+    save_sirt_state(stem, np.arange(8, dtype=np.float32), 2, (2, 2, 2))
+    raw = tmp_path / "c.k2.raw"
+    data = bytearray(raw.read_bytes())
+    data[5] ^= 0x40
+    raw.write_bytes(bytes(data))
+    with pytest.raises(ValueError, match="CRC"):
+        load_sirt_state(stem)
+
+
 @pytest.mark.gpu
 def test_resume_equals_uninterrupted(tmp_path):
     g = ctr.geometry.cone_circular(24, 1.0, 20, 48, 40, 1.0, 120.0, 240.0)

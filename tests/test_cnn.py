@@ -182,3 +182,40 @@ def test_learned_sirt_judged_against_plain_step(ctx):
     report("learned_sirt[judged vs plain]", residuals=np.round(r, 5).tolist(), accepted=acc.tolist())
     assert r[1] <= rp[1] * (1 + 1e-6)          # step 1 starts from the same iterate as the plain step
     assert np.all(np.diff(r) <= 1e-7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile,halo", [(48, 8), (32, 4)])
+def test_cnn_tiled_equals_whole_volume(ctx, tile, halo):
+    """Tiles with halos of at least the receptive radius (4) and input boxes
+    cropped at the volume faces give the whole-volume output bit for bit."""
+    rng = np.random.default_rng(9)
+    shape = (150, 90, 70)
+    n = 150 * 90 * 70
+    p = ctr.cnn_init_params(8, 1.0)
+    p[1728:1760] = rng.uniform(-0.1, 0.1, 32)
+    x = rng.uniform(0, 1, n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    net = ctr.Cnn(ctx, p)
+    try:
+        whole = net.forward(shape, x, g, 0.8)
+        tiled = net.forward_tiled(shape, x, g, 0.8, tile=tile, halo=halo)
+    finally:
+        net.close()
+    report(f"cnn_tiled[{tile}, halo {halo}]", identical=bool(np.array_equal(whole, tiled)),
+           max_abs=float(np.abs(whole - tiled).max()))
+    assert np.array_equal(whole, tiled)
+
+
+@pytest.mark.gpu
+def test_learned_sirt_patchwise_equals_whole(ctx):
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    net = ctr.Cnn(ctx, ctr.cnn_init_params(13, 0.5))
+    try:
+        x0, r0, a0 = ctx.learned_sirt3d(g, b, net, 4)
+        x1, r1, a1 = ctx.learned_sirt3d(g, b, net, 4, cnn_tile=12)
+    finally:
+        net.close()
+    report("learned_sirt[patch-wise]", identical=bool(np.array_equal(x0, x1)), accepted=a1.tolist())
+    assert np.array_equal(x0, x1) and np.array_equal(r0, r1) and np.array_equal(a0, a1)

@@ -64,3 +64,35 @@ def test_fp_masked_voxel_reads_are_bit_identical(ctx, monkeypatch):
             c.close()
     assert np.array_equal(out["0"][0], out["1"][0])
     assert np.array_equal(out["0"][1], out["1"][1])
+
+
+def test_fp_non_finite_voxels_stay_on_their_rays(ctx, monkeypatch):
+    """The library asks for finite volumes.  A non-finite voxel makes the
+    projection of the rays through it non-finite in both read paths; a ray
+    that only grazes it (a zero-length segment) may come out finite or not,
+    depending on the path.  Every ray that passes nowhere near it keeps its
+    value, bit for bit, in both paths."""
+    rng = np.random.default_rng(44)
+    g = ctr.geometry.cone_circular(40, 1.0, 30, 72, 56, 1.0, 150.0, 300.0)
+    vol = rng.random((40, 40, 40)).astype(np.float32)
+    k = (17, 22, 9)
+    bad = vol.copy()
+    bad[k] = np.inf
+    near = np.zeros_like(vol)
+    near[k[0] - 1:k[0] + 2, k[1] - 1:k[1] + 2, k[2] - 1:k[2] + 2] = 1.0
+    one = np.zeros_like(vol)
+    one[k] = 1.0
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("CTR_FP_SKIP", flag)
+        c = ctr.Context(0)
+        try:
+            out[flag] = np.array(c.cone_fp(g, bad.ravel()), copy=True)
+            ref = np.array(c.cone_fp(g, vol.ravel()), copy=True)
+            through = np.array(c.cone_fp(g, one.ravel()), copy=True) > 0
+            close = np.array(c.cone_fp(g, near.ravel()), copy=True) > 0
+        finally:
+            c.close()
+        assert not np.isfinite(out[flag][through]).any(), flag
+        assert np.array_equal(out[flag][~close], ref[~close]), flag
+    assert through.sum() > 0 and (~close).sum() > 0.9 * g.nray

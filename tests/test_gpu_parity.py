@@ -460,3 +460,42 @@ def test_par_fbp_disk_full_size(ctx):
            ring_mean=float(img[ring].mean() / A))
     assert abs(img[inner].mean() / A - 1) < 1e-3 and img[inner].std() / A < 1e-3
     assert abs(img[ring].mean() / A) < 2e-3
+
+
+def test_fdk_rows_in_path_matches_masked_path_and_oracle(ctx, oracle):
+    """A detector narrow in u and tall in v: tiles whose footprint leaves the
+    detector only in u take the clamped interior loop (rows-in).  The plan
+    must contain such pairs; the result must equal the masked loop's
+    (CTR_BP_ROWSIN=0) bit for bit and the oracle away from the u edges."""
+    import os
+    import ctr
+    n = 48
+    beta = ((2 * np.pi / 40) * np.arange(1, 41)).astype(np.float32)
+    p = (0.9 * (np.arange(30) - 14.5)).astype(np.float32)          # +-13 mm: the 48 mm volume leaves it in u
+    xi = (0.8 * (np.arange(110) - 54.5) + 0.05).astype(np.float32)  # +-43.6 mm: every row inside
+    g = ctr.ConeGeom(n, n, n, (48.0, 48.0, 48.0), 150.0, beta, p, xi)
+    og = oracle.Cone(n, n, n, (48.0, 48.0, 48.0), 150.0, beta, p, xi)
+    stats = ctx.bp_plan_stats(g)
+    y = np.random.default_rng(61).standard_normal(g.nray).astype(np.float32)
+    fast = ctx.fdk_bp(g, y)
+    os.environ["CTR_BP_ROWSIN"] = "0"
+    try:
+        masked = ctx.fdk_bp(g, y)
+    finally:
+        del os.environ["CTR_BP_ROWSIN"]
+    with oracle.interp_mode(1):
+        ref = oracle.fdk_bp(og, y)
+    # columns whose centre projects within 1e-3 mm of the first/last column in some view
+    c = (np.arange(n) + 0.5) - n / 2
+    X, Y = c[None, :], c[:, None]
+    near = np.zeros((n, n), bool)
+    for b in beta.astype(np.float64):
+        P = (X * np.cos(b) + Y * np.sin(b)) * 150.0 / (150.0 - (-X * np.sin(b) + Y * np.cos(b)))
+        near |= (np.abs(P - p[0]) < 1e-3) | (np.abs(P - p[-1]) < 1e-3)
+    keep = ~np.broadcast_to(near, (n, n, n)).ravel()
+    e, m = errs(fast[keep], ref[keep])
+    report("fdk_bp[rows-in]", **stats, identical_to_masked=bool(np.array_equal(fast, masked)), rel_l2=e, max_rel=m,
+           edge_columns=int(near.sum()))
+    assert stats["rows_in"] > 0.2 * stats["pairs"]
+    assert np.array_equal(fast, masked)
+    assert e < 3e-5 and m < 1e-4

@@ -491,3 +491,271 @@ def test_fdk_error_slice_linear_and_z_mirror_symmetric(oracle):
     sym = (f + f[::-1]).astype(np.float32)
     e = oracle.fdk_error_slice(a, sym.ravel()).reshape(24, 16, 16)
     assert np.abs(e - e[:, ::-1, :]).max() < 1e-5 * np.abs(e).max()
+
+
+# ---- bilinear interpolation on affine data ----------------------------------
+# Linear interpolation reproduces data that are affine in the sample index
+# exactly, so a backprojection of projections R[b][ix][ip] = a_b ip + c_b ix + e_b
+# has the closed form sum_b (a_b tp + c_b tx + e_b) * weight_b, with (tp, tx)
+# the fractional sample indices of the voxel's detector point.  Per-view
+# coefficients and a != c catch a swapped axis, a flipped weight or a wrong
+# view; interpolation mode 1 runs on a grid that is not exact in float32,
+# mode 0 (the reference's first-pair pitch) on one that is.
+
+def _frac_index(pos32, x):
+    """Fractional index of x on the uniform grid through the end samples."""
+    p0, pn, n = float(pos32[0]), float(pos32[-1]), len(pos32)
+    return (x - p0) / ((pn - p0) / (n - 1))
+
+
+def _on_detector(pos32, x):
+    lo, hi = min(float(pos32[0]), float(pos32[-1])), max(float(pos32[0]), float(pos32[-1]))
+    return (x >= lo) & (x < hi)
+
+
+def _affine_views(rng, nb):
+    # multiples of 1/64: the projections are exact in float32
+    a, c, e = (rng.integers(-64, 65, nb) / 64.0 for _ in range(3))
+    return a, c, e
+
+
+_BILINEAR_GRIDS = [(1, 0.7, 0.55), (0, 1.0, 0.75)]   # (interp mode, column pitch, row pitch)
+
+
+def _cone_affine_case(oracle, mode, pp, px, seed):
+    og = small_cone(oracle, nt=8, ns=9, nz=7, size=(8.0, 9.9, 6.3), np_=17, nx=13, pp=pp, px=px, D=30.0)
+    nb, np_, nx = len(og.beta), len(og.p), len(og.xi)
+    a, c, e = _affine_views(np.random.default_rng(seed), nb)
+    R = (a[:, None, None] * np.arange(np_)[None, None, :] + c[:, None, None] * np.arange(nx)[None, :, None]
+         + e[:, None, None])
+    assert np.array_equal(R.astype(np.float32).astype(np.float64), R)
+    x, y, z, r = _mag_geometry(og)
+    views = []
+    for k, b in enumerate(og.beta.astype(np.float64)):
+        u = x[None, None, :] * np.cos(b) + y[None, :, None] * np.sin(b)
+        w = -x[None, None, :] * np.sin(b) + y[None, :, None] * np.cos(b)
+        m = og.s.D / (og.s.D - w)
+        P = (u * m) + 0 * z[:, None, None]
+        X = z[:, None, None] * m
+        ok = _on_detector(og.p, P) & _on_detector(og.xi, X)
+        val = a[k] * _frac_index(og.p, P) + c[k] * _frac_index(og.xi, X) + e[k]
+        views.append((b, ok, val, m))
+    return og, R.astype(np.float32).ravel(), views, (x, y, z, r)
+
+
+@pytest.mark.parametrize("mode,pp,px", _BILINEAR_GRIDS)
+def test_fdk_bp_affine_closed_form(oracle, mode, pp, px):
+    """FDK.cu:80-123: bilinear sample x mag^2 x dbeta."""
+    og, R, views, _ = _cone_affine_case(oracle, mode, pp, px, 41)
+    dbeta = abs(float(og.beta[1]) - float(og.beta[0]))
+    ref = sum(np.where(ok, val * m * m * dbeta, 0.0) for _, ok, val, m in views)
+    with oracle.interp_mode(mode):
+        out = oracle.fdk_bp(og, R).reshape(ref.shape)
+    assert np.abs(out - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.count_nonzero(ref) > 0.5 * ref.size
+
+
+@pytest.mark.parametrize("mode,pp,px", _BILINEAR_GRIDS)
+def test_cone_bp_ref_affine_closed_form(oracle, mode, pp, px):
+    """Backprojection3D.cu:68-288: bilinear sample x chord of the
+    source-to-voxel-centre line through the voxel."""
+    og, R, views, (x, y, z, r) = _cone_affine_case(oracle, mode, pp, px, 42)
+    D = og.s.D
+    ref = 0.0
+    for b, ok, val, _ in views:
+        dx = x[None, None, :] + D * np.sin(b) + 0 * z[:, None, None] + 0 * y[None, :, None]
+        dy = y[None, :, None] - D * np.cos(b) + 0 * dx
+        dz = z[:, None, None] + 0 * dx
+        with np.errstate(divide="ignore"):
+            half = np.minimum(np.minimum(r[0] / 2 / np.abs(dx), r[1] / 2 / np.abs(dy)), r[2] / 2 / np.abs(dz))
+        ref = ref + np.where(ok, val * 2 * half * np.sqrt(dx * dx + dy * dy + dz * dz), 0.0)
+    with oracle.interp_mode(mode):
+        out = oracle.cone_bp_ref(og, R).reshape(ref.shape)
+    assert np.abs(out - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+def _fan_case(oracle, pp, equiangular, nb=13):
+    import ctr
+    D, n = 40.0, 11
+    beta = (2 * np.pi / nb) * np.arange(1, nb + 1)
+    if equiangular:
+        g = ctr.geometry.fan_equiangular(n, 1.1, nb, 27, 0.9, D, 2 * D)
+        p = g.p
+    else:
+        p = pp * (np.arange(27) - 13.0)
+    og = oracle.Fan(n, n - 2, (n * 1.1, (n - 2) * 0.95), D, beta, p)
+    x = (np.arange(n) + 0.5) * 1.1 - n * 1.1 / 2
+    y = (np.arange(n - 2) + 0.5) * 0.95 - (n - 2) * 0.95 / 2
+    return og, x[None, :], y[:, None]
+
+
+def _fan_affine(rng, og):
+    a, _, e = _affine_views(rng, len(og.beta))
+    R = a[:, None] * np.arange(len(og.p))[None, :] + e[:, None]
+    return a, e, R.astype(np.float32).ravel()
+
+
+@pytest.mark.parametrize("mode,pp", [(1, 0.7), (0, 0.5)])
+def test_fan_fbp_flat_affine_closed_form(oracle, mode, pp):
+    """FBPfan.cu:45-91: linear sample at p/U, weighted 1/U^2 x dbeta (float angles)."""
+    og, x, y = _fan_case(oracle, pp, False)
+    a, e, R = _fan_affine(np.random.default_rng(43), og)
+    p32 = og.p.astype(np.float32)
+    b32 = og.beta.astype(np.float32).astype(np.float64)
+    D, dbeta = og.s.D, abs(b32[1] - b32[0])
+    ref = 0.0
+    for k, b in enumerate(b32):
+        u = x * np.cos(b) + y * np.sin(b)
+        w = -x * np.sin(b) + y * np.cos(b)
+        U = (D - w) / D
+        ok = _on_detector(p32, u / U)
+        ref = ref + np.where(ok, (a[k] * _frac_index(p32, u / U) + e[k]) / (U * U) * dbeta, 0.0)
+    with oracle.interp_mode(mode):
+        out = oracle.fan_fbp_bp(og, R, False).reshape(ref.shape)
+    assert np.abs(out - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+def test_fan_fbp_equiangular_affine_closed_form(oracle):
+    """FanProjection.m:299-324: linear sample at gamma = atan2(u, |w - D|) on the
+    channel-angle grid, weighted 1/L^2, times dbeta."""
+    og, x, y = _fan_case(oracle, None, True)
+    a, e, R = _fan_affine(np.random.default_rng(44), og)
+    gam = np.arctan(og.p / og.s.D)            # channel angles of the double positions
+    D, dbeta = og.s.D, abs(og.beta[1] - og.beta[0])
+    ref = 0.0
+    for k, b in enumerate(og.beta):
+        u = x * np.cos(b) + y * np.sin(b)
+        w = -x * np.sin(b) + y * np.cos(b)
+        gb = np.arctan2(u, np.abs(w - D))
+        ok = _on_detector(gam, gb)
+        ref = ref + np.where(ok, (a[k] * _frac_index(gam, gb) + e[k]) / (u * u + (w - D) ** 2), 0.0)
+    ref = ref * dbeta
+    with oracle.interp_mode(1):
+        out = oracle.fan_fbp_bp(og, R, True).reshape(ref.shape)
+    assert np.abs(out - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("mode,pp", [(1, 0.7), (0, 0.5)])
+def test_fan_bp_ref_affine_closed_form(oracle, mode, pp):
+    """BackprojectionFan2D.cu:36-113 (entry face corrected): linear sample at
+    p/U times the chord of the source-to-pixel-centre line through the pixel."""
+    og, x, y = _fan_case(oracle, pp, False)
+    a, e, R = _fan_affine(np.random.default_rng(45), og)
+    p32 = og.p.astype(np.float32)
+    D = og.s.D
+    rt, rs = og.s.size[0] / og.s.nt, og.s.size[1] / og.s.ns
+    ref = 0.0
+    for k, b in enumerate(og.beta.astype(np.float32).astype(np.float64)):
+        u = x * np.cos(b) + y * np.sin(b)
+        w = -x * np.sin(b) + y * np.cos(b)
+        Pd = u / ((D - w) / D)
+        dx, dy = x + D * np.sin(b) + 0 * y, y - D * np.cos(b) + 0 * x
+        with np.errstate(divide="ignore"):
+            chord = np.sqrt(dx * dx + dy * dy) * np.minimum(rt / np.abs(dx), rs / np.abs(dy))
+        ok = _on_detector(p32, Pd)
+        ref = ref + np.where(ok, (a[k] * _frac_index(p32, Pd) + e[k]) * chord, 0.0)
+    with oracle.interp_mode(mode):
+        out = oracle.fan_bp_ref(og, R).reshape(ref.shape)
+    assert np.abs(out - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.count_nonzero(ref) == ref.size
+
+
+def test_fan_bp_ref_quirk_chord_to_far_end(oracle):
+    """CTR_REF_QUIRKS (BackprojectionFan2D.cu:54-109): the entry point is the
+    crossing of the source-to-centre line with the pixel's t-low edge when
+    that lies in [0, s_high], else with its s-low edge when that lies in
+    [0, t_high]; the weight is 2 |entry - F|, F the ray's end point 2D from
+    the source.  Ones in, so the sum over views of those lengths comes out."""
+    og, x, y = _fan_case(oracle, 0.5, False)
+    D = og.s.D
+    nt, ns = og.s.nt, og.s.ns
+    rt, rs = og.s.size[0] / nt, og.s.size[1] / ns
+    Ct, Cs = og.s.size[0] / 2, og.s.size[1] / 2
+    p32 = og.p.astype(np.float32)
+    T = np.arange(nt)[None, :] + 0 * y
+    S = np.arange(ns)[:, None] + 0 * x
+    ref = np.zeros((ns, nt))
+    for b in og.beta.astype(np.float32).astype(np.float64):
+        u = x * np.cos(b) + y * np.sin(b)
+        w = -x * np.sin(b) + y * np.cos(b)
+        Pd = u / ((D - w) / D)
+        src = np.array([Ct - D * np.sin(b), Cs + D * np.cos(b)])
+        th = np.arctan(Pd / D)
+        F = (Ct + 2 * D * np.sin(th) * np.cos(b) - (D - 2 * D * np.cos(th)) * np.sin(b),
+             Cs + 2 * D * np.sin(th) * np.sin(b) + (D - 2 * D * np.cos(th)) * np.cos(b))
+        # the source-to-F line crosses t = T rt at s = src_s + (T rt - src_t) (F_s - src_s)/(F_t - src_t)
+        tl = T * rt
+        s_at = src[1] + (tl - src[0]) * (F[1] - src[1]) / (F[0] - src[0])
+        sl = S * rs
+        t_at = src[0] + (sl - src[1]) * (F[0] - src[0]) / (F[1] - src[1])
+        use_t = (s_at >= 0) & (s_at <= (S + 1) * rs)
+        use_s = ~use_t & (t_at >= 0) & (t_at <= (T + 1) * rt)
+        et = np.where(use_t, tl, t_at)
+        es = np.where(use_t, s_at, sl)
+        L = 2 * np.sqrt((et - F[0]) ** 2 + (es - F[1]) ** 2)
+        ok = _on_detector(p32, Pd) & (use_t | use_s)
+        ref += np.where(ok, L, 0.0)
+    out = oracle.fan_bp_ref(og, np.ones(og.nray, np.float32), quirks=True).reshape(ns, nt)
+    assert np.abs(out - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+# ---- the error slice's weight factor ------------------------------------------
+
+def test_fdk_error_slice_weight_for_volume_linear_in_z(oracle):
+    """IterativeFDKerrCompute.cu:66-70 weights each difference of two voxels
+    by 0.5 LR^2 sqrt(1 + (P/D)^2) G[f] D/sqrt(D^2 + P^2) dU dP dbeta, and the
+    two square-root factors cancel.  For a volume k (16 + 2 j + i)/16 (k, j, i
+    the z, s, t voxel indices) every difference is exact in float32,
+    (k' - k) (16 + 2 j' + i')/16, so with random (asymmetric) taps G the
+    sum is fixed by the voxel indices the kernel looks up (restated from
+    :37-58 in float32, as the kernel computes them) and the weight
+    0.5 LR^2 G[f] dU dP dbeta.  A dropped or extra factor, a wrong power of
+    LR or a mirrored tap index fails it."""
+    n, t = 16, 5
+    from ctr.geometry import iterative_fdk_args
+    A = iterative_fdk_args(n, n_views=24, pint=1.6)
+    u = A["u"][::3]
+    rng = np.random.default_rng(46)
+    G = rng.standard_normal(len(A["g"]))
+    a = oracle.FdkErr(n, n, n, A["size"], A["D"], A["beta"], u, A["p"], G, t)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    vol = (k * (16 + 2 * j + i) / 16.0).astype(np.float32)
+    out = oracle.fdk_error_slice(a, vol.ravel()).reshape(len(A["beta"]), n, n)
+
+    f32 = np.float32
+    D, size = A["D"], A["size"]
+    r = [size[q] / n for q in range(3)]
+    C = [size[q] / 2 for q in range(3)]
+    P = A["p"]
+    PInt = abs(P[1] - P[0])
+    dbeta = abs(f32(A["beta"][1]) - f32(A["beta"][0]))
+    dU = abs(f32(u[1]) - f32(u[0]))
+    ref = np.zeros_like(out)
+    for ib, beta in enumerate(A["beta"]):
+        cb, sb = f32(np.cos(np.float64(beta))), f32(np.sin(np.float64(beta)))
+        it = f32((t + 0.5) * r[0] - C[0])
+        for z in range(n):
+            iz = f32((z + 0.5) * r[2] - C[2])
+            for s in range(n):
+                is_ = f32((s + 0.5) * r[1] - C[1])
+                dt = f32(it * cb) + f32(is_ * sb)
+                ds = f32(is_ * cb) - f32(it * sb)
+                LR = f32(D / (D - np.float64(ds)))
+                dtl = np.float64(f32(dt * LR))
+                ff = np.floor((dtl - P) / PInt) + len(P)                       # [np]
+                for U in u:
+                    U = f32(U)
+                    num = f32(f32(ds - U) * iz)
+                    Zd = f32(np.float64(iz) + np.float64(num) / (D - np.float64(ds)))
+                    T = (P * (D - np.float64(U)) / D).astype(f32)
+                    X = f32(T * cb) - f32(U * sb)
+                    Y = f32(T * sb) + f32(U * cb)
+                    fx = np.floor((X.astype(np.float64) + C[0]) / r[0])
+                    fy = np.floor((Y.astype(np.float64) + C[1]) / r[1])
+                    fz = np.floor((np.float64(Zd) + C[2]) / r[2])
+                    ok = (ff >= 0) & (ff < len(G)) & (fx >= 0) & (fx < n) & (fy >= 0) & (fy < n) & (fz >= 0) & (fz < n)
+                    diff = (fz - z) * (16 + 2 * fy + fx) / 16.0
+                    w = 0.5 * np.float64(LR) ** 2 * G[np.where(ok, ff, 0).astype(int)] * dU * PInt * dbeta
+                    ref[ib, z, s] += np.sum(np.where(ok, diff * w, 0.0))
+    assert np.abs(ref).max() > 0
+    assert np.abs(out - ref).max() < 1e-9 * np.abs(ref).max()
