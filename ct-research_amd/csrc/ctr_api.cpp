@@ -129,6 +129,10 @@ struct ctr_ctx {
     // the buffer may be reallocated, and sirt_norm sets it once V is written
     uint64_t vinv_key = 0;
     int fault = 0;   // CTR_FAULT test hook: 1 alloc, 2 launch, 3 copy
+    // device address of the library's fixed scratch array on this device
+    // (the allocation-free filter's tables), looked up at creation
+    void *scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
     std::vector<ctr_cnn *> nets;   // networks created on this context (released with it)
     std::map<uint64_t, Tuning> tune;   // autotuned choices per cone geometry
 };
@@ -1242,6 +1246,7 @@ ctr_status ctr_ctx_create(int device, ctr_ctx **out)
     if (e != hipSuccess) return fail(nullptr, CTR_EHIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
     ctr_ctx *ctx = new ctr_ctx();
     ctx->device = device;
+    ctx->scratch_err = noalloc_scratch(device, &ctx->scratch);
     ctx->fault = parse_fault();
     e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1410,6 +1415,44 @@ ctr_status ctr_cone_filter(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, con
     TRY(stage_out(ctx, dev, false, kWsTmp, filtered, nray3(g), &df));
     TRY(cone_filter_impl(ctx, g, filter, dp, df));
     return finish_out(ctx, dev, df, filtered, nray3(g));
+}
+
+ctr_status ctr_cone_filter_noalloc(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj,
+                                   float *filtered, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(check_filter_args(ctx, filter));
+    if (!(flags & CTR_DEVICE_PTRS))
+        return fail(ctx, CTR_EINVAL, "the allocation-free filter takes device pointers (CTR_DEVICE_PTRS)");
+    if (!proj || !filtered) return fail(ctx, CTR_EINVAL, "data pointer is NULL");
+    if (g->n_p > kNoAllocMaxCols || g->n_xi > kNoAllocMaxRows)
+        return fail(ctx, CTR_ETOOBIG, "the allocation-free filter holds at most %d columns and %d rows (got %d x %d)",
+                    kNoAllocMaxCols, kNoAllocMaxRows, g->n_p, g->n_xi);
+    if (!ctx->scratch)
+        return fail(ctx, CTR_EHIP, "no scratch array on device %d: %s", ctx->device, hipGetErrorString(ctx->scratch_err));
+    const int n = g->n_p, L = fft_filter_length(n);
+    const double d = (double)std::fabs(g->p[1] - g->p[0]);
+    unsigned char *base = (unsigned char *)ctx->scratch;
+    float *pf = (float *)base, *xf = (float *)(base + kNoAllocOffXf);
+    double *taps = (double *)(base + kNoAllocOffTaps), *spec = (double *)(base + kNoAllocOffSpec);
+    if (ctx->fault == 3) return fail(ctx, CTR_EFAULT_INJECTED, "CTR_FAULT=copy: host-to-device copy");
+    HIP_TRY(ctx, hipMemcpyAsync(pf, g->p, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(xf, g->xi, sizeof(float) * g->n_xi, hipMemcpyHostToDevice, ctx->stream));
+    TRY(check_launch(ctx, launch_filter_tables(filter, (double)(float)d, n, L, taps, spec, ctx->stream),
+                     "filter tables"));
+    FilterParams F{};
+    F.np = n;
+    F.nx = g->n_xi;
+    F.rows = g->n_beta * g->n_xi;
+    F.mode = 0;
+    F.D = (float)g->distance;
+    F.pd = pf;
+    F.xd = xf;
+    F.taps = taps;
+    F.spec = L ? spec : nullptr;
+    F.fft_len = L;
+    return check_launch(ctx, launch_row_filter(F, proj, filtered, ctx->stream), "ramp filter");
 }
 
 ctr_status ctr_fdk_bp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *filtered, float *vol, unsigned flags)

@@ -271,6 +271,25 @@ int fft_filter_length(int np);
 hipError_t launch_box_copy(const float *src, int sx, int sy, int sx0, int sy0, int sz0, float *dst, int dx, int dy,
                            int dx0, int dy0, int dz0, int wx, int wy, int wz, hipStream_t st);
 
+// Allocation-free ramp filter: the library's fixed device array (one per
+// GPU, file-scope in k_filter.hip) holds, per call, the detector column and
+// row positions, the two-sided taps and the FFT spectrum.
+constexpr size_t kNoAllocScratchBytes = 128 * 1024;
+constexpr int kNoAllocMaxCols = 2048;     // taps: 2 * 2048 - 1 doubles
+constexpr int kNoAllocMaxRows = 8192;
+constexpr size_t kNoAllocOffXf = 8192;            // [kNoAllocMaxRows] float row positions
+constexpr size_t kNoAllocOffTaps = 40960;         // [2 * kNoAllocMaxCols - 1] double taps
+constexpr size_t kNoAllocOffSpec = 73728;         // [<= 2048] double spectrum
+static_assert(kNoAllocOffXf >= sizeof(float) * kNoAllocMaxCols, "scratch layout");
+static_assert(kNoAllocOffTaps - kNoAllocOffXf >= sizeof(float) * kNoAllocMaxRows, "scratch layout");
+static_assert(kNoAllocOffSpec - kNoAllocOffTaps >= sizeof(double) * (2 * kNoAllocMaxCols - 1), "scratch layout");
+static_assert(kNoAllocOffSpec + sizeof(double) * 2048 <= kNoAllocScratchBytes, "scratch layout");
+// Device address of the array on `device` (which must be current), looked up
+// once per device and kept.
+hipError_t noalloc_scratch(int device, void **ptr);
+// Taps h2[k] = d h[|k - (n-1)|] of the half SL / RL flat-detector filter and,
+// when L > 0, the spectrum of its circular kernel divided by L.
+hipError_t launch_filter_tables(int filter, double d, int n, int L, double *taps, double *spec, hipStream_t st);
 
 hipError_t launch_fan_bp_ref(const FanParams &P, const float *proj, float *img, hipStream_t st);
 hipError_t launch_fan_adjoint(const FanParams &P, const float *proj, float *img, const SirtUpdate *upd,

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 namespace ctr {
 namespace {
@@ -423,6 +424,88 @@ hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out,
     if (need <= 8) return launch_k<8, 4>(P, in, out, st);
     if (P.np <= 1024) return launch_k<16, 2>(P, in, out, st);
     return launch_k<16, 1>(P, in, out, st);
+}
+
+// ---------------------------------------------------------------------------
+// Tables of the allocation-free ramp filter.  `scratch` is a file-scope
+// device array: each GPU holds its own copy for the life of the process, and
+// every call on that GPU overwrites it.  Host code never passes the array's
+// name to a kernel (in host code that is a stand-in, not a device address):
+// noalloc_scratch looks up the device address with hipGetSymbolAddress once
+// per device, and the callers use that address.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices] = {};
+void *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Half-scaled flat-detector taps times the pitch, h[k] d (the closed forms of
+// the library's tap table: Shepp-Logan rounded through float as the
+// reference's float taps, FDK_GPU/FDK.cu:6-19; Ram-Lak zero at even k).
+__device__ double flat_tap(int filter, double d, int k)
+{
+    const double pi = 3.141592653589793;
+    double v;
+    if (filter == 0) {
+        v = -1.0 / ((d * pi) * (d * pi)) / (4.0 * k * k - 1.0);
+        v = (double)(float)v;
+    } else if (k == 0) {
+        v = 1.0 / (8.0 * d * d);
+    } else if (k % 2 == 0) {
+        v = 0.0;
+    } else {
+        v = -1.0 / (2.0 * pi * pi * (double)k * k * d * d);
+    }
+    return v * d;
+}
+
+// Block b writes taps[256 b ..] and, with L > 0, spec[256 b ..]: the one-sided
+// taps and the cosine table sit in LDS, and each thread sums one spectral
+// line H[k] = h[0] + 2 sum_m h[m] cos(2 pi k m / L) in FP64.
+__global__ __launch_bounds__(256) void k_filter_tables(int filter, double d, int n, int L, double *__restrict__ taps,
+                                                      double *__restrict__ spec)
+{
+    __shared__ double h[kNoAllocMaxCols / 2];
+    __shared__ double cosl[2048];
+    const int t = threadIdx.x, e = blockIdx.x * 256 + t;
+    if (e < 2 * n - 1) taps[e] = flat_tap(filter, d, abs(e - (n - 1)));
+    if (L == 0) return;   // direct convolution (uniform over the grid): no spectrum
+    for (int m = t; m < n; m += 256) h[m] = flat_tap(filter, d, m);
+    for (int i = t; i < L; i += 256) cosl[i] = cospi(2.0 * i / L);
+    __syncthreads();
+    if (e >= L) return;
+    double acc = 0.0;
+    for (int m = n - 1; m >= 1; --m) acc = fma(2.0 * h[m], cosl[(int)(((long long)e * m) & (L - 1))], acc);
+    spec[e] = (h[0] + acc) / L;
+}
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void **ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_err[device] = e;
+        g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+        g_scratch_done[device] = true;
+    }
+    *ptr = g_scratch_ptr[device];
+    return g_scratch_err[device];
+}
+
+hipError_t launch_filter_tables(int filter, double d, int n, int L, double *taps, double *spec, hipStream_t st)
+{
+    if (n < 1 || n > kNoAllocMaxCols || L > 2048 || (L & (L - 1)) != 0) return hipErrorInvalidValue;
+    const int blocks = std::max((2 * n - 1 + 255) / 256, L / 256);
+    hipLaunchKernelGGL(k_filter_tables, dim3(blocks), dim3(256), 0, st, filter, d, n, L, taps, spec);
+    return hipGetLastError();
 }
 
 }  // namespace ctr

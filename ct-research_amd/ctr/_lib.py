@@ -33,7 +33,7 @@ EXPORTS = [
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
-    "ctr_cone_bp_plan_stats", "ctr_cnn_forward_tiled",
+    "ctr_cone_bp_plan_stats", "ctr_cone_filter_noalloc", "ctr_cnn_forward_tiled",
 ]
 
 CNN_NUM_PARAMS = 57985
@@ -165,6 +165,8 @@ def lib():
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_uint]
         L.ctr_cone_bp_plan_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.ctr_cone_filter_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_uint]
         L.ctr_cnn_train_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,

@@ -206,6 +206,18 @@ ctr_status ctr_cone_bp_adjoint(ctr_ctx *ctx, const ctr_cone_geom *g, const float
 ctr_status ctr_cone_filter(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj, float *filtered,
                            unsigned flags);
 
+/* ctr_cone_filter without any allocation, for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Data pointers must be device memory
+ * (CTR_DEVICE_PTRS is required).  The detector positions, the taps and the
+ * FFT spectrum go to a fixed 128 KiB device array of the library, one per
+ * GPU, held until the process ends: at most 2048 detector columns and 8192
+ * rows (CTR_ETOOBIG beyond).  Every call on a GPU rewrites that array, so
+ * calls that overlap on one GPU (other streams, threads or contexts) race.
+ * The outputs agree with ctr_cone_filter's to FP64 rounding of the spectrum. */
+ctr_status ctr_cone_filter_noalloc(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, const float *proj,
+                                   float *filtered, unsigned flags);
+
 /* FDK backprojection of filtered projections: bilinear sample x mag^2 x dbeta.
  * Replaces the BackProjection kernel of FDK_GPU/FDK.cu:50-126. */
 ctr_status ctr_fdk_bp(ctr_ctx *ctx, const ctr_cone_geom *g, const float *filtered, float *vol, unsigned flags);
