@@ -27,7 +27,8 @@ thread_local std::string g_last_error;
 // before the context is looked at) happened on the thread.
 thread_local unsigned long long g_error_seq = 0;
 
-enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsT, kWsYn, kWsCount };
+enum Slot { kWsIn = 0, kWsOut, kWsTmp, kWsR, kWsVinv, kWsPart, kWsSums, kWsG, kWsS, kWsD, kWsX2, kWsR2, kWsT, kWsYn,
+            kWsMask, kWsXs, kWsCount };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -494,6 +495,8 @@ BpParams cone_bp_params_from(const ConeTables &T, const ctr_cone_geom *g, unsign
     P.rt = (float)(g->size[0] / g->nt);
     P.rs = (float)(g->size[1] / g->ns);
     P.rz = (float)(g->size[2] / g->nz);
+    P.rt_d = g->size[0] / g->nt;
+    P.rs_d = g->size[1] / g->ns;
     P.zc0 = (float)(0.5 * g->size[2] / g->nz - g->size[2] / 2);
     P.D = (float)g->distance;
     // uniform grid through the end samples: a pitch taken from the first two
@@ -2597,6 +2600,185 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
         if (o->accepted) o->accepted[k] = acc;
     }
     if (cur != x) HIP_TRY(ctx, hipMemcpyAsync(x, cur, nvox * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return finish_out(ctx, dev, x, x_in, nvox);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// SIRT emulated by a restricted Chebyshev polynomial
+// ===========================================================================
+namespace {
+
+// Chebyshev coefficients c_j (j = 0..K) of q(mu) = p_N(lambda mu) on mu in
+// [0, 1], interpolated at the K + 1 Chebyshev nodes, where p_N(s) =
+// sum_{i<N} (1 - s)^i = (1 - (1 - s)^N) / s: from x0, N plain SIRT steps
+// x <- x + lambda (g - M x) (M = V A'W A, spectrum in [0, 1]) give
+// x0 + p_N(lambda M) s0 with s0 the plain step at x0.
+std::vector<double> emulation_coeffs(int N, int K, double lambda)
+{
+    const double pi = 3.141592653589793;
+    const int n = K + 1;
+    std::vector<double> f(n), c(n);
+    for (int k = 0; k < n; ++k) {
+        const double mu = 0.5 * (std::cos(pi * (k + 0.5) / n) + 1.0), sv = lambda * mu;
+        if (sv < 1e-12) f[k] = N;
+        else if (sv < 1.0) f[k] = -std::expm1(N * std::log1p(-sv)) / sv;
+        else f[k] = (1.0 - std::pow(1.0 - sv, N)) / sv;
+    }
+    for (int j = 0; j < n; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < n; ++k) acc += f[k] * std::cos(pi * j * (k + 0.5) / n);
+        c[j] = 2.0 * acc / n;
+    }
+    c[0] *= 0.5;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+ctr_status ctr_sirt3d_emulated(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_in, float *x_in, ctr_cnn *net,
+                               const ctr_emulated_opts *o, unsigned flags)
+{
+    TRY(check_cone(g, 1));
+    TRY(need_ctx(ctx));
+    TRY(maybe_tune(ctx, g, flags));
+    if (!o) return fail(ctx, CTR_EINVAL, "options are NULL");
+    if (o->n_start < 0 || o->n_polish < 0) return fail(ctx, CTR_EINVAL, "iteration counts must be >= 0");
+    if (o->degree < 1 || o->degree > 1024) return fail(ctx, CTR_EINVAL, "degree must be in 1..1024");
+    if (o->n_target <= o->n_start) return fail(ctx, CTR_EINVAL, "n_target must exceed n_start");
+    if (!(o->lambda > 0.0f && o->lambda < 2.0f)) return fail(ctx, CTR_EINVAL, "lambda must be in (0, 2)");
+    if (std::isnan(o->min_val) || std::isnan(o->max_val) || o->min_val > o->max_val)
+        return fail(ctx, CTR_EINVAL, "clamp bounds must satisfy min_val <= max_val");
+    if (!std::isfinite(o->support_threshold)) return fail(ctx, CTR_EINVAL, "support_threshold must be finite");
+    if (o->cnn_tile < 0 || (o->cnn_tile > 0 && o->cnn_tile < 8)) return fail(ctx, CTR_EINVAL, "cnn_tile must be 0 or >= 8");
+    if (net) {
+        TRY(need_net(net));
+        if (net->ctx != ctx) return fail(ctx, CTR_EINVAL, "network belongs to another context");
+    }
+    TRY(cone_tables(ctx, g));
+    hipStream_t st = ctx->stream;
+    const RayParams R = cone_ray_params(ctx, g);
+    const BpParams P = cone_bp_params(ctx, g, 0);
+    TRY(ensure_footprints(ctx, P, ctx->foot_adj, true));
+    const BpPlan AL = ctx->foot_adj.plan;
+    const int4 *AF = (const int4 *)ctx->foot_adj.buf.ptr;
+    const size_t nvox = nvox3(g), nray = nray3(g);
+    const bool dev = flags & CTR_DEVICE_PTRS;
+    const float *b;
+    float *x;
+    TRY(stage_in(ctx, dev, kWsIn, b_in, nray, &b));
+    TRY(stage_out(ctx, dev, false, kWsOut, x_in, nvox, &x));
+    TRY(reserve_vinv(ctx, nvox * sizeof(float)));
+    for (int sl : {kWsG, kWsS, kWsD, kWsX2, kWsMask, kWsXs}) TRY(buf_reserve(ctx, ctx->ws[sl], nvox * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsR2], nray * sizeof(float)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * 4));
+    const bool tiled = net && o->cnn_tile > 0 && (g->nt > o->cnn_tile || g->ns > o->cnn_tile || g->nz > o->cnn_tile);
+    if (net) {
+        if (tiled) TRY(buf_reserve(ctx, net->full, nvox * sizeof(float)));
+        else TRY(cnn_reserve(net, nvox, false));
+    }
+    float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr;
+    float *cur = (float *)ctx->ws[kWsS].ptr, *prev = (float *)ctx->ws[kWsD].ptr, *acc = (float *)ctx->ws[kWsX2].ptr;
+    float *mask = (float *)ctx->ws[kWsMask].ptr, *xs = (float *)ctx->ws[kWsXs].ptr;
+    float *r = (float *)ctx->ws[kWsR].ptr, *zero = (float *)ctx->ws[kWsR2].ptr;
+    double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
+    TRY(sirt_norm(ctx, fnv(ctx->foot_adj.key, "cone", 4), nvox, st, [&](float *v) {
+        return check_launch(ctx, launch_adjoint(P, AL, AF, nullptr, v, nullptr, 1, st), "SIRT column sums");
+    }));
+    TRY(check_launch(ctx, launch_fill(zero, 0.0f, nray, st), "fill"));
+    TRY(check_launch(ctx, launch_fill(x, 0.0f, nvox, st), "fill"));
+    const bool fpT = tuning_for(ctx, ctx->cone.key).fp_transpose != 0;
+    auto host_sums = [&](double *out2) -> ctr_status {
+        HIP_TRY(ctx, hipMemcpyAsync(out2, sums, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        return CTR_OK;
+    };
+    double h[2];
+    int nb = 0;
+    TRY(check_launch(ctx, launch_sum_sq_sum(b, nray, part, &nb, st), "norm"));
+    TRY(check_launch(ctx, launch_reduce_partials(part, nb, sums, st), "reduce"));
+    TRY(host_sums(h));
+    const double bn = std::sqrt(h[0]) > 0 ? std::sqrt(h[0]) : 1.0;
+    // r = W (bb - A xx); with res, ||b - A xx|| / ||b|| (a host synchronisation)
+    auto residual = [&](const float *xx, const float *bb, double *res) -> ctr_status {
+        int nblk = 0;
+        RayParams Rx = R;
+        TRY(fp_transposed(ctx, Rx, xx, ctx->ws[kWsT], st, fpT));
+        TRY(check_launch(ctx, launch_ray_fp_residual(Rx, xx, bb, r, part, &nblk, st), "SIRT residual"));
+        if (!res) return CTR_OK;
+        TRY(check_launch(ctx, launch_reduce_partials(part, nblk, sums, st), "reduce"));
+        double hh[2];
+        TRY(host_sums(hh));
+        *res = std::sqrt(hh[0]) / bn;
+        return CTR_OK;
+    };
+    // n plain clamped SIRT iterations on x (SIRT3D.m:78-82)
+    auto plain = [&](int n) -> ctr_status {
+        SirtUpdate U{vinv, x, o->lambda, o->min_val, o->max_val};
+        for (int k = 0; k < n; ++k) {
+            TRY(residual(x, b, nullptr));
+            TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, nullptr, &U, 0, st), "SIRT update"));
+        }
+        return CTR_OK;
+    };
+    TRY(plain(o->n_start));
+    double res_start = 0.0;
+    TRY(residual(x, b, &res_start));
+    // support: the network's prediction from (x_k, s_k / rms(s_k)), or the iterate's own
+    if (net) {
+        TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
+        TRY(check_launch(ctx, launch_sirt_step(vinv, gb, o->lambda, nvox, cur, st), "SIRT step"));
+        TRY(check_launch(ctx, launch_sum_sq_sum(cur, nvox, part, &nb, st), "step norm"));
+        TRY(check_launch(ctx, launch_reduce_partials(part, nb, sums, st), "reduce"));
+        TRY(host_sums(h));
+        const double rms = std::sqrt(h[0] / (double)nvox);
+        const float gscale = rms > 0.0 && std::isfinite(rms) ? (float)(1.0 / rms) : 0.0f;
+        const CnnShape S{g->nt, g->ns, g->nz};
+        const float *prob = (const float *)net->out.ptr;
+        if (tiled) {
+            TRY(cnn_forward_tiled(net, S, x, cur, gscale, (float *)net->full.ptr, o->cnn_tile, 8));
+            prob = (const float *)net->full.ptr;
+        } else {
+            TRY(cnn_forward_dev(net, S, x, cur, gscale, false));
+        }
+        TRY(check_launch(ctx, launch_threshold_mask(prob, o->support_threshold, nvox, mask, st), "support"));
+    } else {
+        TRY(check_launch(ctx, launch_threshold_mask(x, o->min_val, nvox, mask, st), "support"));
+    }
+    // x_s + p_N(lambda M_S) s_S on the support S, N = n_target - n_start
+    const std::vector<double> c = emulation_coeffs(o->n_target - o->n_start, o->degree, o->lambda);
+    TRY(check_launch(ctx, launch_mul(x, mask, nvox, xs, st), "masked start"));
+    TRY(residual(xs, b, nullptr));
+    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
+    TRY(check_launch(ctx, launch_cheb_start(gb, vinv, mask, o->lambda, (float)c[0], nvox, cur, acc, st), "emulation"));
+    for (int j = 0; j < o->degree; ++j) {
+        TRY(residual(cur, zero, nullptr));   // r = -W A w_j
+        TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "emulation operator"));
+        TRY(check_launch(ctx, launch_cheb_step(gb, vinv, mask, cur, prev, acc, (float)c[j + 1], j == 0, nvox, st),
+                         "emulation"));
+        std::swap(cur, prev);
+    }
+    float *xe = prev;   // free once the recurrence is done
+    TRY(check_launch(ctx, launch_add_clamp(xs, acc, o->min_val, o->max_val, nvox, xe, st), "emulated iterate"));
+    double res_emul = 0.0;
+    TRY(residual(xe, b, &res_emul));
+    // the plain projector judges the emulated iterate: kept only if it lowers the residual
+    const int fell_back = res_emul < res_start ? 0 : 1;
+    if (!fell_back) HIP_TRY(ctx, hipMemcpyAsync(x, xe, nvox * sizeof(float), hipMemcpyDeviceToDevice, st));
+    TRY(plain(o->n_polish));
+    double res_end = 0.0;
+    if (o->residual_l2 || o->fell_back) TRY(residual(x, b, &res_end));
+    if (o->residual_l2) {
+        o->residual_l2[0] = res_start;
+        o->residual_l2[1] = res_emul;
+        o->residual_l2[2] = res_end;
+    }
+    if (o->fell_back) *o->fell_back = fell_back;
     return finish_out(ctx, dev, x, x_in, nvox);
 }
 

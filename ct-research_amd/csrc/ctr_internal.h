@@ -71,6 +71,7 @@ struct BpParams {
                             // input multiplied by it (this table itself for A'1)
     const double *view_d;   // [nb][4] sin, cos, source t, source s in FP64 (the projector's rays)
     const double *pdd;      // [np] detector columns in FP64
+    double rt_d, rs_d;      // voxel pitch in FP64
 };
 
 // The ray norm of detector sample (p, xi), in the float operations that the
@@ -266,6 +267,18 @@ hipError_t launch_iter_inc(int *iter, hipStream_t st);
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
 // transform length of the FFT filter path for np columns (0: direct convolution)
 int fft_filter_length(int np);
+// Restricted Chebyshev emulation of SIRT: mask = v > thr; out = a * m; the
+// masked plain step w0 = mask lambda V g with acc = c0 w0; one recurrence
+// term w_{j+1} = mask (2 (2M - I) w_j - w_{j-1}) (first: mask (2M - I) w_0)
+// from g = A'(-W A w_j), written over w_{j-1}, with acc += c w_{j+1};
+// out = clamp(a + b).
+hipError_t launch_threshold_mask(const float *v, float thr, size_t n, float *mask, hipStream_t st);
+hipError_t launch_mul(const float *a, const float *m, size_t n, float *out, hipStream_t st);
+hipError_t launch_cheb_start(const float *g, const float *vinv, const float *mask, float lambda, float c0, size_t n,
+                             float *w0, float *acc, hipStream_t st);
+hipError_t launch_cheb_step(const float *g, const float *vinv, const float *mask, const float *cur, float *prev,
+                            float *acc, float c, int first, size_t n, hipStream_t st);
+hipError_t launch_add_clamp(const float *a, const float *b, float lo, float hi, size_t n, float *out, hipStream_t st);
 // Copy a wx x wy x wz box from src (row sx, plane sx*sy) at (sx0, sy0, sz0) to
 // dst (row dx, plane dx*dy) at (dx0, dy0, dz0).
 hipError_t launch_box_copy(const float *src, int sx, int sy, int sx0, int sy0, int sz0, float *dst, int dx, int dy,

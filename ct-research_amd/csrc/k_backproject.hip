@@ -534,7 +534,13 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
         const int rlo_lim = fits ? f.y : 0, rhi_lim = fits ? f.y + H - 1 : P.nx - 1;
         cu0 = max(cu0, clo_lim);
         cu1 = min(cu1, chi_lim);
-        const float tlo = fmaf(fi, P.rt, -vw.st), slo = fmaf(fj, P.rs, -vw.ss);   // voxel low faces relative to S
+        // voxel low faces relative to the source, in FP64: for a ray nearly
+        // parallel to a slab (small a or b) the slab's parameter interval is
+        // this offset divided by that component, so its rounding is
+        // magnified; formed in float from a float source position it moved
+        // segment ends by up to 3e-3 mm at 256^3 / D = 600 mm
+        const float tlo = (float)((double)i * P.rt_d - P.view_d[4 * b + 2]);
+        const float slo = (float)((double)j * P.rs_d - P.view_d[4 * b + 3]);
         for (int cb = cu0; cb <= cu1; cb += CM) {
             float clo[CM], chi[CM];
 #pragma unroll

@@ -198,6 +198,56 @@ __global__ __launch_bounds__(256) void k_box_copy(const float *__restrict__ src,
     }
 }
 
+// ---- restricted Chebyshev emulation of SIRT (ctr_sirt3d_emulated) ----------
+__global__ __launch_bounds__(256) void k_threshold_mask(const float *__restrict__ v, float thr, size_t n,
+                                                        float *__restrict__ mask)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) mask[i] = v[i] > thr ? 1.0f : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_mul(const float *__restrict__ a, const float *__restrict__ m, size_t n,
+                                             float *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = a[i] * m[i];
+}
+
+// w0 = mask lambda V g (the masked plain step, g = A' W (b - A x)), acc = c0 w0
+__global__ __launch_bounds__(256) void k_cheb_start(const float *__restrict__ g, const float *__restrict__ vinv,
+                                                    const float *__restrict__ mask, float lambda, float c0, size_t n,
+                                                    float *__restrict__ w0, float *__restrict__ acc)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float w = mask[i] * (lambda * vinv[i] * g[i]);
+    w0[i] = w;
+    acc[i] = c0 * w;
+}
+
+// One term of the recurrence w_{j+1} = 2 (2M - I) w_j - w_{j-1} (w_1 = (2M - I) w_0),
+// restricted to the mask: g = A' (-W A w_j), so M w_j = -V g.  w_{j+1}
+// overwrites w_{j-1} in prev; acc += c w_{j+1}.
+__global__ __launch_bounds__(256) void k_cheb_step(const float *__restrict__ g, const float *__restrict__ vinv,
+                                                   const float *__restrict__ mask, const float *__restrict__ cur,
+                                                   float *__restrict__ prev, float *__restrict__ acc, float c,
+                                                   int first, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float xw = 2.0f * (-vinv[i] * g[i]) - cur[i];   // (2M - I) w_j
+    const float wn = mask[i] * (first ? xw : 2.0f * xw - prev[i]);
+    prev[i] = wn;
+    acc[i] = fmaf(c, wn, acc[i]);
+}
+
+__global__ __launch_bounds__(256) void k_add_clamp(const float *__restrict__ a, const float *__restrict__ b, float lo,
+                                                   float hi, size_t n, float *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = fminf(fmaxf(a[i] + b[i], lo), hi);
+}
+
 hipError_t launch_box_copy(const float *src, int sx, int sy, int sx0, int sy0, int sz0, float *dst, int dx, int dy,
                            int dx0, int dy0, int dz0, int wx, int wy, int wz, hipStream_t st)
 {
@@ -206,6 +256,40 @@ hipError_t launch_box_copy(const float *src, int sx, int sy, int sx0, int sy0, i
     const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_box_copy, dim3(blocks), dim3(256), 0, st, src, sx, sy, sx0, sy0, sz0, dst, dx, dy, dx0, dy0,
                        dz0, wx, wy, wz);
+    return hipGetLastError();
+}
+
+static unsigned ew_blocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_threshold_mask(const float *v, float thr, size_t n, float *mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_threshold_mask, dim3(ew_blocks(n)), dim3(256), 0, st, v, thr, n, mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_mul(const float *a, const float *m, size_t n, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mul, dim3(ew_blocks(n)), dim3(256), 0, st, a, m, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cheb_start(const float *g, const float *vinv, const float *mask, float lambda, float c0, size_t n,
+                             float *w0, float *acc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cheb_start, dim3(ew_blocks(n)), dim3(256), 0, st, g, vinv, mask, lambda, c0, n, w0, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_cheb_step(const float *g, const float *vinv, const float *mask, const float *cur, float *prev,
+                            float *acc, float c, int first, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cheb_step, dim3(ew_blocks(n)), dim3(256), 0, st, g, vinv, mask, cur, prev, acc, c, first, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_clamp(const float *a, const float *b, float lo, float hi, size_t n, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_add_clamp, dim3(ew_blocks(n)), dim3(256), 0, st, a, b, lo, hi, n, out);
     return hipGetLastError();
 }
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "ctr_sirt3d", "ctr_sirt2d",
     "ctr_cnn_create", "ctr_cnn_destroy", "ctr_cnn_set_params", "ctr_cnn_get_params", "ctr_cnn_forward",
     "ctr_cnn_train_step", "ctr_sirt3d_step", "ctr_learned_sirt3d", "ctr_sirt3d_batched", "ctr_autotune_cone",
-    "ctr_cone_bp_plan_stats", "ctr_cone_filter_noalloc", "ctr_cnn_forward_tiled",
+    "ctr_cone_bp_plan_stats", "ctr_cone_filter_noalloc", "ctr_cnn_forward_tiled", "ctr_sirt3d_emulated",
 ]
 
 CNN_NUM_PARAMS = 57985
@@ -102,6 +102,13 @@ class LearnedOptsC(ctypes.Structure):
                 ("max_val", ctypes.c_float), ("use_x0", ctypes.c_int),
                 ("residual_l2", ctypes.c_void_p), ("accepted", ctypes.c_void_p), ("judge_vs_plain", ctypes.c_int),
                 ("cnn_tile", ctypes.c_int)]
+
+
+class EmulatedOptsC(ctypes.Structure):
+    _fields_ = [("n_start", ctypes.c_int), ("n_target", ctypes.c_int), ("degree", ctypes.c_int),
+                ("n_polish", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
+                ("max_val", ctypes.c_float), ("support_threshold", ctypes.c_float), ("cnn_tile", ctypes.c_int),
+                ("residual_l2", ctypes.c_void_p), ("fell_back", ctypes.c_void_p)]
 
 
 _lib = None
@@ -176,6 +183,8 @@ def lib():
                                          ctypes.c_uint]
         L.ctr_sirt3d_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_float, ctypes.c_uint]
+        L.ctr_sirt3d_emulated.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
         L.ctr_learned_sirt3d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
         _lib = L
@@ -612,6 +621,16 @@ class Context:
                                           float(lam), 0))
         return s
 
+    def sirt3d_emulated(self, g, b, net=None, n_start=5, n_target=210, degree=28, n_polish=2, lam=1.0, lo=0.0,
+                        hi=np.inf, threshold=0.5, cnn_tile=0, flags=0):
+        """SIRT from zero: n_start plain iterations, the rest up to n_target
+        emulated by a degree-`degree` Chebyshev polynomial on the support the
+        network predicts (or the iterate's own), residual-judged, then
+        n_polish plain iterations.  Returns (x, [res_start, res_emulated,
+        res_end], fell_back)."""
+        return _emulated(self, g, b, net, n_start, n_target, degree, n_polish, lam, lo, hi, threshold, cnn_tile,
+                         flags)
+
     def learned_sirt3d(self, g, b, net, n_steps, lam=1.0, lo=0.0, hi=np.inf, x0=None, judge_vs_plain=False, flags=0,
                        cnn_tile=0):
         """Residual-judged learned SIRT; returns (x, residual_l2[n_steps+1], accepted[n_steps]).
@@ -633,6 +652,25 @@ class Context:
             self._check(fn(self.h, ctypes.byref(g.c), bb.ctypes.data, x.ctypes.data, net.h, ctypes.byref(opts),
                            flags))
         return x, r2, acc[:n_steps]
+
+
+def _emulated(ctx, g, b, net, n_start, n_target, degree, n_polish, lam, lo, hi, threshold, cnn_tile, flags):
+    res = np.zeros(3)
+    fb = np.zeros(1, dtype=np.int32)
+    opts = EmulatedOptsC(int(n_start), int(n_target), int(degree), int(n_polish), float(lam), float(lo), float(hi),
+                         float(threshold), int(cnn_tile), res.ctypes.data, fb.ctypes.data)
+    fn = lib().ctr_sirt3d_emulated
+    nh = net.h if net is not None else None
+    if _is_device(b):
+        import torch
+        ctx._bind_torch_stream()
+        x = torch.empty(g.nvox, dtype=torch.float32, device=b.device)
+        ctx._check(fn(ctx.h, ctypes.byref(g.c), b.data_ptr(), x.data_ptr(), nh, ctypes.byref(opts), DEVICE_PTRS | flags))
+    else:
+        bb = _f32(b)
+        x = np.zeros(g.nvox, dtype=np.float32)
+        ctx._check(fn(ctx.h, ctypes.byref(g.c), bb.ctypes.data, x.ctypes.data, nh, ctypes.byref(opts), flags))
+    return x, res, int(fb[0])
 
 
 def cnn_init_params(seed=0, last_scale=0.0):

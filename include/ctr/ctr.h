@@ -453,6 +453,39 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b,
 ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, float *x, ctr_cnn *net,
                               const ctr_learned_opts *opts, unsigned flags);
 
+/* ---- SIRT emulated by a restricted Chebyshev polynomial (no counterpart
+ * in the reference) ------------------------------------------------------
+ *
+ * Plain clamped SIRT from zero (SIRT3D.m:67-92) for n_start iterations, then
+ * the remaining n_target - n_start iterations emulated in degree operator
+ * applications: on a support S, x = x_s + p(lambda M_S) s_S, where M = V A'W A,
+ * M_S its restriction to S, s_S the masked plain step at the masked start
+ * x_s, and p the Chebyshev interpolant of p_N(s) = (1 - (1 - s)^N)/s on the
+ * spectrum [0, lambda].  S is {net(x, s / rms(s)) > support_threshold} with a
+ * support network (its output is a probability that the voxel is positive
+ * after n_target plain iterations), else {x > min_val}.  The plain projector
+ * judges the clamped result: it replaces x_{n_start} only when its data
+ * residual is lower, else the plain iterate stays.  Then n_polish plain
+ * clamped iterations.  One operator application costs about one SIRT
+ * iteration, so the call costs about n_start + degree + n_polish + 1 of
+ * them. */
+typedef struct ctr_emulated_opts {
+    int n_start;            /* plain iterations before the emulation */
+    int n_target;           /* plain iterations the result stands for, counted from zero */
+    int degree;             /* Chebyshev degree: operator applications of the emulation (1..1024) */
+    int n_polish;           /* plain iterations after it */
+    float lambda;           /* relaxation, in (0, 2) */
+    float min_val, max_val; /* clamp of every iterate */
+    float support_threshold;/* support from the network's output above this (with a network) */
+    int cnn_tile;           /* network tiles as in ctr_learned_opts */
+    double *residual_l2;    /* optional host [3]: ||b - A x|| / ||b|| after the start, of the emulated
+                               iterate, at the end */
+    int *fell_back;         /* optional host: 1 when the emulated iterate did not lower the residual */
+} ctr_emulated_opts;
+
+ctr_status ctr_sirt3d_emulated(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b, float *x, ctr_cnn *support_net,
+                               const ctr_emulated_opts *opts, unsigned flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -219,3 +219,52 @@ def test_learned_sirt_patchwise_equals_whole(ctx):
         net.close()
     report("learned_sirt[patch-wise]", identical=bool(np.array_equal(x0, x1)), accepted=a1.tolist())
     assert np.array_equal(x0, x1) and np.array_equal(r0, r1) and np.array_equal(a0, a1)
+
+
+@pytest.mark.gpu
+def test_emulated_sirt_is_exact_without_clamp(ctx):
+    """Without the clamp and on the whole volume the emulation is linear; a
+    Chebyshev degree of N - 1 interpolates the degree-(N-1) polynomial p_N
+    exactly, so the result is plain unclamped SIRT of n_target iterations."""
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    xe, res, fb = ctx.sirt3d_emulated(g, b, None, n_start=2, n_target=10, degree=7, n_polish=0, lo=-np.inf)
+    xp, rp, _ = ctx.sirt3d(g, b, 10, lo=-np.inf)
+    e = _rel(xe, xp)
+    report("sirt3d_emulated[exact]", x_rel=e, residuals=np.round(res, 6).tolist(), plain=float(rp[-1]), fell_back=fb)
+    assert e < 1e-4 and fb == 0
+    assert abs(res[1] - rp[-1]) < 1e-4 * rp[-1]
+
+
+@pytest.mark.gpu
+def test_emulated_sirt_judged_by_residual(ctx):
+    """A degree too low for the emulated count is still judged: the clamped
+    result is kept only when it lowers the residual; otherwise the output is
+    plain SIRT of n_start + n_polish iterations."""
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    for degree, n_target in ((12, 60), (1, 400)):
+        x, res, fb = ctx.sirt3d_emulated(g, b, None, n_start=3, n_target=n_target, degree=degree, n_polish=2)
+        report(f"sirt3d_emulated[K={degree}, N={n_target}]", residuals=np.round(res, 6).tolist(), fell_back=fb)
+        assert float(np.min(x)) >= 0.0
+        if fb:
+            xp, _, _ = ctx.sirt3d(g, b, 5)
+            assert _rel(x, xp) < 1e-5
+        else:
+            assert res[1] < res[0] and res[2] <= res[1] * (1 + 1e-6)
+
+
+@pytest.mark.gpu
+def test_emulated_sirt_with_support_network_runs(ctx):
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    net = ctr.Cnn(ctx, ctr.cnn_init_params(14, 0.5))
+    try:
+        x0, r0, f0 = ctx.sirt3d_emulated(g, b, net, n_start=3, n_target=40, degree=8, n_polish=1, threshold=0.0)
+        x1, r1, f1 = ctx.sirt3d_emulated(g, b, net, n_start=3, n_target=40, degree=8, n_polish=1, threshold=0.0,
+                                         cnn_tile=12)
+    finally:
+        net.close()
+    report("sirt3d_emulated[network support]", residuals=np.round(r0, 6).tolist(), fell_back=f0,
+           tiled_identical=bool(np.array_equal(x0, x1)))
+    assert np.array_equal(x0, x1) and float(np.min(x0)) >= 0.0

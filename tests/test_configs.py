@@ -300,8 +300,8 @@ class TestC2Cone:
         """The reference BP and the exact adjoint on 8 views against the
         oracle, and the adjoint identity over all 360.  The adjoint's
         interval ends are float parameters measured from the source, 600 mm
-        away, which bounds its agreement with the FP64 oracle here at
-        rel-L2 3e-5 and 1e-3 of the largest voxel."""
+        away (about 6e-5 mm each), which bounds its agreement with the FP64
+        oracle here at rel-L2 2e-5 and 1e-4 of the largest voxel."""
         g, _ = c2
         sel = slice(3, 360, 45)
         gs = sub_cone(g, sel)
@@ -317,7 +317,7 @@ class TestC2Cone:
         report("C2 bp", bp_ref_rel_l2=e1, bp_ref_max_rel=m1, adjoint_rel_l2=e2, adjoint_max_rel=m2,
                adjoint_gap_360=gap, **{f"adj_{k}": v for k, v in err_summary(adj, adj_ref).items()})
         assert e1 < 1e-5 and m1 < 1e-4
-        assert e2 < 3e-5 and m2 < 1e-3
+        assert e2 < 2e-5 and m2 < 1e-4
         assert gap < 1e-5
 
     def test_sirt_three_iterations(self, ctx, c2):
