@@ -186,24 +186,42 @@ struct FftPlan {
     static constexpr int NBF = RL > 1 ? 8 / RL : 1;           // last-pass butterflies per thread
 };
 
-// Per-thread twiddle bases, computed once per thread; the powers w^r are
-// formed by recurrence in each pass (relative error below 8 ulp).
+// Twiddle bases of the passes, in a block-shared LDS table filled once per
+// block (kept in registers they held 24 VGPRs across the whole row loop and
+// cost the kernel a wave per SIMD): [0, 8) w_64^i (radix-8 pass 2), [8, 72)
+// w_512^i (pass 3), [72, 72 + G) w_L^j.  The last pass's butterfly j + G m
+// needs w_L^(j + G m) = w_L^j w_8^m (G = L/8).  Powers w^r are formed by
+// recurrence in each pass (relative error below 8 ulp).
 template <int L>
 struct FftTwiddles {
-    double2 p2, p3;                     // radix-8 passes 2 and 3: w^(j mod Ns) of Ns R = 64, 512
-    double2 c[FftPlan<L>::NBF];         // last pass: w_L^(j + G m)
+    static constexpr int kOff512 = 8, kOffL = 72;
+    static constexpr int kSize = kOffL + (FftPlan<L>::RL > 1 ? FftPlan<L>::G : 0);
 };
 
 template <int L>
-__device__ __forceinline__ FftTwiddles<L> fft_twiddles(int j)
+__device__ __forceinline__ void fft_twiddle_table(double2 *tab)
 {
-    using F = FftPlan<L>;
-    FftTwiddles<L> tw;
-    tw.p2 = twiddle(j & 7, 64);
-    tw.p3 = twiddle(j & 63, 512);
-#pragma unroll
-    for (int m = 0; m < F::NBF; ++m) tw.c[m] = twiddle(j + F::G * m, L);
-    return tw;
+    using T = FftTwiddles<L>;
+    for (int i = threadIdx.x; i < T::kSize; i += kFftThreads) {
+        if (i < T::kOff512) tab[i] = twiddle(i, 64);
+        else if (i < T::kOffL) tab[i] = twiddle(i - T::kOff512, 512);
+        else tab[i] = twiddle(i - T::kOffL, L);
+    }
+}
+
+// w_8^m = exp(-2 pi i m / 8)
+__device__ __forceinline__ double2 w8(int m)
+{
+    switch (m & 7) {
+    case 0: return make_double2(1.0, 0.0);
+    case 1: return make_double2(kR2, -kR2);
+    case 2: return make_double2(0.0, -1.0);
+    case 3: return make_double2(-kR2, -kR2);
+    case 4: return make_double2(-1.0, 0.0);
+    case 5: return make_double2(-kR2, kR2);
+    case 6: return make_double2(0.0, 1.0);
+    default: return make_double2(kR2, kR2);
+    }
 }
 
 // v[r] *= w^r for r = 1..n-1
@@ -235,20 +253,22 @@ __device__ __forceinline__ void fft8_pass(double2 *v, double2 *buf, int j, int N
 }
 
 // One forward transform of the 8 points v[t] = x[j + G t] of this thread's
-// group; on return v[t] = X[j + G t].  buf is the group's padded LDS slot.
+// group; on return v[t] = X[j + G t].  buf is the group's padded LDS slot,
+// tab the block's twiddle table.
 template <int L>
-__device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const FftTwiddles<L> &tw)
+__device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const double2 *tab)
 {
     using F = FftPlan<L>;
+    using T = FftTwiddles<L>;
     fft8_pass<L>(v, buf, j, 1);
     if constexpr (F::NP8 == 2) {
         // L = 256: radix 8 (Ns = 8), then radix 4 (Ns = 64)
-        twiddle_powers(v, 8, tw.p2);
+        twiddle_powers(v, 8, tab[j & 7]);
         fft8_pass<L>(v, buf, j, 8);
     } else {
-        twiddle_powers(v, 8, tw.p2);
+        twiddle_powers(v, 8, tab[j & 7]);
         fft8_pass<L>(v, buf, j, 8);
-        twiddle_powers(v, 8, tw.p3);
+        twiddle_powers(v, 8, tab[T::kOff512 + (j & 63)]);
         if constexpr (F::RL == 1) {
             // L = 512: the third radix-8 pass is the last; thread j's outputs are
             // X[j + 64 r] = X[j + G r], already in place
@@ -265,7 +285,7 @@ __device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const
             double2 u[F::RL];
 #pragma unroll
             for (int r = 0; r < F::RL; ++r) u[r] = v[m + F::NBF * r];
-            twiddle_powers(u, F::RL, tw.c[m]);
+            twiddle_powers(u, F::RL, m ? cmul(tab[T::kOffL + j], w8(m)) : tab[T::kOffL + j]);
             if constexpr (F::RL == 2) dft2(u[0], u[1]);
             else dft4(u[0], u[1], u[2], u[3]);
 #pragma unroll
@@ -281,10 +301,11 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
     constexpr int G = FftPlan<L>::G, GPB = kFftThreads / G;
     constexpr int SLOT = L + L / 8;
     __shared__ double2 lds[GPB * SLOT];
+    __shared__ double2 tab[FftTwiddles<L>::kSize];
     const int grp = threadIdx.x / G, j = threadIdx.x % G;
     double2 *buf = lds + grp * SLOT;
     const int np = P.np;
-    const FftTwiddles<L> tw = fft_twiddles<L>(j);
+    fft_twiddle_table<L>(tab);   // published by the first barrier of the row loop
 
     // with one spectrum per detector row, a pair is one row index in two consecutive views
     const int nviews = P.rows / P.nx;
@@ -343,13 +364,13 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
         double2 v[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) v[t] = buf[fft_pad(j + G * t)];
-        fft_group<L>(v, buf, j, tw);
+        fft_group<L>(v, buf, j, tab);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const double h = spec[j + G * t];   // H[k] / L, real
             v[t] = make_double2(v[t].x * h, -v[t].y * h);
         }
-        fft_group<L>(v, buf, j, tw);
+        fft_group<L>(v, buf, j, tab);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int q = j + G * t;
