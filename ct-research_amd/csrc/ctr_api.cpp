@@ -2654,6 +2654,7 @@ ctr_status ctr_sirt3d_emulated(ctr_ctx *ctx, const ctr_cone_geom *g, const float
         return fail(ctx, CTR_EINVAL, "clamp bounds must satisfy min_val <= max_val");
     if (!std::isfinite(o->support_threshold)) return fail(ctx, CTR_EINVAL, "support_threshold must be finite");
     if (o->cnn_tile < 0 || (o->cnn_tile > 0 && o->cnn_tile < 8)) return fail(ctx, CTR_EINVAL, "cnn_tile must be 0 or >= 8");
+    if (o->support_degree < 0 || o->support_degree > 1024) return fail(ctx, CTR_EINVAL, "support_degree must be in 0..1024");
     if (net) {
         TRY(need_net(net));
         if (net->ctx != ctx) return fail(ctx, CTR_EINVAL, "network belongs to another context");
@@ -2750,21 +2751,30 @@ ctr_status ctr_sirt3d_emulated(ctr_ctx *ctx, const ctr_cone_geom *g, const float
     } else {
         TRY(check_launch(ctx, launch_threshold_mask(x, o->min_val, nvox, mask, st), "support"));
     }
-    // x_s + p_N(lambda M_S) s_S on the support S, N = n_target - n_start
-    const std::vector<double> c = emulation_coeffs(o->n_target - o->n_start, o->degree, o->lambda);
-    TRY(check_launch(ctx, launch_mul(x, mask, nvox, xs, st), "masked start"));
-    TRY(residual(xs, b, nullptr));
-    TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
-    TRY(check_launch(ctx, launch_cheb_start(gb, vinv, mask, o->lambda, (float)c[0], nvox, cur, acc, st), "emulation"));
-    for (int j = 0; j < o->degree; ++j) {
-        TRY(residual(cur, zero, nullptr));   // r = -W A w_j
-        TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "emulation operator"));
-        TRY(check_launch(ctx, launch_cheb_step(gb, vinv, mask, cur, prev, acc, (float)c[j + 1], j == 0, nvox, st),
-                         "emulation"));
-        std::swap(cur, prev);
+    // x_s + p_N(lambda M_S) s_S on the support S, N = n_target - n_start, into out
+    auto emulate = [&](int degree, float *out) -> ctr_status {
+        const std::vector<double> c = emulation_coeffs(o->n_target - o->n_start, degree, o->lambda);
+        float *w = cur, *wp = prev;
+        TRY(check_launch(ctx, launch_mul(x, mask, nvox, xs, st), "masked start"));
+        TRY(residual(xs, b, nullptr));
+        TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "SIRT gradient"));
+        TRY(check_launch(ctx, launch_cheb_start(gb, vinv, mask, o->lambda, (float)c[0], nvox, w, acc, st), "emulation"));
+        for (int j = 0; j < degree; ++j) {
+            TRY(residual(w, zero, nullptr));   // r = -W A w_j
+            TRY(check_launch(ctx, launch_adjoint(P, AL, AF, r, gb, nullptr, 0, st), "emulation operator"));
+            TRY(check_launch(ctx, launch_cheb_step(gb, vinv, mask, w, wp, acc, (float)c[j + 1], j == 0, nvox, st),
+                             "emulation"));
+            std::swap(w, wp);
+        }
+        return check_launch(ctx, launch_add_clamp(xs, acc, o->min_val, o->max_val, nvox, out, st), "emulated iterate");
+    };
+    // a first, low-degree pass refines the support: where its clamped result is positive
+    if (o->support_degree > 0) {
+        TRY(emulate(o->support_degree, prev));
+        TRY(check_launch(ctx, launch_threshold_mask(prev, o->min_val, nvox, mask, st), "support"));
     }
-    float *xe = prev;   // free once the recurrence is done
-    TRY(check_launch(ctx, launch_add_clamp(xs, acc, o->min_val, o->max_val, nvox, xe, st), "emulated iterate"));
+    float *xe = prev;   // the recurrence's buffers are free once it is done
+    TRY(emulate(o->degree, xe));
     double res_emul = 0.0;
     TRY(residual(xe, b, &res_emul));
     // the plain projector judges the emulated iterate: kept only if it lowers the residual

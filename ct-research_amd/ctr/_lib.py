@@ -108,7 +108,7 @@ class EmulatedOptsC(ctypes.Structure):
     _fields_ = [("n_start", ctypes.c_int), ("n_target", ctypes.c_int), ("degree", ctypes.c_int),
                 ("n_polish", ctypes.c_int), ("lambda_", ctypes.c_float), ("min_val", ctypes.c_float),
                 ("max_val", ctypes.c_float), ("support_threshold", ctypes.c_float), ("cnn_tile", ctypes.c_int),
-                ("residual_l2", ctypes.c_void_p), ("fell_back", ctypes.c_void_p)]
+                ("support_degree", ctypes.c_int), ("residual_l2", ctypes.c_void_p), ("fell_back", ctypes.c_void_p)]
 
 
 _lib = None
@@ -622,14 +622,14 @@ class Context:
         return s
 
     def sirt3d_emulated(self, g, b, net=None, n_start=5, n_target=210, degree=28, n_polish=2, lam=1.0, lo=0.0,
-                        hi=np.inf, threshold=0.5, cnn_tile=0, flags=0):
+                        hi=np.inf, threshold=0.5, cnn_tile=0, support_degree=0, flags=0):
         """SIRT from zero: n_start plain iterations, the rest up to n_target
         emulated by a degree-`degree` Chebyshev polynomial on the support the
         network predicts (or the iterate's own), residual-judged, then
         n_polish plain iterations.  Returns (x, [res_start, res_emulated,
         res_end], fell_back)."""
         return _emulated(self, g, b, net, n_start, n_target, degree, n_polish, lam, lo, hi, threshold, cnn_tile,
-                         flags)
+                         support_degree, flags)
 
     def learned_sirt3d(self, g, b, net, n_steps, lam=1.0, lo=0.0, hi=np.inf, x0=None, judge_vs_plain=False, flags=0,
                        cnn_tile=0):
@@ -654,11 +654,12 @@ class Context:
         return x, r2, acc[:n_steps]
 
 
-def _emulated(ctx, g, b, net, n_start, n_target, degree, n_polish, lam, lo, hi, threshold, cnn_tile, flags):
+def _emulated(ctx, g, b, net, n_start, n_target, degree, n_polish, lam, lo, hi, threshold, cnn_tile, support_degree,
+              flags):
     res = np.zeros(3)
     fb = np.zeros(1, dtype=np.int32)
     opts = EmulatedOptsC(int(n_start), int(n_target), int(degree), int(n_polish), float(lam), float(lo), float(hi),
-                         float(threshold), int(cnn_tile), res.ctypes.data, fb.ctypes.data)
+                         float(threshold), int(cnn_tile), int(support_degree), res.ctypes.data, fb.ctypes.data)
     fn = lib().ctr_sirt3d_emulated
     nh = net.h if net is not None else None
     if _is_device(b):

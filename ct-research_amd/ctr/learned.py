@@ -28,6 +28,14 @@ import numpy as np
 from . import _lib, geometry, phantoms
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "learned_sirt_256_90.npz")
+CORRECTION_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "emulated_correction_256_90.npz")
+
+# The emulated SIRT of the sparse-view problem: 8 plain iterations, the rest
+# of 215 emulated by a degree-28 Chebyshev polynomial on the iterate's
+# support; the two-pass variant starts at 6, refines the support with a
+# degree-14 pass and polishes with 2 plain iterations.
+EMULATED = dict(n_start=8, n_target=215, degree=28, n_polish=0)
+EMULATED_TWO_PASS = dict(n_start=6, n_target=215, degree=28, n_polish=2, support_degree=14)
 STARTS = (0, 1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48)
 AHEAD = 16
 SNAPSHOTS = tuple(sorted(set(STARTS) | {k + AHEAD for k in STARTS}))
@@ -174,7 +182,88 @@ def evaluate(ctx, net, seed=2000, n=256, views=90, n_plain=200, n_learned=10, lo
     return out
 
 
+def collect_correction(ctx, seeds, n=256, views=90, log=print):
+    """Records (x_e, s_e, rms, target) for the correction network: x_e the
+    emulated SIRT (EMULATED), s_e the plain step there, target
+    (x_200 - x_e - s_e) / rms(s_e) with x_200 plain SIRT of 200 iterations."""
+    import torch
+    recs = []
+    for seed in seeds:
+        t0 = time.perf_counter()
+        g, vol, b = make_problem(ctx, seed, n, views)
+        xe, _, _ = ctx.sirt3d_emulated(g, b, None, **EMULATED)
+        s = ctx.sirt3d_step(g, b, xe)
+        rms = float(torch.sqrt(torch.mean(s * s)))
+        x200, _, _ = ctx.sirt3d(g, b, 200, residuals=False)
+        recs.append((xe, s, rms, (x200 - xe - s) / rms))
+        torch.cuda.synchronize()
+        log(f"  phantom {seed}: {time.perf_counter() - t0:.1f} s", flush=True)
+    return recs
+
+
 def m2_measure(ctx, seed=2000, n=256, views=90, n_plain=200):
+    """The time-to-residual comparison on the held-out problem, as one dict.
+    Plain SIRT's residual after n_plain iterations is the target; the plain
+    time is the time plain SIRT takes to first reach it (per-iteration time
+    times the iteration).  Two methods:
+      * emulated + learned: EMULATED, then one step of the correction network
+        judged against the plain step (the packaged correction weights);
+      * two-pass emulated: EMULATED_TWO_PASS (no network).
+    Each reports its wall time, residual, image rel-L2 against plain SIRT's
+    200th iterate, the speed-up and the speed-up at equal residual (plain
+    time to the method's residual over the method's time)."""
+    import torch
+    g, vol, b = make_problem(ctx, seed, n, views)
+    ctx.sirt3d(g, b, 2)
+    ctx.sirt3d_emulated(g, b, None, n_start=2, n_target=10, degree=2, n_polish=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xp, rp, _ = ctx.sirt3d(g, b, n_plain)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    target = float(rp[-1])
+    k_first = int(np.argmax(rp <= target))
+    t_iter = t_plain / n_plain
+    nxp = float(torch.linalg.norm(xp))
+
+    def summary(name, x, res, t):
+        k_eq = int(np.argmax(rp <= res)) if np.any(rp <= res) else n_plain
+        return {"method": name, "s": round(t, 4), "residual": res, "reached_target": bool(res <= target),
+                "image_rel_l2_vs_plain200": float(torch.linalg.norm(x - xp)) / nxp,
+                "speedup": round(t_iter * k_first / t, 3),
+                "plain_iters_for_residual": k_eq, "speedup_at_equal_residual": round(t_iter * k_eq / t, 3)}
+
+    out = {"problem": f"{n}^3 random ellipsoids (seed {seed}), {views} views, 512^2 detector, Poisson I0=1e5",
+           "target_residual": target, "plain_iters_to_target": k_first, "plain_s_to_target": round(t_iter * k_first, 4),
+           "plain_ms_per_iter": round(1e3 * t_iter, 3), "methods": []}
+    net = None
+    if os.path.exists(CORRECTION_WEIGHTS):
+        net = _lib.Cnn(ctx, _lib.cnn_init_params(0, 0.0))
+        with np.load(CORRECTION_WEIGHTS, allow_pickle=False) as f:
+            net.set_params(f["params"])
+        xe, _, _ = ctx.sirt3d_emulated(g, b, None, **EMULATED)     # warm the network path too
+        ctx.learned_sirt3d(g, b, net, 1, x0=xe, judge_vs_plain=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xe, _, _ = ctx.sirt3d_emulated(g, b, None, **EMULATED)
+        xl, rl, acc = ctx.learned_sirt3d(g, b, net, 1, x0=xe, judge_vs_plain=True)
+        torch.cuda.synchronize()
+        out["methods"].append(dict(summary("emulated + learned correction", xl, float(rl[-1]),
+                                           time.perf_counter() - t0), accepted=int(acc[0]), **EMULATED))
+        net.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x2, r2, fb = ctx.sirt3d_emulated(g, b, None, **EMULATED_TWO_PASS)
+    torch.cuda.synchronize()
+    out["methods"].append(dict(summary("two-pass emulated", x2, float(r2[2]), time.perf_counter() - t0),
+                               fell_back=fb, **EMULATED_TWO_PASS))
+    best = min(out["methods"], key=lambda m: m["image_rel_l2_vs_plain200"])
+    out["best_image"] = best["method"]
+    torch.cuda.empty_cache()
+    return out
+
+
+def m2_measure_cnn_step(ctx, seed=2000, n=256, views=90, n_plain=200):
     """The time-to-residual comparison on the held-out problem, as one dict:
     plain SIRT's 200-iteration residual is the target; the plain time is the
     time plain SIRT takes to first reach it; the learned time is the wall
@@ -197,7 +286,7 @@ def m2_measure(ctx, seed=2000, n=256, views=90, n_plain=200):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=("train", "eval", "all"))
+    ap.add_argument("cmd", choices=("train", "eval", "all", "train-correction", "m2"))
     ap.add_argument("--out", default="learned_sirt_weights.npz")
     ap.add_argument("--weights", default=None)
     ap.add_argument("--phantoms", type=int, default=6)
@@ -211,6 +300,21 @@ def main(argv=None):
     import torch
     ctx = _lib.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    if args.cmd == "m2":
+        print(json.dumps(m2_measure(ctx, seed=args.seed, n=args.n)), flush=True)
+        ctx.close()
+        return 0
+    if args.cmd == "train-correction":
+        net = _lib.Cnn(ctx, _lib.cnn_init_params(2, 0.01))
+        print(f"collecting {args.phantoms} training phantoms (emulated SIRT and 200-iteration targets)", flush=True)
+        recs = collect_correction(ctx, range(3000, 3000 + args.phantoms), n=args.n)
+        print(f"training {args.steps} steps on {len(recs)} records", flush=True)
+        losses = train(net, recs, args.n, steps=args.steps, lr=args.lr)
+        np.savez(args.out, params=net.get_params(), losses=np.array(losses))
+        print(f"saved {args.out}", flush=True)
+        net.close()
+        ctx.close()
+        return 0
     net = _lib.Cnn(ctx, _lib.cnn_init_params(0, 0.0))
     if args.weights:
         with np.load(args.weights, allow_pickle=False) as f:

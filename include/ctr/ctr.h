@@ -463,12 +463,13 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
  * x_s, and p the Chebyshev interpolant of p_N(s) = (1 - (1 - s)^N)/s on the
  * spectrum [0, lambda].  S is {net(x, s / rms(s)) > support_threshold} with a
  * support network (its output is a probability that the voxel is positive
- * after n_target plain iterations), else {x > min_val}.  The plain projector
- * judges the clamped result: it replaces x_{n_start} only when its data
- * residual is lower, else the plain iterate stays.  Then n_polish plain
- * clamped iterations.  One operator application costs about one SIRT
- * iteration, so the call costs about n_start + degree + n_polish + 1 of
- * them. */
+ * after n_target plain iterations), else {x > min_val}.  With support_degree
+ * > 0 a first emulation of that degree runs on S, and the main one on the
+ * support of its clamped result.  The plain projector judges the clamped
+ * result: it replaces x_{n_start} only when its data residual is lower, else
+ * the plain iterate stays.  Then n_polish plain clamped iterations.  One
+ * operator application costs about one SIRT iteration, so the call costs
+ * about n_start + support_degree + degree + n_polish + 2 of them. */
 typedef struct ctr_emulated_opts {
     int n_start;            /* plain iterations before the emulation */
     int n_target;           /* plain iterations the result stands for, counted from zero */
@@ -478,6 +479,7 @@ typedef struct ctr_emulated_opts {
     float min_val, max_val; /* clamp of every iterate */
     float support_threshold;/* support from the network's output above this (with a network) */
     int cnn_tile;           /* network tiles as in ctr_learned_opts */
+    int support_degree;     /* > 0: a first emulation of this degree refines the support */
     double *residual_l2;    /* optional host [3]: ||b - A x|| / ||b|| after the start, of the emulated
                                iterate, at the end */
     int *fell_back;         /* optional host: 1 when the emulated iterate did not lower the residual */

@@ -268,3 +268,20 @@ def test_emulated_sirt_with_support_network_runs(ctx):
     report("sirt3d_emulated[network support]", residuals=np.round(r0, 6).tolist(), fell_back=f0,
            tiled_identical=bool(np.array_equal(x0, x1)))
     assert np.array_equal(x0, x1) and float(np.min(x0)) >= 0.0
+
+
+@pytest.mark.gpu
+def test_emulated_sirt_two_pass_support(ctx):
+    """A first low-degree pass refines the support; the result stays judged and
+    clamped, and an exact-degree two-pass run without the clamp is still plain
+    unclamped SIRT (the first pass then leaves every voxel in the support)."""
+    g, vol = _small_problem()
+    b = ctx.cone_fp(g, vol)
+    x, res, fb = ctx.sirt3d_emulated(g, b, None, n_start=3, n_target=60, degree=12, n_polish=1, support_degree=4)
+    report("sirt3d_emulated[two-pass]", residuals=np.round(res, 6).tolist(), fell_back=fb)
+    assert float(np.min(x)) >= 0.0
+    assert fb == 1 or (res[1] < res[0] and res[2] <= res[1] * (1 + 1e-6))
+    xe, _, _ = ctx.sirt3d_emulated(g, b, None, n_start=2, n_target=10, degree=7, n_polish=0, lo=-np.inf,
+                                   support_degree=3)
+    xp, _, _ = ctx.sirt3d(g, b, 10, lo=-np.inf)
+    assert _rel(xe, xp) < 1e-4
