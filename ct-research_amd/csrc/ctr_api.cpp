@@ -86,6 +86,7 @@ struct PhantomTable {
 
 struct FootCache {
     DevBuf buf;
+    DevBuf unfit;           // one int: footprints too large for the adjoint's LDS
     uint64_t key = 0;
     BpPlan plan{};
 };
@@ -543,7 +544,7 @@ Tuning tuning_for(const ctr_ctx *ctx, uint64_t key)
 ctr_status ensure_footprints_key(ctr_ctx *ctx, const BpParams &P, FootCache &fc, bool adjoint, uint64_t base_key)
 {
     const Tuning tu = tuning_for(ctx, base_key);
-    const BpPlan L = adjoint ? adjoint_plan(P, tu.adj_cm) : bp_plan(P, tu.bp_pitch);
+    BpPlan L = adjoint ? adjoint_plan(P, tu.adj_cm) : bp_plan(P, tu.bp_pitch);
     uint64_t key = base_key;
     key = fnv(key, &L, sizeof L);
     if (fc.buf.ptr && fc.key == key) return CTR_OK;
@@ -551,6 +552,21 @@ ctr_status ensure_footprints_key(ctr_ctx *ctx, const BpParams &P, FootCache &fc,
     if (s != CTR_OK) return s;
     s = check_launch(ctx, launch_bp_footprints(P, L, (int4 *)fc.buf.ptr, ctx->stream), "footprint table");
     if (s != CTR_OK) return s;
+    // the exact adjoint has a leaner variant for plans whose footprints all
+    // fit LDS: counted once per geometry (not while a graph is captured)
+    L.fit_all = 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (adjoint && hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+        s = buf_reserve(ctx, fc.unfit, sizeof(int));
+        if (s != CTR_OK) return s;
+        s = check_launch(ctx, launch_foot_unfit((const int4 *)fc.buf.ptr, L.nfoot, (int *)fc.unfit.ptr, ctx->stream),
+                         "footprint count");
+        if (s != CTR_OK) return s;
+        int unfit = -1;
+        HIP_TRY(ctx, hipMemcpyAsync(&unfit, fc.unfit.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        L.fit_all = unfit == 0 ? 1 : 0;
+    }
     fc.key = key;
     fc.plan = L;
     return CTR_OK;
@@ -2863,7 +2879,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     std::vector<FpJob> fj(n);
     std::vector<ReduceJob> rj(n);
     std::vector<std::vector<AdjointJob>> groups(3), ones_groups(3);   // by LDS pitch 40 / 72 / 136
-    std::vector<int> group_blocks(3, 0), group_cm(3, 4);
+    std::vector<int> group_blocks(3, 0), group_cm(3, 4), group_fit(3, 1);
     const int pitches[3] = {40, 72, 136};
     int fp_total = 0;
     for (int p = 0; p < n; ++p) {
@@ -2913,6 +2929,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
                                              group_blocks[gi], nwg});
         group_blocks[gi] += nwg;
         group_cm[gi] = L.cm;
+        group_fit[gi] = group_fit[gi] && L.fit_all;
         if (!o->use_x0) TRY(check_launch(ctx, launch_fill(q.xp, 0.0f, q.nvox, st), "fill"));
     }
     // job tables in device memory (adjoint groups packed one after another)
@@ -2942,7 +2959,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     // column sums V = 1/(A'1 + 1e-9) and ||b|| per problem
     for (int gi = 0; gi < 3; ++gi)
         if (!groups[gi].empty())
-            TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], dones + gofs[gi], (int)groups[gi].size(),
+            TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], group_fit[gi] != 0, dones + gofs[gi], (int)groups[gi].size(),
                                                          group_blocks[gi], 0, 1, st), "batched column sums"));
     for (int p = 0; p < n; ++p) {
         BatchProblem &q = B.probs[p];
@@ -2971,7 +2988,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         if (!update) return CTR_OK;
         for (int gi = 0; gi < 3; ++gi)
             if (!groups[gi].empty())
-                TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], dadj + gofs[gi], (int)groups[gi].size(),
+                TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], group_fit[gi] != 0, dadj + gofs[gi], (int)groups[gi].size(),
                                                              group_blocks[gi], 1, 0, s), "batched update"));
         return check_launch(ctx, launch_iter_inc(diter, s), "iteration counter");
     };

@@ -198,6 +198,7 @@ struct BpPlan {
     int faces;              // footprints of voxel boxes (exact adjoint) instead of voxel centres
     int tz, wp, capf, hmax;
     int cm;                 // exact adjoint: candidate detector columns per chunk
+    int fit_all;            // exact adjoint: every footprint fits LDS (set once the table is built)
     int tiles_t, tiles_s, tiles_z;
     size_t nfoot;
 };
@@ -214,6 +215,8 @@ hipError_t launch_adjoint(const BpParams &P, const BpPlan &L, const int4 *foot, 
 // Fills P.adj_inva, adj_invb, adj_invxi and adj_norm (the pointers name the
 // destinations; P's views, pd and xd must be on the device).
 hipError_t launch_adjoint_tables(const BpParams &P, hipStream_t st);
+// out[0] = number of footprints that are neither empty nor fit LDS (one block).
+hipError_t launch_foot_unfit(const int4 *foot, size_t n, int *out, hipStream_t st);
 // out[b][r][c] = in[b][r][c] * P.adj_norm[r][c] over all views.
 hipError_t launch_scale_by_norm(const BpParams &P, const float *in, float *out, hipStream_t st);
 
@@ -227,8 +230,10 @@ struct AdjointJob {
     SirtUpdate U;
     int block0, nblocks;    // this job's blocks in the launch
 };
-hipError_t launch_adjoint_batched(int wp, int cm, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
-                                  int ones_input, hipStream_t st);
+// fit_all: every job's plan has fit_all set (the variant without the
+// global-memory path for footprints too large for LDS).
+hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob *jobs, int njobs, int nblocks,
+                                  int sirt, int ones_input, hipStream_t st);
 
 // Batched SIRT residual pass: r = (b - A x)/(A1 + 1e-9) and per-block (sum e^2, sum |e|) per job.
 // The job of flat block `bid` in a batched launch: the last job whose first

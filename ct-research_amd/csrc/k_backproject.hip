@@ -447,10 +447,29 @@ __device__ __forceinline__ void column_box(const BpParams &P, const BpView &vw, 
 __device__ __forceinline__ void slice_rows(const BpParams &P, float zb, float zt, float mmin, float mmax, int &rv0,
                                            int &rv1)
 {
-    const float a = zb * mmin, b = zb * mmax, c = zt * mmin, d = zt * mmax;
-    float v0 = fmaf(fminf(fminf(a, b), fminf(c, d)), P.v_scale, P.v_off);
-    float v1 = fmaf(fmaxf(fmaxf(a, b), fmaxf(c, d)), P.v_scale, P.v_off);
-    if (v0 > v1) {
+    float zlo, zhi;
+    if (mmin > 0.0f) {
+        // zb < zt, and rounding is monotone, so the extreme products of
+        // {zb, zt} x {mmin, mmax} follow from the signs of zb and zt (the
+        // same for the whole block: a branch on uniform values)
+        if (zb >= 0.0f) {
+            zlo = zb * mmin;
+            zhi = zt * mmax;
+        } else if (zt <= 0.0f) {
+            zlo = zb * mmax;
+            zhi = zt * mmin;
+        } else {
+            zlo = zb * mmax;
+            zhi = zt * mmax;
+        }
+    } else {
+        const float a = zb * mmin, b = zb * mmax, c = zt * mmin, d = zt * mmax;
+        zlo = fminf(fminf(a, b), fminf(c, d));
+        zhi = fmaxf(fmaxf(a, b), fmaxf(c, d));
+    }
+    float v0 = fmaf(zlo, P.v_scale, P.v_off);
+    float v1 = fmaf(zhi, P.v_scale, P.v_off);
+    if (P.v_scale < 0.0f) {
         const float t = v0;
         v0 = v1;
         v1 = t;
@@ -460,20 +479,31 @@ __device__ __forceinline__ void slice_rows(const BpParams &P, float zb, float zt
     rv1 = (int)floorf(fminf(v1 + ev, (float)P.nx));
 }
 
+// Rows of the staged footprint and floats of LDS per adjoint block: the
+// footprint rows, the column tables, and the z-slab interval table of
+// (row, slice) pairs.
+template <int TZ, int WP, int CAPF>
+struct AdjLds {
+    static constexpr int kHmax = (CAPF - 2 * WP - 2 * kCMax) / (WP + 1);
+    static constexpr int kFloats = CAPF + 2 * TZ * kHmax;
+};
+
 // Exact adjoint (voxel-driven gather of Siddon segment lengths).  LDS holds,
 // for the view's footprint rectangle: y*|d| per sample (|d| = sqrt(p^2 +
-// D^2 + xi^2)), 1/a and 1/b per column and 1/xi per row, so a candidate
-// (ray, voxel) pair costs 5 VALU and one immediate-offset LDS read.
-template <int TZ, int WP, int CAPF, int CM>
+// D^2 + xi^2)), 1/a and 1/b per column, and per (row, slice) the z-slab
+// parameter interval [zl, zh] of that row's rays (every thread of the block
+// covers the same slices), so a candidate (ray, voxel) pair costs 4 VALU and
+// one immediate-offset LDS read, and a candidate row one 64-bit read.
+template <int TZ, int WP, int CAPF, int CM, bool FITALL>
 __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, float *__restrict__ vol, const BpParams &P,
                                              const int4 *__restrict__ foot, const BpView *__restrict__ views,
                                              const SirtUpdate &U, int sirt, int ones, int tile, float *lds)
 {
-    constexpr int HMAX = (CAPF - 2 * WP - 2 * kCMax) / (WP + 1);
+    constexpr int HMAX = AdjLds<TZ, WP, CAPF>::kHmax;
     float *yd = lds;                                // [HMAX][WP] (+ kCMax zero slack)
     float *inva = lds + HMAX * WP + kCMax;          // [WP + kCMax]
     float *invb = inva + WP + kCMax;                // [WP]
-    float *invxi = invb + WP;                       // [HMAX]
+    float2 *ztab = reinterpret_cast<float2 *>(lds + CAPF);   // [HMAX][TZ]: (zl, zh)
     const int tt = tile % P.tiles_t, rest = tile / P.tiles_t;
     const int ts = rest % P.tiles_s, tz = rest / P.tiles_s;
     const int tx = threadIdx.x & (kTX - 1), ty = threadIdx.x / kTX;
@@ -494,8 +524,11 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
     for (int b = 0; b < P.nb; ++b) {
         const int4 f = ft[b];
         if (f.w & kFootEmpty) continue;
+        // FITALL: the host found every footprint of the plan small enough
+        // for LDS and launched the variant without the global-memory path
+        if (FITALL && !(f.w & kFootFits)) continue;
         const int W = f.z & 0xffff, H = f.z >> 16;
-        const bool fits = (f.w & kFootFits) != 0;
+        const bool fits = FITALL || (f.w & kFootFits) != 0;
         const BpView vw = views[b];
         // the input times the ray norm; A'1 reads the norm table for every view
         const float *src = ones ? P.adj_norm : proj + (size_t)b * P.nx * P.np;
@@ -517,10 +550,14 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                 const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
                     (void *)(P.adj_invb + (size_t)b * P.np), 0, P.np * (int)sizeof(float), kBufDword3);
                 dma_span(rb, invb, 0u, f.x, W, lane);
-            } else if (wave == 2) {
-                const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *)P.adj_invxi, 0, P.nx * (int)sizeof(float), kBufDword3);
-                dma_span(rx, invxi, 0u, f.y, H, lane);
+            }
+            // z-slab intervals of the footprint rows for the tile's slices,
+            // formed as the global path below forms them (bit for bit)
+            for (int e = threadIdx.x; e < H * TZ; e += kThreads) {
+                const int r = e / TZ, k = e - r * TZ;
+                float zl, zh;
+                slab_interval(fmaf((float)k, P.rz, zbase), P.rz, P.adj_invxi[f.y + r], zl, zh);
+                ztab[e] = make_float2(zl, zh);
             }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
@@ -573,9 +610,14 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
                 rv1 = min(rv1, rhi_lim);
                 float a_k = 0.0f;
                 for (int r = rv0; r <= rv1; ++r) {
-                    const float ix = fits ? invxi[r - f.y] : P.adj_invxi[r];
                     float zl, zh;
-                    slab_interval(zb, P.rz, ix, zl, zh);
+                    if (fits) {
+                        const float2 zz = ztab[(r - f.y) * TZ + k];
+                        zl = zz.x;
+                        zh = zz.y;
+                    } else {
+                        slab_interval(zb, P.rz, P.adj_invxi[r], zl, zh);
+                    }
                     if (fits) {
                         const float *sy = yd + (r - f.y) * WP + (cb - f.x);
 #pragma unroll
@@ -612,36 +654,56 @@ __device__ __forceinline__ void adjoint_tile(const float *__restrict__ proj, flo
 }
 
 // The per-pair work is short dependent chains (slab clip, LDS read, fma), so
-// the adjoint runs latency-bound at low occupancy.  Four z-slices per thread
-// at four waves per SIMD measured 43.4 ms at 256^3 / 360 views, against
-// 56.6 ms for eight slices at the two waves that their 197 registers allow,
-// 48 ms for eight slices forced to three waves (spills) and 50 ms for four
-// slices at three.  With the staging reduced to copies and 3-column chunks:
-// four slices at five waves (96 VGPRs, a few spills) 31.2 ms, six slices at
-// four waves (128 VGPRs) 30.0 ms, eight at three 33.0 ms and eight at four
-// (spills) 31.9 ms, all with the input scaling pass.
-template <int TZ, int WP, int CAPF, int CM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
+// the adjoint runs latency-bound and occupancy decides its speed.  At 256^3 /
+// 360 views / 512^2 with the z-interval table and without the global path
+// (79-90 VGPRs): six slices at four waves 21.0 ms, at five 18.3 ms, at six
+// (spills) 17.3 ms; eight at four 19.3 ms; five at six 17.3 ms; four at six
+// 16.9 ms, at seven (spills) 17.6 ms, at eight 18.9 ms; three at seven or
+// eight 19.4 ms; two at eight 22.6 ms.  The variant with the global path
+// keeps four waves.
+template <int TZ, int WP, int CAPF, int CM, bool FITALL>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(FITALL ? 6 : 4, FITALL ? 6 : 4))) void k_adjoint(const float *__restrict__ proj, float *__restrict__ vol, BpParams P,
                                                       const int4 *__restrict__ foot, const BpView *__restrict__ views,
                                                       SirtUpdate U, int sirt, int ones)
 {
-    __shared__ float lds[CAPF];
-    adjoint_tile<TZ, WP, CAPF, CM>(proj, vol, P, foot, views, U, sirt, ones, xcd_tile(blockIdx.x, gridDim.x), lds);
+    __shared__ __align__(16) float lds[AdjLds<TZ, WP, CAPF>::kFloats];
+    adjoint_tile<TZ, WP, CAPF, CM, FITALL>(proj, vol, P, foot, views, U, sirt, ones, xcd_tile(blockIdx.x, gridDim.x),
+                                           lds);
 }
 
 // Many independent problems in one launch: block b belongs to the job whose
 // block range holds it (a scalar scan of the small job table).
-template <int TZ, int WP, int CAPF, int CM>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
+template <int TZ, int WP, int CAPF, int CM, bool FITALL>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(FITALL ? 6 : 4, FITALL ? 6 : 4))) void k_adjoint_batched(const AdjointJob *__restrict__ jobs, int njobs, int sirt,
                                                               int ones)
 {
-    __shared__ float lds[CAPF];
+    __shared__ __align__(16) float lds[AdjLds<TZ, WP, CAPF>::kFloats];
     const int p = find_job(jobs, njobs, (int)blockIdx.x);
     const BpParams P = jobs[p].P;
     const SirtUpdate U = jobs[p].U;
     const int local = (int)blockIdx.x - jobs[p].block0;
-    adjoint_tile<TZ, WP, CAPF, CM>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
-                                   xcd_tile(local, jobs[p].nblocks), lds);
+    adjoint_tile<TZ, WP, CAPF, CM, FITALL>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
+                                           xcd_tile(local, jobs[p].nblocks), lds);
+}
+
+// Number of (tile, view) footprints of a plan that are not empty and do not
+// fit the adjoint's LDS buffers; one block.
+__global__ __launch_bounds__(1024) void k_foot_unfit(const int4 *__restrict__ foot, long n, int *__restrict__ out)
+{
+    __shared__ int part[16];
+    int c = 0;
+    for (long e = threadIdx.x; e < n; e += 1024) {
+        const int w = foot[e].w;
+        c += !(w & kFootEmpty) && !(w & kFootFits);
+    }
+    for (int m = 32; m > 0; m >>= 1) c += __shfl_xor(c, m, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; ++w) t += part[w];
+        out[0] = t;
+    }
 }
 
 // Geometry tables of the exact adjoint: per (view, column) the reciprocals
@@ -680,7 +742,13 @@ __global__ __launch_bounds__(256) void k_scale_by_norm(const float *__restrict__
 constexpr int kBpTZ = 16;
 // (row pitch, rows) of the two FDK/BP_REF footprint buffers, 2 x 32 KB at most
 constexpr int kBpShapes[3][2] = {{64, 64}, {72, 56}, {80, 50}};
-constexpr int kAdjTZ = 6, kAdjCap = 6144;
+// Four z-slices per thread, six waves per SIMD for the all-fit variant (79
+// VGPRs): the adjoint is latency-bound, so occupancy pays more than sharing
+// the per-(column, view) set-up over more slices.  LDS floats per block by
+// footprint pitch: six blocks of 26 KB fit a CU at pitch 136 (42 rows), 16 KB
+// at the narrower pitches (53 and 97 rows).
+constexpr int kAdjTZ = 4;
+constexpr int adj_cap(int wp) { return wp == 136 ? 6144 : 4096; }
 
 }  // namespace
 
@@ -734,7 +802,7 @@ BpPlan adjoint_plan(const BpParams &P, int cm)
     int want = cm;
     if (const char *e = std::getenv("CTR_ADJ_CM")) want = std::atoi(e);
     L.cm = (want == 3 || want == 6) ? want : 4;
-    L.capf = kAdjCap;
+    L.capf = adj_cap(L.wp);
     L.hmax = (L.capf - 2 * L.wp - 2 * kCMax) / (L.wp + 1);
     L.nfoot = (size_t)L.tiles_t * L.tiles_s * L.tiles_z * P.nb;
     return L;
@@ -791,21 +859,29 @@ hipError_t launch_bp_tiled(int kind, const BpParams &Pin, const BpPlan &L, const
                           : launch_bp_kind<kBpRef>(P, L, foot, proj, vol, st);
 }
 
-template <int CM>
+template <int CM, bool FITALL>
 static hipError_t launch_adjoint_cm(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj,
                                     float *vol, const SirtUpdate &U, int sirt, int ones_input, hipStream_t st)
 {
     const int nwg = L.tiles_t * L.tiles_s * L.tiles_z;
     if (L.wp == 40)
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, kAdjCap, CM>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
-                           P.views, U, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 40, adj_cap(40), CM, FITALL>), dim3(nwg), dim3(kThreads), 0, st, proj, vol,
+                           P, foot, P.views, U, sirt, ones_input);
     else if (L.wp == 72)
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, kAdjCap, CM>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P, foot,
-                           P.views, U, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 72, adj_cap(72), CM, FITALL>), dim3(nwg), dim3(kThreads), 0, st, proj, vol,
+                           P, foot, P.views, U, sirt, ones_input);
     else
-        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, kAdjCap, CM>), dim3(nwg), dim3(kThreads), 0, st, proj, vol, P,
-                           foot, P.views, U, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint<kAdjTZ, 136, adj_cap(136), CM, FITALL>), dim3(nwg), dim3(kThreads), 0, st, proj,
+                           vol, P, foot, P.views, U, sirt, ones_input);
     return hipGetLastError();
+}
+
+template <int CM>
+static hipError_t launch_adjoint_fit(const BpParams &P, const BpPlan &L, const int4 *foot, const float *proj,
+                                     float *vol, const SirtUpdate &U, int sirt, int ones_input, hipStream_t st)
+{
+    return L.fit_all ? launch_adjoint_cm<CM, true>(P, L, foot, proj, vol, U, sirt, ones_input, st)
+                     : launch_adjoint_cm<CM, false>(P, L, foot, proj, vol, U, sirt, ones_input, st);
 }
 
 hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot, const float *proj, float *vol,
@@ -821,35 +897,49 @@ hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot
         U = *upd;
         sirt = 1;
     }
-    if (L.cm == 3) return launch_adjoint_cm<3>(P, L, foot, proj, vol, U, sirt, ones_input, st);
-    if (L.cm == 4) return launch_adjoint_cm<4>(P, L, foot, proj, vol, U, sirt, ones_input, st);
-    return launch_adjoint_cm<6>(P, L, foot, proj, vol, U, sirt, ones_input, st);
+    if (L.cm == 3) return launch_adjoint_fit<3>(P, L, foot, proj, vol, U, sirt, ones_input, st);
+    if (L.cm == 4) return launch_adjoint_fit<4>(P, L, foot, proj, vol, U, sirt, ones_input, st);
+    return launch_adjoint_fit<6>(P, L, foot, proj, vol, U, sirt, ones_input, st);
 }
 
-template <int CM>
+template <int CM, bool FITALL>
 static hipError_t launch_adjoint_batched_cm(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
                                             int ones_input, hipStream_t st)
 {
     if (wp == 40)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, kAdjCap, CM>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
-                           njobs, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, adj_cap(40), CM, FITALL>), dim3(nblocks), dim3(kThreads), 0, st,
+                           jobs, njobs, sirt, ones_input);
     else if (wp == 72)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, kAdjCap, CM>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
-                           njobs, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, adj_cap(72), CM, FITALL>), dim3(nblocks), dim3(kThreads), 0, st,
+                           jobs, njobs, sirt, ones_input);
     else if (wp == 136)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, kAdjCap, CM>), dim3(nblocks), dim3(kThreads), 0, st, jobs,
-                           njobs, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, adj_cap(136), CM, FITALL>), dim3(nblocks), dim3(kThreads), 0,
+                           st, jobs, njobs, sirt, ones_input);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_batched(int wp, int cm, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
-                                  int ones_input, hipStream_t st)
+template <int CM>
+static hipError_t launch_adjoint_batched_fit(int wp, bool fit_all, const AdjointJob *jobs, int njobs, int nblocks,
+                                             int sirt, int ones_input, hipStream_t st)
 {
-    if (cm == 3) return launch_adjoint_batched_cm<3>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
-    if (cm == 4) return launch_adjoint_batched_cm<4>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
-    return launch_adjoint_batched_cm<6>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
+    return fit_all ? launch_adjoint_batched_cm<CM, true>(wp, jobs, njobs, nblocks, sirt, ones_input, st)
+                   : launch_adjoint_batched_cm<CM, false>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
+}
+
+hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob *jobs, int njobs, int nblocks,
+                                  int sirt, int ones_input, hipStream_t st)
+{
+    if (cm == 3) return launch_adjoint_batched_fit<3>(wp, fit_all, jobs, njobs, nblocks, sirt, ones_input, st);
+    if (cm == 4) return launch_adjoint_batched_fit<4>(wp, fit_all, jobs, njobs, nblocks, sirt, ones_input, st);
+    return launch_adjoint_batched_fit<6>(wp, fit_all, jobs, njobs, nblocks, sirt, ones_input, st);
+}
+
+hipError_t launch_foot_unfit(const int4 *foot, size_t n, int *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_foot_unfit, dim3(1), dim3(1024), 0, st, foot, (long)n, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_tables(const BpParams &P, hipStream_t st)

@@ -193,6 +193,15 @@ def test_cone_adjoint(ctx, oracle, case):
     assert e < 1e-4 and m < 1e-3
 
 
+def test_cone_adjoint_cases_cover_both_plan_variants(ctx, oracle):
+    """The exact adjoint launches a leaner variant when every footprint of
+    the plan fits LDS, and the one with the global-memory path otherwise:
+    the cases above run both."""
+    glob = {case: ctx.bp_plan_stats(make_cone(case, oracle)[0], adjoint=True)["global"] for case in CASES}
+    print(f"[case] adjoint footprints on the global path: {glob}", flush=True)
+    assert any(v == 0 for v in glob.values()) and any(v > 0 for v in glob.values()), glob
+
+
 def fan_case(equiangular=False, n=48, nb=40, nbins=80):
     D = 100.0
     size = (n * 0.9, n * 0.9)
