@@ -388,21 +388,24 @@ __global__ __launch_bounds__(256) void k_fdk_quirk(const float *__restrict__ pro
 constexpr int kCMax = 6;   // LDS slack after each staged row and column table: the widest candidate chunk
 
 // Slab interval of one axis for a ray parameterised S + lam*dir: the voxel
-// [lo_rel, lo_rel + pitch] (relative to S) holds lam in [l1, l2] sorted;
-// inv = 1/dir, with inv == 0 marking a ray parallel to the slab (inside iff
-// the source coordinate lies in [lo, lo + pitch)): the whole line, or the
-// empty interval [inf, inf] (lo <= hi always).
+// [lo_rel, lo_rel + pitch] (relative to S) holds lam in [l1, l2], sorted.
+// inv = 1/dir from slab_rcp, finite: a ray parallel to the slab (dir 0) has
+// inv = 1e30, and its interval, [lo_rel, lo_rel + pitch] * 1e30, spans every
+// parameter of the volume (|lam| < 1e4) when S lies inside the slab and
+// none of them when it lies outside (an end at exactly 0 when S is on a
+// face: a voxel in front of the source has lam > 0 either way).
 __device__ __forceinline__ void slab_interval(float lo_rel, float pitch, float inv, float &lo, float &hi)
 {
-    if (inv != 0.0f) {
-        const float l1 = lo_rel * inv, l2 = fmaf(pitch, inv, l1);
-        lo = fminf(l1, l2);
-        hi = fmaxf(l1, l2);
-    } else {
-        const bool in = lo_rel <= 0.0f && 0.0f < lo_rel + pitch;
-        lo = in ? -kInf : kInf;
-        hi = kInf;
-    }
+    const float l1 = lo_rel * inv, l2 = fmaf(pitch, inv, l1);
+    lo = fminf(l1, l2);
+    hi = fmaxf(l1, l2);
+}
+
+// 1/x for slab_interval: finite, 1e30 in magnitude at most (x = 0 gives 1e30).
+__device__ __forceinline__ float slab_rcp(double x)
+{
+    const double r = x != 0.0 ? 1.0 / x : 1e30;
+    return (float)fmin(fmax(r, -1e30), 1e30);
 }
 
 // |[zl, zh] n [lo, hi]| for lo <= hi finite and zl <= zh (either may be
@@ -415,7 +418,6 @@ __device__ __forceinline__ float overlap(float zl, float zh, float lo, float hi)
     return __builtin_amdgcn_fmed3f(zh, lo, hi) - __builtin_amdgcn_fmed3f(zl, lo, hi);
 }
 
-__device__ __forceinline__ float safe_rcp(float x) { return x != 0.0f ? 1.0f / x : 0.0f; }
 
 // Column footprint of one voxel column (box corners): candidate detector
 // columns [cu0, cu1] and the magnification range of the column box.
@@ -722,13 +724,13 @@ __global__ __launch_bounds__(256) void k_adjoint_tables(BpParams P, float *__res
         const int b = (int)(e / P.np), c = (int)(e % P.np);
         const double sb = P.view_d[4 * b + 0], cb = P.view_d[4 * b + 1], p = P.pdd[c], D = (double)P.D;
         const double a = p * cb + D * sb, bq = p * sb - D * cb;
-        inva[e] = a != 0.0 ? (float)(1.0 / a) : 0.0f;
-        invb[e] = bq != 0.0 ? (float)(1.0 / bq) : 0.0f;
+        inva[e] = slab_rcp(a);
+        invb[e] = slab_rcp(bq);
     }
     if (e < nrc) {
         const int r = (int)(e / P.np), c = (int)(e % P.np);
         norm[e] = ray_norm(P.pd[c], P.xd[r], P.D);
-        if (c == 0) invxi[r] = safe_rcp(P.xd[r]);
+        if (c == 0) invxi[r] = slab_rcp((double)P.xd[r]);
     }
 }
 
