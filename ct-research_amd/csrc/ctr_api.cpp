@@ -557,14 +557,16 @@ ctr_status ensure_footprints_key(ctr_ctx *ctx, const BpParams &P, FootCache &fc,
     L.fit_all = 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (adjoint && hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
-        s = buf_reserve(ctx, fc.unfit, sizeof(int));
+        s = buf_reserve(ctx, fc.unfit, sizeof(int) * kFootCountBlocks);
         if (s != CTR_OK) return s;
         s = check_launch(ctx, launch_foot_unfit((const int4 *)fc.buf.ptr, L.nfoot, (int *)fc.unfit.ptr, ctx->stream),
                          "footprint count");
         if (s != CTR_OK) return s;
-        int unfit = -1;
-        HIP_TRY(ctx, hipMemcpyAsync(&unfit, fc.unfit.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        int counts[kFootCountBlocks];
+        HIP_TRY(ctx, hipMemcpyAsync(counts, fc.unfit.ptr, sizeof counts, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        long long unfit = 0;
+        for (int c : counts) unfit += c;
         L.fit_all = unfit == 0 ? 1 : 0;
     }
     fc.key = key;
@@ -1818,7 +1820,7 @@ static ctr_status diff_sums(ctr_ctx *ctx, const float *x, const float *ref, size
     const float *dx, *dr;
     TRY(stage_in(ctx, dev, kWsIn, x, n, &dx));
     TRY(stage_in(ctx, dev, kWsTmp, ref, n, &dr));
-    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * 4096));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (4096 + kReduceSlack)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2));
     int nb = 0;
     double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
@@ -1983,7 +1985,7 @@ ctr_status sirt_loop(ctr_ctx *ctx, size_t nvox, size_t nray, const float *b_in, 
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
     TRY(reserve_vinv(ctx, nvox * sizeof(float)));
     const size_t nsum = (size_t)o->n_iter + 2;
-    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(nblk_fp, 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((size_t)std::max(nblk_fp, 4096) + kReduceSlack)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * nsum));
     float *r = (float *)ctx->ws[kWsR].ptr, *vinv = (float *)ctx->ws[kWsVinv].ptr;
     double *part = (double *)ctx->ws[kWsPart].ptr, *sums = (double *)ctx->ws[kWsSums].ptr;
@@ -2408,7 +2410,7 @@ ctr_status ctr_cnn_train_step(ctr_cnn *net, int n_patch, int nx, int ny, int nz,
     TRY(cnn_reserve(net, n, true));
     const int nblk = wgrad_blocks(S);
     TRY(buf_reserve(ctx, net->partial, (size_t)nblk * (32 * 32 * 27 + 32) * sizeof(float)));
-    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((n + 255) / 256 + 1)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((n + 255) / 256 + 1 + kReduceSlack)));
     if (!dev) {
         TRY(buf_reserve(ctx, net->xin, n * sizeof(float)));
         TRY(buf_reserve(ctx, net->gin, n * sizeof(float)));
@@ -2475,7 +2477,7 @@ ctr_status ctr_sirt3d_step(ctr_ctx *ctx, const ctr_cone_geom *g, const float *b_
     TRY(reserve_vinv(ctx, nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsG], nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
-    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((size_t)std::max(ray_fp_blocks(R), 4096) + kReduceSlack)));
     float *vinv = (float *)ctx->ws[kWsVinv].ptr, *gb = (float *)ctx->ws[kWsG].ptr, *r = (float *)ctx->ws[kWsR].ptr;
     double *part = (double *)ctx->ws[kWsPart].ptr;
     TRY(sirt_norm(ctx, fnv(ctx->foot_adj.key, "cone", 4), nvox, st, [&](float *v) {
@@ -2520,7 +2522,7 @@ ctr_status ctr_learned_sirt3d(ctr_ctx *ctx, const ctr_cone_geom *g, const float 
     for (int s : {kWsG, kWsS, kWsD, kWsX2}) TRY(buf_reserve(ctx, ctx->ws[s], nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
     if (o->judge_vs_plain) TRY(buf_reserve(ctx, ctx->ws[kWsR2], nray * sizeof(float)));
-    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((size_t)std::max(ray_fp_blocks(R), 4096) + kReduceSlack)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * 4));
     if (o->cnn_tile < 0) return fail(ctx, CTR_EINVAL, "cnn_tile must be >= 0");
     // patch-wise network: tiles of cnn_tile^3 with 8-voxel halos, workspaces of one tile
@@ -2692,7 +2694,7 @@ ctr_status ctr_sirt3d_emulated(ctr_ctx *ctx, const ctr_cone_geom *g, const float
     for (int sl : {kWsG, kWsS, kWsD, kWsX2, kWsMask, kWsXs}) TRY(buf_reserve(ctx, ctx->ws[sl], nvox * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR], nray * sizeof(float)));
     TRY(buf_reserve(ctx, ctx->ws[kWsR2], nray * sizeof(float)));
-    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * (size_t)std::max(ray_fp_blocks(R), 4096)));
+    TRY(buf_reserve(ctx, ctx->ws[kWsPart], sizeof(double) * 2 * ((size_t)std::max(ray_fp_blocks(R), 4096) + kReduceSlack)));
     TRY(buf_reserve(ctx, ctx->ws[kWsSums], sizeof(double) * 2 * 4));
     const bool tiled = net && o->cnn_tile > 0 && (g->nt > o->cnn_tile || g->ns > o->cnn_tile || g->nz > o->cnn_tile);
     if (net) {
@@ -2909,7 +2911,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
             q.R.volT = (const float *)q.vt.ptr;   // refreshed from x before every residual pass
         }
         q.fp_blocks = fp_job_blocks(q.R, &q.gx, &q.gy);
-        TRY(buf_reserve(ctx, q.part, sizeof(double) * 2 * (size_t)std::max(q.fp_blocks, 4096)));
+        TRY(buf_reserve(ctx, q.part, sizeof(double) * 2 * ((size_t)std::max(q.fp_blocks, 4096) + kReduceSlack)));
         fj[p] = FpJob{q.R, q.xp, q.bp, (float *)q.r.ptr, (double *)q.part.ptr, fp_total, q.gx, q.gy};
         fp_total += q.fp_blocks;
         rj[p] = ReduceJob{(const double *)q.part.ptr, q.fp_blocks};
@@ -2966,7 +2968,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         TRY(check_launch(ctx, launch_reciprocal_eps((float *)q.vinv.ptr, q.nvox, 1e-9f, st), "reciprocal"));
         int nb = 0;
         TRY(check_launch(ctx, launch_sum_sq_sum(q.bp, q.nray, (double *)q.part.ptr, &nb, st), "norm"));
-        TRY(check_launch(ctx, launch_reduce_partials((const double *)q.part.ptr, nb, (double *)B.norms.ptr + 2 * p, st),
+        TRY(check_launch(ctx, launch_reduce_partials((double *)q.part.ptr, nb, (double *)B.norms.ptr + 2 * p, st),
                          "reduce"));
     }
     HIP_TRY(ctx, hipMemsetAsync(diter, 0, sizeof(int), st));   // before capture: not part of the graph

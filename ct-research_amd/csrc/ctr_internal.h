@@ -215,7 +215,9 @@ hipError_t launch_adjoint(const BpParams &P, const BpPlan &L, const int4 *foot, 
 // Fills P.adj_inva, adj_invb, adj_invxi and adj_norm (the pointers name the
 // destinations; P's views, pd and xd must be on the device).
 hipError_t launch_adjoint_tables(const BpParams &P, hipStream_t st);
-// out[0] = number of footprints that are neither empty nor fit LDS (one block).
+// out[0 .. kFootCountBlocks) sum to the number of footprints that are neither
+// empty nor fit LDS.
+constexpr int kFootCountBlocks = 256;
 hipError_t launch_foot_unfit(const int4 *foot, size_t n, int *out, hipStream_t st);
 // out[b][r][c] = in[b][r][c] * P.adj_norm[r][c] over all views.
 hipError_t launch_scale_by_norm(const BpParams &P, const float *in, float *out, hipStream_t st);
@@ -371,7 +373,11 @@ hipError_t launch_diff_sums(const float *x, const float *ref, size_t n, double *
 hipError_t launch_fill(float *dst, float value, size_t n, hipStream_t st);
 hipError_t launch_transpose_st(const float *in, float *out, int nt, int ns, int nz, hipStream_t st);   // [z][t][s] <- [z][s][t]
 hipError_t launch_reciprocal_eps(float *v, size_t n, float eps, hipStream_t st);   // v <- 1/(v+eps)
-hipError_t launch_reduce_partials(const double *partials, int n, double *out2, hipStream_t st);
+// Deterministic sum of n (a, b) pairs into out2[0..1].  Above kReduceDirect
+// pairs it runs in two stages and writes up to kReduceSlack stage pairs after
+// the n pairs: a partial buffer holds 2 (n + kReduceSlack) doubles.
+constexpr int kReduceDirect = 4096, kReduceSlack = 256;
+hipError_t launch_reduce_partials(double *partials, int n, double *out2, hipStream_t st);
 hipError_t launch_sum_sq_sum(const float *x, size_t n, double *partials, int *nblocks, hipStream_t st);  // (sum x^2, sum x)
 
 }  // namespace ctr

@@ -449,26 +449,14 @@ __device__ __forceinline__ void column_box(const BpParams &P, const BpView &vw, 
 __device__ __forceinline__ void slice_rows(const BpParams &P, float zb, float zt, float mmin, float mmax, int &rv0,
                                            int &rv1)
 {
-    float zlo, zhi;
-    if (mmin > 0.0f) {
-        // zb < zt, and rounding is monotone, so the extreme products of
-        // {zb, zt} x {mmin, mmax} follow from the signs of zb and zt (the
-        // same for the whole block: a branch on uniform values)
-        if (zb >= 0.0f) {
-            zlo = zb * mmin;
-            zhi = zt * mmax;
-        } else if (zt <= 0.0f) {
-            zlo = zb * mmax;
-            zhi = zt * mmin;
-        } else {
-            zlo = zb * mmax;
-            zhi = zt * mmax;
-        }
-    } else {
-        const float a = zb * mmin, b = zb * mmax, c = zt * mmin, d = zt * mmax;
-        zlo = fminf(fminf(a, b), fminf(c, d));
-        zhi = fmaxf(fmaxf(a, b), fmaxf(c, d));
-    }
+    // extreme products of {zb, zt} x {mmin, mmax}.  The magnifications are
+    // positive (the geometry check keeps the source orbit outside the
+    // volume, so every voxel corner lies in front of the source), zb < zt,
+    // and rounding is monotone: the smallest product is zb times mmin or
+    // mmax by the sign of zb, the largest zt times mmax or mmin by the sign
+    // of zt (uniform over the block), bit for bit the min and max of all four
+    const float zlo = zb * (zb >= 0.0f ? mmin : mmax);
+    const float zhi = zt * (zt >= 0.0f ? mmax : mmin);
     float v0 = fmaf(zlo, P.v_scale, P.v_off);
     float v1 = fmaf(zhi, P.v_scale, P.v_off);
     if (P.v_scale < 0.0f) {
@@ -689,23 +677,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(FITALL
 }
 
 // Number of (tile, view) footprints of a plan that are not empty and do not
-// fit the adjoint's LDS buffers; one block.
-__global__ __launch_bounds__(1024) void k_foot_unfit(const int4 *__restrict__ foot, long n, int *__restrict__ out)
+// fit the adjoint's LDS buffers: block g counts its grid-stride share into
+// out[g] (the host adds kFootCountBlocks counts).
+__global__ __launch_bounds__(256) void k_foot_unfit(const int4 *__restrict__ foot, long n, int *__restrict__ out)
 {
-    __shared__ int part[16];
+    __shared__ int part[4];
     int c = 0;
-    for (long e = threadIdx.x; e < n; e += 1024) {
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
         const int w = foot[e].w;
         c += !(w & kFootEmpty) && !(w & kFootFits);
     }
     for (int m = 32; m > 0; m >>= 1) c += __shfl_xor(c, m, 64);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < 16; ++w) t += part[w];
-        out[0] = t;
-    }
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
 // Geometry tables of the exact adjoint: per (view, column) the reciprocals
@@ -940,7 +925,7 @@ hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob
 
 hipError_t launch_foot_unfit(const int4 *foot, size_t n, int *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_foot_unfit, dim3(1), dim3(1024), 0, st, foot, (long)n, out);
+    hipLaunchKernelGGL(k_foot_unfit, dim3(kFootCountBlocks), dim3(256), 0, st, foot, (long)n, out);
     return hipGetLastError();
 }
 

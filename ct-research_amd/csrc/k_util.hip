@@ -26,12 +26,14 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// One block sums n pairs (fixed order per lane, then a fixed tree): deterministic.
-__global__ __launch_bounds__(256) void k_reduce_pairs(const double *partials, int n, double *out2)
+// Sum of the pairs [lo, hi) by one block: per lane in a fixed order, then a
+// fixed tree over the lanes and the four waves.  Thread 0 gets the pair.
+__device__ __forceinline__ void block_pair_sum(const double *__restrict__ partials, int lo, int hi, double &sa,
+                                               double &sb)
 {
     __shared__ double red[2][4];
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = lo + (int)threadIdx.x; i < hi; i += 256) {
         a += partials[2 * i];
         b += partials[2 * i + 1];
     }
@@ -42,36 +44,65 @@ __global__ __launch_bounds__(256) void k_reduce_pairs(const double *partials, in
         red[1][threadIdx.x >> 6] = b;
     }
     __syncthreads();
+    sa = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    sb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+}
+
+// Deterministic sum of n pairs.  Up to kReduceDirect pairs one block sums
+// them; above, reduce_groups(n) blocks sum consecutive ranges of
+// ceil(n / groups) pairs into partials[n ..] (stage 1) and one block sums
+// those (stage 2).  The order depends on n only, so every caller that sums
+// the same partials gets the same bits.
+__host__ __device__ __forceinline__ int reduce_groups(int n)
+{
+    return n <= kReduceDirect ? 0 : min((int)kReduceSlack, (n + 2047) / 2048);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_pairs(const double *partials, int n, double *out2)
+{
+    double a, b;
+    block_pair_sum(partials, 0, n, a, b);
     if (threadIdx.x == 0) {
-        out2[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        out2[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        out2[0] = a;
+        out2[1] = b;
     }
 }
 
-// One block per job: sums[(*iter) * njobs + p] = sum of job p's partial pairs.
+// Stage 1 for job blockIdx.y (jobs == nullptr: the single sum of n pairs).
+__global__ __launch_bounds__(256) void k_reduce_stage1(double *partials, int n, const ReduceJob *__restrict__ jobs)
+{
+    double *p = partials;
+    int m = n;
+    if (jobs) {
+        p = const_cast<double *>(jobs[blockIdx.y].partials);
+        m = jobs[blockIdx.y].n;
+    }
+    const int groups = reduce_groups(m);
+    if ((int)blockIdx.x >= groups) return;
+    const int chunk = (m + groups - 1) / groups;
+    const int lo = (int)blockIdx.x * chunk, hi = min(m, lo + chunk);
+    double a, b;
+    block_pair_sum(p, lo, hi, a, b);
+    if (threadIdx.x == 0) {
+        p[2 * (m + (int)blockIdx.x)] = a;
+        p[2 * (m + (int)blockIdx.x) + 1] = b;
+    }
+}
+
+// One block per job: sums[(*iter) * njobs + p] = sum of job p's partial pairs
+// (of its stage-1 pairs when it has more than kReduceDirect).
 __global__ __launch_bounds__(256) void k_reduce_jobs(const ReduceJob *__restrict__ jobs, const int *__restrict__ iter,
                                                      double *__restrict__ sums, int njobs)
 {
-    __shared__ double red[2][4];
     const int p = blockIdx.x;
-    const double *partials = jobs[p].partials;
-    const int n = jobs[p].n;
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        a += partials[2 * i];
-        b += partials[2 * i + 1];
-    }
-    a = wave_sum(a);
-    b = wave_sum(b);
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = a;
-        red[1][threadIdx.x >> 6] = b;
-    }
-    __syncthreads();
+    const int n = jobs[p].n, groups = reduce_groups(n);
+    double a, b;
+    if (groups) block_pair_sum(jobs[p].partials + 2 * (size_t)n, 0, groups, a, b);
+    else block_pair_sum(jobs[p].partials, 0, n, a, b);
     if (threadIdx.x == 0) {
         double *o = sums + 2 * ((size_t)(*iter) * njobs + p);
-        o[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        o[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        o[0] = a;
+        o[1] = b;
     }
 }
 
@@ -305,14 +336,21 @@ hipError_t launch_reciprocal_eps(float *v, size_t n, float eps, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_partials(const double *partials, int n, double *out2, hipStream_t st)
+hipError_t launch_reduce_partials(double *partials, int n, double *out2, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, st, partials, n, out2);
+    const int groups = reduce_groups(n);
+    if (groups == 0) {
+        hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, st, partials, n, out2);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_reduce_stage1, dim3(groups), dim3(256), 0, st, partials, n, (const ReduceJob *)nullptr);
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, st, partials + 2 * (size_t)n, groups, out2);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_jobs(const ReduceJob *jobs, int njobs, const int *iter, double *sums, hipStream_t st)
 {
+    hipLaunchKernelGGL(k_reduce_stage1, dim3(kReduceSlack, njobs), dim3(256), 0, st, (double *)nullptr, 0, jobs);
     hipLaunchKernelGGL(k_reduce_jobs, dim3(njobs), dim3(256), 0, st, jobs, iter, sums, njobs);
     return hipGetLastError();
 }
