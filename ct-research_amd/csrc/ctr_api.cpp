@@ -45,6 +45,8 @@ struct ConeTables {
     std::vector<float> h_zq;        // [nz + 32][2]: zs^2, hz/|zs| per slice (zs = slice centre - source z)
     DevBuf d_view, d_p, d_xi, d_bp, d_pf, d_xf, d_zq;
     DevBuf d_ainv, d_binv, d_xinv, d_norm;   // exact-adjoint tables, built on the device
+    DevBuf d_fw;                              // FDK pre-weights [nx][np] (FP64), built on first filter use
+    uint64_t fw_key = 0;
 };
 
 struct FanTables {
@@ -1073,11 +1075,26 @@ ctr_status cone_filter_impl(ctr_ctx *ctx, const ctr_cone_geom *g, int filter, co
     F.rows = g->n_beta * g->n_xi;
     F.mode = 0;
     F.D = (float)g->distance;
+    F.Dd = g->distance;
     F.pd = (const float *)ctx->cone.d_pf.ptr;
     F.xd = (const float *)ctx->cone.d_xf.ptr;
     F.taps = (const double *)ctx->taps.d.ptr;
     F.spec = ctx->taps.fft_len ? (const double *)ctx->taps.d_spec.ptr : nullptr;
     F.fft_len = ctx->taps.fft_len;
+    // the FP64 pre-weights once per geometry (at most 32 MiB; beyond, the
+    // filter forms the same weights per sample)
+    const size_t nw = (size_t)g->n_p * g->n_xi;
+    if (nw <= ((size_t)1 << 22)) {
+        ConeTables &T = ctx->cone;
+        if (T.fw_key != T.key || !T.d_fw.ptr) {
+            T.fw_key = 0;
+            TRY(buf_reserve(ctx, T.d_fw, nw * sizeof(double)));
+            TRY(check_launch(ctx, launch_cone_weights(g->distance, F.pd, F.xd, g->n_p, g->n_xi, (double *)T.d_fw.ptr,
+                                                      ctx->stream), "pre-weight table"));
+            T.fw_key = T.key;
+        }
+        F.pw = (const double *)T.d_fw.ptr;
+    }
     return check_launch(ctx, launch_row_filter(F, in, out, ctx->stream), "ramp filter");
 }
 
@@ -1097,6 +1114,7 @@ ctr_status fan_filter_impl(ctr_ctx *ctx, const ctr_fan_geom *g, int detector, in
     F.rows = g->n_beta;
     F.mode = detector == CTR_FAN_EQUIANGULAR ? 1 : 0;
     F.D = (float)g->distance;
+    F.Dd = g->distance;
     F.pd = (const float *)ctx->fan.d_pf.ptr;
     F.xd = (const float *)ctx->fan.d_xf.ptr;
     F.colw = (const float *)ctx->fan.d_colw.ptr;
@@ -1149,6 +1167,7 @@ ctr_status curved_filter_impl(ctr_ctx *ctx, const ctr_curved_geom *g, const floa
     F.rows = g->n_beta * g->n_xi;
     F.mode = 3;
     F.D = (float)g->distance;
+    F.Dd = g->distance;
     F.spec = (const double *)T.d_spec.ptr;
     F.fft_len = T.fft_len;
     F.spec_stride = T.fft_len;
@@ -1468,6 +1487,7 @@ ctr_status ctr_cone_filter_noalloc(ctr_ctx *ctx, const ctr_cone_geom *g, int fil
     F.rows = g->n_beta * g->n_xi;
     F.mode = 0;
     F.D = (float)g->distance;
+    F.Dd = g->distance;
     F.pd = pf;
     F.xd = xf;
     F.taps = taps;

@@ -97,6 +97,9 @@ struct FilterParams {
                             // 3: curved short scan, w = (D/P) cos(g P) Parker(b P, start P, g P, fan P)
                             //    with P = rowp[row % nx], b = betad[row / nx], g = gamd[p]
     float D;
+    double Dd;              // mode 0: D in FP64 (the pre-weight is formed in FP64)
+    const double *pw;       // mode 0: [nx][np] FP64 weights D / sqrt(D^2 + p^2 + xi^2) (launch_cone_weights),
+                            //         or null: the same weights formed per sample
     const float *pd;        // [np] column positions
     const float *xd;        // [nx] row positions
     const float *colw;      // [np] column weights (mode 1)
@@ -272,6 +275,8 @@ hipError_t launch_reduce_jobs(const ReduceJob *jobs, int njobs, const int *iter,
 hipError_t launch_iter_inc(int *iter, hipStream_t st);
 
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st);
+// pw[x][q] = D / sqrt(D^2 + pd[q]^2 + xd[x]^2) in FP64 (the mode-0 weight table).
+hipError_t launch_cone_weights(double D, const float *pd, const float *xd, int np, int nx, double *pw, hipStream_t st);
 // transform length of the FFT filter path for np columns (0: direct convolution)
 int fft_filter_length(int np);
 // Restricted Chebyshev emulation of SIRT: mask = v > thr; out = a * m; the

@@ -19,6 +19,28 @@
 namespace ctr {
 namespace {
 
+// FDK cosine pre-weight of FDK.cu:36 and the float store of the weighted
+// sample: r * D / sqrt(D^2 + p^2 + xi^2) formed in FP64 operation by
+// operation (no contraction) from the float detector positions, rounded to
+// float once.  The ramp-filtered row is ~300 times smaller in norm than its
+// input, so a last-bit difference in the stored samples shows at ~3e-6 in
+// the output: the same arithmetic as the FP64 oracle keeps it out.
+__device__ __forceinline__ double cone_weight64(double D, float p, float xi)
+{
+    const double pd = p, xd = xi;
+    return D / sqrt(__dadd_rn(__dadd_rn(__dmul_rn(D, D), __dmul_rn(pd, pd)), __dmul_rn(xd, xd)));
+}
+__device__ __forceinline__ float cone_preweight(float r, double w) { return (float)__dmul_rn((double)r, w); }
+
+__global__ __launch_bounds__(256) void k_cone_weights(double D, const float *__restrict__ pd,
+                                                      const float *__restrict__ xd, int np, int nx,
+                                                      double *__restrict__ pw)
+{
+    const long n = (long)np * nx;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256)
+        pw[e] = cone_weight64(D, pd[e % np], xd[e / np]);
+}
+
 template <int K>
 struct FilterLayout {
     // 2K zeros in front: the window advance after the last input sample reads
@@ -52,16 +74,13 @@ __global__ __launch_bounds__(64 *RW) void k_row_filter(const float *__restrict__
         for (int q = lane; q < RL; q += 64) {
             double v = 0.0;
             if (q < np) {
-                float wgt;
                 if (P.mode == 0) {
-                    const float pq = P.pd[q];
-                    wgt = P.D / sqrtf(P.D * P.D + pq * pq + xi * xi);
-                } else if (P.mode == 1) {
-                    wgt = P.colw[q];
+                    const double w = P.pw ? P.pw[(size_t)ix * np + q] : cone_weight64(P.Dd, P.pd[q], xi);
+                    v = (double)cone_preweight(src[q], w);
                 } else {
-                    wgt = 1.0f;
+                    const float wgt = P.mode == 1 ? P.colw[q] : 1.0f;
+                    v = (double)(src[q] * wgt);   // rounded to float like the in-place store of FDK.cu:36
                 }
-                v = (double)(src[q] * wgt);   // rounded to float like the in-place store of FDK.cu:36
             }
             wrow[q] = v;
         }
@@ -323,47 +342,56 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
         const double bp0 = MODE == 3 && has0 ? P.betad[row0 / P.nx] * prow : 0.0;
         const double bp1 = MODE == 3 && has1 ? P.betad[row1 / P.nx] * prow : 0.0;
         const float *src0 = in + (size_t)row0 * np, *src1 = in + (size_t)row1 * np;
-        // the weighted samples go through LDS: a rolled loop keeps the
-        // weight arithmetic of one sample live at a time
-        __syncthreads();   // the group's previous readers of buf are done
+        double2 v[8];
+        if constexpr (MODE == 3) {
+            // the Parker-weighted samples go through LDS: a rolled loop keeps
+            // the weight arithmetic of one sample live at a time
+            __syncthreads();   // the group's previous readers of buf are done
 #pragma unroll 1
-        for (int t = 0; t < 8; ++t) {
-            const int q = j + G * t;
-            double a = 0.0, b = 0.0;
-            if (q < np) {
-                float w0, w1;
-                if constexpr (MODE == 0) {
-                    const float pq = P.pd[q];
-                    w0 = P.D / sqrtf(P.D * P.D + pq * pq + xi0 * xi0);
-                    w1 = P.D / sqrtf(P.D * P.D + pq * pq + xi1 * xi1);
-                } else if constexpr (MODE == 1) {
-                    w0 = w1 = P.colw[q];
-                } else if constexpr (MODE == 2) {
-                    w0 = w1 = 1.0f;
-                } else {
+            for (int t = 0; t < 8; ++t) {
+                const int q = j + G * t;
+                double a = 0.0, b = 0.0;
+                if (q < np) {
                     // ImplementUI.cu:43-66: (D/P) cos(g P) Parker(b P, start P, g P, fan P)
                     const double gp = P.gamd[q] * prow;
                     const float base_w = (float)((double)P.D / prow) * cosf((float)gp);
                     const double stp = P.start * prow, fanp = P.fan * prow;
-                    w0 = base_w * parker_weight(bp0, stp, gp, fanp);
-                    w1 = base_w * parker_weight(bp1, stp, gp, fanp);
-                }
-                if constexpr (MODE == 3) {
+                    const float w0 = base_w * parker_weight(bp0, stp, gp, fanp);
+                    const float w1 = base_w * parker_weight(bp1, stp, gp, fanp);
                     // the reference keeps the weighted samples in FP64 (ImplementUI.cu:66)
                     if (has0) a = (double)src0[q] * (double)w0;
                     if (has1) b = (double)src1[q] * (double)w1;
-                } else {
-                    // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
-                    if (has0) a = (double)(src0[q] * w0);
-                    if (has1) b = (double)(src1[q] * w1);
                 }
+                buf[fft_pad(q)] = make_double2(a, b);
             }
-            buf[fft_pad(q)] = make_double2(a, b);
-        }
-        __syncthreads();
-        double2 v[8];
+            __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 8; ++t) v[t] = buf[fft_pad(j + G * t)];
+            for (int t = 0; t < 8; ++t) v[t] = buf[fft_pad(j + G * t)];
+        } else {
+            // thread j holds samples j + G t: loaded and weighted in
+            // registers (the first pass's barrier publishes nothing of them)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int q = j + G * t;
+                double a = 0.0, b = 0.0;
+                if (q < np) {
+                    if constexpr (MODE == 0) {
+                        const float pq = P.pd[q];
+                        if (has0) a = (double)cone_preweight(src0[q], cone_weight64(P.Dd, pq, xi0));
+                        if (has1) b = (double)cone_preweight(src1[q], cone_weight64(P.Dd, pq, xi1));
+                    } else if constexpr (MODE == 4) {
+                        if (has0) a = (double)cone_preweight(src0[q], P.pw[(size_t)(row0 % P.nx) * np + q]);
+                        if (has1) b = (double)cone_preweight(src1[q], P.pw[(size_t)(row1 % P.nx) * np + q]);
+                    } else {
+                        const float w = MODE == 1 ? P.colw[q] : 1.0f;
+                        // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
+                        if (has0) a = (double)(src0[q] * w);
+                        if (has1) b = (double)(src1[q] * w);
+                    }
+                }
+                v[t] = make_double2(a, b);
+            }
+        }
         fft_group<L>(v, buf, j, tab);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -399,7 +427,11 @@ hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStr
     const int need = (npairs + GPB - 1) / GPB;
     const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
     switch (P.mode) {
-    case 0: hipLaunchKernelGGL((k_fft_filter<L, 0>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
+    case 0:
+        // mode 4: mode 0 with the FP64 weight table
+        if (P.pw) hipLaunchKernelGGL((k_fft_filter<L, 4>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+        else hipLaunchKernelGGL((k_fft_filter<L, 0>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
+        break;
     case 1: hipLaunchKernelGGL((k_fft_filter<L, 1>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
     case 2: hipLaunchKernelGGL((k_fft_filter<L, 2>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
     case 3: hipLaunchKernelGGL((k_fft_filter<L, 3>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
@@ -425,6 +457,13 @@ int fft_filter_length(int np)
     int L = 256;
     while (L < 2 * np - 1) L *= 2;
     return L <= 2048 ? L : 0;
+}
+
+hipError_t launch_cone_weights(double D, const float *pd, const float *xd, int np, int nx, double *pw, hipStream_t st)
+{
+    const long n = (long)np * nx, blocks = std::min((n + 255) / 256, 8192L);
+    hipLaunchKernelGGL(k_cone_weights, dim3((unsigned)blocks), dim3(256), 0, st, D, pd, xd, np, nx, pw);
+    return hipGetLastError();
 }
 
 hipError_t launch_row_filter(const FilterParams &P, const float *in, float *out, hipStream_t st)

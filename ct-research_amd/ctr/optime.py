@@ -16,7 +16,8 @@ operators run the short scan of ImplementUIData.m: 512 x 512 x 50 voxels,
 channel angles, uniform random projections).  fan_sysmat builds the
 system matrix of the fan configuration C1 (256^2 over 256 mm, 360 views, 512
 equiangular bins at 1e-3 rad, SOD 500); fdk_error_slice runs
-IterativeFDK.m's 128^3 / 360-view slice.
+IterativeFDK.m's 128^3 / 360-view slice; cnn_forward runs the learned-update
+network (4 layers, 32 channels, packaged weights) on the workload's volume.
 """
 import argparse
 import os
@@ -25,7 +26,7 @@ import time
 
 OPS = ("cone_fp", "cone_filter", "fdk_bp", "cone_bp_ref", "cone_adjoint", "sirt3d",
        "par_fp", "par_filter", "par_fbp", "par_adjoint", "curved_filter", "curved_bp", "fan_sysmat",
-       "fdk_error_slice")
+       "fdk_error_slice", "cnn_forward")
 
 
 def parallel_workload():
@@ -108,6 +109,14 @@ def main(argv=None):
         ge = ctr.FdkErrGeom(128, 128, 128, ea["size"], ea["D"], ea["beta"], ea["u"], ea["p"], ea["g"], 63)
         evol = torch.rand(ge.nvox, device="cuda")
         eout = torch.empty(ge.nout, device="cuda")
+    if "cnn_forward" in ops:
+        import numpy as np
+        from . import learned
+        net = ctr.Cnn(ctx, ctr.cnn_init_params(0, 0.0))
+        with np.load(learned.WEIGHTS, allow_pickle=False) as f:
+            net.set_params(f["params"])
+        cx = torch.rand(args.n ** 3, device="cuda")
+        cg = torch.randn(args.n ** 3, device="cuda")
     calls = {
         "cone_fp": lambda: ctx.cone_fp(g, vol, out=proj),
         "cone_filter": lambda: ctx.cone_filter(g, proj, out=filt),
@@ -122,6 +131,7 @@ def main(argv=None):
         "curved_bp": lambda: ctx.curved_bp(gc, cfilt, out=cvol),
         "fan_sysmat": lambda: ctx.fan_sysmat(gf, device=True),
         "fdk_error_slice": lambda: ctx.fdk_error_slice(ge, evol, out=eout),
+        "cnn_forward": lambda: net.forward((args.n, args.n, args.n), cx, cg),
     }
     for o in ops:
         if o == "sirt3d":

@@ -250,6 +250,20 @@ class TestC2Cone:
         report("C2 fp(ones) vs chord", max_abs_mm=worst, rel_l2=rel)
         assert rel < 1e-6 and worst < 2e-3        # chords up to ~440 mm
 
+    def test_filter_matches_oracle_on_view_subset(self, ctx, oracle, c2):
+        """The FFT ramp filter (pre-weight, SL-half and RL-half kernels) at
+        full size against the oracle's FP64 direct convolution on 8 views."""
+        g, vol = c2
+        proj = ctx.cone_fp(g, vol).view(g.proj_shape)
+        sel = slice(3, 360, 45)
+        og = oracle_cone(oracle, g, sel)
+        ph = host(proj[sel])
+        for filt in (ctr.FILTER_SL_HALF, ctr.FILTER_RL_HALF):
+            gpu = host(ctx.cone_filter(g, proj, filt=filt).view(g.proj_shape)[sel])
+            e, m = errs(gpu, oracle.cone_filter(og, ph, filt=filt))
+            report(f"C2 cone_filter[{filt}, 8 of 360 views]", rel_l2=e, max_rel=m)
+            assert e < 1e-6 and m < 1e-5
+
     def test_fdk_bp_matches_oracle(self, ctx, oracle, c2):
         """The Shepp-Logan shadow is truncated by the detector in u at some
         views, so the filtered samples at the detector's u edges are large;
