@@ -27,6 +27,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTaps = 27;
+constexpr int kConvZRun = 16;   // consecutive output slices per x-y tile of a conv block
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
 
@@ -46,10 +47,15 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 31, kh = lane >> 5;
     const int tiles_x = (S.nx + 31) / 32, tiles_y = (S.ny + 15) / 16;
-    const int ntiles = tiles_x * tiles_y * S.nz;
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int tx = tile % tiles_x, rest = tile / tiles_x;
-        const int ty = rest % tiles_y, z = rest / tiles_y;
+    const int zchunks = (S.nz + kConvZRun - 1) / kConvZRun;
+    const int ncols = tiles_x * tiles_y * zchunks;
+    // a block walks one x-y tile through kConvZRun consecutive slices, so
+    // the three input slices of an output slice were read by this CU for the
+    // previous one (its L2 serves the dz reuse)
+    for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x)
+    for (int z = (tcol / (tiles_x * tiles_y)) * kConvZRun,
+             zend = min(S.nz, z + kConvZRun); z < zend; ++z) {
+        const int tx = tcol % tiles_x, ty = (tcol / tiles_x) % tiles_y;
         const int x = tx * 32 + col, y0 = ty * 16 + wave * 4;
         f32x16 acc[4];
 #pragma unroll
@@ -57,39 +63,56 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[g][r] = 0.0f;
 
-#pragma unroll 1
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int zz = z + dz;
-            const bool zok = zz >= 0 && zz < S.nz;
+        // steps (dz, dx, channel chunk); the six activation rows of the next
+        // step are loaded while the twelve MFMAs of the current one run
+        constexpr int NS = 9 * NQ;
+        // the rows come through a buffer descriptor over the (at most three)
+        // input slices of this output slice; a row outside the volume gets
+        // an offset past the descriptor's range, which reads as zero.  With
+        // no branch and no select on the loaded values, the compiler waits
+        // only for the rows the next MFMAs read.
+        const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
+        const size_t plane = (size_t)S.ny * S.nx * CIN;   // bf16 elements per slice
+        const unsigned long long base = (unsigned long long)(in + zlo * plane);
+        const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+        const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(((unsigned long long)bhi << 32) | blo), 0,
+            __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
+        auto load_rows = [&](int st, bf16x8 *rows) {
+            const int q = st % NQ, dxi = (st / NQ) % 3, dzi = st / (3 * NQ);
+            const int zz = z + dzi - 1, xx = x + dxi - 1;
+            const bool zxok = zz >= 0 && zz < S.nz && xx >= 0 && xx < S.nx;
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = x + dx;
-                const bool zxok = zok && xx >= 0 && xx < S.nx;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    bf16x8 rows[6];
-#pragma unroll
-                    for (int rr = 0; rr < 6; ++rr) {
-                        const int yy = y0 - 1 + rr;
-                        bf16x8 v;
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
-                        if (zxok && yy >= 0 && yy < S.ny) {
-                            const size_t vox = ((size_t)zz * S.ny + yy) * S.nx + xx;
-                            v = *reinterpret_cast<const bf16x8 *>(in + vox * CIN + q * 16 + kh * 8);
-                        }
-                        rows[rr] = v;
-                    }
-#pragma unroll
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                        const bf16x8 a = wl[((t * NQ + q) * 32 + col) * 2 + kh];
-#pragma unroll
-                        for (int g = 0; g < 4; ++g)
-                            acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, rows[g + dy + 1], acc[g], 0, 0, 0);
-                    }
-                }
+            for (int rr = 0; rr < 6; ++rr) {
+                const int yy = y0 - 1 + rr;
+                const bool ok = zxok && yy >= 0 && yy < S.ny;
+                const unsigned off =
+                    ok ? (unsigned)(((((size_t)(zz - zlo) * S.ny + yy) * S.nx + xx) * CIN + q * 16 + kh * 8) *
+                                    sizeof(__bf16))
+                       : 0x80000000u;
+                rows[rr] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
             }
+        };
+        auto mma = [&](int st, const bf16x8 *rows) {
+            const int q = st % NQ, dxi = (st / NQ) % 3, dzi = st / (3 * NQ);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int t = dzi * 9 + dy * 3 + dxi;
+                const bf16x8 a = wl[((t * NQ + q) * 32 + col) * 2 + kh];
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, rows[g + dy], acc[g], 0, 0, 0);
+            }
+        };
+        bf16x8 ra[6], rb[6];
+        load_rows(0, ra);
+#pragma unroll 1
+        for (int st = 0; st < NS; st += 2) {
+            load_rows(min(st + 1, NS - 1), rb);
+            mma(st, ra);
+            load_rows(min(st + 2, NS - 1), ra);
+            if (st + 1 < NS) mma(st + 1, rb);
         }
 
         // lane holds D[co][voxel col] for co = 8 s + 4 kh + u (s, u = 0..3)
@@ -117,6 +140,121 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
                     o[u] = (__bf16)v;
                 }
                 *reinterpret_cast<bf16x4 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+            }
+        }
+    }
+}
+
+// Last layer of the network (32 -> 1 channel, linear, FP32 out):
+//   out[v] = b + sum_t sum_ci W[t][ci] a[v + off(t)][ci].
+// A 32-output convolution would spend 31/32 of its MFMA work on padding.
+// Here the taps are the MFMA's M dimension instead: per input slice a block
+// forms P[t][u] = sum_ci W[t][ci] a[u][ci] (v_mfma_f32_32x32x16_bf16, A =
+// W^T with 27 of 32 rows, B = the activations of 32 voxels, K = 32 channels
+// in two steps) for every voxel u of its x-y tile with a one-voxel halo,
+// stores P in LDS, and each output gathers its nine taps of that slice.  A
+// slice contributes to the outputs one below (taps dz = +1), level (dz = 0)
+// and one above (dz = -1): three rolling sums per output, the lowest one
+// complete once the slice is done.  The block walks kConvZRun output slices.
+constexpr int kO1TX = 32, kO1TY = 16, kO1HX = kO1TX + 2, kO1HY = kO1TY + 2;
+constexpr int kO1Halo = kO1HX * kO1HY;                  // 612 voxels
+constexpr int kO1Tiles = (kO1Halo + 31) / 32;           // 20 MFMA column tiles
+constexpr int kO1Stride = kO1Tiles * 32 + 4;            // LDS floats per tap (+4: the two lane halves' taps 4 apart
+                                                        // fall in different banks)
+
+__global__ __launch_bounds__(256) void k_conv3d_out1(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
+                                                     const float *__restrict__ bias, float *__restrict__ out,
+                                                     CnnShape S)
+{
+    __shared__ float pl[kTaps * kO1Stride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 31, kh = lane >> 5;
+    // A = W^T (rows = taps, 27 of 32; K = input channels), from the packed
+    // forward layout [tap][q][co][16] at output channel 0
+    bf16x8 a[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+        if (col < kTaps) v = reinterpret_cast<const bf16x8 *>(wts)[((col * 2 + q) * 32) * 2 + kh];
+        a[q] = v;
+    }
+    const float b0 = bias[0];
+    const size_t plane = (size_t)S.ny * S.nx * 32;   // bf16 elements per slice
+    const int tiles_x = (S.nx + kO1TX - 1) / kO1TX, tiles_y = (S.ny + kO1TY - 1) / kO1TY;
+    const int zruns = (S.nz + kConvZRun - 1) / kConvZRun;
+    const int ncols = tiles_x * tiles_y * zruns;
+    const int ox = threadIdx.x & 31, oy0 = threadIdx.x >> 5;   // outputs (ox, oy0) and (ox, oy0 + 8)
+    for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x) {
+        const int x0 = (tcol % tiles_x) * kO1TX, y0 = ((tcol / tiles_x) % tiles_y) * kO1TY;
+        const int zs = (tcol / (tiles_x * tiles_y)) * kConvZRun, ze = min(S.nz, zs + kConvZRun);
+        float acc_lo[2] = {0.0f, 0.0f}, acc_mid[2] = {0.0f, 0.0f}, acc_hi[2] = {0.0f, 0.0f};
+        for (int zi = zs - 1; zi <= ze; ++zi) {
+            if (zi >= 0 && zi < S.nz) {   // uniform: slices outside the volume contribute zero
+                const unsigned long long base = (unsigned long long)(in + zi * plane);
+                const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+                const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+                const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(((unsigned long long)bhi << 32) | blo), 0,
+                    __builtin_amdgcn_readfirstlane((int)(plane * sizeof(__bf16))), 0x00020000);
+                __syncthreads();   // the previous slice's gathers are done with pl
+                for (int T = wave; T < kO1Tiles; T += 4) {
+                    const int i = T * 32 + col, hx = i % kO1HX, hy = i / kO1HX;
+                    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+                    const bool ok = i < kO1Halo && gx >= 0 && gx < S.nx && gy >= 0 && gy < S.ny;
+                    // a voxel outside the volume reads as zero (offset past the descriptor's range)
+                    const unsigned off =
+                        ok ? (unsigned)((((size_t)gy * S.nx + gx) * 32 + kh * 8) * sizeof(__bf16)) : 0x80000000u;
+                    const bf16x8 b0v = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+                    const bf16x8 b1v =
+                        __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 32u, 0, 0));
+                    f32x16 d;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) d[r] = 0.0f;
+                    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b0v, d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b1v, d, 0, 0, 0);
+                    // lane holds P[8 s + 4 kh + u][voxel T*32 + col]
+#pragma unroll
+                    for (int sg = 0; sg < 4; ++sg)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int t = 8 * sg + 4 * kh + u;
+                            if (t < kTaps) pl[t * kO1Stride + i] = d[4 * sg + u];
+                        }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int oy = oy0 + 8 * k;
+                    float below = 0.0f, level = 0.0f, above = 0.0f;
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                        for (int dx = 0; dx < 3; ++dx) {
+                            const int idx = (oy + dy) * kO1HX + ox + dx;
+                            above += pl[(0 * 9 + dy * 3 + dx) * kO1Stride + idx];   // dz = -1: output zi + 1
+                            level += pl[(1 * 9 + dy * 3 + dx) * kO1Stride + idx];   // dz = 0
+                            below += pl[(2 * 9 + dy * 3 + dx) * kO1Stride + idx];   // dz = +1: output zi - 1
+                        }
+                    acc_hi[k] += above;
+                    acc_mid[k] += level;
+                    acc_lo[k] += below;
+                }
+            }
+            const int zo = zi - 1;   // complete now
+            if (zo >= zs && zo < ze) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int x = x0 + ox, y = y0 + oy0 + 8 * k;
+                    if (x < S.nx && y < S.ny) out[((size_t)zo * S.ny + y) * S.nx + x] = acc_lo[k] + b0;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                acc_lo[k] = acc_mid[k];
+                acc_mid[k] = acc_hi[k];
+                acc_hi[k] = 0.0f;
             }
         }
     }
@@ -349,8 +487,8 @@ __global__ __launch_bounds__(256) void k_learned_candidate(const float *__restri
 
 int conv_grid(const CnnShape &S)
 {
-    const long tiles = (long)((S.nx + 31) / 32) * ((S.ny + 15) / 16) * S.nz;
-    return (int)(tiles < 2048 ? tiles : 2048);
+    const long cols = (long)((S.nx + 31) / 32) * ((S.ny + 15) / 16) * ((S.nz + kConvZRun - 1) / kConvZRun);
+    return (int)(cols < 2048 ? cols : 2048);
 }
 
 }  // namespace
@@ -368,7 +506,13 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
         else return hipErrorInvalidValue;
     } else if (cin_pad == 32) {
         if (epi == kEpiBiasRelu) CTR_CONV(32, kEpiBiasRelu);
-        else if (epi == kEpiLinearF32) CTR_CONV(32, kEpiLinearF32);
+        else if (epi == kEpiLinearF32) {
+            // one real output channel: the taps-as-rows kernel
+            const long cols = (long)((S.nx + kO1TX - 1) / kO1TX) * ((S.ny + kO1TY - 1) / kO1TY) *
+                              ((S.nz + kConvZRun - 1) / kConvZRun);
+            hipLaunchKernelGGL(k_conv3d_out1, dim3((unsigned)std::min(cols, 2048L)), dim3(256), 0, st, in, w, bias,
+                               (float *)out, S);
+        }
         else if (epi == kEpiMask) CTR_CONV(32, kEpiMask);
         else return hipErrorInvalidValue;
     } else {
