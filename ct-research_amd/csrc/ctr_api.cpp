@@ -2901,9 +2901,9 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     std::vector<FpJob> fj(n);
     std::vector<ReduceJob> rj(n);
     std::vector<std::vector<AdjointJob>> groups(3), ones_groups(3);   // by LDS pitch 40 / 72 / 136
-    std::vector<int> group_blocks(3, 0), group_cm(3, 4), group_fit(3, 1);
+    std::vector<int> group_blocks(3, 0), group_max(3, 0), group_cm(3, 4), group_fit(3, 1);
     const int pitches[3] = {40, 72, 136};
-    int fp_total = 0;
+    int fp_total = 0, fp_max = 0;
     for (int p = 0; p < n; ++p) {
         BatchProblem &q = B.probs[p];
         const ctr_cone_geom *g = &pr[p].geom;
@@ -2934,6 +2934,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         TRY(buf_reserve(ctx, q.part, sizeof(double) * 2 * ((size_t)std::max(q.fp_blocks, 4096) + kReduceSlack)));
         fj[p] = FpJob{q.R, q.xp, q.bp, (float *)q.r.ptr, (double *)q.part.ptr, fp_total, q.gx, q.gy};
         fp_total += q.fp_blocks;
+        fp_max = std::max(fp_max, q.fp_blocks);
         rj[p] = ReduceJob{(const double *)q.part.ptr, q.fp_blocks};
         const BpPlan &L = q.F.plan;
         BpParams P = q.P;
@@ -2950,6 +2951,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         ones_groups[gi].push_back(AdjointJob{P, (const int4 *)q.F.buf.ptr, nullptr, (float *)q.vinv.ptr, U,
                                              group_blocks[gi], nwg});
         group_blocks[gi] += nwg;
+        group_max[gi] = std::max(group_max[gi], nwg);
         group_cm[gi] = L.cm;
         group_fit[gi] = group_fit[gi] && L.fit_all;
         if (!o->use_x0) TRY(check_launch(ctx, launch_fill(q.xp, 0.0f, q.nvox, st), "fill"));
@@ -2982,7 +2984,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     for (int gi = 0; gi < 3; ++gi)
         if (!groups[gi].empty())
             TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], group_fit[gi] != 0, dones + gofs[gi], (int)groups[gi].size(),
-                                                         group_blocks[gi], 0, 1, st), "batched column sums"));
+                                                         group_max[gi], 0, 1, st), "batched column sums"));
     for (int p = 0; p < n; ++p) {
         BatchProblem &q = B.probs[p];
         TRY(check_launch(ctx, launch_reciprocal_eps((float *)q.vinv.ptr, q.nvox, 1e-9f, st), "reciprocal"));
@@ -3005,13 +3007,13 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
                 TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, s),
                                  "volume transpose"));
         }
-        TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_total, s), "batched residual"));
+        TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_max, s), "batched residual"));
         TRY(check_launch(ctx, launch_reduce_jobs(drj, n, diter, dsums, s), "batched reduce"));
         if (!update) return CTR_OK;
         for (int gi = 0; gi < 3; ++gi)
             if (!groups[gi].empty())
                 TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], group_fit[gi] != 0, dadj + gofs[gi], (int)groups[gi].size(),
-                                                             group_blocks[gi], 1, 0, s), "batched update"));
+                                                             group_max[gi], 1, 0, s), "batched update"));
         return check_launch(ctx, launch_iter_inc(diter, s), "iteration counter");
     };
     if (o->n_iter > 0) {

@@ -225,37 +225,24 @@ hipError_t launch_foot_unfit(const int4 *foot, size_t n, int *out, hipStream_t s
 // out[b][r][c] = in[b][r][c] * P.adj_norm[r][c] over all views.
 hipError_t launch_scale_by_norm(const BpParams &P, const float *in, float *out, hipStream_t st);
 
-// Batched exact adjoint (independent problems in one launch).  P.tiles_* are
-// the problem's plan; jobs live in device memory, sorted by block0.
+// Batched exact adjoint (independent problems in one launch): job
+// blockIdx.y, its tile blockIdx.x (blocks past the job's tiles leave at
+// once).  P.tiles_* are the problem's plan; jobs live in device memory.
 struct AdjointJob {
     BpParams P;
     const int4 *foot;
     const float *proj;      // weighted residual times the ray norm (ignored when ones_input)
     float *vol;             // output (when not a SIRT update)
     SirtUpdate U;
-    int block0, nblocks;    // this job's blocks in the launch
+    int block0, nblocks;    // the job's first block in the batch's block count, its tiles
 };
 // fit_all: every job's plan has fit_all set (the variant without the
-// global-memory path for footprints too large for LDS).
-hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob *jobs, int njobs, int nblocks,
+// global-memory path for footprints too large for LDS); max_blocks: the
+// largest job's tile count.
+hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob *jobs, int njobs, int max_blocks,
                                   int sirt, int ones_input, hipStream_t st);
 
 // Batched SIRT residual pass: r = (b - A x)/(A1 + 1e-9) and per-block (sum e^2, sum |e|) per job.
-// The job of flat block `bid` in a batched launch: the last job whose first
-// block is <= bid (block0 ascending, jobs[0].block0 == 0), by binary search
-// over uniform values (scalar loads).
-template <typename Job>
-__device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int njobs, int bid)
-{
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bid >= jobs[mid].block0) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 struct FpJob {
     RayParams g;
     const float *vol;
@@ -265,7 +252,8 @@ struct FpJob {
     int block0, gx, gy;     // first block and the job's (columns, rows) block grid; views = z
 };
 int fp_job_blocks(const RayParams &g, int *gx, int *gy);
-hipError_t launch_ray_fp_residual_batched(const FpJob *jobs, int njobs, int nblocks, hipStream_t st);
+// job blockIdx.y, its block blockIdx.x; max_blocks: the largest job's block count
+hipError_t launch_ray_fp_residual_batched(const FpJob *jobs, int njobs, int max_blocks, hipStream_t st);
 // sums[(*iter) * njobs + p] = sum of job p's partial pairs (one block per job)
 struct ReduceJob {
     const double *partials;

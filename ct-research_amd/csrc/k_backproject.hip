@@ -668,10 +668,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(FITALL
                                                               int ones)
 {
     __shared__ __align__(16) float lds[AdjLds<TZ, WP, CAPF>::kFloats];
-    const int p = find_job(jobs, njobs, (int)blockIdx.x);
+    // job blockIdx.y, its block blockIdx.x; blocks past the job's tiles leave at once
+    const int p = blockIdx.y, local = blockIdx.x;
+    if (local >= jobs[p].nblocks) return;
     const BpParams P = jobs[p].P;
     const SirtUpdate U = jobs[p].U;
-    const int local = (int)blockIdx.x - jobs[p].block0;
     adjoint_tile<TZ, WP, CAPF, CM, FITALL>(jobs[p].proj, jobs[p].vol, P, jobs[p].foot, P.views, U, sirt, ones,
                                            xcd_tile(local, jobs[p].nblocks), lds);
 }
@@ -890,37 +891,39 @@ hipError_t launch_adjoint(const BpParams &Pin, const BpPlan &L, const int4 *foot
 }
 
 template <int CM, bool FITALL>
-static hipError_t launch_adjoint_batched_cm(int wp, const AdjointJob *jobs, int njobs, int nblocks, int sirt,
+static hipError_t launch_adjoint_batched_cm(int wp, const AdjointJob *jobs, int njobs, int max_blocks, int sirt,
                                             int ones_input, hipStream_t st)
 {
+    // x: the job's blocks (a multiple of 8, so that blockIdx.x mod 8 stays the XCD), y: the job
+    const dim3 grid((unsigned)((max_blocks + 7) / 8 * 8), (unsigned)njobs);
     if (wp == 40)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, adj_cap(40), CM, FITALL>), dim3(nblocks), dim3(kThreads), 0, st,
-                           jobs, njobs, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 40, adj_cap(40), CM, FITALL>), grid, dim3(kThreads), 0, st, jobs,
+                           njobs, sirt, ones_input);
     else if (wp == 72)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, adj_cap(72), CM, FITALL>), dim3(nblocks), dim3(kThreads), 0, st,
-                           jobs, njobs, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 72, adj_cap(72), CM, FITALL>), grid, dim3(kThreads), 0, st, jobs,
+                           njobs, sirt, ones_input);
     else if (wp == 136)
-        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, adj_cap(136), CM, FITALL>), dim3(nblocks), dim3(kThreads), 0,
-                           st, jobs, njobs, sirt, ones_input);
+        hipLaunchKernelGGL((k_adjoint_batched<kAdjTZ, 136, adj_cap(136), CM, FITALL>), grid, dim3(kThreads), 0, st, jobs,
+                           njobs, sirt, ones_input);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 template <int CM>
-static hipError_t launch_adjoint_batched_fit(int wp, bool fit_all, const AdjointJob *jobs, int njobs, int nblocks,
+static hipError_t launch_adjoint_batched_fit(int wp, bool fit_all, const AdjointJob *jobs, int njobs, int max_blocks,
                                              int sirt, int ones_input, hipStream_t st)
 {
-    return fit_all ? launch_adjoint_batched_cm<CM, true>(wp, jobs, njobs, nblocks, sirt, ones_input, st)
-                   : launch_adjoint_batched_cm<CM, false>(wp, jobs, njobs, nblocks, sirt, ones_input, st);
+    return fit_all ? launch_adjoint_batched_cm<CM, true>(wp, jobs, njobs, max_blocks, sirt, ones_input, st)
+                   : launch_adjoint_batched_cm<CM, false>(wp, jobs, njobs, max_blocks, sirt, ones_input, st);
 }
 
-hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob *jobs, int njobs, int nblocks,
+hipError_t launch_adjoint_batched(int wp, int cm, bool fit_all, const AdjointJob *jobs, int njobs, int max_blocks,
                                   int sirt, int ones_input, hipStream_t st)
 {
-    if (cm == 3) return launch_adjoint_batched_fit<3>(wp, fit_all, jobs, njobs, nblocks, sirt, ones_input, st);
-    if (cm == 4) return launch_adjoint_batched_fit<4>(wp, fit_all, jobs, njobs, nblocks, sirt, ones_input, st);
-    return launch_adjoint_batched_fit<6>(wp, fit_all, jobs, njobs, nblocks, sirt, ones_input, st);
+    if (cm == 3) return launch_adjoint_batched_fit<3>(wp, fit_all, jobs, njobs, max_blocks, sirt, ones_input, st);
+    if (cm == 4) return launch_adjoint_batched_fit<4>(wp, fit_all, jobs, njobs, max_blocks, sirt, ones_input, st);
+    return launch_adjoint_batched_fit<6>(wp, fit_all, jobs, njobs, max_blocks, sirt, ones_input, st);
 }
 
 hipError_t launch_foot_unfit(const int4 *foot, size_t n, int *out, hipStream_t st)

@@ -471,9 +471,11 @@ __global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual(const float *__res
 // Independent problems in one launch: a flat grid over the jobs' block ranges.
 __global__ __launch_bounds__(kBX *kBY) void k_ray_fp_residual_batched(const FpJob *__restrict__ jobs, int njobs)
 {
-    const int p = find_job(jobs, njobs, (int)blockIdx.x);
+    // job blockIdx.y, its block blockIdx.x; blocks past the job's grid leave at once
+    const int p = blockIdx.y, local = blockIdx.x;
+    const int gx = jobs[p].gx, gy = jobs[p].gy;
+    if (local >= gx * gy * jobs[p].g.nb) return;
     const RayParams g = jobs[p].g;
-    const int local = (int)blockIdx.x - jobs[p].block0, gx = jobs[p].gx, gy = jobs[p].gy;
     const int bx = local % gx, by = (local / gx) % gy, ib = local / (gx * gy);
     fp_residual_block(jobs[p].vol, jobs[p].b, jobs[p].r, jobs[p].partials, g, bx, by, ib, (size_t)local);
 }
@@ -525,9 +527,10 @@ int fp_job_blocks(const RayParams &g, int *gx, int *gy)
     return (int)(grid.x * grid.y * grid.z);
 }
 
-hipError_t launch_ray_fp_residual_batched(const FpJob *jobs, int njobs, int nblocks, hipStream_t st)
+hipError_t launch_ray_fp_residual_batched(const FpJob *jobs, int njobs, int max_blocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ray_fp_residual_batched, dim3(nblocks), dim3(kBX, kBY), 0, st, jobs, njobs);
+    hipLaunchKernelGGL(k_ray_fp_residual_batched, dim3((unsigned)max_blocks, (unsigned)njobs), dim3(kBX, kBY), 0, st,
+                       jobs, njobs);
     return hipGetLastError();
 }
 
