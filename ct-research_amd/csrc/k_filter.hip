@@ -127,15 +127,16 @@ __global__ __launch_bounds__(64 *RW) void k_row_filter(const float *__restrict__
 // one complex transform pair, z' = IDFT(DFT(z) H) = y0 + i y1, computed as
 // conj(DFT(conj(DFT(z)) H / L)).  FP64 throughout; H / L comes from the host.
 //
-// Transforms are Stockham passes of radix 8, then one of radix 2 or 4 when
-// L is not a power of 8, over G = L/8 threads that each hold 8 points.  After
-// the last pass thread j holds X[j + G t] (t = 0..7), which is exactly the
-// input distribution of the first pass, so the spectrum multiply and the
-// second transform follow in registers.  LDS holds one padded transform
-// (index i at i + i/8) per group of G threads.  With 8 points per thread the
-// kernel needs about 100 registers, so four workgroups of 256 threads share
-// a CU (a radix-16 design held 16 points per thread and ran at one or two
-// waves per SIMD).
+// Two transform kernels.  k_fft16_filter (below; radix 16, one transform per
+// wave for L <= 1024) serves the flat-detector modes.  k_fft_filter serves
+// the curved-detector mode, whose rows pair by detector row across views and
+// whose Parker weights are staged through LDS: Stockham passes of radix 8,
+// then one of radix 2 or 4 when L is not a power of 8, over G = L/8 threads
+// that each hold 8 points.  After the last pass thread j holds X[j + G t]
+// (t = 0..7), which is exactly the input distribution of the first pass, so
+// the spectrum multiply and the second transform follow in registers.  LDS
+// holds one padded transform (index i at i + i/8) per group of G threads;
+// about 100 registers, four workgroups of 256 threads per CU.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -313,9 +314,11 @@ __device__ __forceinline__ void fft_group(double2 *v, double2 *buf, int j, const
     }
 }
 
-template <int L, int MODE>
-__device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, float *__restrict__ out,
-                                                const FilterParams &P)
+// Curved-detector short scan (mode 3): rows pair by detector row across two
+// consecutive views (one spectrum per detector row).
+template <int L>
+__global__ __launch_bounds__(kFftThreads) void k_fft_filter_curved(const float *__restrict__ in,
+                                                                  float *__restrict__ out, FilterParams P)
 {
     constexpr int G = FftPlan<L>::G, GPB = kFftThreads / G;
     constexpr int SLOT = L + L / 8;
@@ -326,72 +329,43 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
     const int np = P.np;
     fft_twiddle_table<L>(tab);   // published by the first barrier of the row loop
 
-    // with one spectrum per detector row, a pair is one row index in two consecutive views
     const int nviews = P.rows / P.nx;
-    const int npairs = P.spec_stride ? P.nx * ((nviews + 1) / 2) : (P.rows + 1) / 2;
+    const int npairs = P.nx * ((nviews + 1) / 2);
     for (int base = blockIdx.x * GPB; base < npairs; base += gridDim.x * GPB) {
         const int pair = base + grp;   // every group takes part in the barriers, active or not
-        const int row0 = P.spec_stride ? (2 * (pair / P.nx)) * P.nx + pair % P.nx : 2 * pair;
-        const int row1 = P.spec_stride ? row0 + P.nx : row0 + 1;
+        const int row0 = (2 * (pair / P.nx)) * P.nx + pair % P.nx;
+        const int row1 = row0 + P.nx;
         const bool has0 = pair < npairs, has1 = has0 && row1 < P.rows;
-        const float xi0 = MODE == 0 && has0 ? P.xd[row0 % P.nx] : 0.0f;
-        const float xi1 = MODE == 0 && has1 ? P.xd[row1 % P.nx] : 0.0f;
-        const double *spec = P.spec + (P.spec_stride && has0 ? (size_t)(row0 % P.nx) * P.spec_stride : 0);
-        // mode 3: row scale and the scaled view angles of the two rows
-        const double prow = MODE == 3 && has0 ? P.rowp[row0 % P.nx] : 1.0;
-        const double bp0 = MODE == 3 && has0 ? P.betad[row0 / P.nx] * prow : 0.0;
-        const double bp1 = MODE == 3 && has1 ? P.betad[row1 / P.nx] * prow : 0.0;
+        const double *spec = P.spec + (has0 ? (size_t)(row0 % P.nx) * P.spec_stride : 0);
+        // row scale and the scaled view angles of the two rows
+        const double prow = has0 ? P.rowp[row0 % P.nx] : 1.0;
+        const double bp0 = has0 ? P.betad[row0 / P.nx] * prow : 0.0;
+        const double bp1 = has1 ? P.betad[row1 / P.nx] * prow : 0.0;
         const float *src0 = in + (size_t)row0 * np, *src1 = in + (size_t)row1 * np;
         double2 v[8];
-        if constexpr (MODE == 3) {
-            // the Parker-weighted samples go through LDS: a rolled loop keeps
-            // the weight arithmetic of one sample live at a time
-            __syncthreads();   // the group's previous readers of buf are done
+        // the Parker-weighted samples go through LDS: a rolled loop keeps the
+        // weight arithmetic of one sample live at a time
+        __syncthreads();   // the group's previous readers of buf are done
 #pragma unroll 1
-            for (int t = 0; t < 8; ++t) {
-                const int q = j + G * t;
-                double a = 0.0, b = 0.0;
-                if (q < np) {
-                    // ImplementUI.cu:43-66: (D/P) cos(g P) Parker(b P, start P, g P, fan P)
-                    const double gp = P.gamd[q] * prow;
-                    const float base_w = (float)((double)P.D / prow) * cosf((float)gp);
-                    const double stp = P.start * prow, fanp = P.fan * prow;
-                    const float w0 = base_w * parker_weight(bp0, stp, gp, fanp);
-                    const float w1 = base_w * parker_weight(bp1, stp, gp, fanp);
-                    // the reference keeps the weighted samples in FP64 (ImplementUI.cu:66)
-                    if (has0) a = (double)src0[q] * (double)w0;
-                    if (has1) b = (double)src1[q] * (double)w1;
-                }
-                buf[fft_pad(q)] = make_double2(a, b);
+        for (int t = 0; t < 8; ++t) {
+            const int q = j + G * t;
+            double a = 0.0, b = 0.0;
+            if (q < np) {
+                // ImplementUI.cu:43-66: (D/P) cos(g P) Parker(b P, start P, g P, fan P)
+                const double gp = P.gamd[q] * prow;
+                const float base_w = (float)((double)P.D / prow) * cosf((float)gp);
+                const double stp = P.start * prow, fanp = P.fan * prow;
+                const float w0 = base_w * parker_weight(bp0, stp, gp, fanp);
+                const float w1 = base_w * parker_weight(bp1, stp, gp, fanp);
+                // the reference keeps the weighted samples in FP64 (ImplementUI.cu:66)
+                if (has0) a = (double)src0[q] * (double)w0;
+                if (has1) b = (double)src1[q] * (double)w1;
             }
-            __syncthreads();
-#pragma unroll
-            for (int t = 0; t < 8; ++t) v[t] = buf[fft_pad(j + G * t)];
-        } else {
-            // thread j holds samples j + G t: loaded and weighted in
-            // registers (the first pass's barrier publishes nothing of them)
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int q = j + G * t;
-                double a = 0.0, b = 0.0;
-                if (q < np) {
-                    if constexpr (MODE == 0) {
-                        const float pq = P.pd[q];
-                        if (has0) a = (double)cone_preweight(src0[q], cone_weight64(P.Dd, pq, xi0));
-                        if (has1) b = (double)cone_preweight(src1[q], cone_weight64(P.Dd, pq, xi1));
-                    } else if constexpr (MODE == 4) {
-                        if (has0) a = (double)cone_preweight(src0[q], P.pw[(size_t)(row0 % P.nx) * np + q]);
-                        if (has1) b = (double)cone_preweight(src1[q], P.pw[(size_t)(row1 % P.nx) * np + q]);
-                    } else {
-                        const float w = MODE == 1 ? P.colw[q] : 1.0f;
-                        // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
-                        if (has0) a = (double)(src0[q] * w);
-                        if (has1) b = (double)(src1[q] * w);
-                    }
-                }
-                v[t] = make_double2(a, b);
-            }
+            buf[fft_pad(q)] = make_double2(a, b);
         }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[t] = buf[fft_pad(j + G * t)];
         fft_group<L>(v, buf, j, tab);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -410,33 +384,280 @@ __device__ __forceinline__ void fft_filter_body(const float *__restrict__ in, fl
     }
 }
 
-// One kernel per pre-weight mode: the curved-detector weights (FP64 Parker
-// window) need registers that the other modes should not hold.
-template <int L, int MODE>
-__global__ __launch_bounds__(kFftThreads) void k_fft_filter(const float *__restrict__ in, float *__restrict__ out,
-                                                          FilterParams P)
+// ---------------------------------------------------------------------------
+// Radix-16 transforms of length L = 256 .. 2048 over G = L/16 lanes that each
+// hold 16 points: Stockham passes of radix 16 (Ns = 1), 16 (Ns = 16) and,
+// for L > 256, one of radix R3 = L/256 (Ns = 256) in registers.  Lane j holds
+// v[r] = x[j + G r] before and after a transform.  The two exchanges of a
+// transform go through one padded array of 8-byte words (index i at
+// i + i/16, no bank conflicts for any of the three access patterns), real
+// parts first, then imaginary parts: L = 1024 needs 8.7 KB per wave, so
+// sixteen waves fit a CU.  For L <= 1024 a transform lives in one wave (one
+// to four transforms per wave), so no wave waits for another; L = 2048 takes
+// two waves.  Rows are zero-padded to L >= 2np - 1, so the forward
+// transform's first pass sees zeros in v[8..15] (x[j + G r] with
+// j + G r >= L/2 >= np), and of the inverse only outputs j + G t with t < 8
+// are kept (the compiler drops the rest).  Against the radix-8 kernel above
+// at L = 1024: two exchanges per transform instead of three, and no barrier
+// between waves.
+// ---------------------------------------------------------------------------
+namespace f16 {
+template <int L>
+struct Plan {
+    static_assert(L >= 256 && L <= 2048 && (L & (L - 1)) == 0, "length");
+    static constexpr int G = L / 16;                     // lanes per transform
+    static constexpr int R3 = L / 256;                   // radix of the last pass (1: none)
+    static constexpr int NB = 256 / G;                   // last-pass butterflies per lane
+    static constexpr int kWords = L + L / 16;            // padded words of one transform
+    static constexpr int kThreads = G < 64 ? 64 : G;     // workgroup
+    static constexpr int kPerBlock = kThreads / G;       // transforms per workgroup
+};
+
+constexpr double kC = 0.92387953251128675613, kS = 0.38268343236508977173;   // cos(pi/8), sin(pi/8)
+
+// a * w16^E with w16 = exp(-2 pi i / 16)
+template <int E>
+__device__ __forceinline__ double2 mul_w16(double2 a)
 {
-    fft_filter_body<L, MODE>(in, out, P);
+    constexpr int e = E & 15;
+    if constexpr (e == 0) return a;
+    else if constexpr (e == 4) return mul_mi(a);
+    else if constexpr (e == 8) return make_double2(-a.x, -a.y);
+    else if constexpr (e == 12) return make_double2(-a.y, a.x);
+    else if constexpr (e == 2) return make_double2((a.x + a.y) * kR2, (a.y - a.x) * kR2);
+    else if constexpr (e == 6) return make_double2((a.y - a.x) * kR2, -(a.x + a.y) * kR2);
+    else {
+        // cos(2 pi e / 16), -sin(2 pi e / 16)
+        constexpr double c[16] = {1, kC, kR2, kS, 0, -kS, -kR2, -kC, -1, -kC, -kR2, -kS, 0, kS, kR2, kC};
+        constexpr double s[16] = {0, -kS, -kR2, -kC, -1, -kC, -kR2, -kS, 0, kS, kR2, kC, 1, kC, kR2, kS};
+        return cmul(a, make_double2(c[e], s[e]));
+    }
+}
+
+// forward 16-point DFT, natural order in and out, as 4 x 4: DFTs over
+// n2 of v[n1 + 4 n2], twiddles w16^(n1 k2), DFTs over n1.  ZERO_HI: the
+// inputs v[8..15] are zero.
+template <bool ZERO_HI>
+__device__ __forceinline__ void dft16(double2 *v)
+{
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        if constexpr (ZERO_HI) {
+            const double2 a = v[n1], b = v[n1 + 4], mb = mul_mi(b);
+            v[n1] = cadd(a, b);
+            v[n1 + 4] = cadd(a, mb);
+            v[n1 + 8] = csub(a, b);
+            v[n1 + 12] = csub(a, mb);
+        } else {
+            dft4(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]);
+        }
+    }
+    v[5] = mul_w16<1>(v[5]);
+    v[9] = mul_w16<2>(v[9]);
+    v[13] = mul_w16<3>(v[13]);
+    v[6] = mul_w16<2>(v[6]);
+    v[10] = mul_w16<4>(v[10]);
+    v[14] = mul_w16<6>(v[14]);
+    v[7] = mul_w16<3>(v[7]);
+    v[11] = mul_w16<6>(v[11]);
+    v[15] = mul_w16<9>(v[15]);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) dft4(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+    // v[4 k2 + k1] holds X[k2 + 4 k1]
+    double2 t[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2)
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) t[k2 + 4 * k1] = v[4 * k2 + k1];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = t[r];
+}
+
+// Orders the LDS accesses of the lanes of one transform.  Within one wave
+// the LDS instructions execute in issue order, so a wave needs only a
+// compiler fence (no s_waitcnt, no s_barrier); a transform spanning two
+// waves (MULTI) needs the workgroup barrier.
+template <bool MULTI>
+__device__ __forceinline__ void group_sync()
+{
+    if constexpr (MULTI) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// words[wbase + WSTEP r] = v[r], then v[r] = words[rbase + RSTEP r], real
+// parts first, then imaginary parts.  The last sync keeps the next writes
+// to words behind this exchange's reads.
+template <int WSTEP, int RSTEP, bool MULTI>
+__device__ __forceinline__ void exchange(double2 *v, double *words, int wbase, int rbase)
+{
+#pragma unroll
+    for (int r = 0; r < 16; ++r) words[wbase + WSTEP * r] = v[r].x;
+    group_sync<MULTI>();
+    double re[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) re[r] = words[rbase + RSTEP * r];
+    group_sync<MULTI>();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) words[wbase + WSTEP * r] = v[r].y;
+    group_sync<MULTI>();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = make_double2(re[r], words[rbase + RSTEP * r]);
+    group_sync<MULTI>();
+}
+
+// One forward transform of this lane's group; words is the group's LDS
+// array, t256 = w_256^(j mod 16), tL = w_L^j.  ZERO_HI: x[j + G r] = 0 for
+// r >= 8.
+template <int L, bool ZERO_HI>
+__device__ __forceinline__ void transform(double2 *v, double *words, int j, double2 t256, double2 tL)
+{
+    using F = Plan<L>;
+    constexpr int RSTEP = F::G + F::G / 16;   // padded stride of x[j + G r]
+    const int rbase = j + (j >> 4);
+    // the powers of the twiddle bases are formed anew in each transform:
+    // hoisted out of the row loop they would hold about 100 registers
+    asm volatile("" : "+v"(t256.x), "+v"(t256.y), "+v"(tL.x), "+v"(tL.y));
+    dft16<ZERO_HI>(v);
+    // pass 1 (Ns = 1): output r of lane j goes to 16 j + r, padded 17 j + r
+    exchange<1, RSTEP, (F::G > 64)>(v, words, 17 * j, rbase);
+    twiddle_powers(v, 16, t256);
+    dft16<false>(v);
+    // pass 2 (Ns = 16): to (j / 16) 256 + j mod 16 + 16 r, padded (j / 16) 272 + j mod 16 + 17 r
+    exchange<17, RSTEP, (F::G > 64)>(v, words, (j >> 4) * 272 + (j & 15), rbase);
+    if constexpr (F::R3 > 1) {
+        // pass 3 (Ns = 256, radix R3): butterfly b = j + G m reads v[m + NB r] =
+        // x'[b + 256 r] and twiddles by w_L^(b r), where w_L^b = w_L^j w16^m
+#pragma unroll
+        for (int m = 0; m < F::NB; ++m) {
+            double2 base = tL;
+            if (m == 1) base = mul_w16<1>(tL);
+            if (m == 2) base = mul_w16<2>(tL);
+            if (m == 3) base = mul_w16<3>(tL);
+            if (m == 4) base = mul_w16<4>(tL);
+            if (m == 5) base = mul_w16<5>(tL);
+            if (m == 6) base = mul_w16<6>(tL);
+            if (m == 7) base = mul_w16<7>(tL);
+            double2 u[F::R3];
+#pragma unroll
+            for (int r = 0; r < F::R3; ++r) u[r] = v[m + F::NB * r];
+            twiddle_powers(u, F::R3, base);
+            if constexpr (F::R3 == 2) dft2(u[0], u[1]);
+            else if constexpr (F::R3 == 4) dft4(u[0], u[1], u[2], u[3]);
+            else dft8(u);
+#pragma unroll
+            for (int r = 0; r < F::R3; ++r) v[m + F::NB * r] = u[r];
+        }
+    }
+}
+}  // namespace f16
+
+// MODE as k_fft_filter (0, 1, 2, 4; rows are paired as 2p, 2p + 1); each
+// group of G lanes filters one row pair per step.
+// Four waves per SIMD (128 registers); the FP64 weights formed per sample
+// (MODE 0) need more, so three.
+template <int L, int MODE>
+__global__ __launch_bounds__(f16::Plan<L>::kThreads)
+__attribute__((amdgpu_waves_per_eu(MODE == 0 ? 3 : 4, MODE == 0 ? 3 : 4)))
+void k_fft16_filter(const float *__restrict__ in, float *__restrict__ out, FilterParams P)
+{
+    using F = f16::Plan<L>;
+    __shared__ double words[F::kPerBlock * F::kWords];
+    // with one group per workgroup the row pair is uniform (scalar index arithmetic)
+    const int grp = F::kPerBlock == 1 ? 0 : threadIdx.x / F::G, j = threadIdx.x % F::G, np = P.np;
+    double *mine = words + grp * F::kWords;
+    const double2 t256 = twiddle(j & 15, 256), tL = twiddle(j, L);
+    const int npairs = (P.rows + 1) / 2;
+    // every group takes part in the barriers of a two-wave transform; with
+    // several groups per wave a group past the last pair computes on zeros
+    for (int base = blockIdx.x * F::kPerBlock; base < npairs; base += gridDim.x * F::kPerBlock) {
+        const int pair = base + grp;
+        const int row0 = 2 * pair, row1 = row0 + 1;
+        // one group per workgroup: the loop bound already keeps pair < npairs
+        const bool has0 = F::kPerBlock == 1 || pair < npairs, has1 = row1 < P.rows;
+        // every load is in bounds (absent rows and columns read a stand-in
+        // and are zeroed after), so all of them issue before the first wait
+        const int rd0 = has0 ? row0 : 0, rd1 = has1 ? row1 : rd0;
+        // the lane index, opaque per row pair: addresses formed from it are
+        // recomputed per pair rather than hoisted out of the loop and spilled
+        int jr = j;
+        asm volatile("" : "+v"(jr));
+        const float *src0 = in + (size_t)rd0 * np, *src1 = in + (size_t)rd1 * np;
+        const float xi0 = MODE == 0 ? P.xd[rd0 % P.nx] : 0.0f;
+        const float xi1 = MODE == 0 ? P.xd[rd1 % P.nx] : 0.0f;
+        const double *pw0 = MODE == 4 ? P.pw + (size_t)(rd0 % P.nx) * np : nullptr;
+        const double *pw1 = MODE == 4 ? P.pw + (size_t)(rd1 % P.nx) * np : nullptr;
+        double2 v[16];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = jr + F::G * t, qc = q < np ? q : np - 1;
+            double a, b;
+            if constexpr (MODE == 0) {
+                const float pq = P.pd[qc];
+                a = (double)cone_preweight(src0[qc], cone_weight64(P.Dd, pq, xi0));
+                b = (double)cone_preweight(src1[qc], cone_weight64(P.Dd, pq, xi1));
+            } else if constexpr (MODE == 4) {
+                a = (double)cone_preweight(src0[qc], pw0[qc]);
+                b = (double)cone_preweight(src1[qc], pw1[qc]);
+            } else {
+                const float w = MODE == 1 ? P.colw[qc] : 1.0f;
+                // pre-weighted samples rounded to float like the in-place store of FDK.cu:36
+                a = (double)(src0[qc] * w);
+                b = (double)(src1[qc] * w);
+            }
+            v[t] = make_double2(q < np && has0 ? a : 0.0, q < np && has1 ? b : 0.0);
+            v[t + 8] = make_double2(0.0, 0.0);
+        }
+        f16::transform<L, true>(v, mine, j, t256, tL);
+        // H[k] / L (real) at k = j + G t, from a uniform base and a 32-bit offset
+        const char *hbase = reinterpret_cast<const char *>(P.spec);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const double h = *reinterpret_cast<const double *>(hbase + 8u * (unsigned)(jr + F::G * t));
+            v[t] = make_double2(v[t].x * h, -v[t].y * h);
+        }
+        f16::transform<L, false>(v, mine, j, t256, tL);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = jr + F::G * t;
+            if (q < np) {
+                if (has0) out[(size_t)row0 * np + q] = (float)v[t].x;
+                if (has1) out[(size_t)row1 * np + q] = (float)(-v[t].y);
+            }
+        }
+    }
 }
 
 template <int L>
 hipError_t launch_fft(const FilterParams &P, const float *in, float *out, hipStream_t st)
 {
+    if (P.mode != 3 && !P.spec_stride) {
+        using F = f16::Plan<L>;
+        const int npairs = (P.rows + 1) / 2, need = (npairs + F::kPerBlock - 1) / F::kPerBlock;
+        const int cap = 256 * 32 * 64 / F::kThreads;   // pair stride beyond 32 waves per CU
+        const int nblk = need < cap ? need : cap;
+        const dim3 grid(nblk), block(F::kThreads);
+        switch (P.mode) {
+        case 0:
+            // mode 4: mode 0 with the FP64 weight table
+            if (P.pw) hipLaunchKernelGGL((k_fft16_filter<L, 4>), grid, block, 0, st, in, out, P);
+            else hipLaunchKernelGGL((k_fft16_filter<L, 0>), grid, block, 0, st, in, out, P);
+            break;
+        case 1: hipLaunchKernelGGL((k_fft16_filter<L, 1>), grid, block, 0, st, in, out, P); break;
+        default: hipLaunchKernelGGL((k_fft16_filter<L, 2>), grid, block, 0, st, in, out, P); break;
+        }
+        return hipGetLastError();
+    }
+    if (P.mode != 3 || !P.spec_stride) return hipErrorInvalidValue;
     constexpr int GPB = kFftThreads / FftPlan<L>::G;
-    const int npairs = P.spec_stride ? P.nx * ((P.rows / P.nx + 1) / 2) : (P.rows + 1) / 2;
+    const int npairs = P.nx * ((P.rows / P.nx + 1) / 2);
     const int need = (npairs + GPB - 1) / GPB;
     const int nblk = need < 256 * 16 ? need : 256 * 16;   // grid-stride beyond 16 blocks per CU
-    switch (P.mode) {
-    case 0:
-        // mode 4: mode 0 with the FP64 weight table
-        if (P.pw) hipLaunchKernelGGL((k_fft_filter<L, 4>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
-        else hipLaunchKernelGGL((k_fft_filter<L, 0>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
-        break;
-    case 1: hipLaunchKernelGGL((k_fft_filter<L, 1>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
-    case 2: hipLaunchKernelGGL((k_fft_filter<L, 2>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
-    case 3: hipLaunchKernelGGL((k_fft_filter<L, 3>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P); break;
-    default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL((k_fft_filter_curved<L>), dim3(nblk), dim3(kFftThreads), 0, st, in, out, P);
     return hipGetLastError();
 }
 
