@@ -4,14 +4,16 @@
 // kernels, zero padding 1.  Activations are BF16, channels-last
 // ([z][y][x][c], c fastest), weights BF16, accumulation FP32.
 //
-// Convolution = implicit GEMM on v_mfma_f32_32x32x16_bf16:
-//   D[co][voxel] += W[co][ci] * act[ci][voxel]   (M = 32 output channels,
-//   N = 32 voxels along x, K = 16 input channels of one tap)
-// A (weights, 32 x 16 per tap and channel chunk) is staged in LDS once per
-// block; B (activations) is read straight from global memory: lane l takes
-// 8 consecutive channels (16 bytes) of voxel x0 + (l & 31).  A wave owns
-// 32 x R voxels (R = 8 or 4 rows in y) and loads the R + 2 rows around them
-// once per (dz, dx, chunk), reusing them for the 3 dy taps.
+// Convolution = implicit GEMM, D[co][voxel] += W[co][ci] * act[ci][voxel].
+// 16 input channels (the first layer, and the input gradient of the last):
+// k_conv3d on v_mfma_f32_32x32x16_bf16 (M = 32 output channels, N = 32
+// voxels along x, K = 16 input channels of one tap).  32 input channels:
+// k_conv3d_c32 on v_mfma_f32_16x16x32_bf16 (M = 16 output channels in two
+// halves, N = 16 voxels, K = all 32 channels).  A (weights) is staged in
+// LDS once per block; B (activations) is read straight from global memory
+// as 16-byte channel vectors.  A wave owns R rows of voxels along y and
+// loads the R + 2 rows around them once per step, reusing them for the
+// 3 dy taps.
 //
 // The backward pass w.r.t. the input is the same convolution with the taps
 // flipped and the channel roles swapped; its epilogue applies the ReLU
@@ -25,11 +27,12 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTaps = 27;
 constexpr int kConvZRun = 16;   // consecutive output slices per x-y tile of a conv block
-// output rows (y) per wave of k_conv3d: 8 for 32 input channels (24 MFMAs per
-// step of 10 loaded rows, one wave per SIMD), 4 for 16 (two waves per SIMD)
+// output rows (y) per wave of k_conv3d: 4 for 16 input channels (two waves
+// per SIMD)
 constexpr int conv_rows(int cin) { return cin == 32 ? 8 : 4; }
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
@@ -141,6 +144,131 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
                     if (EPI == kEpiBiasRelu) {
                         v = fmaxf(v + bias[c0 + u], 0.0f);
                     } else if (EPI == kEpiMask) {
+                        v = bf2f(mask[vox * 32 + c0 + u]) > 0.0f ? v : 0.0f;
+                    }
+                    o[u] = (__bf16)v;
+                }
+                *reinterpret_cast<bf16x4 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+            }
+        }
+    }
+}
+
+// conv3d with 32 input channels on v_mfma_f32_16x16x32_bf16: M = 16 output
+// channels (two halves), N = 16 voxels along x, K = all 32 input channels
+// of one tap.  Lane l takes channels 8 (l / 16) .. + 8 of voxel
+// x0 + l mod 16, so one load instruction of a wave reads 16 whole voxels
+// (1 KB contiguous), and one (dz, dx) step covers both channel halves:
+// nine steps per output slice where the 32 x 32 x 16 layout above needs
+// eighteen (1.37 -> 1.13 ms per layer at 256^3; both are bound by the
+// vector-memory address unit, TA busy 0.74 and 0.79).  A wave owns 16 x R voxels and
+// loads the R + 2 rows around them once per (dz, dx) step for the 3 dy taps
+// and both output halves.  The A operands come from the same global layout
+// [tap][q][32 co][16 ci], staged in LDS as [tap][half][lane] so a wave's
+// read is one conflict-free 1 KB run.
+constexpr int kC32Rows = 8;   // output rows (y) per wave; four waves per block
+template <int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv3d_c32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
+                                                    const float *__restrict__ bias, const __bf16 *__restrict__ mask,
+                                                    void *__restrict__ out, CnnShape S)
+{
+    constexpr int CIN = 32, R = kC32Rows;
+    constexpr int NW = kTaps * 2 * 64;   // bf16x8 vectors, [tap][half][lane]
+    __shared__ bf16x8 wl[NW];
+    for (int i = threadIdx.x; i < NW; i += 256) {
+        const int l = i & 63, h = (i >> 6) & 1, t = i >> 7;
+        // lane l of half h: co = 16 h + l mod 16, ci = 8 (l / 16) .. + 8 = chunk l / 32, half (l / 16) mod 2
+        wl[i] = reinterpret_cast<const bf16x8 *>(wts)[((t * 2 + (l >> 5)) * 32 + 16 * h + (l & 15)) * 2 + ((l >> 4) & 1)];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kg = lane >> 4;
+    constexpr int TY = 4 * R;   // rows per block tile
+    const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + TY - 1) / TY;
+    const int zchunks = (S.nz + kConvZRun - 1) / kConvZRun;
+    const int ncols = tiles_x * tiles_y * zchunks;
+    for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x)
+    for (int z = (tcol / (tiles_x * tiles_y)) * kConvZRun,
+             zend = min(S.nz, z + kConvZRun); z < zend; ++z) {
+        const int tx = tcol % tiles_x, ty = (tcol / tiles_x) % tiles_y;
+        const int x = tx * 16 + col, y0 = ty * TY + wave * R;
+        f32x4 acc[R][2];
+#pragma unroll
+        for (int g = 0; g < R; ++g)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+        // rows through a buffer descriptor over the (at most three) input
+        // slices; a row outside the volume reads as zero (offset past range)
+        const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
+        const size_t plane = (size_t)S.ny * S.nx * CIN;
+        const unsigned long long base = (unsigned long long)(in + zlo * plane);
+        const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+        const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(((unsigned long long)bhi << 32) | blo), 0,
+            __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
+        auto load_rows = [&](int st, bf16x8 *rows) {
+            const int dxi = st % 3, dzi = st / 3;
+            const int zz = z + dzi - 1, xx = x + dxi - 1;
+            const bool zxok = zz >= 0 && zz < S.nz && xx >= 0 && xx < S.nx;
+#pragma unroll
+            for (int rr = 0; rr < R + 2; ++rr) {
+                const int yy = y0 - 1 + rr;
+                const bool ok = zxok && yy >= 0 && yy < S.ny;
+                const unsigned off =
+                    ok ? (unsigned)(((((size_t)(zz - zlo) * S.ny + yy) * S.nx + xx) * CIN + kg * 8) * sizeof(__bf16))
+                       : 0x80000000u;
+                rows[rr] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+            }
+        };
+        auto mma = [&](int st, const bf16x8 *rows) {
+            const int dxi = st % 3, dzi = st / 3;
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int t = dzi * 9 + dy * 3 + dxi;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const bf16x8 a = wl[(t * 2 + h) * 64 + lane];
+#pragma unroll
+                    for (int g = 0; g < R; ++g)
+                        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, rows[g + dy], acc[g][h], 0, 0, 0);
+                }
+            }
+        };
+        // nine (dz, dx) steps: the rows of step s + 1 are in flight while
+        // step s multiplies; the last step is peeled so that no step's rows
+        // load twice (a third row buffer does not fit 256 registers)
+        constexpr int NS = 9;
+        bf16x8 ra[R + 2], rb[R + 2];
+        load_rows(0, ra);
+#pragma unroll 1
+        for (int st = 0; st < NS - 1; st += 2) {
+            load_rows(st + 1, rb);
+            mma(st, ra);
+            load_rows(st + 2, ra);
+            mma(st + 1, rb);
+        }
+        mma(NS - 1, ra);
+
+        // lane holds D[co][voxel x] for co = 16 h + 4 kg + u (u = 0..3)
+        if (x >= S.nx) continue;
+#pragma unroll
+        for (int g = 0; g < R; ++g) {
+            const int y = y0 + g;
+            if (y >= S.ny) continue;
+            const size_t vox = ((size_t)z * S.ny + y) * S.nx + x;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c0 = 16 * h + 4 * kg;
+                bf16x4 o;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float v = acc[g][h][u];
+                    if (EPI == kEpiBiasRelu) {
+                        v = fmaxf(v + bias[c0 + u], 0.0f);
+                    } else {
                         v = bf2f(mask[vox * 32 + c0 + u]) > 0.0f ? v : 0.0f;
                     }
                     o[u] = (__bf16)v;
@@ -512,7 +640,11 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
         else if (epi == kEpiMask) CTR_CONV(16, kEpiMask);
         else return hipErrorInvalidValue;
     } else if (cin_pad == 32) {
-        if (epi == kEpiBiasRelu) CTR_CONV(32, kEpiBiasRelu);
+        const int ty = 4 * kC32Rows;
+        const long cols = (long)((S.nx + 15) / 16) * ((S.ny + ty - 1) / ty) * ((S.nz + kConvZRun - 1) / kConvZRun);
+        const dim3 g32((unsigned)std::min(cols, 4096L));
+        if (epi == kEpiBiasRelu)
+            hipLaunchKernelGGL((k_conv3d_c32<kEpiBiasRelu>), g32, dim3(256), 0, st, in, w, bias, mask, out, S);
         else if (epi == kEpiLinearF32) {
             // one real output channel: the taps-as-rows kernel
             const long cols = (long)((S.nx + kO1TX - 1) / kO1TX) * ((S.ny + kO1TY - 1) / kO1TY) *
@@ -520,7 +652,8 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
             hipLaunchKernelGGL(k_conv3d_out1, dim3((unsigned)std::min(cols, 2048L)), dim3(256), 0, st, in, w, bias,
                                (float *)out, S);
         }
-        else if (epi == kEpiMask) CTR_CONV(32, kEpiMask);
+        else if (epi == kEpiMask)
+            hipLaunchKernelGGL((k_conv3d_c32<kEpiMask>), g32, dim3(256), 0, st, in, w, bias, mask, out, S);
         else return hipErrorInvalidValue;
     } else {
         return hipErrorInvalidValue;
