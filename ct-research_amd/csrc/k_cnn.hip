@@ -158,15 +158,38 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
 // channels (two halves), N = 16 voxels along x, K = all 32 input channels
 // of one tap.  Lane l takes channels 8 (l / 16) .. + 8 of voxel
 // x0 + l mod 16, so one load instruction of a wave reads 16 whole voxels
-// (1 KB contiguous), and one (dz, dx) step covers both channel halves:
-// nine steps per output slice where the 32 x 32 x 16 layout above needs
-// eighteen (1.37 -> 1.13 ms per layer at 256^3; both are bound by the
-// vector-memory address unit, TA busy 0.74 and 0.79).  A wave owns 16 x R voxels and
-// loads the R + 2 rows around them once per (dz, dx) step for the 3 dy taps
-// and both output halves.  The A operands come from the same global layout
+// (1 KB contiguous) for both channel halves.  A wave owns 16 x R voxels;
+// per dz it loads the R + 2 rows around them once (the centre column x0 +
+// l mod 16) and forms the dx = -1 and dx = +1 operands by shifting that row
+// across the lanes of each channel group, with the two halo voxels (x0 - 1
+// and x0 + 16) from one more load; each row feeds the 3 dy taps and both
+// output halves.  Against three loads per (dz, row), one for each dx:
+// 1.13 -> 0.97 ms per layer at 256^3 (vector-memory unit busy 0.79 -> 0.57),
+// and against the 32 x 32 x 16 layout above (half voxels, two channel
+// steps): 1.37 ms.  The A operands come from the global layout
 // [tap][q][32 co][16 ci], staged in LDS as [tap][half][lane] so a wave's
 // read is one conflict-free 1 KB run.
 constexpr int kC32Rows = 8;   // output rows (y) per wave; four waves per block
+
+// x-neighbour operand from a centre row: a 16-lane DPP row holds the 16
+// voxels x0 .. x0 + 15 of one channel group, so a row shift by one lane
+// gives voxel x0 + col - 1 (CTRL = row_shr:1) or x0 + col + 1 (row_shl:1);
+// the lane whose source lies outside the row keeps `edge`, the halo voxel
+// loaded for it.
+template <int CTRL>
+__device__ __forceinline__ bf16x8 shift_cols(bf16x8 edge, bf16x8 centre)
+{
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 e = __builtin_bit_cast(i32x4, edge), c = __builtin_bit_cast(i32x4, centre);
+    i32x4 r;
+    r[0] = __builtin_amdgcn_update_dpp(e[0], c[0], CTRL, 0xf, 0xf, false);
+    r[1] = __builtin_amdgcn_update_dpp(e[1], c[1], CTRL, 0xf, 0xf, false);
+    r[2] = __builtin_amdgcn_update_dpp(e[2], c[2], CTRL, 0xf, 0xf, false);
+    r[3] = __builtin_amdgcn_update_dpp(e[3], c[3], CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(bf16x8, r);
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShl1 = 0x101;
+
 template <int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv3d_c32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
                                                     const float *__restrict__ bias, const __bf16 *__restrict__ mask,
@@ -177,30 +200,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __shared__ bf16x8 wl[NW];
     for (int i = threadIdx.x; i < NW; i += 256) {
         const int l = i & 63, h = (i >> 6) & 1, t = i >> 7;
-        // lane l of half h: co = 16 h + l mod 16, ci = 8 (l / 16) .. + 8 = chunk l / 32, half (l / 16) mod 2
         wl[i] = reinterpret_cast<const bf16x8 *>(wts)[((t * 2 + (l >> 5)) * 32 + 16 * h + (l & 15)) * 2 + ((l >> 4) & 1)];
     }
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
-    constexpr int TY = 4 * R;   // rows per block tile
+    constexpr int TY = 4 * R;
     const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + TY - 1) / TY;
     const int zchunks = (S.nz + kConvZRun - 1) / kConvZRun;
     const int ncols = tiles_x * tiles_y * zchunks;
+    const bool edge = col == 0 || col == 15;
     for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x)
     for (int z = (tcol / (tiles_x * tiles_y)) * kConvZRun,
              zend = min(S.nz, z + kConvZRun); z < zend; ++z) {
         const int tx = tcol % tiles_x, ty = (tcol / tiles_x) % tiles_y;
-        const int x = tx * 16 + col, y0 = ty * TY + wave * R;
+        const int x0 = tx * 16, x = x0 + col, y0 = ty * TY + wave * R;
+        const int hx = col == 0 ? x0 - 1 : x0 + 16;   // halo voxel of an edge lane
         f32x4 acc[R][2];
 #pragma unroll
         for (int g = 0; g < R; ++g)
 #pragma unroll
             for (int h = 0; h < 2; ++h) acc[g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-        // rows through a buffer descriptor over the (at most three) input
-        // slices; a row outside the volume reads as zero (offset past range)
         const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
         const size_t plane = (size_t)S.ny * S.nx * CIN;
         const unsigned long long base = (unsigned long long)(in + zlo * plane);
@@ -209,10 +231,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(((unsigned long long)bhi << 32) | blo), 0,
             __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
-        auto load_rows = [&](int st, bf16x8 *rows) {
-            const int dxi = st % 3, dzi = st / 3;
-            const int zz = z + dzi - 1, xx = x + dxi - 1;
-            const bool zxok = zz >= 0 && zz < S.nz && xx >= 0 && xx < S.nx;
+        // rows y0 - 1 .. y0 + R of slice z + dzi - 1 at voxel xx (out of range: zero)
+        auto load_rows = [&](int dzi, int xx, bool lane_ok, bf16x8 *rows) {
+            const int zz = z + dzi - 1;
+            const bool zxok = lane_ok && zz >= 0 && zz < S.nz && xx >= 0 && xx < S.nx;
 #pragma unroll
             for (int rr = 0; rr < R + 2; ++rr) {
                 const int yy = y0 - 1 + rr;
@@ -223,36 +245,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 rows[rr] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
             }
         };
-        auto mma = [&](int st, const bf16x8 *rows) {
-            const int dxi = st % 3, dzi = st / 3;
+        // the taps (dzi, dxi, dy) of every row: row r feeds output rows r - dy
+        auto mma_dx = [&](int dzi, int dxi, const bf16x8 *centre, const bf16x8 *halo) {
+            bf16x8 a[3][2];
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const int t = dzi * 9 + dy * 3 + dxi;
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const bf16x8 a = wl[(t * 2 + h) * 64 + lane];
+                for (int h = 0; h < 2; ++h) a[dy][h] = wl[((dzi * 9 + dy * 3 + dxi) * 2 + h) * 64 + lane];
 #pragma unroll
-                    for (int g = 0; g < R; ++g)
-                        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, rows[g + dy], acc[g][h], 0, 0, 0);
+            for (int r = 0; r < R + 2; ++r) {
+                const bf16x8 row = dxi == 1   ? centre[r]
+                                   : dxi == 0 ? shift_cols<kDppRowShr1>(halo[r], centre[r])
+                                              : shift_cols<kDppRowShl1>(halo[r], centre[r]);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const int g = r - dy;
+                    if (g < 0 || g >= R) continue;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dy][h], row, acc[g][h], 0, 0, 0);
                 }
             }
         };
-        // nine (dz, dx) steps: the rows of step s + 1 are in flight while
-        // step s multiplies; the last step is peeled so that no step's rows
-        // load twice (a third row buffer does not fit 256 registers)
-        constexpr int NS = 9;
-        bf16x8 ra[R + 2], rb[R + 2];
-        load_rows(0, ra);
+        // per dz: the centre and halo rows, then the three dx taps (a
+        // prefetch of the next dz's rows does not fit 256 registers with 8
+        // rows per wave; with 6 rows per wave it measured 1.08 ms per layer
+        // at 256^3 against 0.97 for this form)
+        bf16x8 cur[R + 2], hal[R + 2];
 #pragma unroll 1
-        for (int st = 0; st < NS - 1; st += 2) {
-            load_rows(st + 1, rb);
-            mma(st, ra);
-            load_rows(st + 2, ra);
-            mma(st + 1, rb);
+        for (int dzi = 0; dzi < 3; ++dzi) {
+            load_rows(dzi, x, true, cur);
+            load_rows(dzi, hx, edge, hal);
+            mma_dx(dzi, 1, cur, hal);
+            mma_dx(dzi, 0, cur, hal);
+            mma_dx(dzi, 2, cur, hal);
         }
-        mma(NS - 1, ra);
 
-        // lane holds D[co][voxel x] for co = 16 h + 4 kg + u (u = 0..3)
         if (x >= S.nx) continue;
 #pragma unroll
         for (int g = 0; g < R; ++g) {
