@@ -200,7 +200,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __shared__ bf16x8 wl[NW];
     for (int i = threadIdx.x; i < NW; i += 256) {
         const int l = i & 63, h = (i >> 6) & 1, t = i >> 7;
-        wl[i] = reinterpret_cast<const bf16x8 *>(wts)[((t * 2 + (l >> 5)) * 32 + 16 * h + (l & 15)) * 2 + ((l >> 4) & 1)];
+        // A row m of half h is output channel 8 (m / 4) + 4 h + m mod 4, so
+        // that a lane's two accumulator quads are 8 consecutive channels
+        const int m = l & 15, co = 8 * (m >> 2) + 4 * h + (m & 3);
+        wl[i] = reinterpret_cast<const bf16x8 *>(wts)[((t * 2 + (l >> 5)) * 32 + co) * 2 + ((l >> 4) & 1)];
     }
     __syncthreads();
 
@@ -287,22 +290,131 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             const int y = y0 + g;
             if (y >= S.ny) continue;
             const size_t vox = ((size_t)z * S.ny + y) * S.nx + x;
+            // channels 8 kg .. 8 kg + 7: one 16-byte store per lane
+            const int c0 = 8 * kg;
+            bf16x8 o;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int c0 = 16 * h + 4 * kg;
-                bf16x4 o;
+            for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     float v = acc[g][h][u];
                     if (EPI == kEpiBiasRelu) {
-                        v = fmaxf(v + bias[c0 + u], 0.0f);
+                        v = fmaxf(v + bias[c0 + 4 * h + u], 0.0f);
                     } else {
-                        v = bf2f(mask[vox * 32 + c0 + u]) > 0.0f ? v : 0.0f;
+                        v = bf2f(mask[vox * 32 + c0 + 4 * h + u]) > 0.0f ? v : 0.0f;
                     }
-                    o[u] = (__bf16)v;
+                    o[4 * h + u] = (__bf16)v;
                 }
-                *reinterpret_cast<bf16x4 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+            *reinterpret_cast<bf16x8 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+        }
+    }
+}
+
+// First layer (2 -> 32 channels) from the compact input [V][2] (x_k and
+// the scaled SIRT step, 4 bytes per voxel).  K of v_mfma_f32_16x16x32_bf16
+// runs over (tap, channel) pairs instead of channels: 27 taps x 2 channels
+// in two steps of 32 (taps 27..31 are zero padding), so a row of 16 voxels
+// takes 4 MFMAs where the channel-K layout of k_conv3d takes 27 with 14 of
+// every 16 channels zero.  Lane l (column c = l mod 16, group g = l / 16)
+// supplies for step s the taps 16 s + 4 g + jj (jj < 4): one 4-byte load
+// each (both channels of one voxel; the 16 lanes of a group read 64
+// contiguous bytes), bounds-checked per access (outside the volume, and for
+// padding taps, the offset lies past the descriptor's range and the load
+// returns zeros).  On the [V][16] layout the same loads touched eight times
+// the cache lines and took 1.18 ms at 256^3.
+constexpr int kC2Rows = 4;   // output rows per wave task (4: 0.50 ms at 256^3, 8: 0.55, 16: 0.67)
+__global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
+                                                   const float *__restrict__ bias, __bf16 *__restrict__ out, CnnShape S)
+{
+    constexpr int CIN = 2, R = kC2Rows;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kg = lane >> 4;
+    // A[s][h]: W[co][ch][tap] for K index 8 kg + 2 jj + ch, A row col = output
+    // channel 8 (col / 4) + 4 h + col mod 4 (a lane's accumulators are then
+    // 8 consecutive channels)
+    bf16x8 A[2][2];
+    int tdx[2][4], tdy[2][4], tdz[2][4];
+    bool tok[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int t = 16 * s + 4 * kg + jj;
+            tok[s][jj] = t < kTaps;
+            tdz[s][jj] = t / 9 - 1;
+            tdy[s][jj] = (t / 3) % 3 - 1;
+            tdx[s][jj] = t % 3 - 1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int ch = 0; ch < 2; ++ch)
+                    A[s][h][2 * jj + ch] =
+                        t < kTaps ? wts[(t * 32 + 8 * (col >> 2) + 4 * h + (col & 3)) * 16 + ch] : (__bf16)0.0f;
+        }
+
+    const int tiles_x = (S.nx + 15) / 16, groups_y = (S.ny + R - 1) / R;
+    const long ntasks = (long)tiles_x * groups_y * S.nz;
+    for (long task = (long)blockIdx.x * 4 + wave; task < ntasks; task += (long)gridDim.x * 4) {
+        const int tx = (int)(task % tiles_x), gy = (int)((task / tiles_x) % groups_y), z = (int)(task / ((long)tiles_x * groups_y));
+        const int x = tx * 16 + col, y0 = gy * R;
+        const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
+        const size_t plane = (size_t)S.ny * S.nx * CIN;
+        const unsigned long long base = (unsigned long long)(in + zlo * plane);
+        const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+        const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(((unsigned long long)bhi << 32) | blo), 0,
+            __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
+        // per tap: x and z validity and the row-independent part of the offset
+        bool xzok[2][4];
+        int xzoff[2][4];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int xi = x + tdx[s][jj], zi = z + tdz[s][jj];
+                xzok[s][jj] = tok[s][jj] && xi >= 0 && xi < S.nx && zi >= 0 && zi < S.nz;
+                xzoff[s][jj] = (zi - zlo) * S.ny * S.nx + xi;   // voxels, before the row term
             }
+        f32x4 acc[R][2];
+#pragma unroll
+        for (int g = 0; g < R; ++g) {
+            const int y = y0 + g;
+            bf16x8 B[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                typedef int i32x4 __attribute__((ext_vector_type(4)));
+                i32x4 b;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int yi = y + tdy[s][jj];
+                    const bool ok = xzok[s][jj] && yi >= 0 && yi < S.ny;
+                    const unsigned off = ok ? (unsigned)(xzoff[s][jj] + yi * S.nx) * (unsigned)(CIN * sizeof(__bf16))
+                                            : 0x80000000u;
+                    b[jj] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+                }
+                B[s] = __builtin_bit_cast(bf16x8, b);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][h], B[0], f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][h], B[1], acc[g][h], 0, 0, 0);
+            }
+        }
+        // lane holds D[co][voxel x] for co = 8 kg + 4 h + u (u = 0..3): one
+        // 16-byte store per row
+        if (x >= S.nx) continue;
+#pragma unroll
+        for (int g = 0; g < R; ++g) {
+            const int y = y0 + g;
+            if (y >= S.ny) continue;
+            const size_t vox = ((size_t)z * S.ny + y) * S.nx + x;
+            bf16x8 o;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) o[4 * h + u] = (__bf16)fmaxf(acc[g][h][u] + bias[8 * kg + 4 * h + u], 0.0f);
+            *reinterpret_cast<bf16x8 *>(out + vox * 32 + 8 * kg) = o;
         }
     }
 }
@@ -597,6 +709,19 @@ __global__ __launch_bounds__(256) void k_pack_input(const float *__restrict__ x,
     reinterpret_cast<bf16x8 *>(out)[2 * i + 1] = b;
 }
 
+// Compact network input for k_conv3d_c2: [V][2] = (x, g * g_scale).
+__global__ __launch_bounds__(256) void k_pack_input2(const float *__restrict__ x, const float *__restrict__ g,
+                                                     float g_scale, size_t n, __bf16 *__restrict__ out)
+{
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    bf16x2 a;
+    a[0] = (__bf16)x[i];
+    a[1] = (__bf16)(g[i] * g_scale);
+    reinterpret_cast<bf16x2 *>(out)[i] = a;
+}
+
 // Masked mean-squared error of pred against target over the interior of
 // the volume (voxels at least `border` from every face) and its gradient,
 // written as the BF16 delta of the last layer (channel 0 of 16).
@@ -745,6 +870,21 @@ hipError_t launch_adam(float *p, float *m, float *v, const float *g, int n, floa
 hipError_t launch_pack_input(const float *x, const float *g, float g_scale, size_t n, __bf16 *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, g, g_scale, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_input2(const float *x, const float *g, float g_scale, size_t n, __bf16 *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_input2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, g, g_scale, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv3d_first(const __bf16 *in2, const __bf16 *w, const float *bias, __bf16 *out, const CnnShape &S,
+                               hipStream_t st)
+{
+    const long tasks = (long)((S.nx + 15) / 16) * ((S.ny + kC2Rows - 1) / kC2Rows) * S.nz;
+    const dim3 grid((unsigned)std::min((tasks + 3) / 4, 8192L));
+    hipLaunchKernelGGL(k_conv3d_c2, grid, dim3(256), 0, st, in2, w, bias, out, S);
     return hipGetLastError();
 }
 
