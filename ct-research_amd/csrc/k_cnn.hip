@@ -464,30 +464,47 @@ __global__ __launch_bounds__(256) void k_conv3d_out1(const __bf16 *__restrict__ 
         const int x0 = (tcol % tiles_x) * kO1TX, y0 = ((tcol / tiles_x) % tiles_y) * kO1TY;
         const int zs = (tcol / (tiles_x * tiles_y)) * kConvZRun, ze = min(S.nz, zs + kConvZRun);
         float acc_lo[2] = {0.0f, 0.0f}, acc_mid[2] = {0.0f, 0.0f}, acc_hi[2] = {0.0f, 0.0f};
+        // the halo voxels of a slice for this wave's column tiles T = wave +
+        // 4 j, loaded one slice ahead: the loads of slice zi + 1 are in
+        // flight while slice zi is multiplied and gathered
+        constexpr int NJ = (kO1Tiles + 3) / 4;
+        auto load_slice = [&](int zi, bf16x8 (*dst)[2]) {
+            const bool zok = zi >= 0 && zi < S.nz;
+            const unsigned long long base = (unsigned long long)(in + (zok ? zi : 0) * plane);
+            const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+            const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(((unsigned long long)bhi << 32) | blo), 0,
+                __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(__bf16)) : 0), 0x00020000);
+#pragma unroll
+            for (int jt = 0; jt < NJ; ++jt) {
+                const int T = wave + 4 * jt;
+                const int i = T * 32 + col, hx = i % kO1HX, hy = i / kO1HX;
+                const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+                const bool ok = T < kO1Tiles && i < kO1Halo && gx >= 0 && gx < S.nx && gy >= 0 && gy < S.ny;
+                // a voxel outside the volume reads as zero (offset past the descriptor's range)
+                const unsigned off =
+                    ok ? (unsigned)((((size_t)gy * S.nx + gx) * 32 + kh * 8) * sizeof(__bf16)) : 0x80000000u;
+                dst[jt][0] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+                dst[jt][1] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 32u, 0, 0));
+            }
+        };
+        bf16x8 cb[NJ][2], nb[NJ][2];
+        load_slice(zs - 1, cb);
         for (int zi = zs - 1; zi <= ze; ++zi) {
+            if (zi + 1 <= ze) load_slice(zi + 1, nb);
             if (zi >= 0 && zi < S.nz) {   // uniform: slices outside the volume contribute zero
-                const unsigned long long base = (unsigned long long)(in + zi * plane);
-                const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
-                const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
-                const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *)(((unsigned long long)bhi << 32) | blo), 0,
-                    __builtin_amdgcn_readfirstlane((int)(plane * sizeof(__bf16))), 0x00020000);
                 __syncthreads();   // the previous slice's gathers are done with pl
-                for (int T = wave; T < kO1Tiles; T += 4) {
-                    const int i = T * 32 + col, hx = i % kO1HX, hy = i / kO1HX;
-                    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-                    const bool ok = i < kO1Halo && gx >= 0 && gx < S.nx && gy >= 0 && gy < S.ny;
-                    // a voxel outside the volume reads as zero (offset past the descriptor's range)
-                    const unsigned off =
-                        ok ? (unsigned)((((size_t)gy * S.nx + gx) * 32 + kh * 8) * sizeof(__bf16)) : 0x80000000u;
-                    const bf16x8 b0v = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
-                    const bf16x8 b1v =
-                        __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 32u, 0, 0));
+#pragma unroll
+                for (int jt = 0; jt < NJ; ++jt) {
+                    const int T = wave + 4 * jt;
+                    if (T >= kO1Tiles) continue;
+                    const int i = T * 32 + col;
                     f32x16 d;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) d[r] = 0.0f;
-                    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b0v, d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b1v, d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], cb[jt][0], d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], cb[jt][1], d, 0, 0, 0);
                     // lane holds P[8 s + 4 kh + u][voxel T*32 + col]
 #pragma unroll
                     for (int sg = 0; sg < 4; ++sg)
@@ -529,6 +546,13 @@ __global__ __launch_bounds__(256) void k_conv3d_out1(const __bf16 *__restrict__ 
                 acc_lo[k] = acc_mid[k];
                 acc_mid[k] = acc_hi[k];
                 acc_hi[k] = 0.0f;
+            }
+            if (zi + 1 <= ze) {
+#pragma unroll
+                for (int jt = 0; jt < NJ; ++jt) {
+                    cb[jt][0] = nb[jt][0];
+                    cb[jt][1] = nb[jt][1];
+                }
             }
         }
     }
