@@ -154,13 +154,13 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
     }
 }
 
-// conv3d with 32 input channels on v_mfma_f32_16x16x32_bf16: M = 16 output
-// channels (two halves), N = 16 voxels along x, K = all 32 input channels
-// of one tap.  Lane l takes channels 8 (l / 16) .. + 8 of voxel
-// x0 + l mod 16, so one load instruction of a wave reads 16 whole voxels
-// (1 KB contiguous) for both channel halves.  A wave owns 16 x R voxels;
-// per dz it loads the R + 2 rows around them once (the centre column x0 +
-// l mod 16) and forms the dx = -1 and dx = +1 operands by shifting that row
+// conv3d with 32 input channels (k_conv3d_c32 below) on
+// v_mfma_f32_16x16x32_bf16: M = 16 output channels (two halves), N = 16
+// voxels along x, K = all 32 input channels of one tap.  Lane l takes
+// channels 8 (l / 16) .. + 8 of voxel x0 + l mod 16, so one load
+// instruction of a wave reads 16 whole voxels (1 KB contiguous) for both
+// channel halves.  Each input row is loaded once at the centre column
+// x0 + l mod 16; the dx = -1 and dx = +1 operands are that row shifted
 // across the lanes of each channel group, with the two halo voxels (x0 - 1
 // and x0 + 16) from one more load; each row feeds the 3 dy taps and both
 // output halves.  Against three loads per (dz, row), one for each dx:
@@ -169,7 +169,6 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
 // steps): 1.37 ms.  The A operands come from the global layout
 // [tap][q][32 co][16 ci], staged in LDS as [tap][half][lane] so a wave's
 // read is one conflict-free 1 KB run.
-constexpr int kC32Rows = 8;   // output rows (y) per wave; four waves per block
 
 // x-neighbour operand from a centre row: a 16-lane DPP row holds the 16
 // voxels x0 .. x0 + 15 of one channel group, so a row shift by one lane
@@ -190,18 +189,26 @@ __device__ __forceinline__ bf16x8 shift_cols(bf16x8 edge, bf16x8 centre)
 }
 constexpr int kDppRowShr1 = 0x111, kDppRowShl1 = 0x101;
 
+// The 32-channel convolution walks input slices: each input slice zi is
+// loaded once (centre rows prefetched while the previous slice multiplies,
+// halo rows while its own centre-only taps run) and feeds the three output
+// slices zi + 1, zi, zi - 1 through the tap planes dz = -1, 0, +1: three
+// rolling accumulator sets per row, the lowest complete (and stored) once
+// slice zi is done.  A wave owns 16 x 4 outputs; a block runs 32 output
+// slices of its 16 x 16 tile.  Against loading the three input slices of
+// every output slice (8 rows per wave): vector-memory unit busy 0.52 ->
+// 0.30, 0.96 -> 0.93 ms per layer at 256^3.
+constexpr int kC32zRows = 4, kC32zRun = 32;   // output rows per wave, output slices per block task
 template <int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv3d_c32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
                                                     const float *__restrict__ bias, const __bf16 *__restrict__ mask,
                                                     void *__restrict__ out, CnnShape S)
 {
-    constexpr int CIN = 32, R = kC32Rows;
+    constexpr int CIN = 32, R = kC32zRows;
     constexpr int NW = kTaps * 2 * 64;   // bf16x8 vectors, [tap][half][lane]
     __shared__ bf16x8 wl[NW];
     for (int i = threadIdx.x; i < NW; i += 256) {
         const int l = i & 63, h = (i >> 6) & 1, t = i >> 7;
-        // A row m of half h is output channel 8 (m / 4) + 4 h + m mod 4, so
-        // that a lane's two accumulator quads are 8 consecutive channels
         const int m = l & 15, co = 8 * (m >> 2) + 4 * h + (m & 3);
         wl[i] = reinterpret_cast<const bf16x8 *>(wts)[((t * 2 + (l >> 5)) * 32 + co) * 2 + ((l >> 4) & 1)];
     }
@@ -211,44 +218,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int col = lane & 15, kg = lane >> 4;
     constexpr int TY = 4 * R;
     const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + TY - 1) / TY;
-    const int zchunks = (S.nz + kConvZRun - 1) / kConvZRun;
+    const int zchunks = (S.nz + kC32zRun - 1) / kC32zRun;
     const int ncols = tiles_x * tiles_y * zchunks;
     const bool edge = col == 0 || col == 15;
-    for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x)
-    for (int z = (tcol / (tiles_x * tiles_y)) * kConvZRun,
-             zend = min(S.nz, z + kConvZRun); z < zend; ++z) {
+    const size_t plane = (size_t)S.ny * S.nx * CIN;
+    for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x) {
         const int tx = tcol % tiles_x, ty = (tcol / tiles_x) % tiles_y;
+        const int z0 = (tcol / (tiles_x * tiles_y)) * kC32zRun, z1 = min(S.nz, z0 + kC32zRun);
         const int x0 = tx * 16, x = x0 + col, y0 = ty * TY + wave * R;
         const int hx = col == 0 ? x0 - 1 : x0 + 16;   // halo voxel of an edge lane
-        f32x4 acc[R][2];
-#pragma unroll
-        for (int g = 0; g < R; ++g)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) acc[g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-
-        const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
-        const size_t plane = (size_t)S.ny * S.nx * CIN;
-        const unsigned long long base = (unsigned long long)(in + zlo * plane);
-        const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
-        const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
-        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(((unsigned long long)bhi << 32) | blo), 0,
-            __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
-        // rows y0 - 1 .. y0 + R of slice z + dzi - 1 at voxel xx (out of range: zero)
-        auto load_rows = [&](int dzi, int xx, bool lane_ok, bf16x8 *rows) {
-            const int zz = z + dzi - 1;
-            const bool zxok = lane_ok && zz >= 0 && zz < S.nz && xx >= 0 && xx < S.nx;
+        // rows y0 - 1 .. y0 + R of input slice zi at voxel xx (outside the volume: zero)
+        auto load_slice = [&](int zi, int xx, bool lane_ok, bf16x8 *rows) {
+            const bool zok = zi >= 0 && zi < S.nz;
+            const unsigned long long base = (unsigned long long)(in + (zok ? zi : 0) * plane);
+            const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+            const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(((unsigned long long)bhi << 32) | blo), 0,
+                __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(__bf16)) : 0), 0x00020000);
+            const bool xok = lane_ok && xx >= 0 && xx < S.nx;
 #pragma unroll
             for (int rr = 0; rr < R + 2; ++rr) {
                 const int yy = y0 - 1 + rr;
-                const bool ok = zxok && yy >= 0 && yy < S.ny;
+                const bool ok = xok && yy >= 0 && yy < S.ny;
                 const unsigned off =
-                    ok ? (unsigned)(((((size_t)(zz - zlo) * S.ny + yy) * S.nx + xx) * CIN + kg * 8) * sizeof(__bf16))
-                       : 0x80000000u;
+                    ok ? (unsigned)((((size_t)yy * S.nx + xx) * CIN + kg * 8) * sizeof(__bf16)) : 0x80000000u;
                 rows[rr] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
             }
         };
-        // the taps (dzi, dxi, dy) of every row: row r feeds output rows r - dy
+        f32x4 acc[3][R][2];   // [0]: output zi - 1, [1]: zi, [2]: zi + 1
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int g = 0; g < R; ++g)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) acc[q][g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        // taps (dzi, dxi, dy) of every row into the output slice acc[2 - dzi]
         auto mma_dx = [&](int dzi, int dxi, const bf16x8 *centre, const bf16x8 *halo) {
             bf16x8 a[3][2];
 #pragma unroll
@@ -266,46 +271,76 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     if (g < 0 || g >= R) continue;
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
-                        acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dy][h], row, acc[g][h], 0, 0, 0);
+                        acc[2 - dzi][g][h] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dy][h], row, acc[2 - dzi][g][h], 0, 0, 0);
                 }
             }
         };
-        // per dz: the centre and halo rows, then the three dx taps (a
-        // prefetch of the next dz's rows does not fit 256 registers with 8
-        // rows per wave; with 6 rows per wave it measured 1.08 ms per layer
-        // at 256^3 against 0.97 for this form)
-        bf16x8 cur[R + 2], hal[R + 2];
+        // centre rows are prefetched one slice ahead; the halo rows of slice
+        // zi go out when its centre-only taps (dx = 0) start
+        bf16x8 cur[R + 2], nxt[R + 2], chal[R + 2];
+        load_slice(z0 - 1, x, true, cur);
 #pragma unroll 1
-        for (int dzi = 0; dzi < 3; ++dzi) {
-            load_rows(dzi, x, true, cur);
-            load_rows(dzi, hx, edge, hal);
-            mma_dx(dzi, 1, cur, hal);
-            mma_dx(dzi, 0, cur, hal);
-            mma_dx(dzi, 2, cur, hal);
-        }
-
-        if (x >= S.nx) continue;
+        for (int zi = z0 - 1; zi <= z1; ++zi) {
+            if (zi + 1 <= z1) load_slice(zi + 1, x, true, nxt);
+            if (zi >= 0 && zi < S.nz) {   // uniform: a slice outside the volume contributes zero
+                load_slice(zi, hx, edge, chal);
+                // the empty asm statements keep each step's weight reads (and
+                // their registers) from being hoisted over the others
+                // plane dzi feeds output slice zi + 1 - dzi: skipped (uniform)
+                // where that slice lies outside this run
 #pragma unroll
-        for (int g = 0; g < R; ++g) {
-            const int y = y0 + g;
-            if (y >= S.ny) continue;
-            const size_t vox = ((size_t)z * S.ny + y) * S.nx + x;
-            // channels 8 kg .. 8 kg + 7: one 16-byte store per lane
-            const int c0 = 8 * kg;
-            bf16x8 o;
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    float v = acc[g][h][u];
-                    if (EPI == kEpiBiasRelu) {
-                        v = fmaxf(v + bias[c0 + 4 * h + u], 0.0f);
-                    } else {
-                        v = bf2f(mask[vox * 32 + c0 + 4 * h + u]) > 0.0f ? v : 0.0f;
-                    }
-                    o[4 * h + u] = (__bf16)v;
+                for (int dzi = 0; dzi < 3; ++dzi) {
+                    if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
+                    mma_dx(dzi, 1, cur, chal);
+                    asm volatile("" ::: "memory");
                 }
-            *reinterpret_cast<bf16x8 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+#pragma unroll
+                for (int dzi = 0; dzi < 3; ++dzi) {
+                    if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
+                    mma_dx(dzi, 0, cur, chal);
+                    asm volatile("" ::: "memory");
+                    mma_dx(dzi, 2, cur, chal);
+                    asm volatile("" ::: "memory");
+                }
+            }
+            // output slice zi - 1 has all three planes now
+            const int zo = zi - 1;
+            if (zo >= z0 && zo < z1 && x < S.nx) {
+#pragma unroll
+                for (int g = 0; g < R; ++g) {
+                    const int y = y0 + g;
+                    if (y >= S.ny) continue;
+                    const size_t vox = ((size_t)zo * S.ny + y) * S.nx + x;
+                    const int c0 = 8 * kg;
+                    bf16x8 o;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            float v = acc[0][g][h][u];
+                            if (EPI == kEpiBiasRelu) {
+                                v = fmaxf(v + bias[c0 + 4 * h + u], 0.0f);
+                            } else {
+                                v = bf2f(mask[vox * 32 + c0 + 4 * h + u]) > 0.0f ? v : 0.0f;
+                            }
+                            o[4 * h + u] = (__bf16)v;
+                        }
+                    *reinterpret_cast<bf16x8 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < R; ++g)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    acc[0][g][h] = acc[1][g][h];
+                    acc[1][g][h] = acc[2][g][h];
+                    acc[2][g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
+            if (zi + 1 <= z1) {
+#pragma unroll
+                for (int r = 0; r < R + 2; ++r) cur[r] = nxt[r];
+            }
         }
     }
 }
@@ -817,9 +852,9 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
         else if (epi == kEpiMask) CTR_CONV(16, kEpiMask);
         else return hipErrorInvalidValue;
     } else if (cin_pad == 32) {
-        const int ty = 4 * kC32Rows;
-        const long cols = (long)((S.nx + 15) / 16) * ((S.ny + ty - 1) / ty) * ((S.nz + kConvZRun - 1) / kConvZRun);
-        const dim3 g32((unsigned)std::min(cols, 4096L));
+        const long cols = (long)((S.nx + 15) / 16) * ((S.ny + 4 * kC32zRows - 1) / (4 * kC32zRows)) *
+                          ((S.nz + kC32zRun - 1) / kC32zRun);
+        const dim3 g32((unsigned)std::min(cols, 8192L));
         if (epi == kEpiBiasRelu)
             hipLaunchKernelGGL((k_conv3d_c32<kEpiBiasRelu>), g32, dim3(256), 0, st, in, w, bias, mask, out, S);
         else if (epi == kEpiLinearF32) {
