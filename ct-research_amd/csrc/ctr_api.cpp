@@ -2161,7 +2161,8 @@ const float *layer_bias(const ctr_cnn *net, int l)
 ctr_status cnn_reserve(ctr_cnn *net, size_t nvox, bool train)
 {
     ctr_ctx *ctx = net->ctx;
-    TRY(buf_reserve(ctx, net->in2, nvox * 2 * sizeof(__bf16)));
+    // the padded compact input holds nvox + 2 ny nz <= 3 nvox voxels
+    TRY(buf_reserve(ctx, net->in2, 3 * nvox * 2 * sizeof(__bf16)));
     if (train) TRY(buf_reserve(ctx, net->a0, nvox * 16 * sizeof(__bf16)));
     TRY(buf_reserve(ctx, net->a1, nvox * 32 * sizeof(__bf16)));
     TRY(buf_reserve(ctx, net->a2, nvox * 32 * sizeof(__bf16)));
@@ -2186,7 +2187,7 @@ ctr_status cnn_forward_dev(ctr_cnn *net, const CnnShape &S, const float *x, cons
     // the first layer reads the compact input; the [V][16] layout feeds the
     // first layer's weight gradient when the backward pass follows
     __bf16 *in2 = (__bf16 *)net->in2.ptr;
-    TRY(check_launch(ctx, launch_pack_input2(x, g, g_scale, n, in2, ctx->stream), "CNN input"));
+    TRY(check_launch(ctx, launch_pack_input2(x, g, g_scale, S, in2, ctx->stream), "CNN input"));
     if (keep) TRY(check_launch(ctx, launch_pack_input(x, g, g_scale, n, a0, ctx->stream), "CNN input"));
     TRY(check_launch(ctx, launch_conv3d_first(in2, (const __bf16 *)net->wf[0].ptr, layer_bias(net, 0), a1, S,
                                               ctx->stream), "conv 1"));

@@ -345,47 +345,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// First layer (2 -> 32 channels) from the compact input [V][2] (x_k and
-// the scaled SIRT step, 4 bytes per voxel).  K of v_mfma_f32_16x16x32_bf16
-// runs over (tap, channel) pairs instead of channels: 27 taps x 2 channels
-// in two steps of 32 (taps 27..31 are zero padding), so a row of 16 voxels
-// takes 4 MFMAs where the channel-K layout of k_conv3d takes 27 with 14 of
-// every 16 channels zero.  Lane l (column c = l mod 16, group g = l / 16)
-// supplies for step s the taps 16 s + 4 g + jj (jj < 4): one 4-byte load
-// each (both channels of one voxel; the 16 lanes of a group read 64
-// contiguous bytes), bounds-checked per access (outside the volume, and for
-// padding taps, the offset lies past the descriptor's range and the load
-// returns zeros).  On the [V][16] layout the same loads touched eight times
-// the cache lines and took 1.18 ms at 256^3.
-constexpr int kC2Rows = 4;   // output rows per wave task (4: 0.50 ms at 256^3, 8: 0.55, 16: 0.67)
+// First layer (2 -> 32 channels) from the compact padded input (x_k and the
+// scaled SIRT step, 4 bytes per voxel, see k_pack_input2).  K of
+// v_mfma_f32_16x16x32_bf16 runs over (tap, channel) pairs instead of
+// channels: 27 taps x 2 channels in two steps of 32, so a row of 16 voxels
+// takes 4 MFMAs where a channel-K layout takes 27 with 14 of every 16
+// channels zero.  Lane l (column c = l mod 16, group g = l / 16) holds in
+// step s the three taps dx = -1, 0, +1 of tap row (dz, dy) = 4 s + g (one
+// 12-byte load of the padded row: voxels x - 1 .. x + 1, both channels)
+// and, for s = 0 and g < 3, the tap (dz, dy, dx) = (+1, +1, g - 1) of the
+// ninth tap row (one more 4-byte load); the remaining slots have zero
+// weights.  Three loads per row of 16 outputs instead of eight.  Rows and
+// slices outside the volume: the offset lies past the descriptor's range
+// and the load returns zeros.
+constexpr int kC2Rows = 4;   // output rows per wave task
 __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
                                                    const float *__restrict__ bias, __bf16 *__restrict__ out, CnnShape S)
 {
-    constexpr int CIN = 2, R = kC2Rows;
+    constexpr int R = kC2Rows;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
-    // A[s][h]: W[co][ch][tap] for K index 8 kg + 2 jj + ch, A row col = output
-    // channel 8 (col / 4) + 4 h + col mod 4 (a lane's accumulators are then
-    // 8 consecutive channels)
+    const int px = S.nx + 2;   // padded row length (voxels)
+    // A[s][h]: K index 8 kg + 2 jj + ch; A row col = output channel
+    // 8 (col / 4) + 4 h + col mod 4 (a lane's accumulators are then 8
+    // consecutive channels)
     bf16x8 A[2][2];
-    int tdx[2][4], tdy[2][4], tdz[2][4];
-    bool tok[2][4];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s) {
+        const int r = 4 * s + kg, dz = r / 3, dy = r % 3;   // tap row of the triplet
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const int t = 16 * s + 4 * kg + jj;
-            tok[s][jj] = t < kTaps;
-            tdz[s][jj] = t / 9 - 1;
-            tdy[s][jj] = (t / 3) % 3 - 1;
-            tdx[s][jj] = t % 3 - 1;
+        for (int h = 0; h < 2; ++h) {
+            const int co = 8 * (col >> 2) + 4 * h + (col & 3);
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int jj = 0; jj < 4; ++jj) {
+                int t = -1;
+                if (jj < 3) t = 9 * dz + 3 * dy + jj;
+                else if (s == 0 && kg < 3) t = 24 + kg;   // tap row 8: (dz, dy) = (2, 2), dx = kg
 #pragma unroll
                 for (int ch = 0; ch < 2; ++ch)
-                    A[s][h][2 * jj + ch] =
-                        t < kTaps ? wts[(t * 32 + 8 * (col >> 2) + 4 * h + (col & 3)) * 16 + ch] : (__bf16)0.0f;
+                    A[s][h][2 * jj + ch] = t >= 0 ? wts[(t * 32 + co) * 16 + ch] : (__bf16)0.0f;
+            }
         }
+    }
 
     const int tiles_x = (S.nx + 15) / 16, groups_y = (S.ny + R - 1) / R;
     const long ntasks = (long)tiles_x * groups_y * S.nz;
@@ -393,24 +394,14 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
         const int tx = (int)(task % tiles_x), gy = (int)((task / tiles_x) % groups_y), z = (int)(task / ((long)tiles_x * groups_y));
         const int x = tx * 16 + col, y0 = gy * R;
         const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
-        const size_t plane = (size_t)S.ny * S.nx * CIN;
+        const size_t plane = (size_t)S.ny * px * 2;   // bf16 elements per padded slice
         const unsigned long long base = (unsigned long long)(in + zlo * plane);
         const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
         const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(((unsigned long long)bhi << 32) | blo), 0,
             __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
-        // per tap: x and z validity and the row-independent part of the offset
-        bool xzok[2][4];
-        int xzoff[2][4];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const int xi = x + tdx[s][jj], zi = z + tdz[s][jj];
-                xzok[s][jj] = tok[s][jj] && xi >= 0 && xi < S.nx && zi >= 0 && zi < S.nz;
-                xzoff[s][jj] = (zi - zlo) * S.ny * S.nx + xi;   // voxels, before the row term
-            }
+        const bool xok = x < S.nx;   // padded voxels x - 1 .. x + 1 exist
         f32x4 acc[R][2];
 #pragma unroll
         for (int g = 0; g < R; ++g) {
@@ -418,15 +409,24 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
             bf16x8 B[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                typedef int i32x4 __attribute__((ext_vector_type(4)));
-                i32x4 b;
-#pragma unroll
-                for (int jj = 0; jj < 4; ++jj) {
-                    const int yi = y + tdy[s][jj];
-                    const bool ok = xzok[s][jj] && yi >= 0 && yi < S.ny;
-                    const unsigned off = ok ? (unsigned)(xzoff[s][jj] + yi * S.nx) * (unsigned)(CIN * sizeof(__bf16))
-                                            : 0x80000000u;
-                    b[jj] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+                typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
+                const int r = 4 * s + kg, zi = z + r / 3 - 1, yi = y + r % 3 - 1;
+                const bool ok = xok && zi >= 0 && zi < S.nz && yi >= 0 && yi < S.ny;
+                // padded voxel x - 1 + 1 = x: the first of the three neighbours
+                const unsigned off = ok ? (unsigned)(((zi - zlo) * S.ny + yi) * px + x) * 4u : 0x80000000u;
+                const u32x3 t3 = __builtin_amdgcn_raw_buffer_load_b96(rsrc, off, 0, 0);
+                u32x4 b;
+                b[0] = t3[0];
+                b[1] = t3[1];
+                b[2] = t3[2];
+                b[3] = 0u;
+                if (s == 0) {
+                    // tap (+1, +1, kg - 1) for kg < 3: padded voxel x + kg of row y + 1, slice z + 1
+                    const int ze = z + 1, ye = y + 1;
+                    const bool oke = xok && kg < 3 && ze < S.nz && ye < S.ny;
+                    const unsigned offe = oke ? (unsigned)(((ze - zlo) * S.ny + ye) * px + x + kg) * 4u : 0x80000000u;
+                    b[3] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, offe, 0, 0);
                 }
                 B[s] = __builtin_bit_cast(bf16x8, b);
             }
@@ -768,17 +768,26 @@ __global__ __launch_bounds__(256) void k_pack_input(const float *__restrict__ x,
     reinterpret_cast<bf16x8 *>(out)[2 * i + 1] = b;
 }
 
-// Compact network input for k_conv3d_c2: [V][2] = (x, g * g_scale).
+// Compact network input for k_conv3d_c2: (x, g * g_scale) per voxel, rows
+// padded with one zero voxel on each side ([z][y][nx + 2][2]), so that the
+// three x-neighbours of any voxel are one 12-byte load.
 __global__ __launch_bounds__(256) void k_pack_input2(const float *__restrict__ x, const float *__restrict__ g,
-                                                     float g_scale, size_t n, __bf16 *__restrict__ out)
+                                                     float g_scale, CnnShape S, __bf16 *__restrict__ out)
 {
     typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const size_t n = (size_t)S.nx * S.ny * S.nz;
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const size_t row = i / S.nx;
+    const int xi = (int)(i - row * S.nx);
     bf16x2 a;
     a[0] = (__bf16)x[i];
     a[1] = (__bf16)(g[i] * g_scale);
-    reinterpret_cast<bf16x2 *>(out)[i] = a;
+    bf16x2 *o = reinterpret_cast<bf16x2 *>(out) + row * (S.nx + 2);
+    o[xi + 1] = a;
+    const bf16x2 zero = {(__bf16)0.0f, (__bf16)0.0f};
+    if (xi == 0) o[0] = zero;
+    if (xi == S.nx - 1) o[S.nx + 1] = zero;
 }
 
 // Masked mean-squared error of pred against target over the interior of
@@ -932,9 +941,11 @@ hipError_t launch_pack_input(const float *x, const float *g, float g_scale, size
     return hipGetLastError();
 }
 
-hipError_t launch_pack_input2(const float *x, const float *g, float g_scale, size_t n, __bf16 *out, hipStream_t st)
+hipError_t launch_pack_input2(const float *x, const float *g, float g_scale, const CnnShape &S, __bf16 *out,
+                              hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pack_input2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, g, g_scale, n, out);
+    const size_t n = (size_t)S.nx * S.ny * S.nz;
+    hipLaunchKernelGGL(k_pack_input2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, g, g_scale, S, out);
     return hipGetLastError();
 }
 
