@@ -56,8 +56,10 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(37, 19, 11), (64, 16, 4), (8, 40, 5)])
+@pytest.mark.parametrize("shape", [(37, 19, 11), (64, 16, 4), (8, 40, 5), (1, 7, 6), (17, 33, 34)])
 def test_cnn_forward_matches_torch(ctx, shape):
+    """Partial tiles in x and y, a one-voxel-wide volume (padded input rows
+    of three voxels), and a z extent past one 32-slice run."""
     rng = np.random.default_rng(1)
     nx, ny, nz = shape
     p = ctr.cnn_init_params(3, 1.0)
