@@ -161,43 +161,60 @@ __global__ __launch_bounds__(256) void k_conv3d(const __bf16 *__restrict__ in, c
 // instruction of a wave reads 16 whole voxels (1 KB contiguous) for both
 // channel halves.  Each input row is loaded once at the centre column
 // x0 + l mod 16; the dx = -1 and dx = +1 operands are that row shifted
-// across the lanes of each channel group, with the two halo voxels (x0 - 1
-// and x0 + 16) from one more load; each row feeds the 3 dy taps and both
-// output halves.  Against three loads per (dz, row), one for each dx:
+// across the lanes of each channel group, with the halo voxels (x0 - 1 and
+// x0 + 16) of all rows from one more load; each row feeds the 3 dy taps and
+// both output halves.  Against three loads per (dz, row), one for each dx:
 // 1.13 -> 0.97 ms per layer at 256^3 (vector-memory unit busy 0.79 -> 0.57),
 // and against the 32 x 32 x 16 layout above (half voxels, two channel
 // steps): 1.37 ms.  The A operands come from the global layout
 // [tap][q][32 co][16 ci], staged in LDS as [tap][half][lane] so a wave's
 // read is one conflict-free 1 KB run.
 
-// x-neighbour operand from a centre row: a 16-lane DPP row holds the 16
+// x-neighbour operands from a centre row: a 16-lane DPP row holds the 16
 // voxels x0 .. x0 + 15 of one channel group, so a row shift by one lane
-// gives voxel x0 + col - 1 (CTRL = row_shr:1) or x0 + col + 1 (row_shl:1);
-// the lane whose source lies outside the row keeps `edge`, the halo voxel
-// loaded for it.
-template <int CTRL>
-__device__ __forceinline__ bf16x8 shift_cols(bf16x8 edge, bf16x8 centre)
+// gives voxel x0 + col - 1 (row_shr:1) or x0 + col + 1 (row_shl:1).  The lane
+// whose source lies outside the row (col 0, col 15) takes the halo voxel
+// instead.  The halo voxels of all R + 2 rows of a slice arrive in one
+// register, per channel group: lane 1 + r holds x0 - 1 of row r, lane
+// 15 - (R + 2) + r holds x0 + 16 of row r.  A first DPP move brings row r's
+// halo voxel to col 0 (row_shl:1 + r) or col 15 (row_shr:R + 2 - r); the row
+// shift then leaves that lane as it is (its source is out of the row).
+template <int SHIFT, int PICK>
+__device__ __forceinline__ bf16x8 shift_cols(bf16x8 halo, bf16x8 centre)
 {
     typedef int i32x4 __attribute__((ext_vector_type(4)));
-    const i32x4 e = __builtin_bit_cast(i32x4, edge), c = __builtin_bit_cast(i32x4, centre);
+    const i32x4 e = __builtin_bit_cast(i32x4, halo), c = __builtin_bit_cast(i32x4, centre);
     i32x4 r;
-    r[0] = __builtin_amdgcn_update_dpp(e[0], c[0], CTRL, 0xf, 0xf, false);
-    r[1] = __builtin_amdgcn_update_dpp(e[1], c[1], CTRL, 0xf, 0xf, false);
-    r[2] = __builtin_amdgcn_update_dpp(e[2], c[2], CTRL, 0xf, 0xf, false);
-    r[3] = __builtin_amdgcn_update_dpp(e[3], c[3], CTRL, 0xf, 0xf, false);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        r[i] = __builtin_amdgcn_update_dpp(__builtin_amdgcn_update_dpp(0, e[i], PICK, 0xf, 0xf, true), c[i], SHIFT,
+                                           0xf, 0xf, false);
     return __builtin_bit_cast(bf16x8, r);
 }
-constexpr int kDppRowShr1 = 0x111, kDppRowShl1 = 0x101;
+constexpr int kDppRowShl = 0x100, kDppRowShr = 0x110;   // + shift, 1 .. 15
+// rows ROW .. NR - 1: left[] = the dx = -1 operands; centre[] becomes the dx = +1 operands
+template <int ROW, int NR>
+__device__ __forceinline__ void shift_rows(bf16x8 halo, bf16x8 *centre, bf16x8 *left)
+{
+    static_assert(2 * NR < 16, "halo lanes of the left and right edges overlap");
+    if constexpr (ROW < NR) {
+        left[ROW] = shift_cols<kDppRowShr + 1, kDppRowShl + 1 + ROW>(halo, centre[ROW]);
+        centre[ROW] = shift_cols<kDppRowShl + 1, kDppRowShr + NR - ROW>(halo, centre[ROW]);
+        shift_rows<ROW + 1, NR>(halo, centre, left);
+    }
+}
 
 // The 32-channel convolution walks input slices: each input slice zi is
-// loaded once (centre rows prefetched while the previous slice multiplies,
-// halo rows while its own centre-only taps run) and feeds the three output
-// slices zi + 1, zi, zi - 1 through the tap planes dz = -1, 0, +1: three
-// rolling accumulator sets per row, the lowest complete (and stored) once
-// slice zi is done.  A wave owns 16 x 4 outputs; a block runs 32 output
-// slices of its 16 x 16 tile.  Against loading the three input slices of
-// every output slice (8 rows per wave): vector-memory unit busy 0.52 ->
-// 0.30, 0.96 -> 0.93 ms per layer at 256^3.
+// loaded once (centre and halo rows prefetched while the previous slice
+// multiplies) and feeds the three output slices zi + 1, zi, zi - 1 through
+// the tap planes dz = -1, 0, +1: three rolling accumulator sets per row, the
+// lowest complete (and stored) once slice zi is done.  A wave owns 16 x 4
+// outputs; a block runs 32 output slices of its 16 x 16 tile.  Against
+// loading the three input slices of every output slice (8 rows per wave):
+// vector-memory unit busy 0.52 -> 0.30, 0.96 -> 0.93 ms per layer at 256^3.
+// Forming the shifted rows once per slice instead of once per tap plane,
+// and the halo of all rows in one load and one register instead of one per
+// row: 0.94 -> 0.86 ms on the same GPU.
 constexpr int kC32zRows = 4, kC32zRun = 32;   // output rows per wave, output slices per block task
 template <int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv3d_c32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
@@ -220,31 +237,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + TY - 1) / TY;
     const int zchunks = (S.nz + kC32zRun - 1) / kC32zRun;
     const int ncols = tiles_x * tiles_y * zchunks;
-    const bool edge = col == 0 || col == 15;
     const size_t plane = (size_t)S.ny * S.nx * CIN;
     for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x) {
         const int tx = tcol % tiles_x, ty = (tcol / tiles_x) % tiles_y;
         const int z0 = (tcol / (tiles_x * tiles_y)) * kC32zRun, z1 = min(S.nz, z0 + kC32zRun);
         const int x0 = tx * 16, x = x0 + col, y0 = ty * TY + wave * R;
-        const int hx = col == 0 ? x0 - 1 : x0 + 16;   // halo voxel of an edge lane
-        // rows y0 - 1 .. y0 + R of input slice zi at voxel xx (outside the volume: zero)
-        auto load_slice = [&](int zi, int xx, bool lane_ok, bf16x8 *rows) {
+        // buffer descriptor over input slice zi (outside the volume: an empty
+        // range, every load reads zero)
+        auto slice_rsrc = [&](int zi) {
             const bool zok = zi >= 0 && zi < S.nz;
             const unsigned long long base = (unsigned long long)(in + (zok ? zi : 0) * plane);
             const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
             const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
-            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(((unsigned long long)bhi << 32) | blo), 0,
-                __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(__bf16)) : 0), 0x00020000);
-            const bool xok = lane_ok && xx >= 0 && xx < S.nx;
+            return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)bhi << 32) | blo), 0,
+                                                     __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(__bf16)) : 0),
+                                                     0x00020000);
+        };
+        // centre rows y0 - 1 .. y0 + R of input slice zi at voxel x (outside
+        // the volume: zero)
+        auto load_slice = [&](int zi, bf16x8 *rows) {
+            const __amdgpu_buffer_rsrc_t rsrc = slice_rsrc(zi);
+            // lane part of the offset, recomputed per slice rather than kept
+            // in registers across the loop (the empty asm stops the hoist); a
+            // lane outside the volume starts at 2^31, past the descriptor's range
+            unsigned xoff = x < S.nx ? (unsigned)((x * CIN + kg * 8) * sizeof(__bf16)) : 0x80000000u;
+            asm volatile("" : "+v"(xoff));
 #pragma unroll
             for (int rr = 0; rr < R + 2; ++rr) {
-                const int yy = y0 - 1 + rr;
-                const bool ok = xok && yy >= 0 && yy < S.ny;
-                const unsigned off =
-                    ok ? (unsigned)((((size_t)yy * S.nx + xx) * CIN + kg * 8) * sizeof(__bf16)) : 0x80000000u;
+                const int yy = y0 - 1 + rr;   // uniform
+                const unsigned off = yy >= 0 && yy < S.ny
+                                         ? xoff + (unsigned)((size_t)yy * S.nx * CIN * sizeof(__bf16))
+                                         : 0x80000000u;
                 rows[rr] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
             }
+        };
+        // the 2 (R + 2) halo voxels of slice zi in one register (see shift_cols)
+        auto load_halo = [&](int zi) {
+            const int rr = col <= R + 2 ? col - 1 : col - (15 - (R + 2));
+            const int xx = col <= R + 2 ? x0 - 1 : x0 + 16, yy = y0 - 1 + rr;
+            const bool ok = (col >= 1 && col <= R + 2) || (col >= 15 - (R + 2) && col <= 14);
+            const unsigned off = ok && xx >= 0 && xx < S.nx && yy >= 0 && yy < S.ny
+                                     ? (unsigned)((((size_t)yy * S.nx + xx) * CIN + kg * 8) * sizeof(__bf16))
+                                     : 0x80000000u;
+            return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(slice_rsrc(zi), off, 0, 0));
         };
         f32x4 acc[3][R][2];   // [0]: output zi - 1, [1]: zi, [2]: zi + 1
 #pragma unroll
@@ -253,8 +288,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int g = 0; g < R; ++g)
 #pragma unroll
                 for (int h = 0; h < 2; ++h) acc[q][g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        // taps (dzi, dxi, dy) of every row into the output slice acc[2 - dzi]
-        auto mma_dx = [&](int dzi, int dxi, const bf16x8 *centre, const bf16x8 *halo) {
+        // taps (dzi, dxi, dy) of every row into the output slice acc[2 - dzi];
+        // `rows` holds the operands of column offset dxi - 1.  With `init`
+        // (the first step into a fresh output slice) the first product of
+        // each accumulator starts from zero.
+        auto mma_dx = [&](int dzi, int dxi, const bf16x8 *rows, bool init) {
             bf16x8 a[3][2];
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy)
@@ -262,47 +300,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int h = 0; h < 2; ++h) a[dy][h] = wl[((dzi * 9 + dy * 3 + dxi) * 2 + h) * 64 + lane];
 #pragma unroll
             for (int r = 0; r < R + 2; ++r) {
-                const bf16x8 row = dxi == 1   ? centre[r]
-                                   : dxi == 0 ? shift_cols<kDppRowShr1>(halo[r], centre[r])
-                                              : shift_cols<kDppRowShl1>(halo[r], centre[r]);
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy) {
                     const int g = r - dy;
                     if (g < 0 || g >= R) continue;
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
-                        acc[2 - dzi][g][h] =
-                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dy][h], row, acc[2 - dzi][g][h], 0, 0, 0);
+                        acc[2 - dzi][g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            a[dy][h], rows[r], init && dy == 0 ? f32x4{0.0f, 0.0f, 0.0f, 0.0f} : acc[2 - dzi][g][h],
+                            0, 0, 0);
                 }
             }
         };
-        // centre rows are prefetched one slice ahead; the halo rows of slice
-        // zi go out when its centre-only taps (dx = 0) start
-        bf16x8 cur[R + 2], nxt[R + 2], chal[R + 2];
-        load_slice(z0 - 1, x, true, cur);
+        // The centre and halo rows of slice zi + 1 load while slice zi
+        // multiplies.  The shifted rows (dx = -1, +1) are formed once per
+        // slice and feed all three tap planes; the dx = +1 rows overwrite the
+        // centre rows, which the slice no longer needs.
+        bf16x8 cur[R + 2], nxt[R + 2], sr[R + 2];
+        load_slice(z0 - 1, cur);
+        bf16x8 hal = load_halo(z0 - 1), nhal;
 #pragma unroll 1
         for (int zi = z0 - 1; zi <= z1; ++zi) {
-            if (zi + 1 <= z1) load_slice(zi + 1, x, true, nxt);
+            const bool more = zi + 1 <= z1;
+            if (more) {
+                load_slice(zi + 1, nxt);
+                nhal = load_halo(zi + 1);
+            }
             if (zi >= 0 && zi < S.nz) {   // uniform: a slice outside the volume contributes zero
-                load_slice(zi, hx, edge, chal);
                 // the empty asm statements keep each step's weight reads (and
-                // their registers) from being hoisted over the others
-                // plane dzi feeds output slice zi + 1 - dzi: skipped (uniform)
-                // where that slice lies outside this run
+                // their registers) from being hoisted over the others.  Plane
+                // dzi feeds output slice zi + 1 - dzi: skipped (uniform) where
+                // that slice lies outside this run, whose accumulators are
+                // then never stored.
 #pragma unroll
                 for (int dzi = 0; dzi < 3; ++dzi) {
                     if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
-                    mma_dx(dzi, 1, cur, chal);
+                    mma_dx(dzi, 1, cur, dzi == 0);
+                    asm volatile("" ::: "memory");
+                }
+                shift_rows<0, R + 2>(hal, cur, sr);
+#pragma unroll
+                for (int dzi = 0; dzi < 3; ++dzi) {
+                    if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
+                    mma_dx(dzi, 0, sr, false);
                     asm volatile("" ::: "memory");
                 }
 #pragma unroll
                 for (int dzi = 0; dzi < 3; ++dzi) {
                     if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
-                    mma_dx(dzi, 0, cur, chal);
-                    asm volatile("" ::: "memory");
-                    mma_dx(dzi, 2, cur, chal);
+                    mma_dx(dzi, 2, cur, false);
                     asm volatile("" ::: "memory");
                 }
+            } else {
+#pragma unroll
+                for (int g = 0; g < R; ++g)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) acc[2][g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             }
             // output slice zi - 1 has all three planes now
             const int zo = zi - 1;
@@ -335,11 +388,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int h = 0; h < 2; ++h) {
                     acc[0][g][h] = acc[1][g][h];
                     acc[1][g][h] = acc[2][g][h];
-                    acc[2][g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                 }
-            if (zi + 1 <= z1) {
+            if (more) {
 #pragma unroll
                 for (int r = 0; r < R + 2; ++r) cur[r] = nxt[r];
+                hal = nhal;
             }
         }
     }
