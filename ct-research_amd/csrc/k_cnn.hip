@@ -404,18 +404,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // channels: 27 taps x 2 channels in two steps of 32, so a row of 16 voxels
 // takes 4 MFMAs where a channel-K layout takes 27 with 14 of every 16
 // channels zero.  Lane l (column c = l mod 16, group g = l / 16) holds in
-// step s the three taps dx = -1, 0, +1 of tap row (dz, dy) = 4 s + g (one
-// 12-byte load of the padded row: voxels x - 1 .. x + 1, both channels)
-// and, for s = 0 and g < 3, the tap (dz, dy, dx) = (+1, +1, g - 1) of the
-// ninth tap row (one more 4-byte load); the remaining slots have zero
-// weights.  Three loads per row of 16 outputs instead of eight.  Rows and
-// slices outside the volume: the offset lies past the descriptor's range
-// and the load returns zeros.
-constexpr int kC2Rows = 4;   // output rows per wave task
+// step s the three taps dx = -1, 0, +1 of tap row (dz, dy) = 4 s + g (three
+// consecutive voxels of the padded row, both channels) and, for s = 0 and
+// g < 3, the tap (dz, dy, dx) = (+1, +1, g - 1) of the ninth tap row; the
+// remaining slots have zero weights.
+// A block owns a 16 x 16 tile and walks kC2ZRun output slices: each input
+// slice's 18 x 18 window (with the one-voxel halo) is loaded from global
+// memory once, into a ring of four LDS slots, and the operands are read from
+// LDS.  The slice two ahead loads while the current output slice is formed.
+// Against loading the operands of every output row straight from global
+// memory (each input voxel read 27 times through the L1; texture-unit
+// bound): 0.45 -> 0.26 ms at 256^3, now bound by its 1 GB of output writes.
+constexpr int kC2Rows = 4;                               // output rows per wave
+constexpr int kC2TY = 4 * kC2Rows;                       // tile rows per block
+constexpr int kC2ZRun = 32;                              // output slices per block task
+constexpr int kC2HX = 18, kC2HY = kC2TY + 2;             // staged window of one input slice
+constexpr int kC2Win = kC2HX * kC2HY;                    // 324 voxels (one dword each)
+constexpr int kC2Ld = (kC2Win + 255) / 256;              // window dwords per thread
 __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
                                                    const float *__restrict__ bias, __bf16 *__restrict__ out, CnnShape S)
 {
     constexpr int R = kC2Rows;
+    __shared__ unsigned ring[4][kC2Win];   // slice zi in slot (zi + 1) mod 4
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
     const int px = S.nx + 2;   // padded row length (voxels)
@@ -440,69 +450,103 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
             }
         }
     }
+    float bv[8];   // bias of this lane's 8 output channels
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bv[e] = bias[8 * kg + e];
 
-    const int tiles_x = (S.nx + 15) / 16, groups_y = (S.ny + R - 1) / R;
-    const long ntasks = (long)tiles_x * groups_y * S.nz;
-    for (long task = (long)blockIdx.x * 4 + wave; task < ntasks; task += (long)gridDim.x * 4) {
-        const int tx = (int)(task % tiles_x), gy = (int)((task / tiles_x) % groups_y), z = (int)(task / ((long)tiles_x * groups_y));
-        const int x = tx * 16 + col, y0 = gy * R;
-        const int zlo = max(z - 1, 0), zhi = min(z + 1, S.nz - 1);
-        const size_t plane = (size_t)S.ny * px * 2;   // bf16 elements per padded slice
-        const unsigned long long base = (unsigned long long)(in + zlo * plane);
-        const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
-        const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
-        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(((unsigned long long)bhi << 32) | blo), 0,
-            __builtin_amdgcn_readfirstlane((int)((zhi - zlo + 1) * plane * sizeof(__bf16))), 0x00020000);
-        const bool xok = x < S.nx;   // padded voxels x - 1 .. x + 1 exist
-        f32x4 acc[R][2];
+    const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + kC2TY - 1) / kC2TY;
+    const int zruns = (S.nz + kC2ZRun - 1) / kC2ZRun;
+    const int ntasks = tiles_x * tiles_y * zruns;
+    const size_t plane = (size_t)S.ny * px * 2;   // bf16 elements per padded slice
+    for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
+        const int x0 = (task % tiles_x) * 16, y0 = ((task / tiles_x) % tiles_y) * kC2TY;
+        const int z0 = (task / (tiles_x * tiles_y)) * kC2ZRun, z1 = min(S.nz, z0 + kC2ZRun);
+        const int x = x0 + col;
+        // window of input slice zi (padded columns x0 .. x0 + 17, rows
+        // y0 - 1 .. y0 + 16): dword i of the window goes to thread i mod 256;
+        // outside the volume it reads as zero (offset past the descriptor)
+        auto fetch = [&](int zi, unsigned (&v)[kC2Ld]) {
+            const bool zok = zi >= 0 && zi < S.nz;
+            const unsigned long long base = (unsigned long long)(in + (zok ? zi : 0) * plane);
+            const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+            const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(((unsigned long long)bhi << 32) | blo), 0,
+                __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(__bf16)) : 0), 0x00020000);
 #pragma unroll
-        for (int g = 0; g < R; ++g) {
-            const int y = y0 + g;
-            bf16x8 B[2];
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-                typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
-                const int r = 4 * s + kg, zi = z + r / 3 - 1, yi = y + r % 3 - 1;
-                const bool ok = xok && zi >= 0 && zi < S.nz && yi >= 0 && yi < S.ny;
-                // padded voxel x - 1 + 1 = x: the first of the three neighbours
-                const unsigned off = ok ? (unsigned)(((zi - zlo) * S.ny + yi) * px + x) * 4u : 0x80000000u;
-                const u32x3 t3 = __builtin_amdgcn_raw_buffer_load_b96(rsrc, off, 0, 0);
-                u32x4 b;
-                b[0] = t3[0];
-                b[1] = t3[1];
-                b[2] = t3[2];
-                b[3] = 0u;
-                if (s == 0) {
-                    // tap (+1, +1, kg - 1) for kg < 3: padded voxel x + kg of row y + 1, slice z + 1
-                    const int ze = z + 1, ye = y + 1;
-                    const bool oke = xok && kg < 3 && ze < S.nz && ye < S.ny;
-                    const unsigned offe = oke ? (unsigned)(((ze - zlo) * S.ny + ye) * px + x + kg) * 4u : 0x80000000u;
-                    b[3] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, offe, 0, 0);
-                }
-                B[s] = __builtin_bit_cast(bf16x8, b);
+            for (int k = 0; k < kC2Ld; ++k) {
+                const int i = (int)threadIdx.x + 256 * k, hy = i / kC2HX, hx = i - hy * kC2HX;
+                const int xx = x0 + hx, yy = y0 - 1 + hy;
+                const bool ok = i < kC2Win && xx < px && yy >= 0 && yy < S.ny;
+                const unsigned off = ok ? (unsigned)((yy * px + xx) * 4) : 0x80000000u;
+                v[k] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
             }
+        };
+        auto stash = [&](int zi, const unsigned (&v)[kC2Ld]) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][h], B[0], f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-                acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][h], B[1], acc[g][h], 0, 0, 0);
+            for (int k = 0; k < kC2Ld; ++k) {
+                const int i = (int)threadIdx.x + 256 * k;
+                if (i < kC2Win) ring[(zi + 1) & 3][i] = v[k];
             }
+        };
+        unsigned v[kC2Ld];
+        __syncthreads();   // the previous task's reads of the ring are done
+#pragma unroll 1
+        for (int zi = z0 - 1; zi <= z0 + 1; ++zi) {
+            fetch(zi, v);
+            stash(zi, v);
         }
-        // lane holds D[co][voxel x] for co = 8 kg + 4 h + u (u = 0..3): one
-        // 16-byte store per row
-        if (x >= S.nx) continue;
+        __syncthreads();
+#pragma unroll 1
+        for (int zo = z0; zo < z1; ++zo) {
+            // output slice zo + 1 needs input slice zo + 2
+            const bool more = zo + 2 <= z1;
+            if (more) fetch(zo + 2, v);
+            f32x4 acc[R][2];
 #pragma unroll
-        for (int g = 0; g < R; ++g) {
-            const int y = y0 + g;
-            if (y >= S.ny) continue;
-            const size_t vox = ((size_t)z * S.ny + y) * S.nx + x;
-            bf16x8 o;
+            for (int g = 0; g < R; ++g) {
+                const int wy = wave * R + g;   // window row of output row y0 + wy, minus one
+                bf16x8 B[2];
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+                for (int s = 0; s < 2; ++s) {
+                    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                    const int r = 4 * s + kg, dz = r / 3, dy = r % 3;
+                    const unsigned *w = &ring[(zo + dz) & 3][(wy + dy) * kC2HX + col];
+                    u32x4 b;
+                    b[0] = w[0];
+                    b[1] = w[1];
+                    b[2] = w[2];
+                    b[3] = 0u;
+                    // tap (+1, +1, kg - 1) for kg < 3: slice zo + 1, row y + 1, padded column x + kg
+                    if (s == 0 && kg < 3) b[3] = ring[(zo + 2) & 3][(wy + 2) * kC2HX + col + kg];
+                    B[s] = __builtin_bit_cast(bf16x8, b);
+                }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) o[4 * h + u] = (__bf16)fmaxf(acc[g][h][u] + bias[8 * kg + 4 * h + u], 0.0f);
-            *reinterpret_cast<bf16x8 *>(out + vox * 32 + 8 * kg) = o;
+                for (int h = 0; h < 2; ++h) {
+                    acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][h], B[0], f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                    acc[g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][h], B[1], acc[g][h], 0, 0, 0);
+                }
+            }
+            // lane holds D[co][voxel x] for co = 8 kg + 4 h + u (u = 0..3): one
+            // 16-byte store per row
+            if (x < S.nx) {
+#pragma unroll
+                for (int g = 0; g < R; ++g) {
+                    const int y = y0 + wave * R + g;
+                    if (y >= S.ny) continue;
+                    const size_t vox = ((size_t)zo * S.ny + y) * S.nx + x;
+                    bf16x8 o;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) o[4 * h + u] = (__bf16)fmaxf(acc[g][h][u] + bv[4 * h + u], 0.0f);
+                    *reinterpret_cast<bf16x8 *>(out + vox * 32 + 8 * kg) = o;
+                }
+            }
+            // slot of slice zo + 2 last held slice zo - 2, read by output slice
+            // zo - 1 before the previous barrier
+            if (more) stash(zo + 2, v);
+            __syncthreads();
         }
     }
 }
@@ -1005,8 +1049,8 @@ hipError_t launch_pack_input2(const float *x, const float *g, float g_scale, con
 hipError_t launch_conv3d_first(const __bf16 *in2, const __bf16 *w, const float *bias, __bf16 *out, const CnnShape &S,
                                hipStream_t st)
 {
-    const long tasks = (long)((S.nx + 15) / 16) * ((S.ny + kC2Rows - 1) / kC2Rows) * S.nz;
-    const dim3 grid((unsigned)std::min((tasks + 3) / 4, 8192L));
+    const long tasks = (long)((S.nx + 15) / 16) * ((S.ny + kC2TY - 1) / kC2TY) * ((S.nz + kC2ZRun - 1) / kC2ZRun);
+    const dim3 grid((unsigned)std::min(tasks, 8192L));
     hipLaunchKernelGGL(k_conv3d_c2, grid, dim3(256), 0, st, in2, w, bias, out, S);
     return hipGetLastError();
 }
