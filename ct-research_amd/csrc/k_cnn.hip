@@ -562,9 +562,12 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
 // slice contributes to the outputs one below (taps dz = +1), level (dz = 0)
 // and one above (dz = -1): three rolling sums per output, the lowest one
 // complete once the slice is done.  The block walks kConvZRun output slices.
-constexpr int kO1TX = 32, kO1TY = 16, kO1HX = kO1TX + 2, kO1HY = kO1TY + 2;
-constexpr int kO1Halo = kO1HX * kO1HY;                  // 612 voxels
-constexpr int kO1Tiles = (kO1Halo + 31) / 32;           // 20 MFMA column tiles
+// A 16 x 16 tile (38 KB of LDS, four blocks per CU) against 32 x 16 (70 KB,
+// two blocks): 0.28 -> 0.27 ms at 256^3, despite the larger halo share.
+constexpr int kO1TX = 16, kO1TY = 16, kO1HX = kO1TX + 2, kO1HY = kO1TY + 2;
+constexpr int kO1K = kO1TX * kO1TY / 256, kO1Step = 256 / kO1TX;   // outputs per thread, their row step
+constexpr int kO1Halo = kO1HX * kO1HY;                  // 324 voxels
+constexpr int kO1Tiles = (kO1Halo + 31) / 32;           // 11 MFMA column tiles
 constexpr int kO1Stride = kO1Tiles * 32 + 4;            // LDS floats per tap (+4: the two lane halves' taps 4 apart
                                                         // fall in different banks)
 
@@ -591,11 +594,13 @@ __global__ __launch_bounds__(256) void k_conv3d_out1(const __bf16 *__restrict__ 
     const int tiles_x = (S.nx + kO1TX - 1) / kO1TX, tiles_y = (S.ny + kO1TY - 1) / kO1TY;
     const int zruns = (S.nz + kConvZRun - 1) / kConvZRun;
     const int ncols = tiles_x * tiles_y * zruns;
-    const int ox = threadIdx.x & 31, oy0 = threadIdx.x >> 5;   // outputs (ox, oy0) and (ox, oy0 + 8)
+    const int ox = threadIdx.x % kO1TX, oy0 = threadIdx.x / kO1TX;   // outputs (ox, oy0 + kO1Step k)
     for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x) {
         const int x0 = (tcol % tiles_x) * kO1TX, y0 = ((tcol / tiles_x) % tiles_y) * kO1TY;
         const int zs = (tcol / (tiles_x * tiles_y)) * kConvZRun, ze = min(S.nz, zs + kConvZRun);
-        float acc_lo[2] = {0.0f, 0.0f}, acc_mid[2] = {0.0f, 0.0f}, acc_hi[2] = {0.0f, 0.0f};
+        float acc_lo[kO1K], acc_mid[kO1K], acc_hi[kO1K];
+#pragma unroll
+        for (int k = 0; k < kO1K; ++k) acc_lo[k] = acc_mid[k] = acc_hi[k] = 0.0f;
         // the halo voxels of a slice for this wave's column tiles T = wave +
         // 4 j, loaded one slice ahead: the loads of slice zi + 1 are in
         // flight while slice zi is multiplied and gathered
@@ -648,8 +653,8 @@ __global__ __launch_bounds__(256) void k_conv3d_out1(const __bf16 *__restrict__ 
                 }
                 __syncthreads();
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int oy = oy0 + 8 * k;
+                for (int k = 0; k < kO1K; ++k) {
+                    const int oy = oy0 + kO1Step * k;
                     float below = 0.0f, level = 0.0f, above = 0.0f;
 #pragma unroll
                     for (int dy = 0; dy < 3; ++dy)
@@ -668,13 +673,13 @@ __global__ __launch_bounds__(256) void k_conv3d_out1(const __bf16 *__restrict__ 
             const int zo = zi - 1;   // complete now
             if (zo >= zs && zo < ze) {
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int x = x0 + ox, y = y0 + oy0 + 8 * k;
+                for (int k = 0; k < kO1K; ++k) {
+                    const int x = x0 + ox, y = y0 + oy0 + kO1Step * k;
                     if (x < S.nx && y < S.ny) out[((size_t)zo * S.ny + y) * S.nx + x] = acc_lo[k] + b0;
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < kO1K; ++k) {
                 acc_lo[k] = acc_mid[k];
                 acc_mid[k] = acc_hi[k];
                 acc_hi[k] = 0.0f;
@@ -967,7 +972,7 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
             // one real output channel: the taps-as-rows kernel
             const long cols = (long)((S.nx + kO1TX - 1) / kO1TX) * ((S.ny + kO1TY - 1) / kO1TY) *
                               ((S.nz + kConvZRun - 1) / kConvZRun);
-            hipLaunchKernelGGL(k_conv3d_out1, dim3((unsigned)std::min(cols, 2048L)), dim3(256), 0, st, in, w, bias,
+            hipLaunchKernelGGL(k_conv3d_out1, dim3((unsigned)std::min(cols, 8192L)), dim3(256), 0, st, in, w, bias,
                                (float *)out, S);
         }
         else if (epi == kEpiMask)
