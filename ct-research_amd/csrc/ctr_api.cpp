@@ -2130,7 +2130,6 @@ struct ctr_cnn {
     DevBuf params, m, v, grad;
     DevBuf wf[4], wb[4];                    // packed BF16 forward / input-gradient weights
     DevBuf a0, a1, a2, a3, out, d4, da, db; // activations and deltas (one patch or volume)
-    DevBuf in2;                             // compact first-layer input [V][2]
     DevBuf partial, lsum, xin, gin, tin;
     DevBuf tx, tg, full;                    // tiled pass: a tile's input boxes, the whole output
     int step = 0;
@@ -2161,8 +2160,6 @@ const float *layer_bias(const ctr_cnn *net, int l)
 ctr_status cnn_reserve(ctr_cnn *net, size_t nvox, bool train)
 {
     ctr_ctx *ctx = net->ctx;
-    // the padded compact input holds nvox + 2 ny nz <= 3 nvox voxels
-    TRY(buf_reserve(ctx, net->in2, 3 * nvox * 2 * sizeof(__bf16)));
     if (train) TRY(buf_reserve(ctx, net->a0, nvox * 16 * sizeof(__bf16)));
     TRY(buf_reserve(ctx, net->a1, nvox * 32 * sizeof(__bf16)));
     TRY(buf_reserve(ctx, net->a2, nvox * 32 * sizeof(__bf16)));
@@ -2184,13 +2181,11 @@ ctr_status cnn_forward_dev(ctr_cnn *net, const CnnShape &S, const float *x, cons
     const size_t n = (size_t)S.nx * S.ny * S.nz;
     __bf16 *a0 = (__bf16 *)net->a0.ptr, *a1 = (__bf16 *)net->a1.ptr, *a2 = (__bf16 *)net->a2.ptr;
     __bf16 *a3 = keep ? (__bf16 *)net->a3.ptr : a1;
-    // the first layer reads the compact input; the [V][16] layout feeds the
+    // the first layer reads x and g itself; the [V][16] layout feeds the
     // first layer's weight gradient when the backward pass follows
-    __bf16 *in2 = (__bf16 *)net->in2.ptr;
-    TRY(check_launch(ctx, launch_pack_input2(x, g, g_scale, S, in2, ctx->stream), "CNN input"));
     if (keep) TRY(check_launch(ctx, launch_pack_input(x, g, g_scale, n, a0, ctx->stream), "CNN input"));
-    TRY(check_launch(ctx, launch_conv3d_first(in2, (const __bf16 *)net->wf[0].ptr, layer_bias(net, 0), a1, S,
-                                              ctx->stream), "conv 1"));
+    TRY(check_launch(ctx, launch_conv3d_first(x, g, g_scale, (const __bf16 *)net->wf[0].ptr, layer_bias(net, 0), a1,
+                                              S, ctx->stream), "conv 1"));
     TRY(check_launch(ctx, launch_conv3d(32, kEpiBiasRelu, a1, (const __bf16 *)net->wf[1].ptr, layer_bias(net, 1),
                                         nullptr, a2, S, ctx->stream), "conv 2"));
     TRY(check_launch(ctx, launch_conv3d(32, kEpiBiasRelu, a2, (const __bf16 *)net->wf[2].ptr, layer_bias(net, 2),
@@ -2287,7 +2282,7 @@ void cnn_release(ctr_cnn *net)
     ctr_ctx *ctx = net->ctx;
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf *b : {&net->params, &net->m, &net->v, &net->grad, &net->in2, &net->a0, &net->a1, &net->a2, &net->a3, &net->out,
+    for (DevBuf *b : {&net->params, &net->m, &net->v, &net->grad, &net->a0, &net->a1, &net->a2, &net->a3, &net->out,
                       &net->d4, &net->da, &net->db, &net->partial, &net->lsum, &net->xin, &net->gin, &net->tin,
                       &net->tx, &net->tg, &net->full})
         buf_free(*b);

@@ -36,13 +36,9 @@ hipError_t launch_pack_weights(const float *w, int ci_real, int co_real, int cin
 hipError_t launch_adam(float *p, float *m, float *v, const float *g, int n, float lr_t, float b1, float b2, float eps,
                        hipStream_t st);
 hipError_t launch_pack_input(const float *x, const float *g, float g_scale, size_t n, __bf16 *out, hipStream_t st);
-// compact input of the first layer: (x, g * g_scale) per voxel, rows padded
-// with a zero voxel on each side ([z][y][nx + 2][2])
-hipError_t launch_pack_input2(const float *x, const float *g, float g_scale, const CnnShape &S, __bf16 *out,
-                              hipStream_t st);
-// first layer (2 -> 32, bias + ReLU) from the compact input
-hipError_t launch_conv3d_first(const __bf16 *in2, const __bf16 *w, const float *bias, __bf16 *out, const CnnShape &S,
-                               hipStream_t st);
+// first layer (2 -> 32, bias + ReLU) of the input pair (x, g * g_scale)
+hipError_t launch_conv3d_first(const float *x, const float *g, float g_scale, const __bf16 *w, const float *bias,
+                               __bf16 *out, const CnnShape &S, hipStream_t st);
 hipError_t launch_mse_grad(const float *pred, const float *target, const CnnShape &S, int border, float inv_count,
                            __bf16 *delta, double *partial, int *nblocks, hipStream_t st);
 hipError_t launch_learned_candidate(const float *x, const float *step, const float *d, float out_scale, float lo,

@@ -398,8 +398,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// First layer (2 -> 32 channels) from the compact padded input (x_k and the
-// scaled SIRT step, 4 bytes per voxel, see k_pack_input2).  K of
+// First layer (2 -> 32 channels) of the input pair (x_k, g_scale * step):
+// BF16 pairs, 4 bytes per voxel, formed while the input is staged.  K of
 // v_mfma_f32_16x16x32_bf16 runs over (tap, channel) pairs instead of
 // channels: 27 taps x 2 channels in two steps of 32, so a row of 16 voxels
 // takes 4 MFMAs where a channel-K layout takes 27 with 14 of every 16
@@ -409,9 +409,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // g < 3, the tap (dz, dy, dx) = (+1, +1, g - 1) of the ninth tap row; the
 // remaining slots have zero weights.
 // A block owns a 16 x 16 tile and walks kC2ZRun output slices: each input
-// slice's 18 x 18 window (with the one-voxel halo) is loaded from global
-// memory once, into a ring of four LDS slots, and the operands are read from
-// LDS.  The slice two ahead loads while the current output slice is formed.
+// slice's 18 x 18 window (with the one-voxel halo; zero outside the volume)
+// is loaded from the two FP32 inputs once, packed to BF16 pairs into a ring
+// of four LDS slots, and the operands are read from LDS.  The slice two
+// ahead loads while the current output slice is formed.
 // Against loading the operands of every output row straight from global
 // memory (each input voxel read 27 times through the L1; texture-unit
 // bound): 0.45 -> 0.26 ms at 256^3, now bound by its 1 GB of output writes.
@@ -421,14 +422,15 @@ constexpr int kC2ZRun = 32;                              // output slices per bl
 constexpr int kC2HX = 18, kC2HY = kC2TY + 2;             // staged window of one input slice
 constexpr int kC2Win = kC2HX * kC2HY;                    // 324 voxels (one dword each)
 constexpr int kC2Ld = (kC2Win + 255) / 256;              // window dwords per thread
-__global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
+__global__ __launch_bounds__(256) void k_conv3d_c2(const float *__restrict__ xin, const float *__restrict__ gin,
+                                                   float g_scale, const __bf16 *__restrict__ wts,
                                                    const float *__restrict__ bias, __bf16 *__restrict__ out, CnnShape S)
 {
     constexpr int R = kC2Rows;
-    __shared__ unsigned ring[4][kC2Win];   // slice zi in slot (zi + 1) mod 4
+    // slice zi in slot (zi + 1) mod 4; window column c is voxel x0 + c - 1
+    __shared__ unsigned ring[4][kC2Win];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
-    const int px = S.nx + 2;   // padded row length (voxels)
     // A[s][h]: K index 8 kg + 2 jj + ch; A row col = output channel
     // 8 (col / 4) + 4 h + col mod 4 (a lane's accumulators are then 8
     // consecutive channels)
@@ -457,51 +459,59 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
     const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + kC2TY - 1) / kC2TY;
     const int zruns = (S.nz + kC2ZRun - 1) / kC2ZRun;
     const int ntasks = tiles_x * tiles_y * zruns;
-    const size_t plane = (size_t)S.ny * px * 2;   // bf16 elements per padded slice
+    const size_t plane = (size_t)S.ny * S.nx;   // voxels per slice
     for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
         const int x0 = (task % tiles_x) * 16, y0 = ((task / tiles_x) % tiles_y) * kC2TY;
         const int z0 = (task / (tiles_x * tiles_y)) * kC2ZRun, z1 = min(S.nz, z0 + kC2ZRun);
         const int x = x0 + col;
-        // window of input slice zi (padded columns x0 .. x0 + 17, rows
-        // y0 - 1 .. y0 + 16): dword i of the window goes to thread i mod 256;
-        // outside the volume it reads as zero (offset past the descriptor)
-        auto fetch = [&](int zi, unsigned (&v)[kC2Ld]) {
+        // window of input slice zi (voxels x0 - 1 .. x0 + 16, rows y0 - 1 ..
+        // y0 + 16): voxel i of the window goes to thread i mod 256; outside
+        // the volume it reads as zero (offset past the descriptor's range)
+        auto slice_rsrc = [&](const float *p, int zi) {
             const bool zok = zi >= 0 && zi < S.nz;
-            const unsigned long long base = (unsigned long long)(in + (zok ? zi : 0) * plane);
+            const unsigned long long base = (unsigned long long)(p + (zok ? zi : 0) * plane);
             const unsigned bhi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
             const unsigned blo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
-            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(((unsigned long long)bhi << 32) | blo), 0,
-                __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(__bf16)) : 0), 0x00020000);
+            return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)bhi << 32) | blo), 0,
+                                                     __builtin_amdgcn_readfirstlane(zok ? (int)(plane * sizeof(float)) : 0),
+                                                     0x00020000);
+        };
+        auto fetch = [&](int zi, float (&vx)[kC2Ld], float (&vg)[kC2Ld]) {
+            const __amdgpu_buffer_rsrc_t rx = slice_rsrc(xin, zi), rg = slice_rsrc(gin, zi);
 #pragma unroll
             for (int k = 0; k < kC2Ld; ++k) {
                 const int i = (int)threadIdx.x + 256 * k, hy = i / kC2HX, hx = i - hy * kC2HX;
-                const int xx = x0 + hx, yy = y0 - 1 + hy;
-                const bool ok = i < kC2Win && xx < px && yy >= 0 && yy < S.ny;
-                const unsigned off = ok ? (unsigned)((yy * px + xx) * 4) : 0x80000000u;
-                v[k] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+                const int xx = x0 - 1 + hx, yy = y0 - 1 + hy;
+                const bool ok = i < kC2Win && xx >= 0 && xx < S.nx && yy >= 0 && yy < S.ny;
+                const unsigned off = ok ? (unsigned)((yy * S.nx + xx) * 4) : 0x80000000u;
+                vx[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, off, 0, 0));
+                vg[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rg, off, 0, 0));
             }
         };
-        auto stash = [&](int zi, const unsigned (&v)[kC2Ld]) {
+        auto stash = [&](int zi, const float (&vx)[kC2Ld], const float (&vg)[kC2Ld]) {
+            typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
             for (int k = 0; k < kC2Ld; ++k) {
                 const int i = (int)threadIdx.x + 256 * k;
-                if (i < kC2Win) ring[(zi + 1) & 3][i] = v[k];
+                bf16x2 p;
+                p[0] = (__bf16)vx[k];
+                p[1] = (__bf16)(vg[k] * g_scale);
+                if (i < kC2Win) ring[(zi + 1) & 3][i] = __builtin_bit_cast(unsigned, p);
             }
         };
-        unsigned v[kC2Ld];
+        float vx[kC2Ld], vg[kC2Ld];
         __syncthreads();   // the previous task's reads of the ring are done
 #pragma unroll 1
         for (int zi = z0 - 1; zi <= z0 + 1; ++zi) {
-            fetch(zi, v);
-            stash(zi, v);
+            fetch(zi, vx, vg);
+            stash(zi, vx, vg);
         }
         __syncthreads();
 #pragma unroll 1
         for (int zo = z0; zo < z1; ++zo) {
             // output slice zo + 1 needs input slice zo + 2
             const bool more = zo + 2 <= z1;
-            if (more) fetch(zo + 2, v);
+            if (more) fetch(zo + 2, vx, vg);
             f32x4 acc[R][2];
 #pragma unroll
             for (int g = 0; g < R; ++g) {
@@ -517,7 +527,7 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
                     b[1] = w[1];
                     b[2] = w[2];
                     b[3] = 0u;
-                    // tap (+1, +1, kg - 1) for kg < 3: slice zo + 1, row y + 1, padded column x + kg
+                    // tap (+1, +1, kg - 1) for kg < 3: slice zo + 1, row y + 1, window column col + kg
                     if (s == 0 && kg < 3) b[3] = ring[(zo + 2) & 3][(wy + 2) * kC2HX + col + kg];
                     B[s] = __builtin_bit_cast(bf16x8, b);
                 }
@@ -545,7 +555,7 @@ __global__ __launch_bounds__(256) void k_conv3d_c2(const __bf16 *__restrict__ in
             }
             // slot of slice zo + 2 last held slice zo - 2, read by output slice
             // zo - 1 before the previous barrier
-            if (more) stash(zo + 2, v);
+            if (more) stash(zo + 2, vx, vg);
             __syncthreads();
         }
     }
@@ -870,28 +880,6 @@ __global__ __launch_bounds__(256) void k_pack_input(const float *__restrict__ x,
     reinterpret_cast<bf16x8 *>(out)[2 * i + 1] = b;
 }
 
-// Compact network input for k_conv3d_c2: (x, g * g_scale) per voxel, rows
-// padded with one zero voxel on each side ([z][y][nx + 2][2]), so that the
-// three x-neighbours of any voxel are one 12-byte load.
-__global__ __launch_bounds__(256) void k_pack_input2(const float *__restrict__ x, const float *__restrict__ g,
-                                                     float g_scale, CnnShape S, __bf16 *__restrict__ out)
-{
-    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-    const size_t n = (size_t)S.nx * S.ny * S.nz;
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const size_t row = i / S.nx;
-    const int xi = (int)(i - row * S.nx);
-    bf16x2 a;
-    a[0] = (__bf16)x[i];
-    a[1] = (__bf16)(g[i] * g_scale);
-    bf16x2 *o = reinterpret_cast<bf16x2 *>(out) + row * (S.nx + 2);
-    o[xi + 1] = a;
-    const bf16x2 zero = {(__bf16)0.0f, (__bf16)0.0f};
-    if (xi == 0) o[0] = zero;
-    if (xi == S.nx - 1) o[S.nx + 1] = zero;
-}
-
 // Masked mean-squared error of pred against target over the interior of
 // the volume (voxels at least `border` from every face) and its gradient,
 // written as the BF16 delta of the last layer (channel 0 of 16).
@@ -1043,20 +1031,12 @@ hipError_t launch_pack_input(const float *x, const float *g, float g_scale, size
     return hipGetLastError();
 }
 
-hipError_t launch_pack_input2(const float *x, const float *g, float g_scale, const CnnShape &S, __bf16 *out,
-                              hipStream_t st)
-{
-    const size_t n = (size_t)S.nx * S.ny * S.nz;
-    hipLaunchKernelGGL(k_pack_input2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, g, g_scale, S, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_conv3d_first(const __bf16 *in2, const __bf16 *w, const float *bias, __bf16 *out, const CnnShape &S,
-                               hipStream_t st)
+hipError_t launch_conv3d_first(const float *x, const float *g, float g_scale, const __bf16 *w, const float *bias,
+                               __bf16 *out, const CnnShape &S, hipStream_t st)
 {
     const long tasks = (long)((S.nx + 15) / 16) * ((S.ny + kC2TY - 1) / kC2TY) * ((S.nz + kC2ZRun - 1) / kC2ZRun);
     const dim3 grid((unsigned)std::min(tasks, 8192L));
-    hipLaunchKernelGGL(k_conv3d_c2, grid, dim3(256), 0, st, in2, w, bias, out, S);
+    hipLaunchKernelGGL(k_conv3d_c2, grid, dim3(256), 0, st, x, g, g_scale, w, bias, out, S);
     return hipGetLastError();
 }
 
