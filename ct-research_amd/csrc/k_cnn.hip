@@ -209,17 +209,21 @@ __device__ __forceinline__ void shift_rows(bf16x8 halo, bf16x8 *centre, bf16x8 *
 // multiplies) and feeds the three output slices zi + 1, zi, zi - 1 through
 // the tap planes dz = -1, 0, +1: three rolling accumulator sets per row, the
 // lowest complete (and stored) once slice zi is done.  A wave owns 16 x 4
-// outputs; a block runs 32 output slices of its 16 x 16 tile.  Against
+// outputs; a block runs zrun output slices of its 16 x 16 tile.  Against
 // loading the three input slices of every output slice (8 rows per wave):
 // vector-memory unit busy 0.52 -> 0.30, 0.96 -> 0.93 ms per layer at 256^3.
 // Forming the shifted rows once per slice instead of once per tap plane,
 // and the halo of all rows in one load and one register instead of one per
 // row: 0.94 -> 0.86 ms on the same GPU.
-constexpr int kC32zRows = 4, kC32zRun = 32;   // output rows per wave, output slices per block task
+constexpr int kC32zRows = 4;   // output rows per wave
+// Output slices per block task (zrun, a launch argument): 64 where that
+// still leaves at least kC32MinTasks tasks, else 32 or 16.  At 256^3, 64
+// against 32: about 2% less time per layer (fewer halo slices per run).
+constexpr int kC32MinTasks = 1024;
 template <int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv3d_c32(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wts,
                                                     const float *__restrict__ bias, const __bf16 *__restrict__ mask,
-                                                    void *__restrict__ out, CnnShape S)
+                                                    void *__restrict__ out, CnnShape S, int zrun)
 {
     constexpr int CIN = 32, R = kC32zRows;
     constexpr int NW = kTaps * 2 * 64;   // bf16x8 vectors, [tap][half][lane]
@@ -235,12 +239,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int col = lane & 15, kg = lane >> 4;
     constexpr int TY = 4 * R;
     const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + TY - 1) / TY;
-    const int zchunks = (S.nz + kC32zRun - 1) / kC32zRun;
+    const int zchunks = (S.nz + zrun - 1) / zrun;
     const int ncols = tiles_x * tiles_y * zchunks;
     const size_t plane = (size_t)S.ny * S.nx * CIN;
     for (int tcol = blockIdx.x; tcol < ncols; tcol += gridDim.x) {
         const int tx = tcol % tiles_x, ty = (tcol / tiles_x) % tiles_y;
-        const int z0 = (tcol / (tiles_x * tiles_y)) * kC32zRun, z1 = min(S.nz, z0 + kC32zRun);
+        const int z0 = (tcol / (tiles_x * tiles_y)) * zrun, z1 = min(S.nz, z0 + zrun);
         const int x0 = tx * 16, x = x0 + col, y0 = ty * TY + wave * R;
         // buffer descriptor over input slice zi (outside the volume: an empty
         // range, every load reads zero)
@@ -951,11 +955,13 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
         else if (epi == kEpiMask) CTR_CONV(16, kEpiMask);
         else return hipErrorInvalidValue;
     } else if (cin_pad == 32) {
-        const long cols = (long)((S.nx + 15) / 16) * ((S.ny + 4 * kC32zRows - 1) / (4 * kC32zRows)) *
-                          ((S.nz + kC32zRun - 1) / kC32zRun);
+        const long tiles = (long)((S.nx + 15) / 16) * ((S.ny + 4 * kC32zRows - 1) / (4 * kC32zRows));
+        int zrun = 64;
+        while (zrun > 16 && tiles * ((S.nz + zrun - 1) / zrun) < kC32MinTasks) zrun /= 2;
+        const long cols = tiles * ((S.nz + zrun - 1) / zrun);
         const dim3 g32((unsigned)std::min(cols, 8192L));
         if (epi == kEpiBiasRelu)
-            hipLaunchKernelGGL((k_conv3d_c32<kEpiBiasRelu>), g32, dim3(256), 0, st, in, w, bias, mask, out, S);
+            hipLaunchKernelGGL((k_conv3d_c32<kEpiBiasRelu>), g32, dim3(256), 0, st, in, w, bias, mask, out, S, zrun);
         else if (epi == kEpiLinearF32) {
             // one real output channel: the taps-as-rows kernel
             const long cols = (long)((S.nx + kO1TX - 1) / kO1TX) * ((S.ny + kO1TY - 1) / kO1TY) *
@@ -964,7 +970,7 @@ hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w
                                (float *)out, S);
         }
         else if (epi == kEpiMask)
-            hipLaunchKernelGGL((k_conv3d_c32<kEpiMask>), g32, dim3(256), 0, st, in, w, bias, mask, out, S);
+            hipLaunchKernelGGL((k_conv3d_c32<kEpiMask>), g32, dim3(256), 0, st, in, w, bias, mask, out, S, zrun);
         else return hipErrorInvalidValue;
     } else {
         return hipErrorInvalidValue;
