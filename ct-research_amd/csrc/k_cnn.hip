@@ -192,15 +192,15 @@ __device__ __forceinline__ bf16x8 shift_cols(bf16x8 halo, bf16x8 centre)
     return __builtin_bit_cast(bf16x8, r);
 }
 constexpr int kDppRowShl = 0x100, kDppRowShr = 0x110;   // + shift, 1 .. 15
-// rows ROW .. NR - 1: left[] = the dx = -1 operands; centre[] becomes the dx = +1 operands
+// rows ROW .. NR - 1: left[] = the dx = -1 operands, right[] = the dx = +1 operands
 template <int ROW, int NR>
-__device__ __forceinline__ void shift_rows(bf16x8 halo, bf16x8 *centre, bf16x8 *left)
+__device__ __forceinline__ void shift_rows(bf16x8 halo, const bf16x8 *centre, bf16x8 *left, bf16x8 *right)
 {
     static_assert(2 * NR < 16, "halo lanes of the left and right edges overlap");
     if constexpr (ROW < NR) {
         left[ROW] = shift_cols<kDppRowShr + 1, kDppRowShl + 1 + ROW>(halo, centre[ROW]);
-        centre[ROW] = shift_cols<kDppRowShl + 1, kDppRowShr + NR - ROW>(halo, centre[ROW]);
-        shift_rows<ROW + 1, NR>(halo, centre, left);
+        right[ROW] = shift_cols<kDppRowShl + 1, kDppRowShr + NR - ROW>(halo, centre[ROW]);
+        shift_rows<ROW + 1, NR>(halo, centre, left, right);
     }
 }
 
@@ -214,7 +214,9 @@ __device__ __forceinline__ void shift_rows(bf16x8 halo, bf16x8 *centre, bf16x8 *
 // vector-memory unit busy 0.52 -> 0.30, 0.96 -> 0.93 ms per layer at 256^3.
 // Forming the shifted rows once per slice instead of once per tap plane,
 // and the halo of all rows in one load and one register instead of one per
-// row: 0.94 -> 0.86 ms on the same GPU.
+// row: 0.94 -> 0.86 ms on the same GPU.  Running the tap planes in order
+// dz = +1, 0, -1 (store between the first and second), so that the rolling
+// accumulator sets need no copies from slice to slice: 0.87 -> 0.81 ms.
 constexpr int kC32zRows = 4;   // output rows per wave
 // Output slices per block task (zrun, a launch argument): 64 where that
 // still leaves at least kC32MinTasks tasks, else 32 or 16.  At 256^3, 64
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform
     const int col = lane & 15, kg = lane >> 4;
     constexpr int TY = 4 * R;
     const int tiles_x = (S.nx + 15) / 16, tiles_y = (S.ny + TY - 1) / TY;
@@ -285,6 +287,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                                      : 0x80000000u;
             return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(slice_rsrc(zi), off, 0, 0));
         };
+        // starting value of an output slice's accumulators: the bias of this
+        // lane's channels 8 kg + 4 h + u (bias and ReLU epilogue), else zero
+        f32x4 acc0[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc0[h][u] = EPI == kEpiBiasRelu ? bias[8 * kg + 4 * h + u] : 0.0f;
         f32x4 acc[3][R][2];   // [0]: output zi - 1, [1]: zi, [2]: zi + 1
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -297,6 +306,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         // (the first step into a fresh output slice) the first product of
         // each accumulator starts from zero.
         auto mma_dx = [&](int dzi, int dxi, const bf16x8 *rows, bool init) {
+            // a fresh output slice starts from its bias (zero for the masked epilogue)
             bf16x8 a[3][2];
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy)
@@ -311,18 +321,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
                         acc[2 - dzi][g][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            a[dy][h], rows[r], init && dy == 0 ? f32x4{0.0f, 0.0f, 0.0f, 0.0f} : acc[2 - dzi][g][h],
+                            a[dy][h], rows[r], init && dy == 0 ? acc0[h] : acc[2 - dzi][g][h],
                             0, 0, 0);
                 }
             }
         };
         // The centre and halo rows of slice zi + 1 load while slice zi
         // multiplies.  The shifted rows (dx = -1, +1) are formed once per
-        // slice and feed all three tap planes; the dx = +1 rows overwrite the
-        // centre rows, which the slice no longer needs.
-        bf16x8 cur[R + 2], nxt[R + 2], sr[R + 2];
+        // slice and feed all three tap planes.  The planes run in the order
+        // dz = +1 (completing output slice zi - 1, stored right after), 0,
+        // -1 (starting output slice zi + 1): the registers of the stored
+        // slice then carry on as output slice zi, and those of zi as zi + 1,
+        // with no copies between slices.
+        bf16x8 cur[R + 2], nxt[R + 2], sl[R + 2], sh[R + 2];
+        // output: slices of oplane elements, rows of ostride bytes
+        const size_t oplane = (size_t)S.ny * S.nx * 32;
+        const unsigned ostride = (unsigned)S.nx * 32 * sizeof(__bf16);
+        const int xoff_out = x < S.nx ? (int)((x * 32 + 8 * kg) * sizeof(__bf16)) : (int)0x80000000u;
         load_slice(z0 - 1, cur);
         bf16x8 hal = load_halo(z0 - 1), nhal;
+        // plane dzi of slice zi: output slice zi + 1 - dzi, skipped (uniform)
+        // where that slice lies outside this run (its accumulators are then
+        // never stored).  The empty asm statements keep each step's weight
+        // reads (and their registers) from being hoisted over the others.
+        auto plane = [&](int zi, int dzi) {
+            if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) return;
+            mma_dx(dzi, 1, cur, dzi == 0);
+            asm volatile("" ::: "memory");
+            mma_dx(dzi, 0, sl, false);
+            asm volatile("" ::: "memory");
+            mma_dx(dzi, 2, sh, false);
+            asm volatile("" ::: "memory");
+        };
 #pragma unroll 1
         for (int zi = z0 - 1; zi <= z1; ++zi) {
             const bool more = zi + 1 <= z1;
@@ -330,61 +360,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 load_slice(zi + 1, nxt);
                 nhal = load_halo(zi + 1);
             }
-            if (zi >= 0 && zi < S.nz) {   // uniform: a slice outside the volume contributes zero
-                // the empty asm statements keep each step's weight reads (and
-                // their registers) from being hoisted over the others.  Plane
-                // dzi feeds output slice zi + 1 - dzi: skipped (uniform) where
-                // that slice lies outside this run, whose accumulators are
-                // then never stored.
+            const bool zin = zi >= 0 && zi < S.nz;   // uniform: a slice outside the volume contributes zero
+            if (zin) {
+                shift_rows<0, R + 2>(hal, cur, sl, sh);
+                plane(zi, 2);
+            }
+            // output slice zi - 1 has all three planes now: one 16-byte store
+            // per row through a descriptor over that slice (a lane past the
+            // volume's x extent has an offset past its range: no store)
+            const int zo = zi - 1;
+            if (zo >= z0 && zo < z1) {
+                const unsigned long long ob = (unsigned long long)(reinterpret_cast<__bf16 *>(out) + zo * oplane);
+                const __amdgpu_buffer_rsrc_t orsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(ob >> 32)) << 32) |
+                             (unsigned)__builtin_amdgcn_readfirstlane((unsigned)ob)),
+                    0, (int)(oplane * sizeof(__bf16)), 0x00020000);
 #pragma unroll
-                for (int dzi = 0; dzi < 3; ++dzi) {
-                    if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
-                    mma_dx(dzi, 1, cur, dzi == 0);
-                    asm volatile("" ::: "memory");
-                }
-                shift_rows<0, R + 2>(hal, cur, sr);
+                for (int g = 0; g < R; ++g) {
+                    const int y = y0 + g;   // uniform
+                    if (y >= S.ny) continue;
+                    typedef float f32x2 __attribute__((ext_vector_type(2)));
+                    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+                    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                    u32x4 o;
 #pragma unroll
-                for (int dzi = 0; dzi < 3; ++dzi) {
-                    if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
-                    mma_dx(dzi, 0, sr, false);
-                    asm volatile("" ::: "memory");
-                }
+                    for (int j = 0; j < 4; ++j) {
+                        f32x2 v = {acc[0][g][j >> 1][2 * (j & 1)], acc[0][g][j >> 1][2 * (j & 1) + 1]};
+                        if (EPI != kEpiBiasRelu) {
+                            const size_t vox = ((size_t)zo * S.ny + y) * S.nx + x;
 #pragma unroll
-                for (int dzi = 0; dzi < 3; ++dzi) {
-                    if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) continue;
-                    mma_dx(dzi, 2, cur, false);
-                    asm volatile("" ::: "memory");
+                            for (int e = 0; e < 2; ++e)
+                                v[e] = x < S.nx && bf2f(mask[vox * 32 + 8 * kg + 2 * j + e]) > 0.0f ? v[e] : 0.0f;
+                        }
+                        o[j] = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+                    }
+                    if (EPI == kEpiBiasRelu) {
+                        // ReLU on the BF16 bit patterns: a negative value has
+                        // the sign bit set, so it is a negative int16
+                        typedef short i16x8 __attribute__((ext_vector_type(8)));
+                        o = __builtin_bit_cast(u32x4, __builtin_elementwise_max(__builtin_bit_cast(i16x8, o),
+                                                                                 (i16x8){0, 0, 0, 0, 0, 0, 0, 0}));
+                    }
+                    __builtin_amdgcn_raw_buffer_store_b128(o, orsrc, xoff_out, (int)((unsigned)y * ostride), 0);
                 }
+            }
+            if (zin) {
+                plane(zi, 1);
+                plane(zi, 0);
             } else {
 #pragma unroll
                 for (int g = 0; g < R; ++g)
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) acc[2][g][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            }
-            // output slice zi - 1 has all three planes now
-            const int zo = zi - 1;
-            if (zo >= z0 && zo < z1 && x < S.nx) {
-#pragma unroll
-                for (int g = 0; g < R; ++g) {
-                    const int y = y0 + g;
-                    if (y >= S.ny) continue;
-                    const size_t vox = ((size_t)zo * S.ny + y) * S.nx + x;
-                    const int c0 = 8 * kg;
-                    bf16x8 o;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h)
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            float v = acc[0][g][h][u];
-                            if (EPI == kEpiBiasRelu) {
-                                v = fmaxf(v + bias[c0 + 4 * h + u], 0.0f);
-                            } else {
-                                v = bf2f(mask[vox * 32 + c0 + 4 * h + u]) > 0.0f ? v : 0.0f;
-                            }
-                            o[4 * h + u] = (__bf16)v;
-                        }
-                    *reinterpret_cast<bf16x8 *>(reinterpret_cast<__bf16 *>(out) + vox * 32 + c0) = o;
-                }
+                    for (int h = 0; h < 2; ++h) acc[2][g][h] = acc0[h];
             }
 #pragma unroll
             for (int g = 0; g < R; ++g)
