@@ -216,7 +216,8 @@ __device__ __forceinline__ void shift_rows(bf16x8 halo, const bf16x8 *centre, bf
 // and the halo of all rows in one load and one register instead of one per
 // row: 0.94 -> 0.86 ms on the same GPU.  Running the tap planes in order
 // dz = +1, 0, -1 (store between the first and second), so that the rolling
-// accumulator sets need no copies from slice to slice: 0.87 -> 0.81 ms.
+// accumulator sets need no copies from slice to slice: 0.87 -> 0.81 ms;
+// the store after the first step of the second plane: 0.83 -> 0.81 ms.
 constexpr int kC32zRows = 4;   // output rows per wave
 // Output slices per block task (zrun, a launch argument): 64 where that
 // still leaves at least kC32MinTasks tasks, else 32 or 16.  At 256^3, 64
@@ -340,17 +341,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int xoff_out = x < S.nx ? (int)((x * 32 + 8 * kg) * sizeof(__bf16)) : (int)0x80000000u;
         load_slice(z0 - 1, cur);
         bf16x8 hal = load_halo(z0 - 1), nhal;
-        // plane dzi of slice zi: output slice zi + 1 - dzi, skipped (uniform)
-        // where that slice lies outside this run (its accumulators are then
-        // never stored).  The empty asm statements keep each step's weight
+        // step (dzi, dxi) of slice zi: output slice zi + 1 - dzi, skipped
+        // (uniform) where that slice lies outside this run (its accumulators
+        // are then never stored).  The empty asm statements keep each step's weight
         // reads (and their registers) from being hoisted over the others.
-        auto plane = [&](int zi, int dzi) {
+        auto step = [&](int zi, int dzi, int dxi) {
             if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) return;
-            mma_dx(dzi, 1, cur, dzi == 0);
-            asm volatile("" ::: "memory");
-            mma_dx(dzi, 0, sl, false);
-            asm volatile("" ::: "memory");
-            mma_dx(dzi, 2, sh, false);
+            mma_dx(dzi, dxi, dxi == 1 ? cur : dxi == 0 ? sl : sh, dzi == 0 && dxi == 1);
             asm volatile("" ::: "memory");
         };
 #pragma unroll 1
@@ -363,7 +360,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             const bool zin = zi >= 0 && zi < S.nz;   // uniform: a slice outside the volume contributes zero
             if (zin) {
                 shift_rows<0, R + 2>(hal, cur, sl, sh);
-                plane(zi, 2);
+                step(zi, 2, 1);
+                step(zi, 2, 0);
+                step(zi, 2, 2);
+                // one step of the next plane before the store, whose
+                // conversions would otherwise wait on the MFMAs just issued
+                step(zi, 1, 1);
             }
             // output slice zi - 1 has all three planes now: one 16-byte store
             // per row through a descriptor over that slice (a lane past the
@@ -405,8 +407,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 }
             }
             if (zin) {
-                plane(zi, 1);
-                plane(zi, 0);
+                step(zi, 1, 0);
+                step(zi, 1, 2);
+                step(zi, 0, 1);
+                step(zi, 0, 0);
+                step(zi, 0, 2);
             } else {
 #pragma unroll
                 for (int g = 0; g < R; ++g)
