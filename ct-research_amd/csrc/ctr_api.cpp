@@ -2865,15 +2865,23 @@ struct BatchRun {
     DevBuf fp_jobs, red_jobs, adj_jobs, adj_ones_jobs, sums, norms, iter;
     hipStream_t cs = nullptr;
     hipEvent_t ev = nullptr;
+    std::vector<hipStream_t> ws;   // streams of the per-problem launches (joined into cs)
+    std::vector<hipEvent_t> wev;   // their join events, then the fork event
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     explicit BatchRun(ctr_ctx *c, int n) : ctx(c), probs(n) {}
     ~BatchRun()
     {
         if (cs) (void)hipStreamSynchronize(cs);
+        for (hipStream_t w : ws)
+            if (w) (void)hipStreamSynchronize(w);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (ev) (void)hipEventDestroy(ev);
+        for (hipEvent_t e : wev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t w : ws)
+            if (w) (void)hipStreamDestroy(w);
         if (cs) (void)hipStreamDestroy(cs);
         (void)hipStreamSynchronize(ctx->stream);
         for (auto &q : probs) free_batch_problem(q);
@@ -3002,20 +3010,65 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     HIP_TRY(ctx, hipEventCreateWithFlags(&B.ev, hipEventDisableTiming));
     HIP_TRY(ctx, hipEventRecord(B.ev, st));
     HIP_TRY(ctx, hipStreamWaitEvent(B.cs, B.ev, 0));
+    // CTR_BATCH_STREAMS = K > 0 (default 4): each problem runs its own
+    // single-problem kernels (its tuned adjoint variant) on stream p mod K,
+    // forked from and joined into the iteration's stream, so that problems
+    // overlap; 0: the job-table kernels, one launch per phase for all problems.
+    int nstreams = 4;
+    if (const char *e = std::getenv("CTR_BATCH_STREAMS")) nstreams = std::max(0, std::atoi(e));
+    nstreams = std::min(nstreams, n);
+    for (int j = 0; j < nstreams; ++j) {
+        B.ws.push_back(nullptr);
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&B.ws.back(), hipStreamNonBlocking));
+    }
+    for (int j = 0; j <= nstreams && nstreams > 0; ++j) {
+        B.wev.push_back(nullptr);
+        HIP_TRY(ctx, hipEventCreateWithFlags(&B.wev.back(), hipEventDisableTiming));
+    }
+    // residual sums: slot k of problem p at 2 (p nslots + k) with streams,
+    // at 2 (k n + p) with the job-table reduction
+    auto sum_index = [&](int p, size_t k) { return nstreams ? 2 * ((size_t)p * nslots + k) : 2 * (k * n + p); };
     auto iteration = [&](hipStream_t s, bool update) -> ctr_status {
-        for (int p = 0; p < n; ++p) {
-            const BatchProblem &q = B.probs[p];
-            if (q.R.volT)
-                TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, s),
-                                 "volume transpose"));
+        if (nstreams) {
+            HIP_TRY(ctx, hipEventRecord(B.wev[nstreams], s));
+            for (int j = 0; j < nstreams; ++j) HIP_TRY(ctx, hipStreamWaitEvent(B.ws[j], B.wev[nstreams], 0));
+            for (int p = 0; p < n; ++p) {
+                const BatchProblem &q = B.probs[p];
+                hipStream_t t = B.ws[p % nstreams];
+                if (q.R.volT)
+                    TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, t),
+                                     "volume transpose"));
+                int nblk = 0;
+                TRY(check_launch(ctx, launch_ray_fp_residual(q.R, q.xp, q.bp, (float *)q.r.ptr, (double *)q.part.ptr,
+                                                             &nblk, t), "residual"));
+                TRY(check_launch(ctx, launch_reduce_jobs(drj + p, 1, diter, dsums + sum_index(p, 0), t), "reduce"));
+                if (update) {
+                    const SirtUpdate U{(const float *)q.vinv.ptr, q.xp, o->lambda, o->min_val, o->max_val};
+                    TRY(check_launch(ctx, launch_adjoint(q.P, q.F.plan, (const int4 *)q.F.buf.ptr,
+                                                         (const float *)q.r.ptr, nullptr, &U, 0, t), "update"));
+                }
+            }
+            for (int j = 0; j < nstreams; ++j) {
+                HIP_TRY(ctx, hipEventRecord(B.wev[j], B.ws[j]));
+                HIP_TRY(ctx, hipStreamWaitEvent(s, B.wev[j], 0));
+            }
+        } else {
+            for (int p = 0; p < n; ++p) {
+                const BatchProblem &q = B.probs[p];
+                if (q.R.volT)
+                    TRY(check_launch(ctx, launch_transpose_st(q.xp, (float *)q.vt.ptr, q.R.nt, q.R.ns, q.R.nz, s),
+                                     "volume transpose"));
+            }
+            TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_max, s), "batched residual"));
+            TRY(check_launch(ctx, launch_reduce_jobs(drj, n, diter, dsums, s), "batched reduce"));
+            if (update)
+                for (int gi = 0; gi < 3; ++gi)
+                    if (!groups[gi].empty())
+                        TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], group_fit[gi] != 0,
+                                                                     dadj + gofs[gi], (int)groups[gi].size(),
+                                                                     group_max[gi], 1, 0, s), "batched update"));
         }
-        TRY(check_launch(ctx, launch_ray_fp_residual_batched(dfp, n, fp_max, s), "batched residual"));
-        TRY(check_launch(ctx, launch_reduce_jobs(drj, n, diter, dsums, s), "batched reduce"));
         if (!update) return CTR_OK;
-        for (int gi = 0; gi < 3; ++gi)
-            if (!groups[gi].empty())
-                TRY(check_launch(ctx, launch_adjoint_batched(pitches[gi], group_cm[gi], group_fit[gi] != 0, dadj + gofs[gi], (int)groups[gi].size(),
-                                                             group_max[gi], 1, 0, s), "batched update"));
         return check_launch(ctx, launch_iter_inc(diter, s), "iteration counter");
     };
     if (o->n_iter > 0) {
@@ -3038,7 +3091,7 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
     for (int p = 0; p < n; ++p) {
         if (pr[p].residual_l2) {
             const double bn = std::sqrt(hn[2 * p]) > 0 ? std::sqrt(hn[2 * p]) : 1.0;
-            for (size_t k = 0; k < nslots; ++k) pr[p].residual_l2[k] = std::sqrt(hs[2 * (k * n + p)]) / bn;
+            for (size_t k = 0; k < nslots; ++k) pr[p].residual_l2[k] = std::sqrt(hs[sum_index(p, k)]) / bn;
         }
         if (!dev)
             HIP_TRY(ctx, hipMemcpy(pr[p].x, B.probs[p].xp, B.probs[p].nvox * sizeof(float), hipMemcpyDeviceToHost));
