@@ -3071,7 +3071,11 @@ ctr_status ctr_sirt3d_batched(ctr_ctx *ctx, const ctr_sirt_problem *pr, int n, c
         if (!update) return CTR_OK;
         return check_launch(ctx, launch_iter_inc(diter, s), "iteration counter");
     };
-    if (o->n_iter > 0) {
+    bool use_graph = true;
+    if (const char *e = std::getenv("CTR_BATCH_GRAPH")) use_graph = std::atoi(e) != 0;
+    if (o->n_iter > 0 && !use_graph) {
+        for (int k = 0; k < o->n_iter; ++k) TRY(iteration(B.cs, true));
+    } else if (o->n_iter > 0) {
         HIP_TRY(ctx, hipStreamBeginCapture(B.cs, hipStreamCaptureModeThreadLocal));
         ctr_status cap = iteration(B.cs, true);
         hipGraph_t graph = nullptr;
