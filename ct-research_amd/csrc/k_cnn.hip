@@ -975,9 +975,14 @@ int conv_grid(const CnnShape &S, int cin)
 
 }  // namespace
 
+// The kernels address a slice through a buffer descriptor with a 32-bit
+// byte range: one slice of 32 BF16 channels must stay below 2 GiB.
+static bool slice_fits(const CnnShape &S) { return (size_t)S.nx * S.ny * 32 * sizeof(__bf16) < (size_t(1) << 31); }
+
 hipError_t launch_conv3d(int cin_pad, int epi, const __bf16 *in, const __bf16 *w, const float *bias, const __bf16 *mask,
                          void *out, const CnnShape &S, hipStream_t st)
 {
+    if (!slice_fits(S)) return hipErrorInvalidValue;
     const int grid = conv_grid(S, cin_pad);
 #define CTR_CONV(CIN, EPI)                                                                                          \
     hipLaunchKernelGGL((k_conv3d<CIN, EPI>), dim3(grid), dim3(256), 0, st, in, w, bias, mask, out, S)
@@ -1072,6 +1077,7 @@ hipError_t launch_pack_input(const float *x, const float *g, float g_scale, size
 hipError_t launch_conv3d_first(const float *x, const float *g, float g_scale, const __bf16 *w, const float *bias,
                                __bf16 *out, const CnnShape &S, hipStream_t st)
 {
+    if (!slice_fits(S)) return hipErrorInvalidValue;
     const long tasks = (long)((S.nx + 15) / 16) * ((S.ny + kC2TY - 1) / kC2TY) * ((S.nz + kC2ZRun - 1) / kC2ZRun);
     const dim3 grid((unsigned)std::min(tasks, 8192L));
     hipLaunchKernelGGL(k_conv3d_c2, grid, dim3(256), 0, st, x, g, g_scale, w, bias, out, S);
