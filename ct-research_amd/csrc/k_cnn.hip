@@ -208,7 +208,7 @@ __device__ __forceinline__ void shift_rows(bf16x8 halo, const bf16x8 *centre, bf
 // loaded once (centre and halo rows prefetched while the previous slice
 // multiplies) and feeds the three output slices zi + 1, zi, zi - 1 through
 // the tap planes dz = -1, 0, +1: three rolling accumulator sets per row, the
-// lowest complete (and stored) once slice zi is done.  A wave owns 16 x 4
+// lowest complete (and stored) once slice zi's dz = +1 plane is in.  A wave owns 16 x 4
 // outputs; a block runs zrun output slices of its 16 x 16 tile.  Against
 // loading the three input slices of every output slice (8 rows per wave):
 // vector-memory unit busy 0.52 -> 0.30, 0.96 -> 0.93 ms per layer at 256^3.
@@ -305,9 +305,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         // taps (dzi, dxi, dy) of every row into the output slice acc[2 - dzi];
         // `rows` holds the operands of column offset dxi - 1.  With `init`
         // (the first step into a fresh output slice) the first product of
-        // each accumulator starts from zero.
+        // each accumulator starts from acc0 (the bias, or zero).
         auto mma_dx = [&](int dzi, int dxi, const bf16x8 *rows, bool init) {
-            // a fresh output slice starts from its bias (zero for the masked epilogue)
             bf16x8 a[3][2];
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy)
@@ -343,8 +342,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         bf16x8 hal = load_halo(z0 - 1), nhal;
         // step (dzi, dxi) of slice zi: output slice zi + 1 - dzi, skipped
         // (uniform) where that slice lies outside this run (its accumulators
-        // are then never stored).  The empty asm statements keep each step's weight
-        // reads (and their registers) from being hoisted over the others.
+        // are then never stored).  The empty asm statements keep each step's
+        // weight reads (and their registers) from being hoisted over the
+        // others.
         auto step = [&](int zi, int dzi, int dxi) {
             if (zi + 1 - dzi < z0 || zi + 1 - dzi >= z1) return;
             mma_dx(dzi, dxi, dxi == 1 ? cur : dxi == 0 ? sl : sh, dzi == 0 && dxi == 1);
