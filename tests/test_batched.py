@@ -1,6 +1,7 @@
-"""Batched SIRT3D: uneven problems advanced by shared, graph-replayed
-launches must give exactly the iterates of one ctr_sirt3d call per problem
-(the same per-ray and per-tile arithmetic, only the block->job mapping differs)."""
+"""Batched SIRT3D: uneven problems advanced in one graph-replayed iteration
+(per-problem kernels on forked streams, or job-table kernels shared by all
+problems) must give exactly the iterates of one ctr_sirt3d call per problem
+(the same per-ray and per-tile arithmetic)."""
 import numpy as np
 import pytest
 
@@ -38,6 +39,27 @@ def test_batched_sirt_matches_single(ctx, device):
         dx = float(np.abs(xs[i] - x1).max())
         dr = float(np.abs(res[i] - r1).max())
         report(f"sirt3d_batched[{i}, device={device}]", max_dx=dx, max_dres=dr, res_last=float(res[i][-1]))
+        assert dx <= 1e-6 * max(float(np.abs(x1).max()), 1.0)
+        assert dr <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("streams,graph", [("0", "1"), ("4", "0"), ("1", "1"), ("2", "1")])
+def test_batched_sirt_launch_forms_match_single(ctx, monkeypatch, streams, graph):
+    """Every launch form of the batch (job-table kernels, per-problem kernels
+    on 1, 2 or 4 streams, with or without graph capture) gives the single-call
+    iterates."""
+    monkeypatch.setenv("CTR_BATCH_STREAMS", streams)
+    monkeypatch.setenv("CTR_BATCH_GRAPH", graph)
+    probs = _problems()
+    bs = [ctx.cone_fp(g, v) for g, v in probs]
+    n_iter = 5
+    xs, res = ctx.sirt3d_batched([(g, b) for (g, _), b in zip(probs, bs)], n_iter)
+    for i, ((g, _), b) in enumerate(zip(probs, bs)):
+        x1, r1, _ = ctx.sirt3d(g, b, n_iter)
+        dx = float(np.abs(xs[i] - x1).max())
+        dr = float(np.abs(res[i] - r1).max())
+        report(f"sirt3d_batched[{i}, streams={streams}, graph={graph}]", max_dx=dx, max_dres=dr)
         assert dx <= 1e-6 * max(float(np.abs(x1).max()), 1.0)
         assert dr <= 1e-9
 
