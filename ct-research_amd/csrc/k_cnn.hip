@@ -444,15 +444,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // consecutive voxels of the padded row, both channels) and, for s = 0 and
 // g < 3, the tap (dz, dy, dx) = (+1, +1, g - 1) of the ninth tap row; the
 // remaining slots have zero weights.
-// A block owns a 16 x 16 tile and walks kC2ZRun output slices: each input
-// slice's 18 x 18 window (with the one-voxel halo; zero outside the volume)
+// A block owns a 16 x 32 tile (8 rows per wave) and walks kC2ZRun output
+// slices: each input slice's 18 x 34 window (with the one-voxel halo; zero
+// outside the volume)
 // is loaded from the two FP32 inputs once, packed to BF16 pairs into a ring
 // of four LDS slots, and the operands are read from LDS.  The slice two
 // ahead loads while the current output slice is formed.
 // Against loading the operands of every output row straight from global
 // memory (each input voxel read 27 times through the L1; texture-unit
 // bound): 0.45 -> 0.26 ms at 256^3, now bound by its 1 GB of output writes.
-constexpr int kC2Rows = 4;                               // output rows per wave
+// Rows per wave 2 / 4 / 8 / 16: 0.32 / 0.275 / 0.265 / 0.31 ms.
+constexpr int kC2Rows = 8;                               // output rows per wave
 constexpr int kC2TY = 4 * kC2Rows;                       // tile rows per block
 constexpr int kC2ZRun = 32;                              // output slices per block task
 constexpr int kC2HX = 18, kC2HY = kC2TY + 2;             // staged window of one input slice
