@@ -30,6 +30,8 @@ constexpr int kBX = 64, kBY = 4;
 // 16 x 4 measured 18.7 ms at 256^3 / 360 views and 257 ms at 512^3 / 720,
 // 32 x 2 18.8 / 259 ms, 64 x 1 19.1 / 266 ms
 constexpr int kWaveCols = 16;
+static_assert(kWaveCols != 16 || (kBX == 64 && kBY == 4), "16-column wavefronts of 4 rows tile a 64 x 4 block");
+static_assert(kWaveCols != 32 || (kBX == 64 && kBY % 2 == 0), "32-column wavefronts of 2 rows need an even block height");
 
 // Column and row of this thread's ray within its block.
 __device__ __forceinline__ void ray_of_thread(int &dcol, int &drow)
